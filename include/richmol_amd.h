@@ -1,0 +1,130 @@
+/*
+ * richmol_amd: MI355X (gfx950) device path for the field-driven rotational
+ * TDSE step of richmol (richmol.tdse.TDSE.update).
+ *
+ * C ABI: plain pointers and sizes, no C++ or torch types.  Complex numbers are
+ * interleaved (re, im) doubles.  "device" pointers are HIP device addresses
+ * (e.g. torch.Tensor.data_ptr() of a CUDA/HIP tensor); "host" pointers are
+ * ordinary process memory.  Every entry point returns RM_OK (0) or a status
+ * code and never throws; rm_last_error() returns the message of the last
+ * failure on the calling thread.  Kernels are enqueued asynchronously on the
+ * given hipStream_t (passed as void*; NULL = default stream).  Calls on one
+ * plan must be serialised by the caller.
+ *
+ * The operator of a plan is the reference's field-contracted block operator
+ * flattened to a sparse matrix in the reference's full-form state order
+ * (richmol/field.py:659-692):
+ *
+ *   H(F) = sum_e A_e |row_e><col_e|
+ *   A_e  = sum_{p in pairs(e)} K_p * MF_{term_p}                 (K:  field.py:1235)
+ *   MF_t = thr_{leaf_t}( sum_{q in t} fprod_{field_q} * M_q )     (MF: field.py:1115-1142)
+ *
+ * where fprod are the field products (one slot per Cartesian component of
+ * each tensor in a sum) and thr zeroes |MF_t| < thresh (thresh < 0: none).
+ */
+#ifndef RICHMOL_AMD_H
+#define RICHMOL_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RM_OK              0
+#define RM_NOT_CONVERGED   1    /* a column hit maxorder (tdse.py:480-484) */
+#define RM_ERR_ARG        -1
+#define RM_ERR_HIP        -2
+#define RM_ERR_ALLOC      -3
+
+#define RM_MAX_FIELD      32    /* field-product slots per plan */
+#define RM_MAX_LEAF        8    /* tensors summed into one operator */
+#define RM_MAX_KRYLOV    128    /* Krylov dimension bound (reference default maxorder=100) */
+
+typedef struct rm_plan rm_plan;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* rm_last_error(void);
+
+/* Library ABI version. */
+int rm_version(void);
+
+/*
+ * Build a device plan for one operator structure.  Replaces the per-call
+ * block loop of CarTens.vec (richmol/field.py:1212-1243) and the per-step
+ * MF contraction of CarTens.field (field.py:1115-1142): the structure is
+ * uploaded once, afterwards only field products cross the ABI.
+ *
+ *   n                      basis dimension (rows = columns of H)
+ *   nnz, row_ptr[n+1], col_idx[nnz]              CSR pattern (host)
+ *   pair_ptr[nnz+1], pair_term[npair], pair_kval[2*npair]   K factors (host)
+ *   nterm, term_ptr[nterm+1], term_field[nq], term_mval[2*nq], term_leaf[nterm]
+ *                                                M factors per MF element (host)
+ *   nfield <= RM_MAX_FIELD, nleaf <= RM_MAX_LEAF
+ */
+int rm_plan_create(int64_t n,
+                   int64_t nnz, const int64_t* row_ptr, const int32_t* col_idx,
+                   const int64_t* pair_ptr, const int32_t* pair_term,
+                   const double* pair_kval,
+                   int64_t nterm, const int64_t* term_ptr, const int32_t* term_field,
+                   const double* term_mval, const int32_t* term_leaf,
+                   int32_t nfield, int32_t nleaf,
+                   rm_plan** out);
+
+int rm_plan_destroy(rm_plan* plan);
+
+/* Basis dimension and number of stored matrix entries of a plan. */
+int rm_plan_info(const rm_plan* plan, int64_t* n, int64_t* nnz);
+
+/*
+ * Contract the operator with the field of one time step (CarTens.field,
+ * field.py:1073-1142) on the device.  fprod: host, 2*nfield doubles;
+ * thresh: host, nleaf doubles (< 0 = no threshold).  Also records whether
+ * any MF element is non-zero; rm_step skips the Krylov propagation when none
+ * is (the `len(H.mfmat) > 0` test of tdse.py:377).
+ */
+int rm_plan_set_field(rm_plan* plan, const double* fprod, const double* thresh,
+                      void* stream);
+
+/* y[:, b] = H x[:, b] for b < ncols (CarTens.vec, field.py:1145-1245).
+ * x, y: device, complex, column b at offset b*ld (ld >= n). */
+int rm_matvec(rm_plan* plan, const void* x, void* y, int64_t ld, int64_t ncols,
+              void* stream);
+
+/* phase[i] = exp(f/2 * diag[i])  (tdse.py:373).  diag: device, complex n;
+ * phase: device, complex n. */
+int rm_phase_init(int64_t n, const void* diag, double f_re, double f_im,
+                  void* phase, void* stream);
+
+/*
+ * One propagation step for ncols wavepackets (TDSE.update, tdse.py:365-414;
+ * Lanczos expv tdse.py:417-486), FP64:
+ *   phi = P*psi;  phi = expv(f*H, phi) if the field is non-zero;  out = P*phi
+ * (P = phase, or identity when phase == NULL).
+ *   psi_in/psi_out: device, complex, column b at offset b*ld (may alias).
+ *   krylov_fixed > 0: fixed Krylov dimension (tol/maxorder ignored);
+ *   krylov_fixed == 0: the reference's adaptive rule with tol and maxorder.
+ *   kused (device int32[ncols], may be NULL): Krylov dimension used per column
+ *   (0 when the step skipped the expv).
+ *   status (device int32[ncols], may be NULL): 0 ok, RM_NOT_CONVERGED.
+ */
+int rm_step(rm_plan* plan, const void* psi_in, void* psi_out, int64_t ld,
+            int64_t ncols, double f_re, double f_im, const void* phase,
+            int32_t krylov_fixed, int32_t maxorder, double tol,
+            int32_t* kused, int32_t* status, void* stream);
+
+/* psi_out[:, b] = (psi_in[:, b] * phase) * phase: the step when the operator
+ * carries no field at all (tdse.py:375, 394 with the Lanczos skipped). */
+int rm_phase_apply(int64_t n, const void* psi_in, void* psi_out, int64_t ld, int64_t ncols,
+                   const void* phase, void* stream);
+
+/* out[b] = <psi_b| H |psi_b> (complex, device 2*ncols doubles), the
+ * quadratic form of the observable loops in examples/ocs_alignment.py:99-100. */
+int rm_expect(rm_plan* plan, const void* psi, int64_t ld, int64_t ncols,
+              void* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RICHMOL_AMD_H */

@@ -1,0 +1,76 @@
+"""ctypes binding of librichmol_amd.so (the C ABI of include/richmol_amd.h).
+
+The library is built in-tree by ``python -m richmol_amd._build``.  Loading
+fails loudly when it is missing: there is no CPU fallback.
+"""
+import ctypes
+import os
+
+from ._build import LIB_PATH
+
+RM_OK = 0
+RM_NOT_CONVERGED = 1
+RM_MAX_FIELD = 32
+RM_MAX_LEAF = 8
+RM_MAX_KRYLOV = 128
+
+_c_i64p = ctypes.POINTER(ctypes.c_int64)
+_c_i32p = ctypes.POINTER(ctypes.c_int32)
+_c_dblp = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+
+# name -> (restype, argtypes); every symbol declared in include/richmol_amd.h
+SIGNATURES = {
+    "rm_last_error": (ctypes.c_char_p, []),
+    "rm_version": (ctypes.c_int, []),
+    "rm_plan_create": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, _c_i64p, _c_i32p, _c_i64p,
+                                      _c_i32p, _c_dblp, ctypes.c_int64, _c_i64p, _c_i32p, _c_dblp,
+                                      _c_i32p, ctypes.c_int32, ctypes.c_int32,
+                                      ctypes.POINTER(_vp)]),
+    "rm_plan_destroy": (ctypes.c_int, [_vp]),
+    "rm_plan_info": (ctypes.c_int, [_vp, _c_i64p, _c_i64p]),
+    "rm_plan_set_field": (ctypes.c_int, [_vp, _c_dblp, _c_dblp, _vp]),
+    "rm_matvec": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int64, ctypes.c_int64, _vp]),
+    "rm_phase_init": (ctypes.c_int, [ctypes.c_int64, _vp, ctypes.c_double, ctypes.c_double, _vp, _vp]),
+    "rm_step": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_double,
+                               ctypes.c_double, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
+                               _vp, _vp, _vp]),
+    "rm_phase_apply": (ctypes.c_int, [ctypes.c_int64, _vp, _vp, ctypes.c_int64, ctypes.c_int64, _vp, _vp]),
+    "rm_expect": (ctypes.c_int, [_vp, _vp, ctypes.c_int64, ctypes.c_int64, _vp, _vp]),
+}
+
+_LIB = None
+
+
+class RichmolAmdError(RuntimeError):
+    pass
+
+
+def load():
+    """Load the library (once) and attach the ctypes signatures."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RichmolAmdError(
+            f"{LIB_PATH} is missing: build it with `python -m richmol_amd._build` "
+            "(the HIP path has no CPU fallback)")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = lib
+    return lib
+
+
+def check(rc, what=""):
+    if rc < 0:
+        msg = load().rm_last_error().decode(errors="replace")
+        raise RichmolAmdError(f"{what}: {msg} (status {rc})")
+    return rc
+
+
+def ptr(arr, ctype):
+    """ctypes pointer to the data of a C-contiguous numpy array."""
+    return arr.ctypes.data_as(ctypes.POINTER(ctype))

@@ -1,0 +1,269 @@
+"""Device plan: a CarTens operator flattened once and uploaded to the GPU.
+
+The reference applies its operator block by block every matvec
+(richmol/field.py:1212-1243) and re-contracts the M tensors with the field
+every step (field.py:1115-1142).  Here the block structure is flattened once
+into a sparse matrix in the full-form state order (field.py:659-692) whose
+entries remember how they are made:
+
+    A_e  = sum_p K_p * MF_{term_p}
+    MF_t = thr( sum_q fprod[slot_q] * M_q )
+
+so that per step only the field products (<= 32 scalars) go to the device,
+where ``k_field_contract`` and ``k_assemble`` rebuild the values.
+"""
+import collections
+
+import numpy as np
+
+from . import _lib
+from .field import block_offsets
+
+_CACHE = collections.OrderedDict()
+_CACHE_MAX = 16
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise _lib.RichmolAmdError("no HIP device visible: the propagation runs only on the GPU")
+    return torch
+
+
+def _stream_handle(stream=None):
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return s.cuda_stream
+
+
+def leaf_carts(leaf):
+    """Sorted Cartesian components present in a leaf tensor's M matrices."""
+    return sorted({c for Jp in leaf.mmat for sp in leaf.mmat[Jp]
+                   for w in leaf.mmat[Jp][sp] for c in leaf.mmat[Jp][sp][w]})
+
+
+def _basis_signature(T):
+    return (tuple(T.Jlist1), tuple(T.Jlist2),
+            tuple((J, tuple(T.symlist1[J]), tuple(T.dim1[J][s] for s in T.symlist1[J])) for J in T.Jlist1),
+            tuple((J, tuple(T.symlist2[J]), tuple(T.dim2[J][s] for s in T.symlist2[J])) for J in T.Jlist2))
+
+
+def structure_key(T):
+    return (tuple(leaf._uid for leaf, _, _ in T.terms()), _basis_signature(T))
+
+
+def flatten_operator(T):
+    """Host arrays of the plan (see module docstring)."""
+    off1, n1 = block_offsets(T, 1)
+    off2, n2 = block_offsets(T, 2)
+    R, C, PT, PK = [], [], [], []
+    TF, TV, TL, TCNT = [], [], [], []
+    nterm = 0
+    slot_base = 0
+    slots = []
+    terms = T.terms()
+    for li, (leaf, _, _) in enumerate(terms):
+        carts = leaf_carts(leaf)
+        cidx = {c: slot_base + i for i, c in enumerate(carts)}
+        slots.append(carts)
+        for Jp in sorted(set(leaf.kmat.keys()) & set(leaf.mmat.keys())):
+            J1, J2 = Jp
+            for sp in sorted(set(leaf.kmat[Jp].keys()) & set(leaf.mmat[Jp].keys())):
+                s1, s2 = sp
+                if (J1, s1) not in off1 or (J2, s2) not in off2:
+                    continue
+                dk1, dk2 = T.dim_k1[J1][s1], T.dim_k2[J2][s2]
+                dm2 = T.dim_m2[J2][s2]
+                Kb, Mb = leaf.kmat[Jp][sp], leaf.mmat[Jp][sp]
+                for w in sorted(set(Kb.keys()) & set(Mb.keys()), key=str):
+                    K = Kb[w].tocoo()
+                    nzk = K.data != 0
+                    kr, kc, kv = K.row[nzk], K.col[nzk], K.data[nzk].astype(np.complex128)
+                    if kv.size == 0:
+                        continue
+                    mr, mc, ms, mv = [], [], [], []
+                    for c in sorted(Mb[w].keys()):
+                        M = Mb[w][c].tocoo()
+                        nz = M.data != 0
+                        mr.append(M.row[nz]); mc.append(M.col[nz])
+                        ms.append(np.full(int(nz.sum()), cidx[c], dtype=np.int32))
+                        mv.append(M.data[nz].astype(np.complex128))
+                    if not mr:
+                        continue
+                    mr = np.concatenate(mr); mc = np.concatenate(mc)
+                    ms = np.concatenate(ms); mv = np.concatenate(mv)
+                    if mr.size == 0:
+                        continue
+                    key = mr.astype(np.int64) * dm2 + mc
+                    ukey, inv = np.unique(key, return_inverse=True)
+                    nt = ukey.size
+                    order = np.argsort(inv, kind="stable")
+                    TF.append(ms[order]); TV.append(mv[order])
+                    TCNT.append(np.bincount(inv, minlength=nt))
+                    TL.append(np.full(nt, li, dtype=np.int32))
+                    ur, uc = ukey // dm2, ukey % dm2
+                    R.append((off1[(J1, s1)] + ur[:, None] * dk1 + kr[None, :]).ravel())
+                    C.append((off2[(J2, s2)] + uc[:, None] * dk2 + kc[None, :]).ravel())
+                    PT.append(np.broadcast_to((nterm + np.arange(nt))[:, None], (nt, kv.size)).ravel())
+                    PK.append(np.broadcast_to(kv[None, :], (nt, kv.size)).ravel())
+                    nterm += nt
+        slot_base += len(carts)
+    if R:
+        R = np.concatenate(R).astype(np.int64); C = np.concatenate(C).astype(np.int64)
+        PT = np.concatenate(PT).astype(np.int32); PK = np.concatenate(PK)
+        TF = np.concatenate(TF).astype(np.int32); TV = np.concatenate(TV)
+        TL = np.concatenate(TL).astype(np.int32); TCNT = np.concatenate(TCNT)
+    else:
+        R = C = np.zeros(0, np.int64); PT = TF = TL = np.zeros(0, np.int32)
+        PK = TV = np.zeros(0, np.complex128); TCNT = np.zeros(0, np.int64)
+    order = np.lexsort((C, R))
+    R, C, PT, PK = R[order], C[order], PT[order], PK[order]
+    if R.size:
+        first = np.ones(R.size, dtype=bool)
+        first[1:] = (R[1:] != R[:-1]) | (C[1:] != C[:-1])
+        ent = np.flatnonzero(first)
+    else:
+        ent = np.zeros(0, np.int64)
+    nnz = ent.size
+    pair_ptr = np.append(ent, R.size).astype(np.int64)
+    col_idx = C[ent].astype(np.int32)
+    row_ptr = np.zeros(n1 + 1, dtype=np.int64)
+    np.cumsum(np.bincount(R[ent], minlength=n1), out=row_ptr[1:])
+    term_ptr = np.zeros(nterm + 1, dtype=np.int64)
+    if nterm:
+        np.cumsum(TCNT, out=term_ptr[1:])
+    return dict(n=n1, n2=n2, nnz=nnz, row_ptr=row_ptr, col_idx=col_idx, pair_ptr=pair_ptr,
+                pair_term=PT, pair_kval=np.ascontiguousarray(PK), nterm=nterm, term_ptr=term_ptr,
+                term_field=TF, term_mval=np.ascontiguousarray(TV), term_leaf=TL,
+                slots=slots, nfield=slot_base, nleaf=len(terms))
+
+
+class DevicePlan:
+    """rm_plan handle plus the slot layout of the field products."""
+
+    def __init__(self, T):
+        import ctypes
+        self.lib = _lib.load()
+        arr = flatten_operator(T)
+        if arr["n"] != arr["n2"]:
+            raise ValueError("bra and ket bases differ: the propagator needs a square operator")
+        if arr["nfield"] > _lib.RM_MAX_FIELD:
+            raise ValueError(f"{arr['nfield']} field slots exceed {_lib.RM_MAX_FIELD}")
+        if arr["nleaf"] > _lib.RM_MAX_LEAF:
+            raise ValueError(f"{arr['nleaf']} summed tensors exceed {_lib.RM_MAX_LEAF}")
+        self.n = arr["n"]
+        self.nnz = arr["nnz"]
+        self.slots = arr["slots"]
+        self.nfield = arr["nfield"]
+        self.nleaf = arr["nleaf"]
+        self.nterm = arr["nterm"]
+        self._arrays = arr
+        h = ctypes.c_void_p()
+        c_i64, c_i32, c_dbl = ctypes.c_int64, ctypes.c_int32, ctypes.c_double
+        _lib.check(self.lib.rm_plan_create(
+            self.n, self.nnz, _lib.ptr(arr["row_ptr"], c_i64), _lib.ptr(arr["col_idx"], c_i32),
+            _lib.ptr(arr["pair_ptr"], c_i64), _lib.ptr(arr["pair_term"], c_i32),
+            _lib.ptr(arr["pair_kval"].view(np.float64), c_dbl), self.nterm,
+            _lib.ptr(arr["term_ptr"], c_i64), _lib.ptr(arr["term_field"], c_i32),
+            _lib.ptr(arr["term_mval"].view(np.float64), c_dbl), _lib.ptr(arr["term_leaf"], c_i32),
+            self.nfield, self.nleaf, ctypes.byref(h)), "rm_plan_create")
+        self.handle = h
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            try:
+                self.lib.rm_plan_destroy(h)
+            except Exception:
+                pass
+            self.handle = None
+
+    # ---------------------------------------------------------------- field
+    def field_arrays(self, T):
+        """Field products and thresholds of T's terms in slot order."""
+        fprod = np.zeros(max(self.nfield, 1), dtype=np.complex128)
+        thresh = np.full(max(self.nleaf, 1), -1.0)
+        base = 0
+        for li, (leaf, fp, thr) in enumerate(T.terms()):
+            carts = self.slots[li]
+            if fp is not None:
+                for i, c in enumerate(carts):
+                    fprod[base + i] = complex(fp.get(c, 0.0))
+            if thr is not None and thr > 0:
+                thresh[li] = float(thr)
+            base += len(carts)
+        return fprod, thresh
+
+    def set_field(self, T, stream=None):
+        import ctypes
+        fprod, thresh = self.field_arrays(T)
+        _lib.check(self.lib.rm_plan_set_field(
+            self.handle, _lib.ptr(fprod.view(np.float64), ctypes.c_double),
+            _lib.ptr(thresh, ctypes.c_double), ctypes.c_void_p(_stream_handle(stream))),
+            "rm_plan_set_field")
+
+    # ------------------------------------------------------------- kernels
+    def matvec(self, x, y, stream=None):
+        """y[b] = H x[b] for torch complex128 (B, N) device tensors."""
+        import ctypes
+        _check_cols(x, self.n); _check_cols(y, self.n)
+        _lib.check(self.lib.rm_matvec(self.handle, ctypes.c_void_p(x.data_ptr()),
+                                      ctypes.c_void_p(y.data_ptr()), x.stride(0), x.shape[0],
+                                      ctypes.c_void_p(_stream_handle(stream))), "rm_matvec")
+
+    def step(self, psi_in, psi_out, f, phase=None, krylov_fixed=0, maxorder=100, tol=1e-15,
+             kused=None, status=None, stream=None):
+        import ctypes
+        _check_cols(psi_in, self.n); _check_cols(psi_out, self.n)
+        if psi_in.stride(0) != psi_out.stride(0):
+            raise ValueError("psi_in and psi_out need the same row stride")
+        vp = ctypes.c_void_p
+        _lib.check(self.lib.rm_step(
+            self.handle, vp(psi_in.data_ptr()), vp(psi_out.data_ptr()), psi_in.stride(0),
+            psi_in.shape[0], float(f.real), float(f.imag),
+            vp(phase.data_ptr() if phase is not None else None), int(krylov_fixed), int(maxorder),
+            float(tol), vp(kused.data_ptr() if kused is not None else None),
+            vp(status.data_ptr() if status is not None else None),
+            vp(_stream_handle(stream))), "rm_step")
+
+    def expect(self, psi, out, stream=None):
+        import ctypes
+        _check_cols(psi, self.n)
+        _lib.check(self.lib.rm_expect(self.handle, ctypes.c_void_p(psi.data_ptr()), psi.stride(0),
+                                      psi.shape[0], ctypes.c_void_p(out.data_ptr()),
+                                      ctypes.c_void_p(_stream_handle(stream))), "rm_expect")
+
+
+def _check_cols(t, n):
+    import torch
+    if t.dtype != torch.complex128:
+        raise TypeError(f"expected complex128 wavepackets, got {t.dtype}")
+    if not t.is_cuda:
+        raise TypeError("wavepackets must be device tensors")
+    if t.dim() != 2 or t.shape[1] != n or t.stride(1) != 1 or t.stride(0) < n:
+        raise ValueError(f"expected (ncols, {n}) rows with unit inner stride, got {tuple(t.shape)}")
+
+
+def plan_for(T):
+    """Cached DevicePlan for T's structure (leaf tensors + basis)."""
+    key = structure_key(T)
+    p = _CACHE.get(key)
+    if p is None:
+        p = DevicePlan(T)
+        _CACHE[key] = p
+        while len(_CACHE) > _CACHE_MAX:
+            _CACHE.popitem(last=False)
+    else:
+        _CACHE.move_to_end(key)
+    return p
+
+
+def apply(T, X):
+    """H X for a numpy (B, N) array through the device (CarTens.vec)."""
+    torch = _torch()
+    p = plan_for(T)
+    x = torch.from_numpy(np.ascontiguousarray(X, dtype=np.complex128)).cuda()
+    y = torch.empty_like(x)
+    p.set_field(T)
+    p.matvec(x, y)
+    return y.cpu().numpy()

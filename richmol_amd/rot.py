@@ -1,0 +1,254 @@
+"""Operator builders for rigid rotors: the K/M tensors the propagator consumes.
+
+Produces CarTens objects with the layout richmol's LabTensor produces
+(richmol/rot/labtens.py:114-231, tens_proj 411-530, matelem 533-638):
+
+  K_w[bra, ket] = sum_k conj(c_bra,k) c_ket,k' (-1)^|k| sum_s 3j(J2 w J1; k' s -k) UsT[w][s]
+  M_w,cart[m1, m2] = sqrt((2J1+1)(2J2+1)) (-1)^|m1| sum_s Ux[cart, (w,s)] 3j(J2 w J1; m2 s -m1)
+
+with (J1, m1, k) the bra and (J2, m2, k') the ket, UsT = Us . tensor the
+spherical components of the molecular-frame tensor, Ux the spherical-to-
+Cartesian matrix, and c the symmetric-top (Wang) coefficients of the basis.
+Linear rotors use the single k = 0 function with c = 1 (J even) or i (J odd),
+as richmol/rot/basis.py:729-801 does.  The Cartesian-spherical transformation
+table is the one richmol uses (labtens.py:89-112) so operators built here
+equal the reference's on the same molecule.
+"""
+import functools
+import math
+from fractions import Fraction
+
+import numpy as np
+from scipy.sparse import csr_matrix
+
+from .field import CarTens
+
+# Cartesian -> spherical tensor transformation, rows (omega, sigma), columns
+# Cartesian components in the order of CART[rank].
+_S2 = math.sqrt(2.0)
+_US = {
+    1: np.array([[_S2 / 2, -_S2 * 1j / 2, 0],
+                 [0, 0, 1.0],
+                 [-_S2 / 2, -_S2 * 1j / 2, 0]], dtype=np.complex128),
+    2: np.array([[-1 / math.sqrt(3), 0, 0, 0, -1 / math.sqrt(3), 0, 0, 0, -1 / math.sqrt(3)],
+                 [0, 0, -0.5, 0, 0, 0.5j, 0.5, -0.5j, 0],
+                 [0, 1 / _S2 * 1j, 0, -1 / _S2 * 1j, 0, 0, 0, 0, 0],
+                 [0, 0, -0.5, 0, 0, -0.5j, 0.5, 0.5j, 0],
+                 [0.5, -0.5j, 0, -0.5j, -0.5, 0, 0, 0, 0],
+                 [0, 0, 0.5, 0, 0, -0.5j, 0.5, -0.5j, 0],
+                 [-1 / math.sqrt(6), 0, 0, 0, -1 / math.sqrt(6), 0, 0, 0, math.sqrt(6) / 3],
+                 [0, 0, -0.5, 0, 0, -0.5j, -0.5, -0.5j, 0],
+                 [0.5, 0.5j, 0, 0.5j, -0.5, 0, 0, 0, 0]], dtype=np.complex128),
+}
+_UX = {r: np.linalg.pinv(u) for r, u in _US.items()}
+CART = {1: ["x", "y", "z"], 2: ["xx", "xy", "xz", "yx", "yy", "yz", "zx", "zy", "zz"]}
+IRREPS = {1: [(1, -1), (1, 0), (1, 1)], 2: [(o, s) for o in range(3) for s in range(-o, o + 1)]}
+_EPS = np.finfo(np.complex128).eps
+
+
+@functools.lru_cache(maxsize=None)
+def _fact(n):
+    return math.factorial(n)
+
+
+@functools.lru_cache(maxsize=1 << 20)
+def wigner3j(j1, j2, j3, m1, m2, m3):
+    """Wigner 3j symbol for integer arguments (Racah formula, exact rational
+    arithmetic, one final rounding)."""
+    if m1 + m2 + m3 != 0:
+        return 0.0
+    if j3 < abs(j1 - j2) or j3 > j1 + j2:
+        return 0.0
+    if abs(m1) > j1 or abs(m2) > j2 or abs(m3) > j3:
+        return 0.0
+    kmin = max(0, j2 - j3 - m1, j1 - j3 + m2)
+    kmax = min(j1 + j2 - j3, j1 - m1, j2 + m2)
+    s = Fraction(0)
+    for k in range(kmin, kmax + 1):
+        den = (_fact(k) * _fact(j3 - j2 + k + m1) * _fact(j3 - j1 + k - m2) *
+               _fact(j1 + j2 - j3 - k) * _fact(j1 - k - m1) * _fact(j2 - k + m2))
+        s += Fraction((-1) ** k, den)
+    if s == 0:
+        return 0.0
+    tri = Fraction(_fact(j1 + j2 - j3) * _fact(j1 - j2 + j3) * _fact(-j1 + j2 + j3),
+                   _fact(j1 + j2 + j3 + 1))
+    pre = tri * (_fact(j1 + m1) * _fact(j1 - m1) * _fact(j2 + m2) * _fact(j2 - m2) *
+                 _fact(j3 + m3) * _fact(j3 - m3))
+    val = math.sqrt(float(pre * s * s))
+    sign = (1 if s > 0 else -1) * (-1) ** ((j1 - j2 - m3) % 2)
+    return sign * val
+
+
+def spherical_components(tens):
+    """(rank, {omega: {sigma: UsT}}, {omega: {sigma: Ux column}}) of a
+    molecular-frame tensor (labtens.py:237-282 for the Us/Ux pruning)."""
+    tens = np.asarray(tens)
+    rank = tens.ndim
+    if rank not in (1, 2) or any(d != 3 for d in tens.shape):
+        raise ValueError(f"input tensor has inappropriate shape: '{tens.shape}'")
+    if np.any(np.isnan(tens)):
+        raise ValueError("some of elements of input tensor are NaN")
+    flat = np.array([tens.item(tuple("xyz".index(c) for c in cart)) for cart in CART[rank]])
+    us = _US[rank].copy()
+    ux = _UX[rank].copy()
+    us[np.abs(us) < _EPS] = 0
+    ux[np.abs(ux) < _EPS] = 0
+    ust = us @ flat
+    comps, ucols = {}, {}
+    for i, (o, s) in enumerate(IRREPS[rank]):
+        if abs(ust[i]) > _EPS:
+            comps.setdefault(o, {})[s] = ust[i]
+        ucols.setdefault(o, {})[s] = ux[:, i]
+    return rank, comps, ucols
+
+
+class RotorBasis:
+    """Block basis of a rigid rotor: per J one symmetry block (label 'A' for
+    linear rotors) with m = -J..J (optionally filtered) and K functions given
+    by coefficient columns over primitive k.
+
+    ``kfun[J][sym]`` is (kprim list, coefficient matrix nprim x nstate),
+    ``energy[J][sym]`` the field-free energies of the states.
+    """
+
+    def __init__(self, jlist, mlist, kfun, energy):
+        self.Jlist = list(jlist)
+        self.mlist = mlist
+        self.kfun = kfun
+        self.energy = energy
+        self.symlist = {J: list(kfun[J].keys()) for J in self.Jlist}
+
+    def attrs(self, T):
+        T.Jlist1 = list(self.Jlist); T.Jlist2 = list(self.Jlist)
+        T.symlist1 = {J: list(s) for J, s in self.symlist.items()}
+        T.symlist2 = {J: list(s) for J, s in self.symlist.items()}
+        T.dim_m1 = {J: {s: len(self.mlist[J]) for s in self.symlist[J]} for J in self.Jlist}
+        T.dim_k1 = {J: {s: self.kfun[J][s][1].shape[1] for s in self.symlist[J]} for J in self.Jlist}
+        T.dim1 = {J: {s: T.dim_m1[J][s] * T.dim_k1[J][s] for s in self.symlist[J]} for J in self.Jlist}
+        T.dim_m2 = {J: dict(v) for J, v in T.dim_m1.items()}
+        T.dim_k2 = {J: dict(v) for J, v in T.dim_k1.items()}
+        T.dim2 = {J: dict(v) for J, v in T.dim1.items()}
+        T.quanta_m1 = {J: {s: list(self.mlist[J]) for s in self.symlist[J]} for J in self.Jlist}
+        T.quanta_m2 = T.quanta_m1
+        T.quanta_k1 = {J: {s: [(str(i), e) for i, e in enumerate(self.energy[J][s])]
+                           for s in self.symlist[J]} for J in self.Jlist}
+        T.quanta_k2 = T.quanta_k1
+
+
+def linear_basis(B, Jmax, Jmin=0, m_filter=None, J_filter=None):
+    """Linear rotor basis, energies B J(J+1) (rot/solution.py:239-251)."""
+    jl, ml, kf, en = [], {}, {}, {}
+    for J in range(Jmin, Jmax + 1):
+        if J_filter is not None and not J_filter(J):
+            continue
+        ms = [m for m in range(-J, J + 1) if m_filter is None or m_filter(J, m)]
+        if not ms:
+            continue
+        jl.append(J)
+        ml[J] = ms
+        c = 1.0 if J % 2 == 0 else 1j      # Wang function |J,0,tau=J mod 2> (basis.py:790-795)
+        kf[J] = {"A": ([0], np.array([[c]], dtype=np.complex128))}
+        en[J] = {"A": np.array([B * J * (J + 1)], dtype=np.float64)}
+    return RotorBasis(jl, ml, kf, en)
+
+
+def hamiltonian(basis):
+    """Field-free Hamiltonian as a rank-0 tensor: K = diag(E), M = identity
+    (labtens.py:285-308 with the eigenbasis of H0)."""
+    T = CarTens()
+    basis.attrs(T)
+    T.rank, T.cart, T.os = 0, ["0"], [(0, 0)]
+    T.kmat, T.mmat = {}, {}
+    for J in basis.Jlist:
+        for s in basis.symlist[J]:
+            dm = len(basis.mlist[J])
+            T.kmat.setdefault((J, J), {})[(s, s)] = {0: csr_matrix(np.diag(basis.energy[J][s]).astype(np.complex128))}
+            T.mmat.setdefault((J, J), {})[(s, s)] = {0: {"0": csr_matrix(np.eye(dm, dtype=np.complex128))}}
+    return T
+
+
+def _k_block(comps_w, w, J1, kf1, J2, kf2):
+    """K_w[bra J1 states, ket J2 states] (labtens.py:482-496 + overlap_k)."""
+    kp1, c1 = kf1
+    kp2, c2 = kf2
+    prim = np.zeros((len(kp1), len(kp2)), dtype=np.complex128)
+    for a, k1 in enumerate(kp1):
+        for b, k2 in enumerate(kp2):
+            me = 0.0
+            for s, u in comps_w.items():
+                t = wigner3j(J2, w, J1, k2, s, -k1)
+                if t != 0.0:
+                    me += t * u
+            prim[a, b] = me * (-1) ** abs(k1)
+    return c1.conj().T @ prim @ c2
+
+
+def _m_block(ucols_w, w, J1, ms1, J2, ms2, carts):
+    """M_w,cart[m1, m2] for every cart (labtens.py:501-523 + overlap_m)."""
+    out = {c: np.zeros((len(ms1), len(ms2)), dtype=np.complex128) for c in carts}
+    fac = math.sqrt((2 * J1 + 1) * (2 * J2 + 1))
+    for a, m1 in enumerate(ms1):
+        for b, m2 in enumerate(ms2):
+            s = m1 - m2
+            if s not in ucols_w:
+                continue
+            t = wigner3j(J2, w, J1, m2, s, -m1)
+            if t == 0.0:
+                continue
+            col = ucols_w[s]
+            for i, c in enumerate(carts):
+                if col[i] != 0:
+                    out[c][a, b] += col[i] * t * fac * (-1) ** abs(m1)
+    return out
+
+
+def _build(basis, rank, carts, comps, ucols):
+    T = CarTens()
+    basis.attrs(T)
+    T.rank, T.cart = rank, list(carts)
+    T.os = [(o, s) for o in sorted(comps) for s in sorted(comps[o])]
+    T.kmat, T.mmat = {}, {}
+    dJ = max(comps.keys()) if comps else 0
+    for J2 in basis.Jlist:
+        for J1 in basis.Jlist:
+            if abs(J1 - J2) > dJ:
+                continue
+            for s1 in basis.symlist[J1]:
+                for s2 in basis.symlist[J2]:
+                    kd, md = {}, {}
+                    for w in sorted(comps):
+                        K = _k_block(comps[w], w, J1, basis.kfun[J1][s1], J2, basis.kfun[J2][s2])
+                        K[np.abs(K) < _EPS] = 0
+                        if not np.any(K):
+                            continue
+                        Ms = _m_block(ucols[w], w, J1, basis.mlist[J1], J2, basis.mlist[J2], carts)
+                        mm = {}
+                        for c, M in Ms.items():
+                            M[np.abs(M) < _EPS] = 0
+                            if np.any(M):
+                                mm[c] = csr_matrix(M)
+                        if mm:
+                            kd[w] = csr_matrix(K)
+                            md[w] = mm
+                    if kd:
+                        T.kmat.setdefault((J1, J2), {})[(s1, s2)] = kd
+                        T.mmat.setdefault((J1, J2), {})[(s1, s2)] = md
+    return T
+
+
+def tensor(tens, basis):
+    """Lab-frame operator of a rank-1 or rank-2 molecular-frame tensor (in au)."""
+    rank, comps, ucols = spherical_components(tens)
+    return _build(basis, rank, CART[rank], comps, ucols)
+
+
+def named(name, basis):
+    """'cos2theta' (cos^2 theta - 1/3) or 'costheta' as rank-0 tensors
+    (labtens.py:362-385)."""
+    if name.lower() == "cos2theta":
+        comps, ucols = {2: {0: 1.0}}, {2: {0: np.array([2.0 / 3.0])}}
+    elif name.lower() == "costheta":
+        comps, ucols = {1: {0: 1.0}}, {1: {0: np.array([1.0])}}
+    else:
+        raise ValueError(f"unknown name for tensor operator: '{name}'")
+    return _build(basis, 0, ["0"], comps, ucols)

@@ -1,0 +1,261 @@
+"""Drop-in time propagator: the interface of richmol.tdse.TDSE on the GPU.
+
+Same constructor, ``time_grid``, ``init_state`` and ``update(H, vecs, **kw)
+-> (vecs, t)`` as richmol/tdse.py:27-414.  ``update`` runs the split-operator
+step and the Lanczos expv of every wavepacket in one HIP launch
+(``rm_step``): the reference's Python loop over rows (tdse.py:379-382) and
+per-matvec block loop (field.py:1212-1243) become one workgroup per
+wavepacket.
+
+``vecs`` may be a numpy (R, N) complex array (copied to the device and back,
+like the reference returns a new array) or a torch complex128 device tensor
+(stays on the device; a new tensor is returned).
+"""
+import functools
+
+import numpy as np
+from scipy.sparse import diags, csr_matrix
+
+from . import units
+from .field import CarTens
+
+
+def update_counter(func):
+    """tdse.py:11-24: return (vecs, t_2[icall]), counting calls per name."""
+    @functools.wraps(func)
+    def wrapper(self, *args, **kwargs):
+        vecs = func(self, *args, **kwargs)
+        name = func.__name__
+        if name not in self._ncalls:
+            self._ncalls[name] = 0
+        icall = self._ncalls[name]
+        t = self._time_grid[1][icall]
+        self._ncalls[name] += 1
+        return vecs, t
+    return wrapper
+
+
+class TDSE:
+    """Time propagator (richmol/tdse.py:27-143).
+
+    Kwargs: t_start, t_end, dt, t_units ('fs', 'ps', 'ns', 'au'),
+    enr_units ('invcm', 'MHz').
+    """
+
+    def __init__(self, **kwargs):
+        self._ncalls = dict()
+        if "t_start" in kwargs:
+            assert type(kwargs["t_start"]) in [int, float], \
+                f"`initial time `t_start` has bad type: '{type(kwargs['t_start'])}', (use 'int', 'float')"
+            self._t_start = kwargs["t_start"]
+        else:
+            self._t_start = 0.0
+        assert "t_end" in kwargs, "terminal time 't_end' not found in kwargs"
+        assert type(kwargs["t_end"]) in [int, float], \
+            f"terminal time `t_end` has bad type: '{type(kwargs['t_end'])}', (use 'int', 'float')"
+        assert kwargs["t_end"] > self._t_start, \
+            f"terminal time `t_end` has bad value: '{kwargs['t_end']}', (must be > '{self._t_start}')"
+        self._t_end = kwargs["t_end"]
+        assert "dt" in kwargs, "time step `dt` not found in kwargs"
+        assert type(kwargs["dt"]) in [int, float], \
+            f"time step `dt` has bad type: '{type(kwargs['dt'])}', (use 'int', 'float')"
+        assert kwargs["dt"] <= (self._t_end - self._t_start), \
+            f"time step `dt` has bad value : '{kwargs['dt']}', (must be <= '{self._t_end - self._t_start}')"
+        self._dt = kwargs["dt"]
+        t_units = units.T_UNITS
+        if "t_units" in kwargs:
+            assert type(kwargs["t_units"]) == str, \
+                f"time units `t_units` has bad type: '{type(kwargs['t_units'])}', (use 'str')"
+            assert kwargs["t_units"] in list(t_units.keys()), \
+                f"time units `t_units` has bad value: '{kwargs['t_units']}', (use 'fs', 'ps', 'ns', 'au')"
+            self._t_to_s = t_units[kwargs["t_units"]]
+        else:
+            self._t_to_s = t_units["ps"]
+        enr = units.enr_units()
+        if "enr_units" in kwargs:
+            assert type(kwargs["enr_units"]) == str, \
+                f"energy units `enr_units` has bad type: '{type(kwargs['enr_units'])}', (use 'str')"
+            assert kwargs["enr_units"].lower() in list(enr.keys()), \
+                f"energy units `enr_units` has bad value: '{kwargs['enr_units']}', (use 'invcm', 'MHz')"
+            self._enr_to_J = enr[kwargs["enr_units"].lower()]
+        else:
+            self._enr_to_J = enr["invcm"]
+
+    @property
+    def exp_fac(self):
+        """Exponent factor f of exp(f H) per step (tdse.py:336-337)."""
+        return -1j * self._dt * self._t_to_s * self._enr_to_J / units.HBAR
+
+    def time_grid(self, grid_type="equidistant"):
+        """tdse.py:146-174: centres t_c; (t_1, t_2, t_c) kept for update()."""
+        assert grid_type.lower() in ["equidistant"], \
+            f"time grid type `grid_type` has bad value: '{grid_type}', (use 'equidistant')"
+        n = int((self._t_end - self._t_start) / self._dt)
+        t1 = np.linspace(self._t_start, self._t_end, num=n, endpoint=False)
+        t2 = t1 + self._dt
+        tc = t1 + self._dt / 2
+        self._time_grid = (t1, t2, tc)
+        return tc
+
+    def init_state(self, H, temp=None, thresh=1e-3, zpe=None, sparse=False):
+        """Eigenvectors of H, Boltzmann-weighted (tdse.py:177-262).
+
+        Host setup (dense eigh, or the identity when H is a rank-0 diagonal
+        Hamiltonian), as in the reference; rows are complex128.
+        """
+        if temp is not None:
+            assert temp >= 0, f"temperature `temp` has negative value: '{temp}'"
+        assert thresh >= 0, f"partition function threshold `thresh` has negative or zero value: '{thresh}'"
+        is_diag = False
+        cart = getattr(H, "cart", None)
+        if cart is not None and len(cart) > 0 and cart[0] == "0" and not H.is_sum:
+            H.field([0, 0, 1])
+            is_diag = True
+        try:
+            H.mfmat
+        except AttributeError:
+            raise AttributeError("hamiltonian `H` has inappropriate type (must be Hamiltonian)") from None
+        if is_diag:
+            enrs = H.tomat(form="full", repres="csr_matrix").diagonal()
+            vecs = diags(np.ones(len(enrs)), format="csr")
+        else:
+            enrs, v = np.linalg.eigh(H.tomat(form="full", repres="dense"))
+            vecs = csr_matrix(v)
+        if zpe is not None:
+            assert zpe <= abs(enrs[0]), \
+                f"zero-point energy `zpe` has a value that is larger than the lowest energy: zpe = '{zpe}' > emin = '{enrs[0]}'"
+        else:
+            zpe = enrs[0]
+        enrs = enrs - zpe
+        vecs = vecs.transpose().tocsr()
+        if temp is None:
+            pass
+        elif temp == 0:
+            vecs = vecs.getrow(0)
+        else:
+            e = enrs * self._enr_to_J
+            beta = 1.0 / (units.BOLTZMANN * temp)
+            w = np.exp(-beta * e)
+            w /= np.sum(w)
+            inds = [i for i in range(len(w)) if (1 - np.sum(w[: i + 1])) > thresh]
+            vecs = vecs[inds].multiply(np.sqrt(w[inds])[:, None]).tocsr()
+        if not sparse:
+            vecs = np.asarray(vecs.toarray())
+        return vecs.astype(np.complex128)
+
+    @update_counter
+    def update(self, H, vecs, **kwargs):
+        """Propagate wavepackets (rows of ``vecs``) by one time step.
+
+        Kwargs as tdse.py:276-286: H0 (diagonal field-free CarTens: split
+        operator), matvec_lib ('hip'), propag ('internal' = the reference's
+        adaptive Lanczos, here on the device), tol (default 1e-15).
+        Device extensions: maxorder (default 100), krylov_fixed (fixed
+        Krylov dimension, default 0 = adaptive), stream (torch stream).
+        """
+        from . import plan as _plan
+        import torch
+
+        if "H0" in kwargs:
+            assert isinstance(kwargs["H0"], CarTens), \
+                f"field-free Hamiltonian `H0` is of bad (sub-)class: '{type(kwargs['H0'])}', " \
+                f"(must be (sub-class of) '{CarTens}')"
+            H0 = kwargs["H0"]
+        else:
+            H0 = None
+        matvec_lib = kwargs.get("matvec_lib", "hip")
+        if matvec_lib != "hip":
+            raise ValueError(f"matvec_lib='{matvec_lib}' selects a CPU backend of the reference; "
+                             "this package runs matvec_lib='hip' only")
+        propag = kwargs.get("propag", "internal")
+        assert type(propag) is str, f"propagator 'propag' has bad type: '{type(propag)}', (must be 'str')"
+        assert propag in ["external", "internal", "hip"], \
+            f"propagator 'propag' has bad value: '{propag}', (use 'external', 'internal')"
+        if propag == "external":
+            raise NotImplementedError("propag='external' (Expokit zhexpv) is not provided; "
+                                      "use the Lanczos propagator ('internal')")
+        if "tol" in kwargs:
+            assert type(kwargs["tol"]) in [int, float], \
+                f"tolerance `tol` has bad type: '{type(kwargs['tol'])}', (must be 'int', 'float')"
+            assert 0 < kwargs["tol"] <= 1, \
+                f"tolerance `tol` has bad value: '{kwargs['tol']}', (must be > 0 and <= 1)"
+            tol = kwargs["tol"]
+        else:
+            tol = 1e-15
+        maxorder = int(kwargs.get("maxorder", 100))
+        krylov_fixed = int(kwargs.get("krylov_fixed", 0))
+        stream = kwargs.get("stream", None)
+
+        f = self.exp_fac
+        on_device = isinstance(vecs, torch.Tensor)
+        if on_device:
+            psi = vecs
+            if psi.dtype != torch.complex128 or not psi.is_cuda:
+                raise TypeError("device wavepackets must be complex128 tensors on the GPU")
+            if psi.dim() != 2 or psi.stride(1) != 1:
+                psi = psi.contiguous()
+        else:
+            arr = np.asarray(vecs)
+            if arr.ndim != 2:
+                raise ValueError("vecs must be a 2-D array of wavepacket rows")
+            psi = torch.from_numpy(np.ascontiguousarray(arr, dtype=np.complex128)).cuda()
+
+        if H is None or (not H.is_sum and "_fprod" not in H.__dict__):
+            if H0 is None:
+                raise AttributeError("you need to multiply tensor with field before applying it to a vector")
+            plan = None
+        else:
+            plan = _plan.plan_for(H)
+
+        phase = None
+        if H0 is not None:
+            if "_exp_fac_H0" not in self.__dict__:
+                h0 = H0.tomat(form="full", cart="0")
+                assert (h0 - diags(h0.diagonal())).nnz == 0, \
+                    "field-free Hamiltonian `H0` is not diagonal -> split-operator approach not implemented"
+                d = torch.from_numpy(np.ascontiguousarray(h0.diagonal(), dtype=np.complex128)).cuda()
+                ph = torch.empty_like(d)
+                _plan_phase(d, f, ph, stream)
+                self._exp_fac_H0 = ph
+            phase = self._exp_fac_H0
+            if phase.numel() != psi.shape[1]:
+                raise ValueError("H0 dimension does not match the wavepackets")
+
+        out = torch.empty_like(psi)
+        ncols = psi.shape[0]
+        if plan is None:
+            # no field on the operator: only the two H0 phases (tdse.py:375, 377, 394)
+            _phase_apply(psi, out, phase, stream)
+        else:
+            status = torch.zeros(ncols, dtype=torch.int32, device=psi.device)
+            kused = torch.zeros(ncols, dtype=torch.int32, device=psi.device)
+            plan.set_field(H, stream)
+            plan.step(psi, out, f, phase=phase, krylov_fixed=krylov_fixed, maxorder=maxorder,
+                      tol=tol, kused=kused, status=status, stream=stream)
+            self.last_krylov_dims = kused
+            if krylov_fixed == 0 and bool((status != 0).any()):
+                raise ValueError(f"Lanczos reached maximum order of '{maxorder}' without convergence")
+        if on_device:
+            return out
+        return out.cpu().numpy()
+
+
+def _phase_apply(psi, out, phase, stream=None):
+    import ctypes
+    from . import _lib
+    from .plan import _stream_handle
+    lib = _lib.load()
+    _lib.check(lib.rm_phase_apply(psi.shape[1], ctypes.c_void_p(psi.data_ptr()),
+                                  ctypes.c_void_p(out.data_ptr()), psi.stride(0), psi.shape[0],
+                                  ctypes.c_void_p(phase.data_ptr()),
+                                  ctypes.c_void_p(_stream_handle(stream))), "rm_phase_apply")
+
+
+def _plan_phase(diag, f, out, stream=None):
+    import ctypes
+    from . import _lib
+    from .plan import _stream_handle
+    lib = _lib.load()
+    _lib.check(lib.rm_phase_init(diag.numel(), ctypes.c_void_p(diag.data_ptr()), float(f.real),
+                                 float(f.imag), ctypes.c_void_p(out.data_ptr()),
+                                 ctypes.c_void_p(_stream_handle(stream))), "rm_phase_init")

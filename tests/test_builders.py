@@ -1,0 +1,116 @@
+"""Operator builders (richmol_amd.rot, richmol_amd.trove) against closed forms,
+sympy's exact 3j symbols and the reference's data files."""
+import os
+import random
+
+import numpy as np
+import pytest
+
+from _fixtures import linear_rotor, ocs_fixture
+from oracle.cartens import OracleCarTens
+from richmol_amd import rot
+
+
+def test_wigner3j_vs_sympy():
+    wigner_3j = pytest.importorskip("sympy.physics.wigner").wigner_3j
+    rnd = random.Random(0)
+    for _ in range(400):
+        j1 = rnd.randint(0, 60)
+        j2 = rnd.randint(0, 2)
+        j3 = rnd.randint(abs(j1 - j2), j1 + j2)
+        m1 = rnd.randint(-j1, j1)
+        m2 = rnd.randint(-j2, j2)
+        a = rot.wigner3j(j1, j2, j3, m1, m2, -m1 - m2)
+        b = float(wigner_3j(j1, j2, j3, m1, m2, -m1 - m2))
+        assert abs(a - b) <= 2e-16 + 1e-15 * abs(b)
+    assert rot.wigner3j(2, 1, 1, 0, 0, 1) == 0.0
+
+
+def test_cos2_closed_form():
+    """<J M|cos^2 theta|J M> - 1/3 = 2/3 (J(J+1) - 3M^2) / ((2J-1)(2J+3)) and
+    <J M|cos^2|J+2 M> = sqrt(((J+1)^2-M^2)((J+2)^2-M^2)/((2J+1)(2J+3)^2(2J+5))) up to sign."""
+    h0, H, cos2, bas = linear_rotor(8)
+    C = OracleCarTens.from_attrs(cos2).tomat(form="full", cart="0").toarray()
+    idx = {}
+    i = 0
+    for J in bas.Jlist:
+        for m in bas.mlist[J]:
+            idx[(J, m)] = i
+            i += 1
+    for (J, m), i in idx.items():
+        assert abs(C[i, i] - 2 / 3 * (J * (J + 1) - 3 * m * m) / ((2 * J - 1) * (2 * J + 3))) < 1e-14
+        if (J + 2, m) in idx:
+            off = np.sqrt(((J + 1) ** 2 - m * m) * ((J + 2) ** 2 - m * m)
+                          / ((2 * J + 1) * (2 * J + 3) ** 2 * (2 * J + 5)))
+            assert abs(abs(C[i, idx[(J + 2, m)]]) - off) < 1e-14
+    assert np.abs(C - C.conj().T).max() == 0.0
+
+
+def test_polarizability_is_aperp_plus_dalpha_cos2():
+    """For a Z field, alpha_ZZ = alpha_perp + (alpha_par - alpha_perp) cos^2 theta."""
+    bas = rot.linear_basis(0.2029, 10)
+    pol = rot.tensor(np.diag([26.0, 26.0, 53.6]), bas)
+    P = OracleCarTens.from_attrs(pol)
+    P.field([0, 0, 1.0])
+    A = P.tomat(form="full").toarray()
+    C = OracleCarTens.from_attrs(rot.named("cos2theta", bas)).tomat(form="full", cart="0").toarray()
+    n = A.shape[0]
+    assert np.abs(A - (26.0 * np.eye(n) + 27.6 * (C + np.eye(n) / 3))).max() < 1e-13
+    # X field: alpha_XX has Delta m = 0, +-2 couplings and the same trace
+    P.field([1.0, 0, 0])
+    AX = P.tomat(form="full").toarray()
+    assert abs(np.trace(AX) - np.trace(A)) < 1e-10 * abs(np.trace(A))
+    assert np.abs(AX - AX.conj().T).max() < 1e-13
+
+
+def test_costheta_closed_form():
+    bas = rot.linear_basis(0.2029, 6)
+    C = OracleCarTens.from_attrs(rot.named("costheta", bas)).tomat(form="full", cart="0").toarray()
+    idx = {}
+    i = 0
+    for J in bas.Jlist:
+        for m in bas.mlist[J]:
+            idx[(J, m)] = i
+            i += 1
+    for (J, m), i in idx.items():
+        if (J + 1, m) in idx:
+            expect = np.sqrt(((J + 1) ** 2 - m * m) / ((2 * J + 1) * (2 * J + 3)))
+            assert abs(abs(C[i, idx[(J + 1, m)]]) - expect) < 1e-14
+        assert C[i, i] == 0
+
+
+def test_linear_energies_and_dims():
+    h0, H, cos2, bas = linear_rotor(40)
+    n = sum(len(bas.mlist[J]) for J in bas.Jlist)
+    assert n == 1681
+    d = OracleCarTens.from_attrs(h0).tomat(form="full", cart="0").diagonal().real
+    assert np.allclose(d, [0.2029 * J * (J + 1) for J in range(41) for m in range(-J, J + 1)], rtol=0, atol=1e-12)
+
+
+REF_OCS = "/root/reference/tests/benchmarks/data/alignment_ocs/matelem/"
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_OCS), reason="reference tree not present")
+def test_trove_reader_matches_fixture():
+    """The .rchm reader on the reference files reproduces the committed fixture,
+    and its M tensor for J=0 -> 2 equals the 3j construction of rot.tensor."""
+    from richmol_amd.trove import CarTensTrove
+    filt = lambda **kw: (kw.get("J", 0) <= 30 and kw.get("J", 0) % 2 == 0) and kw.get("m", 0.0) == 0.0  # noqa
+    states = REF_OCS + "ocs_energies_j0_j30.rchm"
+    a = CarTensTrove(states, REF_OCS + "ocs_matelem_alpha_j<j1>_j<j2>.rchm", bra=filt, ket=filt)
+    h0f, af, d = ocs_fixture()
+    A1 = OracleCarTens.from_attrs(a)
+    A2 = OracleCarTens.from_attrs(af)
+    for F in ([0, 0, 1.0], [1.0, 0, 0], [0.3, -0.5, 0.7]):
+        A1.field(F); A2.field(F)
+        assert np.abs(A1.tomat(form="full").toarray() - A2.tomat(form="full").toarray()).max() < 1e-14
+    # full m basis: the file's M tensors equal the 3j construction (up to the
+    # K normalisation, which carries the molecule's polarizability)
+    full = CarTensTrove(states, REF_OCS + "ocs_matelem_alpha_j<j1>_j<j2>.rchm",
+                        bra=lambda **kw: kw.get("J", 0) <= 2, ket=lambda **kw: kw.get("J", 0) <= 2)
+    bas = rot.linear_basis(1.0, 2)
+    mine = rot.tensor(np.diag([1.0, 1.0, 2.0]), bas)
+    for c in ("xx", "xz", "zz", "xy"):
+        Mf = full.mmat[(0.0, 2.0)][("A", "A")][2][c].toarray()
+        Mm = mine.mmat[(0, 2)][("A", "A")][2][c].toarray()
+        assert np.abs(Mf - Mm).max() < 1e-8

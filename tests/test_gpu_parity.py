@@ -1,0 +1,199 @@
+"""Device path vs the FP64 oracle on the same inputs (HIP kernels through the C ABI).
+
+Tolerances: matvec 1e-13 relative (same sums, different order); propagation
+1e-10 absolute in every wavepacket amplitude and in <cos^2 theta>, the bar
+the reference-parity requirement sets.
+"""
+import numpy as np
+import pytest
+
+from _fixtures import gaussian_pulse_field, linear_rotor, ocs_fixture
+from oracle.cartens import OracleCarTens, flatten, unflatten
+from oracle.tdse import OracleTDSE
+
+pytestmark = pytest.mark.gpu
+
+TOL_PSI = 1e-10
+
+
+def _rand_cols(rng, ncol, n, unit=True):
+    X = rng.normal(size=(ncol, n)) + 1j * rng.normal(size=(ncol, n))
+    if unit:
+        X /= np.linalg.norm(X, axis=1)[:, None]
+    return X
+
+
+def _oracle_vec(O, X):
+    return np.array([flatten(O, O.vec(unflatten(O, x))) for x in X])
+
+
+@pytest.mark.parametrize("F", [[0.0, 0.0, 1.3e9], [4e8, 0.0, 1.1e9], [2e8, -3e8, 7e8]])
+def test_matvec_linear_rotor(device, F):
+    from richmol_amd import plan
+    h0, H, cos2, bas = linear_rotor(9)
+    H.field(F)
+    O = OracleCarTens.from_attrs(H)
+    O.field(F)
+    n = sum(len(bas.mlist[J]) for J in bas.Jlist)
+    X = _rand_cols(np.random.default_rng(1), 3, n)
+    Yd = plan.apply(H, X)
+    Yr = _oracle_vec(O, X)
+    assert np.abs(Yd - Yr).max() <= 1e-13 * np.abs(Yr).max()
+
+
+def test_matvec_sum_and_threshold(device):
+    """H0 + V sum (add_cartens) and the MF element threshold of field()."""
+    from richmol_amd import plan
+    h0, H, cos2, bas = linear_rotor(7)
+    F = [3e8, 0.0, 9e8]
+    H.field(F, thresh=3e16)          # drops field products below 3e16 and small MF entries
+    S = h0 + H
+    Oh0 = OracleCarTens.from_attrs(h0)
+    OH = OracleCarTens.from_attrs(H)
+    OH.field(F, thresh=3e16)
+    OS = Oh0 + OH
+    n = sum(OS.dim2[J]["A"] for J in OS.Jlist2)
+    X = _rand_cols(np.random.default_rng(2), 2, n)
+    Yd = plan.apply(S, X)
+    Yr = _oracle_vec(OS, X)
+    assert np.abs(Yd - Yr).max() <= 1e-13 * np.abs(Yr).max()
+
+
+def _run_pair(h0, H, fields, X, *, split=True, tol=1e-15, fixed=None, dt=10.0, t_units="fs"):
+    """Propagate X with the device TDSE and the oracle; return max deviations."""
+    import torch
+    from richmol_amd.tdse import TDSE
+    Oh0 = OracleCarTens.from_attrs(h0)
+    OH = OracleCarTens.from_attrs(H)
+    nst = len(fields)
+    td = TDSE(t_start=0.0, t_end=dt * nst, dt=dt, t_units=t_units)
+    to = OracleTDSE(t_start=0.0, t_end=dt * nst, dt=dt, t_units=t_units)
+    td.time_grid(); to.time_grid()
+    psi_d = torch.from_numpy(X.copy()).to("cuda:0")
+    psi_o = X.copy()
+    dev, kdev = 0.0, 0
+    kw = {} if fixed is None else {"krylov_fixed": fixed}
+    for F in fields:
+        H.field(F)
+        OH.field(F)
+        stats = []
+        if split:
+            psi_d, t_d = td.update(H, psi_d, H0=h0, tol=tol, **kw)
+            psi_o, t_o = to.update(OH, psi_o, H0=Oh0, tol=tol, fixed_m=fixed, stats=stats)
+        else:
+            S = h0 + H
+            OS = Oh0 + OH
+            psi_d, t_d = td.update(S, psi_d, tol=tol, **kw)
+            psi_o, t_o = to.update(OS, psi_o, tol=tol, fixed_m=fixed, stats=stats)
+        assert t_d == t_o
+        dev = max(dev, np.abs(psi_d.cpu().numpy() - psi_o).max())
+        if stats:
+            kd = td.last_krylov_dims.cpu().numpy()
+            kdev = max(kdev, int(np.abs(kd - np.array(stats)).max()))
+    return dev, kdev, psi_d.cpu().numpy(), psi_o
+
+
+@pytest.mark.parametrize("split", [True, False])
+def test_step_adaptive_xz_field(device, split):
+    """Adaptive Lanczos, X+Z field (Delta m = 0, +-1, +-2 couplings), 4 wavepackets."""
+    h0, H, cos2, bas = linear_rotor(6)
+    n = sum(len(bas.mlist[J]) for J in bas.Jlist)
+    X = _rand_cols(np.random.default_rng(3), 4, n)
+    amps = np.linspace(2e8, 1.6e9, 8)
+    fields = [[0.6 * a, 0.0, a] for a in amps]
+    dev, kdev, _, _ = _run_pair(h0, H, fields, X, split=split)
+    assert dev <= TOL_PSI, dev
+    assert kdev == 0
+
+
+def test_step_fixed_krylov(device):
+    h0, H, cos2, bas = linear_rotor(6)
+    n = sum(len(bas.mlist[J]) for J in bas.Jlist)
+    X = _rand_cols(np.random.default_rng(4), 3, n)
+    fields = [[0.0, 0.0, a] for a in np.linspace(5e8, 1.9e9, 6)]
+    dev, kdev, _, _ = _run_pair(h0, H, fields, X, fixed=12)
+    assert dev <= TOL_PSI, dev
+
+
+def test_step_unnormalised_rows(device):
+    """Boltzmann-weighted rows have norm < 1; the reference Lanczos does not
+    normalise its first vector (tdse.py:443-447) and the device must agree."""
+    h0, H, cos2, bas = linear_rotor(5)
+    n = sum(len(bas.mlist[J]) for J in bas.Jlist)
+    X = _rand_cols(np.random.default_rng(5), 3, n) * np.array([[0.2], [0.55], [1.7]])
+    fields = [[0.0, 0.0, a] for a in np.linspace(8e8, 1.5e9, 5)]
+    dev, kdev, _, _ = _run_pair(h0, H, fields, X)
+    assert dev <= TOL_PSI, dev
+    assert kdev == 0
+
+
+def test_step_zero_field_skips_lanczos(device):
+    """Zero field: mfmat is empty, update applies only the two H0 phases."""
+    h0, H, cos2, bas = linear_rotor(4)
+    n = sum(len(bas.mlist[J]) for J in bas.Jlist)
+    X = _rand_cols(np.random.default_rng(6), 2, n)
+    fields = [[0.0, 0.0, 0.0]] * 3 + [[0.0, 0.0, 1e9]] + [[0.0, 0.0, 0.0]] * 2
+    dev, kdev, _, _ = _run_pair(h0, H, fields, X)
+    assert dev <= TOL_PSI, dev
+
+
+def test_linear_rotor_jmax40_ensemble_slice(device):
+    """Headline basis (J<=40, all m, N=1681), 6 thermal-like columns, steps at
+    the pulse peak, adaptive FP64: wavepackets and <cos^2> against the oracle."""
+    import torch
+    from richmol_amd import plan as _plan
+    h0, H, cos2, bas = linear_rotor(40)
+    n = sum(len(bas.mlist[J]) for J in bas.Jlist)
+    assert n == 1681
+    rng = np.random.default_rng(7)
+    idx = rng.choice(60, size=6, replace=False)
+    X = np.zeros((6, n), dtype=np.complex128)
+    X[np.arange(6), idx] = 1.0
+    t = np.arange(30, 36) * 10.0 + 5.0
+    fields = [[0.0, 0.0, gaussian_pulse_field(tt)] for tt in t]
+    dev, kdev, psi_d, psi_o = _run_pair(h0, H, fields, X)
+    assert dev <= TOL_PSI, dev
+    Oc = OracleCarTens.from_attrs(cos2)
+    C = Oc.tomat(form="full", cart="0")
+    c2_o = np.real(np.einsum("bi,bi->b", psi_o.conj(), (C @ psi_o.T).T)) + 1 / 3
+    cos2.field([0, 0, 1])
+    pc = _plan.plan_for(cos2)
+    pc.set_field(cos2)
+    out = torch.zeros(6, dtype=torch.complex128, device=device)
+    pc.expect(torch.from_numpy(psi_d).to(device), out)
+    c2_d = out.real.cpu().numpy() + 1 / 3
+    assert np.abs(c2_d - c2_o).max() <= TOL_PSI
+
+
+def test_ocs_golden_device(device):
+    """Reference TDSE test system (OCS, even J<=30, m=0, field.txt): device
+    populations against richmol's own Lanczos output pop_lanczos.txt (250 ps,
+    4 decimals) and the first 300 steps against the oracle at 1e-10."""
+    import torch
+    from richmol_amd import units
+    from richmol_amd.tdse import TDSE
+    h0, alpha, d = ocs_fixture()
+    Hbar = alpha * (-0.5 * units.AUpol_x_Vm_to_invcm())
+    ez = d["field_z"]
+    lan = d["pop_lanczos"] / 1e4
+    td = TDSE(t_end=250.0, dt=0.01)
+    td.time_grid()
+    vecs = torch.from_numpy(td.init_state(h0, temp=0)).to(device)
+    Oh0 = OracleCarTens.from_attrs(h0)
+    OH = OracleCarTens.from_attrs(alpha) * (-0.5 * units.AUpol_x_Vm_to_invcm())
+    to = OracleTDSE(t_end=250.0, dt=0.01)
+    to.time_grid()
+    vo = to.init_state(Oh0, temp=0)
+    worst_o, worst_g = 0.0, 0.0
+    for i in range(len(ez)):
+        Hbar.field([0, 0, ez[i]])
+        vecs, t = td.update(Hbar, vecs, H0=h0)
+        if i < 300:
+            OH.field([0, 0, ez[i]])
+            vo, _ = to.update(OH, vo, H0=Oh0)
+            worst_o = max(worst_o, np.abs(vecs.cpu().numpy() - vo).max())
+        if i % 10 == 0:
+            pops = np.abs(vecs[0, :7].cpu().numpy()) ** 2
+            worst_g = max(worst_g, np.abs(np.round(pops, 4) - lan[i // 10]).max())
+    assert worst_o <= TOL_PSI, worst_o
+    assert worst_g <= 1.0001e-4, worst_g   # 4-decimal rounding of the golden file
