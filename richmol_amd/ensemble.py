@@ -1,0 +1,87 @@
+"""Batched propagation of many wavepackets that share one Hamiltonian.
+
+A thermal ensemble (Boltzmann-sampled initial states) or a field-parameter
+scan at fixed structure is one (B, N) block of rows on the device; every time
+step is one field contraction and one ``rm_step`` launch for all B rows
+(the reference loops over rows in Python, tdse.py:379-382).
+"""
+import numpy as np
+
+from . import units
+
+
+def thermal_sample(basis, temp, nsamp, seed=0):
+    """Indices of nsamp basis states |J, m, k> drawn with probability
+    proportional to exp(-E/kT) (all m equally weighted), seeded."""
+    E = np.array([e for J in basis.Jlist for s in basis.symlist[J]
+                  for m in basis.mlist[J] for e in basis.energy[J][s]])
+    kT = units.BOLTZMANN * temp * units.J_to_invcm()
+    w = np.exp(-(E - E.min()) / kT)
+    p = w / w.sum()
+    rng = np.random.default_rng(seed)
+    return rng.choice(len(E), size=nsamp, p=p)
+
+
+def basis_rows(n, idx):
+    """Unit wavepackets |idx_b> as a (B, N) complex128 array."""
+    X = np.zeros((len(idx), n), dtype=np.complex128)
+    X[np.arange(len(idx)), idx] = 1.0
+    return X
+
+
+class Ensemble:
+    """(B, N) wavepackets on the device propagated with a shared operator.
+
+    H: CarTens interaction (its field is set per step by ``step(F)``);
+    H0: diagonal field-free CarTens (split operator) or None;
+    f: exponent factor of the step (TDSE.exp_fac).
+    """
+
+    def __init__(self, H, H0, f, psi0, krylov_fixed=0, maxorder=100, tol=1e-15, stream=None):
+        import torch
+        from . import plan as _plan
+        from .tdse import _plan_phase
+        self.H = H
+        self.f = complex(f)
+        self.krylov_fixed = int(krylov_fixed)
+        self.maxorder = int(maxorder)
+        self.tol = float(tol)
+        self.stream = stream
+        self.plan = _plan.plan_for(H)
+        dev = torch.device("cuda", torch.cuda.current_device())
+        self.psi = torch.as_tensor(np.ascontiguousarray(psi0, dtype=np.complex128)).to(dev)
+        self.out = torch.empty_like(self.psi)
+        B = self.psi.shape[0]
+        self.kused = torch.zeros(B, dtype=torch.int32, device=dev)
+        self.status = torch.zeros(B, dtype=torch.int32, device=dev)
+        self.phase = None
+        if H0 is not None:
+            h0 = H0.tomat(form="full", cart="0")
+            d = torch.from_numpy(np.ascontiguousarray(h0.diagonal(), dtype=np.complex128)).to(dev)
+            self.phase = torch.empty_like(d)
+            _plan_phase(d, self.f, self.phase, stream)
+
+    def step(self, F, thresh=None):
+        """Set the field of the interaction and advance every row by one step."""
+        self.H.field(F, thresh)
+        self.plan.set_field(self.H, self.stream)
+        self.launch()
+
+    def launch(self):
+        """The Krylov step of all rows with the field already on the device."""
+        self.plan.step(self.psi, self.out, self.f, phase=self.phase, krylov_fixed=self.krylov_fixed,
+                       maxorder=self.maxorder, tol=self.tol, kused=self.kused, status=self.status,
+                       stream=self.stream)
+        self.psi, self.out = self.out, self.psi
+
+    def expect(self, obs_plan, out=None):
+        """<psi_b|O|psi_b> for every row (complex, device)."""
+        import torch
+        if out is None:
+            out = torch.empty(self.psi.shape[0], dtype=torch.complex128, device=self.psi.device)
+        obs_plan.expect(self.psi, out, self.stream)
+        return out
+
+    def check(self):
+        if self.krylov_fixed == 0 and bool((self.status != 0).any()):
+            raise ValueError(f"Lanczos reached maximum order of '{self.maxorder}' without convergence")
