@@ -83,6 +83,10 @@ def main():
     ap.add_argument("--cpu-cols", type=int, default=16)
     ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernel", default="auto",
+                    help="auto | global | resident | ell | ell:<i> (explicit ELL configuration)")
+    ap.add_argument("--per-step", action="store_true",
+                    help="one host call per step instead of one rm_propagate call for the run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -108,23 +112,33 @@ def main():
     fields = [[0.0, 0.0, pulse((i + 0.5) * dt)] for i in range(nst)]
 
     ens = ensemble.Ensemble(H, h0, f, psi0, krylov_fixed=args.krylov)
-    for i in range(args.warmup):
-        ens.step(fields[i])
-    torch.cuda.synchronize()
-
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-          for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        ens.H.field(fields[args.warmup + k])
-        ens.plan.set_field(ens.H)
-        ev[k][0].record()
-        ens.launch()
-        ev[k][1].record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kern_ms = sum(a.elapsed_time(b) for a, b in ev) / args.steps
+    ens.set_kernel(args.kernel)
+    if args.per_step:
+        # one Python call per step (the drop-in TDSE.update pattern)
+        for i in range(args.warmup):
+            ens.step(fields[i])
+        torch.cuda.synchronize()
+        ens.plan.timing(True)
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            ens.step(fields[args.warmup + k])
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+    else:
+        # the whole run on the device: field products for all steps, then one
+        # rm_propagate call (field contraction + Krylov step per time step)
+        ens.propagate(fields[:args.warmup])
+        torch.cuda.synchronize()
+        ens.plan.timing(True)
+        t0 = time.perf_counter()
+        ens.propagate(fields[args.warmup:])
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+    tot_ms, nlaunch = ens.plan.kernel_time()
+    ens.plan.timing(False)
+    assert nlaunch == args.steps, (nlaunch, args.steps)
+    kern_ms = tot_ms / nlaunch
+    ens.check()
 
     value = args.ncols * args.steps / wall
     m = args.krylov
@@ -134,8 +148,8 @@ def main():
     achieved = bytes_launch / (kern_ms * 1e-3) / 1e9
 
     # <cos^2 theta> ensemble average after the timed steps (finite, physical)
-    cplan = plan_for(cos2)
     cos2.field([0, 0, 1])
+    cplan = plan_for(cos2)
     cplan.set_field(cos2)
     c2 = float(ens.expect(cplan).real.mean().item()) + 1.0 / 3.0
 
@@ -156,7 +170,9 @@ def main():
                                "512 wavepackets at 1 K, Z pulse 300 fs / 5e11 W/cm^2, dt=10 fs, "
                                "split operator + fixed Krylov m=12, FP64",
                    "jmax": args.jmax, "n_states": n, "wavepackets": args.ncols, "krylov": m,
-                   "dt_fs": dt, "parallelism": "one GPU, one workgroup per wavepacket"},
+                   "dt_fs": dt, "kernel": args.kernel,
+                   "launch": "per-step host calls" if args.per_step else "one rm_propagate call",
+                   "parallelism": "one GPU; wavepackets split over persistent workgroups"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_lanczos_step", "kernel_ms": kern_ms,

@@ -113,6 +113,48 @@ int rm_step(rm_plan* plan, const void* psi_in, void* psi_out, int64_t ld,
             int32_t krylov_fixed, int32_t maxorder, double tol,
             int32_t* kused, int32_t* status, void* stream);
 
+/*
+ * nsteps consecutive steps of the same wavepackets (the user time loop of
+ * examples/ocs_alignment.py:89-100 around TDSE.update, without the host in
+ * between): step s contracts the operator with row s of fprod_dev (device,
+ * nsteps x nfield complex) and advances every column as rm_step does.
+ *   psi: device, in/out; work: device scratch of the same shape (ping-pong).
+ *   kused: device int32[nsteps*ncols] or NULL (Krylov dimension per step).
+ *   status: device int32[ncols] or NULL: RM_NOT_CONVERGED if any step of the
+ *   column hit maxorder (zeroed at the start).
+ * thresh: host, nleaf doubles (< 0 = none), the same for every step.
+ */
+int rm_propagate(rm_plan* plan, void* psi, void* work, int64_t ld, int64_t ncols,
+                 int64_t nsteps, const double* fprod_dev, const double* thresh,
+                 double f_re, double f_im, const void* phase, int32_t krylov_fixed,
+                 int32_t maxorder, double tol, int32_t* kused, int32_t* status,
+                 void* stream);
+
+/* Measurement hooks (no reference counterpart).  With timing enabled, every
+ * Krylov-step launch of rm_step / rm_propagate is bracketed by HIP events on
+ * its stream; rm_plan_kernel_time waits for the last one, returns the summed
+ * launch durations and the number of launches, and resets the record. */
+int rm_plan_timing(rm_plan* plan, int32_t enable);
+int rm_plan_kernel_time(rm_plan* plan, double* total_ms, int64_t* launches);
+
+/* Kernel used by rm_step (no reference counterpart: a device tuning knob).
+ * All variants compute the same step.  RM_KERNEL_AUTO picks the ELL kernel
+ * (persistent workgroups, two wavepackets per workgroup, operator staged in
+ * LDS when it fits) for n <= 4096, else the global-memory kernel. */
+#define RM_KERNEL_AUTO      0
+#define RM_KERNEL_GLOBAL    1
+#define RM_KERNEL_RESIDENT  2
+#define RM_KERNEL_ELL       3
+/* RM_KERNEL_ELL_CFG0 + i: ELL kernel with explicit launch configuration i
+ * (rm_ell_config() describes it); the autotuner's search space. */
+#define RM_KERNEL_ELL_CFG0  16
+int rm_plan_set_kernel(rm_plan* plan, int32_t variant);
+
+/* Launch configuration i of the ELL kernel: threads per workgroup, rows per
+ * thread, wavepackets per workgroup, operator staged in LDS (1/0).
+ * Returns RM_ERR_ARG when i is out of range. */
+int rm_ell_config(int32_t i, int32_t* threads, int32_t* rows, int32_t* cols, int32_t* op_lds);
+
 /* psi_out[:, b] = (psi_in[:, b] * phase) * phase: the step when the operator
  * carries no field at all (tdse.py:375, 394 with the Lanczos skipped). */
 int rm_phase_apply(int64_t n, const void* psi_in, void* psi_out, int64_t ld, int64_t ncols,

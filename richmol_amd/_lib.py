@@ -35,6 +35,13 @@ SIGNATURES = {
     "rm_step": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_double,
                                ctypes.c_double, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
                                _vp, _vp, _vp]),
+    "rm_propagate": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                    _vp, _c_dblp, ctypes.c_double, ctypes.c_double, _vp,
+                                    ctypes.c_int32, ctypes.c_int32, ctypes.c_double, _vp, _vp, _vp]),
+    "rm_plan_timing": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "rm_plan_kernel_time": (ctypes.c_int, [_vp, _c_dblp, _c_i64p]),
+    "rm_plan_set_kernel": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "rm_ell_config": (ctypes.c_int, [ctypes.c_int32, _c_i32p, _c_i32p, _c_i32p, _c_i32p]),
     "rm_phase_apply": (ctypes.c_int, [ctypes.c_int64, _vp, _vp, ctypes.c_int64, ctypes.c_int64, _vp, _vp]),
     "rm_expect": (ctypes.c_int, [_vp, _vp, ctypes.c_int64, ctypes.c_int64, _vp, _vp]),
 }

@@ -17,8 +17,10 @@
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "../../include/richmol_amd.h"
 
@@ -88,6 +90,18 @@ struct rm_plan {
   int32_t step_id = 0;
   void* ws = nullptr;        // Krylov workspace
   size_t ws_bytes = 0;
+  int32_t variant = RM_KERNEL_AUTO;
+  // ELL copy of the matrix: width = max entries per row, column-major
+  // (entry j of row i at j*n + i), padding entries (value 0, column i)
+  int32_t ell_w = 0;
+  int32_t* ell_col = nullptr;
+  double2* ell_val = nullptr;
+  int32_t* ell_pos = nullptr;  // CSR entry -> ELL slot
+  int n_cu = 0;
+  // optional HIP-event timing of the Krylov step launches
+  int timing = 0;
+  int nrec = 0;
+  std::vector<hipEvent_t> ev0, ev1;
 };
 
 template <typename T>
@@ -101,9 +115,12 @@ static int upload(T** dst, const T* src, int64_t count) {
 
 static void plan_free(rm_plan* p) {
   void* bufs[] = {p->row_ptr, p->col, p->pair_ptr, p->pair_term, p->pair_kval, p->term_ptr,
-                  p->term_field, p->term_mval, p->term_leaf, p->mf, p->aval, p->flag, p->ws};
+                  p->term_field, p->term_mval, p->term_leaf, p->mf, p->aval, p->flag, p->ws,
+                  p->ell_col, p->ell_val, p->ell_pos};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
+  for (hipEvent_t e : p->ev0) (void)hipEventDestroy(e);
+  for (hipEvent_t e : p->ev1) (void)hipEventDestroy(e);
   delete p;
 }
 
@@ -170,6 +187,52 @@ extern "C" int rm_plan_create(int64_t n, int64_t nnz, const int64_t* row_ptr, co
     plan_free(p);
     return fail(RM_ERR_HIP, "rm_plan_create: hipMemset failed");
   }
+  // ELL layout
+  int64_t w = 0;
+  for (int64_t i = 0; i < n; ++i) w = row_ptr[i + 1] - row_ptr[i] > w ? row_ptr[i + 1] - row_ptr[i] : w;
+  if (w > INT32_MAX / 2 || (n > 0 && w * n > INT32_MAX)) {
+    plan_free(p);
+    return fail(RM_ERR_ARG, "rm_plan_create: ELL layout (width %lld) too large", (long long)w);
+  }
+  p->ell_w = (int32_t)w;
+  {
+    const int64_t ne = w * n;
+    int32_t* hcol = (int32_t*)malloc(sizeof(int32_t) * (size_t)(ne > 0 ? ne : 1));
+    int32_t* hpos = (int32_t*)malloc(sizeof(int32_t) * (size_t)(nnz > 0 ? nnz : 1));
+    if (!hcol || !hpos) {
+      free(hcol);
+      free(hpos);
+      plan_free(p);
+      return fail(RM_ERR_ALLOC, "rm_plan_create: host allocation");
+    }
+    for (int64_t i = 0; i < n; ++i) {
+      const int64_t len = row_ptr[i + 1] - row_ptr[i];
+      for (int64_t j = 0; j < w; ++j) {
+        if (j < len) {
+          hcol[j * n + i] = col_idx[row_ptr[i] + j];
+          hpos[row_ptr[i] + j] = (int32_t)(j * n + i);
+        } else {
+          hcol[j * n + i] = (int32_t)i;
+        }
+      }
+    }
+    rc = upload(&p->ell_col, hcol, ne);
+    if (!rc) rc = upload(&p->ell_pos, hpos, nnz);
+    if (!rc) rc = upload<double2>(&p->ell_val, nullptr, ne);
+    free(hcol);
+    free(hpos);
+    if (rc) {
+      plan_free(p);
+      return rc;
+    }
+    if (hipMemset(p->ell_val, 0, sizeof(double2) * (size_t)(ne > 0 ? ne : 1)) != hipSuccess) {
+      plan_free(p);
+      return fail(RM_ERR_HIP, "rm_plan_create: hipMemset failed");
+    }
+  }
+  if (hipDeviceGetAttribute(&p->n_cu, hipDeviceAttributeMultiprocessorCount, p->device) != hipSuccess ||
+      p->n_cu <= 0)
+    p->n_cu = 256;
   *out = p;
   return RM_OK;
 }
@@ -196,16 +259,19 @@ struct FieldArgs {
   double thresh[RM_MAX_LEAF];
 };
 
+// fdev: field products of this step in device memory (a row of a table), or
+// NULL to use the kernel-argument copy in fa.
 __global__ void k_field_contract(int64_t nterm, const int64_t* __restrict__ term_ptr,
                                  const int32_t* __restrict__ term_field,
                                  const double2* __restrict__ term_mval,
                                  const int32_t* __restrict__ term_leaf, FieldArgs fa,
+                                 const double2* __restrict__ fdev,
                                  double2* __restrict__ mf, int32_t* __restrict__ flag, int32_t id) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nterm) return;
   double2 acc = make_double2(0.0, 0.0);
   for (int64_t q = term_ptr[t]; q < term_ptr[t + 1]; ++q)
-    acc = cfma(fa.fprod[term_field[q]], term_mval[q], acc);
+    acc = cfma(fdev ? fdev[term_field[q]] : fa.fprod[term_field[q]], term_mval[q], acc);
   double thr = fa.thresh[term_leaf[t]];
   if (thr > 0.0 && hypot(acc.x, acc.y) < thr) acc = make_double2(0.0, 0.0);
   mf[t] = acc;
@@ -214,13 +280,15 @@ __global__ void k_field_contract(int64_t nterm, const int64_t* __restrict__ term
 
 __global__ void k_assemble(int64_t nnz, const int64_t* __restrict__ pair_ptr,
                            const int32_t* __restrict__ pair_term, const double2* __restrict__ pair_kval,
-                           const double2* __restrict__ mf, double2* __restrict__ aval) {
+                           const double2* __restrict__ mf, double2* __restrict__ aval,
+                           const int32_t* __restrict__ ell_pos, double2* __restrict__ ell_val) {
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nnz) return;
   double2 acc = make_double2(0.0, 0.0);
   for (int64_t p = pair_ptr[e]; p < pair_ptr[e + 1]; ++p)
     acc = cfma(pair_kval[p], mf[pair_term[p]], acc);
   aval[e] = acc;
+  ell_val[ell_pos[e]] = acc;
 }
 
 static inline unsigned grid_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
@@ -238,11 +306,12 @@ extern "C" int rm_plan_set_field(rm_plan* p, const double* fprod, const double* 
   p->step_id = (p->step_id + 1) & 0x3fffffff;
   if (p->nterm > 0)
     hipLaunchKernelGGL(k_field_contract, dim3(grid_for(p->nterm, 256)), dim3(256), 0, s, p->nterm,
-                       p->term_ptr, p->term_field, p->term_mval, p->term_leaf, fa, p->mf, p->flag,
-                       p->step_id);
+                       p->term_ptr, p->term_field, p->term_mval, p->term_leaf, fa,
+                       (const double2*)nullptr, p->mf, p->flag, p->step_id);
   if (p->nnz > 0)
     hipLaunchKernelGGL(k_assemble, dim3(grid_for(p->nnz, 256)), dim3(256), 0, s, p->nnz,
-                       p->pair_ptr, p->pair_term, p->pair_kval, p->mf, p->aval);
+                       p->pair_ptr, p->pair_term, p->pair_kval, p->mf, p->aval, p->ell_pos,
+                       p->ell_val);
   HIPCHK(hipGetLastError());
   return RM_OK;
 }
@@ -358,11 +427,72 @@ struct StepArgs {
   int32_t kcap;
   int32_t* kused;
   int32_t* status;
+  // ELL kernel
+  const int32_t* ell_col;
+  const double2* ell_val;
+  int32_t ell_w;
+  int32_t ncols;
+  double2* cscr;    // per column 2*kcap: exp(fT)e0 and its previous value
+  int32_t status_sticky;   // 1: only record failures (multi-step runs)
 };
 
-// Sum NV doubles over the workgroup; every thread receives the same value.
-template <int NV>
-__device__ __forceinline__ void block_sum(double* v, double (*red)[NWAVE][4], int& buf) {
+// Per-column status of a step: RM_NOT_CONVERGED when the Krylov loop hit
+// maxorder; in sticky mode a success does not overwrite an earlier failure.
+__device__ __forceinline__ void put_status(const StepArgs& a, int b, bool bad) {
+  if (a.status && (bad || !a.status_sticky)) a.status[b] = bad ? RM_NOT_CONVERGED : 0;
+}
+
+// DPP row rotation of a double (both 32-bit halves); CTRL is the DPP control.
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+
+// Sum over the 64 lanes of a wave (all lanes active), bitwise identical on
+// every lane: rotations by 8, 4, 2, 1 inside each 16-lane row pair every lane
+// with partners whose partial sums are the same two operands (addition is
+// commutative), then the four row sums are combined in a fixed order.
+__device__ __forceinline__ double wave_allsum(double v) {
+  v += dpp_mov<0x128>(v);   // row_ror:8
+  v += dpp_mov<0x124>(v);   // row_ror:4
+  v += dpp_mov<0x122>(v);   // row_ror:2
+  v += dpp_mov<0x121>(v);   // row_ror:1
+  return (readlane_d(v, 0) + readlane_d(v, 16)) + (readlane_d(v, 32) + readlane_d(v, 48));
+}
+
+// Workgroup sum of NV doubles with DPP wave sums (red: [2][NW][16]).
+template <int NV, int NW>
+__device__ __forceinline__ void block_sum_dpp(double* v, double (*red)[NW][16], int& buf) {
+#pragma unroll
+  for (int i = 0; i < NV; ++i) v[i] = wave_allsum(v[i]);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int i = 0; i < NV; ++i) red[buf][w][i] = v[i];
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < NW; ++j) s += red[buf][j][i];
+    v[i] = s;
+  }
+  buf ^= 1;
+}
+
+// Sum NV doubles over the workgroup (NW waves); every thread receives the same
+// value.  red is double-buffered: one barrier per call suffices because every
+// thread reads red[buf] before it can reach the barrier of the next call.
+template <int NV, int NW>
+__device__ __forceinline__ void block_sum(double* v, double (*red)[NW][4], int& buf) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1)
 #pragma unroll
@@ -376,7 +506,7 @@ __device__ __forceinline__ void block_sum(double* v, double (*red)[NWAVE][4], in
   for (int i = 0; i < NV; ++i) {
     double s = 0.0;
 #pragma unroll
-    for (int j = 0; j < NWAVE; ++j) s += red[buf][j][i];
+    for (int j = 0; j < NW; ++j) s += red[buf][j][i];
     v[i] = s;
   }
   buf ^= 1;
@@ -401,7 +531,7 @@ __device__ __forceinline__ double2 shfl2(double2 v, int src) {
 // lane l owns rows l and l+64 (s <= 128).  Scaling and stepping: exp(A)e0 =
 // (exp(A/nsub))^nsub e0 with |A/nsub|_1 <= 1/2, each factor a Taylor sum.
 __device__ void expm_e0(int s, const double2* alpha, const double* beta, double2 f, double2* c) {
-  const int l = threadIdx.x;
+  const int l = threadIdx.x & 63;
   double2 a[2], bl[2], bu[2];   // A_ii, A_{i,i-1}, A_{i,i+1}
   double col1 = 0.0;
 #pragma unroll
@@ -458,6 +588,628 @@ __device__ void expm_e0(int s, const double2* alpha, const double* beta, double2
   if (l + 64 < s) c[l + 64] = y[1];
 }
 
+// Adaptive stopping metric after iteration k (wave 0 only; tdse.py:473-476).
+// Gram column k of the Krylov basis from the three-term relation
+//   <V_i|V_k> = (<H V_i|V_{k-1}> - a_{k-1} G_{i,k-1} - b_{k-1} G_{i,k-2}) / b_k,
+//   H V_i = b_{i+1} V_{i+1} + a_i V_i + b_i V_{i-1}
+// (skipped after an exact breakdown, whose column the caller computed
+// explicitly), then c = exp(f T_{k+1}) e0 and
+//   conv = |u_k - u_{k-1}|^2 = d^H G d,  d = c - [c_prev, 0];
+// finally c_prev = c.  Returns conv on every lane of wave 0.
+__device__ double adaptive_metric(int k, bool breakdown, double2* G, int kc, const double2* s_alpha,
+                                  const double* s_beta, double2 f, double2* s_c, double2* s_cprev) {
+  const int tid = threadIdx.x & 63;   // lane: any single wave may run this
+  const double beta = s_beta[k];
+  if (!breakdown) {
+    for (int i = tid; i < k; i += 64) {
+      double2 hv;
+      if (i == k - 1) {
+        hv = cconj(s_alpha[k - 1]);
+      } else {
+        hv = cscale(s_beta[i + 1], G[(i + 1) * kc + (k - 1)]);
+        hv = cfma(cconj(s_alpha[i]), G[i * kc + (k - 1)], hv);
+        if (i > 0) hv = cadd(hv, cscale(s_beta[i], G[(i - 1) * kc + (k - 1)]));
+      }
+      double2 t = csub(hv, cmul(s_alpha[k - 1], G[i * kc + (k - 1)]));
+      if (k >= 2) t = csub(t, cscale(s_beta[k - 1], G[i * kc + (k - 2)]));
+      t = make_double2(t.x / beta, t.y / beta);
+      G[i * kc + k] = t;
+      G[k * kc + i] = cconj(t);
+    }
+  }
+  if (tid == 0) G[k * kc + k] = make_double2(1.0, 0.0);
+  __threadfence_block();
+  expm_e0(k + 1, s_alpha, s_beta, f, s_c);
+  __threadfence_block();
+  double part = 0.0;
+  const int s = k + 1;
+  for (int i = tid; i < s; i += 64) {
+    double2 di = csub(s_c[i], i < k ? s_cprev[i] : make_double2(0.0, 0.0));
+    double2 r = make_double2(0.0, 0.0);
+    for (int j = 0; j < s; ++j) {
+      double2 dj = csub(s_c[j], j < k ? s_cprev[j] : make_double2(0.0, 0.0));
+      r = cfma(G[i * kc + j], dj, r);
+    }
+    part += di.x * r.x + di.y * r.y;
+  }
+  part = wave_sum(part);
+  __threadfence_block();
+  for (int i = tid; i < s; i += 64) s_cprev[i] = s_c[i];
+  return part;
+}
+
+// Resident variant: the current Krylov vector lives in LDS (gathered by the
+// sparse matvec), the recurrence vectors V_{k-1}, V_k, W in registers (R rows
+// per thread, rows tid + r*NTR), the basis is written behind to global memory
+// and read once for the final combination (and by the breakdown restart).
+// One barrier separates each LDS overwrite from the gathers of the previous
+// iteration: the alpha reduction (block_sum) that follows every gather.
+template <int NTR, int R>
+__global__ __launch_bounds__(NTR) void k_lanczos_resident(StepArgs a) {
+  constexpr int NW = NTR / 64;
+  extern __shared__ double2 s_x[];
+  __shared__ double red[2][NW][4];
+  __shared__ double2 s_alpha[KMAX];
+  __shared__ double s_beta[KMAX + 1];
+  __shared__ double2 s_c[KMAX];
+  __shared__ double2 s_cprev[KMAX];
+  __shared__ double s_conv;
+  int buf = 0;
+  const int b = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int n = (int)a.n;
+  const double2* psi = a.psi_in + (int64_t)b * a.ld;
+  double2* out = a.psi_out + (int64_t)b * a.ld;
+  double2* V = a.basis + (int64_t)b * a.kcap * a.n;
+  double2* G = a.gram + (int64_t)b * a.kcap * a.kcap;
+  const int kc = a.kcap;
+  const bool fixed = a.fixed_m > 0;
+  const bool skip = a.flag != nullptr && (*a.flag != a.flag_expect);
+  const double2 zero = make_double2(0.0, 0.0);
+
+  double2 vk[R], vp[R], W[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int i = tid + r * NTR;
+    vk[r] = zero;
+    vp[r] = zero;
+    W[r] = zero;
+    if (i < n) {
+      double2 v = psi[i];
+      if (a.phase) v = cmul(v, a.phase[i]);
+      if (skip) {
+        out[i] = a.phase ? cmul(v, a.phase[i]) : v;
+      } else {
+        vk[r] = v;
+        V[i] = v;
+        s_x[i] = v;
+      }
+    }
+  }
+  if (skip) {
+    if (tid == 0) {
+      if (a.kused) a.kused[b] = 0;
+      put_status(a, b, false);
+    }
+    return;
+  }
+  __syncthreads();
+
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int i = tid + r * NTR;
+    if (i < n) {
+      double2 w = spmv_row(i, a.row_ptr, a.col, a.aval, s_x);
+      W[r] = w;
+      acc[0] += w.x * vk[r].x + w.y * vk[r].y;
+      acc[1] += w.x * vk[r].y - w.y * vk[r].x;
+      acc[2] += vk[r].x * vk[r].x + vk[r].y * vk[r].y;
+    }
+  }
+  block_sum<3>(acc, red, buf);
+  double2 alpha = make_double2(acc[0], acc[1]);
+#pragma unroll
+  for (int r = 0; r < R; ++r) W[r] = csub(W[r], cmul(alpha, vk[r]));
+  if (tid == 0) {
+    s_alpha[0] = alpha;
+    s_cprev[0] = make_double2(1.0, 0.0);
+    s_c[0] = make_double2(1.0, 0.0);
+    if (!fixed) G[0] = make_double2(acc[2], 0.0);
+  }
+
+  int k = 1;
+  double conv = 1.0;
+  for (;;) {
+    if (fixed) {
+      if (k >= a.fixed_m) break;
+    } else if (!(k < a.maxorder && conv > a.tol)) {
+      break;
+    }
+    acc[0] = 0.0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[0] += W[r].x * W[r].x + W[r].y * W[r].y;
+    block_sum<1>(acc, red, buf);
+    const double beta = sqrt(acc[0]);
+    const bool breakdown = !(beta != 0.0);
+#pragma unroll
+    for (int r = 0; r < R; ++r) vp[r] = vk[r];
+    if (!breakdown) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) vk[r] = make_double2(W[r].x / beta, W[r].y / beta);
+    } else {
+      // exact breakdown: Gram-Schmidt of a vector of ones against the stored
+      // basis (tdse.py:459-465); each thread owns its rows of every V_j
+#pragma unroll
+      for (int r = 0; r < R; ++r) W[r] = (tid + r * NTR < n) ? make_double2(1.0, 0.0) : zero;
+      for (int j = 0; j < k; ++j) {
+        const double2* Vj = V + (int64_t)j * n;
+        acc[0] = acc[1] = 0.0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int i = tid + r * NTR;
+          if (i < n) {
+            double2 x = Vj[i];
+            acc[0] += x.x * W[r].x + x.y * W[r].y;
+            acc[1] += x.x * W[r].y - x.y * W[r].x;
+          }
+        }
+        block_sum<2>(acc, red, buf);
+        const double2 proj = make_double2(acc[0], acc[1]);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int i = tid + r * NTR;
+          if (i < n) W[r] = csub(W[r], cmul(proj, Vj[i]));
+        }
+      }
+      acc[0] = 0.0;
+#pragma unroll
+      for (int r = 0; r < R; ++r) acc[0] += W[r].x * W[r].x + W[r].y * W[r].y;
+      block_sum<1>(acc, red, buf);
+      const double nv = sqrt(acc[0]);
+#pragma unroll
+      for (int r = 0; r < R; ++r) vk[r] = make_double2(W[r].x / nv, W[r].y / nv);
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int i = tid + r * NTR;
+      if (i < n) {
+        V[(int64_t)k * n + i] = vk[r];
+        s_x[i] = vk[r];
+      }
+    }
+    if (tid == 0) s_beta[k] = beta;
+    __syncthreads();
+
+    if (breakdown && !fixed) {
+      for (int j = 0; j < k; ++j) {
+        const double2* Vj = V + (int64_t)j * n;
+        acc[0] = acc[1] = 0.0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int i = tid + r * NTR;
+          if (i < n) {
+            double2 x = Vj[i];
+            acc[0] += x.x * vk[r].x + x.y * vk[r].y;
+            acc[1] += x.x * vk[r].y - x.y * vk[r].x;
+          }
+        }
+        block_sum<2>(acc, red, buf);
+        if (tid == 0) {
+          G[j * kc + k] = make_double2(acc[0], acc[1]);
+          G[k * kc + j] = make_double2(acc[0], -acc[1]);
+        }
+      }
+    }
+
+    acc[0] = acc[1] = 0.0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int i = tid + r * NTR;
+      if (i < n) {
+        double2 w = spmv_row(i, a.row_ptr, a.col, a.aval, s_x);
+        W[r] = w;
+        acc[0] += w.x * vk[r].x + w.y * vk[r].y;
+        acc[1] += w.x * vk[r].y - w.y * vk[r].x;
+      }
+    }
+    block_sum<2>(acc, red, buf);
+    alpha = make_double2(acc[0], acc[1]);
+#pragma unroll
+    for (int r = 0; r < R; ++r) W[r] = csub(csub(W[r], cmul(alpha, vk[r])), cscale(beta, vp[r]));
+    if (tid == 0) s_alpha[k] = alpha;
+
+    if (!fixed) {
+      __syncthreads();
+      if (tid < 64) {
+        double cv = adaptive_metric(k, breakdown, G, kc, s_alpha, s_beta, a.f, s_c, s_cprev);
+        if (tid == 0) s_conv = cv;
+      }
+      __syncthreads();
+      conv = s_conv;
+    }
+    ++k;
+  }
+
+  __syncthreads();
+  if (fixed && tid < 64 && k > 1) expm_e0(k, s_alpha, s_beta, a.f, s_c);
+  __syncthreads();
+
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int i = tid + r * NTR;
+    if (i < n) {
+      double2 u = zero;
+      for (int j = 0; j < k - 1; ++j) u = cfma(s_c[j], V[(int64_t)j * n + i], u);
+      u = cfma(s_c[k - 1], vk[r], u);
+      if (a.phase) u = cmul(u, a.phase[i]);
+      out[i] = u;
+    }
+  }
+  if (tid == 0) {
+    if (a.kused) a.kused[b] = k;
+    put_status(a, b, !fixed && k >= a.maxorder);
+  }
+}
+
+// W[c][r] = (H x_c)[row tid + r*NTR] for the ELL operator, branch-free: rows
+// past n read row n-1 (a valid address) and are zeroed at the end, so all R
+// loads of one ELL slot j issue together.  x_c = s_x + c*n (LDS).
+template <int NTR, int R, int C, bool OPL>
+__device__ __forceinline__ void ell_apply(double2 (&W)[C][R], int n, int Wd, const double2* s_val,
+                                          const uint16_t* s_col, const int32_t* __restrict__ g_col,
+                                          const double2* __restrict__ g_val, const double2* s_x) {
+  const int tid = threadIdx.x;
+  const double2 zero = make_double2(0.0, 0.0);
+#pragma unroll
+  for (int c = 0; c < C; ++c)
+#pragma unroll
+    for (int r = 0; r < R; ++r) W[c][r] = zero;
+  for (int j = 0; j < Wd; ++j) {
+    int col[R];
+    double2 val[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int i = min(tid + r * NTR, n - 1);
+      const int q = j * n + i;
+      col[r] = OPL ? (int)s_col[q] : g_col[q];
+      val[r] = OPL ? s_val[q] : g_val[q];
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int c = 0; c < C; ++c) W[c][r] = cfma(val[r], s_x[c * n + col[r]], W[c][r]);
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    if (tid + r * NTR >= n)
+#pragma unroll
+      for (int c = 0; c < C; ++c) W[c][r] = zero;
+}
+
+// ELL variant: persistent workgroups, C wavepackets per workgroup advanced in
+// lockstep (shared barriers, batched reductions, one operator read serving C
+// columns), R rows per thread in registers, the C current Krylov vectors in
+// LDS, and (OPL) the ELL operator staged into LDS once per workgroup.
+// Columns of a group that converge early idle until the group is done.
+// FIXED (compile time) selects the fixed Krylov dimension; the adaptive
+// instantiation carries the per-iteration stopping metric.
+template <int NTR, int R, int C, bool OPL, bool FIXED>
+__global__ __launch_bounds__(NTR) void k_lanczos_ell(StepArgs a) {
+  constexpr int NW = NTR / 64;
+  static_assert(C <= NW, "one wave per column for the adaptive metric");
+  extern __shared__ double2 s_dyn[];
+  __shared__ double red[2][NW][16];
+  __shared__ double2 s_alpha[C][KMAX];
+  __shared__ double s_beta[C][KMAX + 1];
+  __shared__ double2 s_cf[C][KMAX];
+  __shared__ double s_conv[C];
+  int buf = 0;
+  const int tid = threadIdx.x;
+  const int n = (int)a.n;
+  const int Wd = a.ell_w;
+  const int kc = a.kcap;
+  double2* s_x = s_dyn;
+  double2* s_val = s_dyn + (size_t)C * n;
+  uint16_t* s_col = (uint16_t*)(s_val + (size_t)Wd * n);
+  constexpr bool fixed = FIXED;
+  const bool skip = a.flag != nullptr && (*a.flag != a.flag_expect);
+  const double2 zero = make_double2(0.0, 0.0);
+
+  if (OPL && !skip) {
+    for (int q = tid; q < Wd * n; q += NTR) {
+      s_val[q] = a.ell_val[q];
+      s_col[q] = (uint16_t)a.ell_col[q];
+    }
+  }
+
+  for (int g = blockIdx.x; g * C < a.ncols; g += gridDim.x) {
+    bool act[C];
+    double2* V[C];
+    double2* G[C];
+    double2* cv[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      const int b = g * C + c;
+      act[c] = b < a.ncols;
+      const int bb = act[c] ? b : 0;
+      V[c] = a.basis + (int64_t)bb * kc * n;
+      G[c] = a.gram + (int64_t)bb * kc * kc;
+      cv[c] = a.cscr + (int64_t)bb * 2 * kc;
+    }
+    double2 vk[C][R], vp[C][R], W[C][R];
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int i = tid + r * NTR;
+        vk[c][r] = zero;
+        vp[c][r] = zero;
+        W[c][r] = zero;
+        if (act[c] && i < n) {
+          const int64_t off = (int64_t)(g * C + c) * a.ld + i;
+          double2 v = a.psi_in[off];
+          if (a.phase) v = cmul(v, a.phase[i]);
+          if (skip) {
+            a.psi_out[off] = a.phase ? cmul(v, a.phase[i]) : v;
+          } else {
+            vk[c][r] = v;
+            V[c][i] = v;
+            s_x[c * n + i] = v;
+          }
+        }
+      }
+    if (skip) {
+      if (tid == 0)
+        for (int c = 0; c < C; ++c)
+          if (act[c]) {
+            if (a.kused) a.kused[g * C + c] = 0;
+            put_status(a, g * C + c, false);
+          }
+      continue;
+    }
+    __syncthreads();
+
+    // w = H V_0 for every column of the group
+    double acc[3 * C];
+#pragma unroll
+    for (int q = 0; q < 3 * C; ++q) acc[q] = 0.0;
+    ell_apply<NTR, R, C, OPL>(W, n, Wd, s_val, s_col, a.ell_col, a.ell_val, s_x);
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const double2 w = W[c][r], v = vk[c][r];
+        acc[3 * c] += w.x * v.x + w.y * v.y;
+        acc[3 * c + 1] += w.x * v.y - w.y * v.x;
+        acc[3 * c + 2] += v.x * v.x + v.y * v.y;
+      }
+    block_sum_dpp<3 * C>(acc, red, buf);
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      const double2 al = make_double2(acc[3 * c], acc[3 * c + 1]);
+#pragma unroll
+      for (int r = 0; r < R; ++r) W[c][r] = csub(W[c][r], cmul(al, vk[c][r]));
+      if (tid == 0) {
+        s_alpha[c][0] = al;
+        if (!fixed && act[c]) {
+          G[c][0] = make_double2(acc[3 * c + 2], 0.0);
+          cv[c][kc] = make_double2(1.0, 0.0);   // c_prev = [1]
+        }
+      }
+    }
+
+    bool live[C];
+    double conv[C];
+    int kk[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      conv[c] = 1.0;
+      kk[c] = 1;
+      live[c] = act[c];
+    }
+    for (int it = 1;; ++it) {
+      bool any = false;
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        if (live[c]) {
+          if (fixed) live[c] = it < a.fixed_m;
+          else live[c] = it < a.maxorder && conv[c] > a.tol;
+        }
+        any |= live[c];
+      }
+      if (!any) break;
+      // beta_it = |W| (tdse.py:453)
+      double bs[C];
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        bs[c] = 0.0;
+        if (live[c])
+#pragma unroll
+          for (int r = 0; r < R; ++r) bs[c] += W[c][r].x * W[c][r].x + W[c][r].y * W[c][r].y;
+      }
+      block_sum_dpp<C>(bs, red, buf);
+      bool brk[C];
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        const double beta = sqrt(bs[c]);
+        bs[c] = beta;
+        brk[c] = live[c] && !(beta != 0.0);
+        if (live[c] && !brk[c]) {
+          const double ib = 1.0 / beta;
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            vp[c][r] = vk[c][r];
+            vk[c][r] = cscale(ib, W[c][r]);
+          }
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        if (!brk[c]) continue;
+        // exact breakdown: Gram-Schmidt of ones against the stored basis (tdse.py:459-465)
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          vp[c][r] = vk[c][r];
+          W[c][r] = (tid + r * NTR < n) ? make_double2(1.0, 0.0) : zero;
+        }
+        for (int j = 0; j < it; ++j) {
+          const double2* Vj = V[c] + (int64_t)j * n;
+          double d[2] = {0.0, 0.0};
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            const int i = tid + r * NTR;
+            if (i < n) {
+              const double2 x = Vj[i];
+              d[0] += x.x * W[c][r].x + x.y * W[c][r].y;
+              d[1] += x.x * W[c][r].y - x.y * W[c][r].x;
+            }
+          }
+          block_sum_dpp<2>(d, red, buf);
+          const double2 proj = make_double2(d[0], d[1]);
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            const int i = tid + r * NTR;
+            if (i < n) W[c][r] = csub(W[c][r], cmul(proj, Vj[i]));
+          }
+        }
+        double nv = 0.0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) nv += W[c][r].x * W[c][r].x + W[c][r].y * W[c][r].y;
+        block_sum_dpp<1>(&nv, red, buf);
+        nv = sqrt(nv);
+#pragma unroll
+        for (int r = 0; r < R; ++r) vk[c][r] = make_double2(W[c][r].x / nv, W[c][r].y / nv);
+      }
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        if (!live[c]) continue;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int i = tid + r * NTR;
+          if (i < n) {
+            V[c][(int64_t)it * n + i] = vk[c][r];
+            s_x[c * n + i] = vk[c][r];
+          }
+        }
+        if (tid == 0) s_beta[c][it] = bs[c];
+      }
+      __syncthreads();
+
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        if (!(brk[c] && !fixed)) continue;
+        for (int j = 0; j < it; ++j) {
+          const double2* Vj = V[c] + (int64_t)j * n;
+          double d[2] = {0.0, 0.0};
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            const int i = tid + r * NTR;
+            if (i < n) {
+              const double2 x = Vj[i];
+              d[0] += x.x * vk[c][r].x + x.y * vk[c][r].y;
+              d[1] += x.x * vk[c][r].y - x.y * vk[c][r].x;
+            }
+          }
+          block_sum_dpp<2>(d, red, buf);
+          if (tid == 0) {
+            G[c][j * kc + it] = make_double2(d[0], d[1]);
+            G[c][it * kc + j] = make_double2(d[0], -d[1]);
+          }
+        }
+      }
+
+      // w = H V_it; alpha = vdot(w, V_it); W = w - alpha V_it - beta V_{it-1}
+      // (finished columns are multiplied too; their W is never read again)
+      double as[2 * C];
+#pragma unroll
+      for (int c = 0; c < C; ++c) as[2 * c] = as[2 * c + 1] = 0.0;
+      ell_apply<NTR, R, C, OPL>(W, n, Wd, s_val, s_col, a.ell_col, a.ell_val, s_x);
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        if (!live[c]) continue;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const double2 w = W[c][r], v = vk[c][r];
+          as[2 * c] += w.x * v.x + w.y * v.y;
+          as[2 * c + 1] += w.x * v.y - w.y * v.x;
+        }
+      }
+      block_sum_dpp<2 * C>(as, red, buf);
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        if (!live[c]) continue;
+        const double2 al = make_double2(as[2 * c], as[2 * c + 1]);
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          W[c][r] = csub(csub(W[c][r], cmul(al, vk[c][r])), cscale(bs[c], vp[c][r]));
+        if (tid == 0) s_alpha[c][it] = al;
+        kk[c] = it + 1;
+      }
+
+      if (!fixed) {
+        __syncthreads();
+        const int w = tid >> 6;
+        if (w < C && live[w]) {
+          const double cvw = adaptive_metric(it, brk[w], G[w], kc, s_alpha[w], s_beta[w], a.f,
+                                             cv[w], cv[w] + kc);
+          if ((tid & 63) == 0) s_conv[w] = cvw;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+          if (live[c]) conv[c] = s_conv[c];
+      }
+    }
+
+    // coefficients of the final combination, into LDS for every wave
+    __syncthreads();
+    {
+      const int w = tid >> 6, l = tid & 63;
+      if (w < C && act[w]) {
+        if (fixed) {
+          if (kk[w] > 1) expm_e0(kk[w], s_alpha[w], s_beta[w], a.f, s_cf[w]);
+          else if (l == 0) s_cf[w][0] = make_double2(1.0, 0.0);
+        } else if (kk[w] == 1) {
+          if (l == 0) s_cf[w][0] = make_double2(1.0, 0.0);
+        } else {
+          for (int j = l; j < kk[w]; j += 64) s_cf[w][j] = cv[w][j];
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      if (!act[c]) continue;
+      const int k = kk[c];
+      double2 u[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) u[r] = zero;
+      for (int j = 0; j < k - 1; ++j) {
+        const double2 cj = s_cf[c][j];
+        const double2* Vj = V[c] + (int64_t)j * n;
+#pragma unroll
+        for (int r = 0; r < R; ++r) u[r] = cfma(cj, Vj[min(tid + r * NTR, n - 1)], u[r]);
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int i = tid + r * NTR;
+        if (i < n) {
+          double2 uu = cfma(s_cf[c][k - 1], vk[c][r], u[r]);
+          if (a.phase) uu = cmul(uu, a.phase[i]);
+          a.psi_out[(int64_t)(g * C + c) * a.ld + i] = uu;
+        }
+      }
+      if (tid == 0) {
+        if (a.kused) a.kused[g * C + c] = k;
+        put_status(a, g * C + c, !fixed && k >= a.maxorder);
+      }
+    }
+    // the next group rewrites s_x only after the barrier that follows its loads
+    __syncthreads();
+  }
+}
+
 __global__ __launch_bounds__(NT) void k_lanczos_step(StepArgs a) {
   __shared__ double red[2][NWAVE][4];
   __shared__ double2 s_alpha[KMAX];
@@ -490,7 +1242,7 @@ __global__ __launch_bounds__(NT) void k_lanczos_step(StepArgs a) {
   if (skip) {
     if (tid == 0) {
       if (a.kused) a.kused[b] = 0;
-      if (a.status) a.status[b] = 0;
+      put_status(a, b, false);
     }
     return;
   }
@@ -594,47 +1346,8 @@ __global__ __launch_bounds__(NT) void k_lanczos_step(StepArgs a) {
     if (!fixed) {
       __syncthreads();
       if (tid < 64) {
-        // Gram column k from the three-term relation (exact-arithmetic identity):
-        //   <V_i|V_k> = (<H V_i|V_{k-1}> - a_{k-1} G_{i,k-1} - b_{k-1} G_{i,k-2}) / b_k
-        //   H V_i = b_{i+1} V_{i+1} + a_i V_i + b_i V_{i-1}
-        if (!breakdown) {
-          for (int i = tid; i < k; i += 64) {
-            double2 hv;
-            if (i == k - 1) {
-              hv = cconj(s_alpha[k - 1]);
-            } else {
-              hv = cscale(s_beta[i + 1], G[(i + 1) * kc + (k - 1)]);
-              hv = cfma(cconj(s_alpha[i]), G[i * kc + (k - 1)], hv);
-              if (i > 0) hv = cadd(hv, cscale(s_beta[i], G[(i - 1) * kc + (k - 1)]));
-            }
-            double2 t = csub(hv, cmul(s_alpha[k - 1], G[i * kc + (k - 1)]));
-            if (k >= 2) t = csub(t, cscale(s_beta[k - 1], G[i * kc + (k - 2)]));
-            t = make_double2(t.x / beta, t.y / beta);
-            G[i * kc + k] = t;
-            G[k * kc + i] = cconj(t);
-          }
-        }
-        if (tid == 0) G[k * kc + k] = make_double2(1.0, 0.0);
-        __threadfence_block();
-        // c = exp(f T_{k+1}) e0 (tdse.py:474)
-        expm_e0(k + 1, s_alpha, s_beta, a.f, s_c);
-        __threadfence_block();
-        // conv = |u_k - u_{k-1}|^2 = d^H G d, d = c - [cprev, 0]  (tdse.py:475-476)
-        double part = 0.0;
-        const int s = k + 1;
-        for (int i = tid; i < s; i += 64) {
-          double2 di = csub(s_c[i], i < k ? s_cprev[i] : make_double2(0.0, 0.0));
-          double2 r = make_double2(0.0, 0.0);
-          for (int j = 0; j < s; ++j) {
-            double2 dj = csub(s_c[j], j < k ? s_cprev[j] : make_double2(0.0, 0.0));
-            r = cfma(G[i * kc + j], dj, r);
-          }
-          part += di.x * r.x + di.y * r.y;
-        }
-        part = wave_sum(part);
-        __threadfence_block();
-        for (int i = tid; i < s; i += 64) s_cprev[i] = s_c[i];
-        if (tid == 0) s_conv = part;
+        double cv = adaptive_metric(k, breakdown, G, kc, s_alpha, s_beta, a.f, s_c, s_cprev);
+        if (tid == 0) s_conv = cv;
       }
       __syncthreads();
       conv = s_conv;
@@ -642,13 +1355,9 @@ __global__ __launch_bounds__(NT) void k_lanczos_step(StepArgs a) {
     ++k;
   }
 
-  if (fixed) {
-    __syncthreads();
-    if (tid < 64 && k > 1) expm_e0(k, s_alpha, s_beta, a.f, s_c);
-    __syncthreads();
-  } else {
-    __syncthreads();
-  }
+  __syncthreads();
+  if (fixed && tid < 64 && k > 1) expm_e0(k, s_alpha, s_beta, a.f, s_c);
+  __syncthreads();
 
   // u = sum_i c_i V_i ; out = u * P  (tdse.py:475, 394)
   for (int64_t i = tid; i < n; i += NT) {
@@ -659,11 +1368,179 @@ __global__ __launch_bounds__(NT) void k_lanczos_step(StepArgs a) {
   }
   if (tid == 0) {
     if (a.kused) a.kused[b] = k;
-    if (a.status) a.status[b] = (!fixed && k >= a.maxorder) ? RM_NOT_CONVERGED : 0;
+    put_status(a, b, !fixed && k >= a.maxorder);
   }
 }
 
+template <int NTR, int R>
+hipError_t launch_resident(const StepArgs& a, unsigned ncols, hipStream_t s) {
+  const size_t lds = sizeof(double2) * (size_t)a.n;
+  static bool attr_set = false;
+  if (!attr_set) {
+    hipError_t e = hipFuncSetAttribute((const void*)k_lanczos_resident<NTR, R>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 8192);
+    if (e != hipSuccess) return e;
+    attr_set = true;
+  }
+  hipLaunchKernelGGL((k_lanczos_resident<NTR, R>), dim3(ncols), dim3(NTR), lds, s, a);
+  return hipGetLastError();
+}
+
+// Largest basis handled by the resident kernel: its Krylov vector fills the
+// dynamic LDS (16 B per state) next to ~6 KB of static LDS.
+constexpr int64_t RESIDENT_MAX_N = 8192;
+constexpr int64_t ELL_MAX_N = 4096;
+constexpr size_t LDS_BYTES = 160 * 1024;
+
+// Launch the ELL kernel if its LDS need fits.  *launched tells whether it
+// did; the return value is the HIP status of the attempt.
+template <int NTR, int R, int C, bool OPL, bool FIXED>
+hipError_t try_launch_ell_k(const StepArgs& a, int ncols, hipStream_t s, int n_cu, bool* launched) {
+  *launched = false;
+  auto kern = k_lanczos_ell<NTR, R, C, OPL, FIXED>;
+  static size_t static_lds = (size_t)-1;
+  if (static_lds == (size_t)-1) {
+    hipFuncAttributes fa;
+    hipError_t e = hipFuncGetAttributes(&fa, (const void*)kern);
+    if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)(LDS_BYTES - fa.sharedSizeBytes));
+    if (e != hipSuccess) return e;
+    static_lds = fa.sharedSizeBytes;
+  }
+  const size_t lds = (size_t)C * a.n * sizeof(double2) +
+                     (OPL ? (size_t)a.ell_w * a.n * (sizeof(double2) + sizeof(uint16_t)) : 0);
+  if (lds + static_lds > LDS_BYTES) return hipSuccess;
+  // The occupancy query does not count LDS beyond 64 KB; with more, the LDS
+  // bound (one workgroup per 160 KB CU) is the binding one.
+  int occ = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)kern, NTR, lds);
+  if (e != hipSuccess) return e;
+  const int occ_lds = (int)(LDS_BYTES / (lds + static_lds));
+  if (occ < 1) occ = 1;
+  if (occ > occ_lds) occ = occ_lds;
+  const int groups = (ncols + C - 1) / C;
+  const int grid = groups < n_cu * occ ? groups : n_cu * occ;
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(NTR), lds, s, a);
+  *launched = true;
+  return hipGetLastError();
+}
+
+template <int NTR, int R, int C, bool OPL>
+hipError_t try_launch_ell(const StepArgs& a, int ncols, hipStream_t s, int n_cu, bool* launched) {
+  if (a.fixed_m > 0) return try_launch_ell_k<NTR, R, C, OPL, true>(a, ncols, s, n_cu, launched);
+  return try_launch_ell_k<NTR, R, C, OPL, false>(a, ncols, s, n_cu, launched);
+}
+
+template <int NTR, int R>
+hipError_t launch_ell(const StepArgs& a, int ncols, hipStream_t s, int n_cu) {
+  bool done = false;
+  hipError_t e;
+  if (NTR <= 512 && R <= 4 && ncols >= 2) {
+    e = try_launch_ell<NTR, R, 2, true>(a, ncols, s, n_cu, &done);
+    if (done || e != hipSuccess) return e;
+  }
+  e = try_launch_ell<NTR, R, 1, true>(a, ncols, s, n_cu, &done);
+  if (done || e != hipSuccess) return e;
+  if (NTR <= 512 && R <= 4 && ncols >= 2) {
+    e = try_launch_ell<NTR, R, 2, false>(a, ncols, s, n_cu, &done);
+    if (done || e != hipSuccess) return e;
+  }
+  e = try_launch_ell<NTR, R, 1, false>(a, ncols, s, n_cu, &done);
+  if (!done && e == hipSuccess) e = hipErrorInvalidConfiguration;
+  return e;
+}
+
+// Explicit ELL configurations (threads, rows per thread, columns per
+// workgroup, operator in LDS), selected by rm_plan_set_kernel(RM_KERNEL_ELL_CFG0 + i).
+struct EllCfg {
+  int ntr, r, c, opl;
+};
+constexpr EllCfg ELL_CFGS[] = {
+    {256, 4, 1, 1}, {256, 4, 2, 1}, {256, 8, 1, 1}, {512, 4, 1, 1}, {512, 4, 2, 1},
+    {512, 8, 1, 1}, {256, 4, 1, 0}, {256, 4, 2, 0}, {256, 8, 1, 0}, {512, 4, 1, 0},
+    {512, 4, 2, 0}, {512, 8, 1, 0}};
+constexpr int N_ELL_CFGS = (int)(sizeof(ELL_CFGS) / sizeof(ELL_CFGS[0]));
+
+hipError_t launch_ell_cfg(int i, const StepArgs& a, int ncols, hipStream_t s, int n_cu, bool* done) {
+  switch (i) {
+    case 0: return try_launch_ell<256, 4, 1, true>(a, ncols, s, n_cu, done);
+    case 1: return try_launch_ell<256, 4, 2, true>(a, ncols, s, n_cu, done);
+    case 2: return try_launch_ell<256, 8, 1, true>(a, ncols, s, n_cu, done);
+    case 3: return try_launch_ell<512, 4, 1, true>(a, ncols, s, n_cu, done);
+    case 4: return try_launch_ell<512, 4, 2, true>(a, ncols, s, n_cu, done);
+    case 5: return try_launch_ell<512, 8, 1, true>(a, ncols, s, n_cu, done);
+    case 6: return try_launch_ell<256, 4, 1, false>(a, ncols, s, n_cu, done);
+    case 7: return try_launch_ell<256, 4, 2, false>(a, ncols, s, n_cu, done);
+    case 8: return try_launch_ell<256, 8, 1, false>(a, ncols, s, n_cu, done);
+    case 9: return try_launch_ell<512, 4, 1, false>(a, ncols, s, n_cu, done);
+    case 10: return try_launch_ell<512, 4, 2, false>(a, ncols, s, n_cu, done);
+    default: return try_launch_ell<512, 8, 1, false>(a, ncols, s, n_cu, done);
+  }
+}
+
+bool ell_cfg_valid(int i, int64_t n, int64_t ncols) {
+  if (i < 0 || i >= N_ELL_CFGS) return false;
+  const EllCfg& c = ELL_CFGS[i];
+  return n <= (int64_t)c.ntr * c.r && (c.c == 1 || ncols >= 2);
+}
+
+hipError_t launch_step(const StepArgs& a, unsigned ncols, hipStream_t s, int variant, int n_cu) {
+  const int64_t n = a.n;
+  if (variant >= RM_KERNEL_ELL_CFG0) {
+    const int i = variant - RM_KERNEL_ELL_CFG0;
+    if (!ell_cfg_valid(i, n, ncols)) return hipErrorInvalidConfiguration;
+    bool done = false;
+    hipError_t e = launch_ell_cfg(i, a, (int)ncols, s, n_cu, &done);
+    if (!done && e == hipSuccess) e = hipErrorInvalidConfiguration;
+    return e;
+  }
+  if ((variant == RM_KERNEL_AUTO || variant == RM_KERNEL_ELL) && n <= ELL_MAX_N) {
+    if (n <= 1024) return launch_ell<256, 4>(a, (int)ncols, s, n_cu);
+    if (n <= 2048) return launch_ell<512, 4>(a, (int)ncols, s, n_cu);
+    return launch_ell<512, 8>(a, (int)ncols, s, n_cu);
+  }
+  if (variant == RM_KERNEL_RESIDENT && n <= RESIDENT_MAX_N) {
+    if (n <= 256) return launch_resident<256, 1>(a, ncols, s);
+    if (n <= 512) return launch_resident<256, 2>(a, ncols, s);
+    if (n <= 1024) return launch_resident<256, 4>(a, ncols, s);
+    if (n <= 2048) return launch_resident<256, 8>(a, ncols, s);
+    if (n <= 4096) return launch_resident<512, 8>(a, ncols, s);
+    return launch_resident<1024, 8>(a, ncols, s);
+  }
+  hipLaunchKernelGGL(k_lanczos_step, dim3(ncols), dim3(NT), 0, s, a);
+  return hipGetLastError();
+}
+
 }  // namespace
+
+extern "C" int rm_ell_config(int32_t i, int32_t* threads, int32_t* rows, int32_t* cols,
+                             int32_t* op_lds) {
+  if (i < 0 || i >= N_ELL_CFGS) return fail(RM_ERR_ARG, "rm_ell_config: no configuration %d", i);
+  if (threads) *threads = ELL_CFGS[i].ntr;
+  if (rows) *rows = ELL_CFGS[i].r;
+  if (cols) *cols = ELL_CFGS[i].c;
+  if (op_lds) *op_lds = ELL_CFGS[i].opl;
+  return RM_OK;
+}
+
+extern "C" int rm_plan_set_kernel(rm_plan* p, int32_t variant) {
+  if (!p) return fail(RM_ERR_ARG, "rm_plan_set_kernel: NULL plan");
+  const bool cfg = variant >= RM_KERNEL_ELL_CFG0 && variant < RM_KERNEL_ELL_CFG0 + N_ELL_CFGS;
+  if (!cfg && (variant < RM_KERNEL_AUTO || variant > RM_KERNEL_ELL))
+    return fail(RM_ERR_ARG, "rm_plan_set_kernel: unknown variant %d", variant);
+  if (cfg && p->n > (int64_t)ELL_CFGS[variant - RM_KERNEL_ELL_CFG0].ntr * ELL_CFGS[variant - RM_KERNEL_ELL_CFG0].r)
+    return fail(RM_ERR_ARG, "rm_plan_set_kernel: n=%lld exceeds ELL configuration %d",
+                (long long)p->n, variant - RM_KERNEL_ELL_CFG0);
+  if (variant == RM_KERNEL_RESIDENT && p->n > RESIDENT_MAX_N)
+    return fail(RM_ERR_ARG, "rm_plan_set_kernel: n=%lld exceeds the resident kernel's %lld states",
+                (long long)p->n, (long long)RESIDENT_MAX_N);
+  if (variant == RM_KERNEL_ELL && p->n > ELL_MAX_N)
+    return fail(RM_ERR_ARG, "rm_plan_set_kernel: n=%lld exceeds the ELL kernel's %lld states",
+                (long long)p->n, (long long)ELL_MAX_N);
+  p->variant = variant;
+  return RM_OK;
+}
 
 static int ensure_ws(rm_plan* p, size_t bytes) {
   if (p->ws_bytes >= bytes) return RM_OK;
@@ -678,46 +1555,172 @@ static int ensure_ws(rm_plan* p, size_t bytes) {
   return RM_OK;
 }
 
+// launch_step, bracketed by HIP events on the same stream when timing is on.
+static hipError_t timed_launch_step(rm_plan* p, const StepArgs& a, unsigned ncols, hipStream_t s) {
+  if (!p->timing) return launch_step(a, ncols, s, p->variant, p->n_cu);
+  if (p->nrec >= (int)p->ev0.size()) {
+    hipEvent_t e0, e1;
+    hipError_t e = hipEventCreate(&e0);
+    if (e != hipSuccess) return e;
+    e = hipEventCreate(&e1);
+    if (e != hipSuccess) {
+      (void)hipEventDestroy(e0);
+      return e;
+    }
+    p->ev0.push_back(e0);
+    p->ev1.push_back(e1);
+  }
+  hipError_t e = hipEventRecord(p->ev0[p->nrec], s);
+  if (e != hipSuccess) return e;
+  e = launch_step(a, ncols, s, p->variant, p->n_cu);
+  if (e != hipSuccess) return e;
+  e = hipEventRecord(p->ev1[p->nrec], s);
+  p->nrec++;
+  return e;
+}
+
+extern "C" int rm_plan_timing(rm_plan* p, int32_t enable) {
+  if (!p) return fail(RM_ERR_ARG, "rm_plan_timing: NULL plan");
+  p->timing = enable ? 1 : 0;
+  p->nrec = 0;
+  return RM_OK;
+}
+
+extern "C" int rm_plan_kernel_time(rm_plan* p, double* total_ms, int64_t* launches) {
+  if (!p) return fail(RM_ERR_ARG, "rm_plan_kernel_time: NULL plan");
+  double tot = 0.0;
+  if (p->nrec > 0) HIPCHK(hipEventSynchronize(p->ev1[p->nrec - 1]));
+  for (int i = 0; i < p->nrec; ++i) {
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, p->ev0[i], p->ev1[i]));
+    tot += ms;
+  }
+  if (total_ms) *total_ms = tot;
+  if (launches) *launches = p->nrec;
+  p->nrec = 0;
+  return RM_OK;
+}
+
+// Validate the step parameters and fill StepArgs (workspace sized for ncols).
+static int step_setup(rm_plan* p, const char* who, int64_t ld, int64_t ncols, double f_re,
+                      double f_im, const void* phase, int32_t krylov_fixed, int32_t maxorder,
+                      double tol, StepArgs* a) {
+  if (!p) return fail(RM_ERR_ARG, "%s: NULL plan", who);
+  if (ncols < 0 || ld < p->n)
+    return fail(RM_ERR_ARG, "%s: bad ld=%lld or ncols=%lld", who, (long long)ld, (long long)ncols);
+  if (ncols > INT32_MAX) return fail(RM_ERR_ARG, "%s: ncols too large", who);
+  if (krylov_fixed < 0 || krylov_fixed > RM_MAX_KRYLOV)
+    return fail(RM_ERR_ARG, "%s: krylov_fixed=%d outside [0, %d]", who, krylov_fixed, RM_MAX_KRYLOV);
+  if (krylov_fixed == 0 && (maxorder < 1 || maxorder > RM_MAX_KRYLOV))
+    return fail(RM_ERR_ARG, "%s: maxorder=%d outside [1, %d]", who, maxorder, RM_MAX_KRYLOV);
+  if (krylov_fixed == 0 && !(tol > 0.0 && tol <= 1.0)) return fail(RM_ERR_ARG, "%s: tol must be in (0, 1]", who);
+  const int kcap = krylov_fixed > 0 ? krylov_fixed : maxorder;
+  const size_t per_col = (size_t)kcap * p->n + (size_t)p->n + (size_t)kcap * kcap + 2 * (size_t)kcap;
+  if (ncols > 0 && p->n > 0) {
+    int rc = ensure_ws(p, sizeof(double2) * per_col * (size_t)ncols);
+    if (rc) return rc;
+  }
+  double2* ws = (double2*)p->ws;
+  memset(a, 0, sizeof(*a));
+  a->n = p->n;
+  a->ld = ld;
+  a->phase = (const double2*)phase;
+  a->row_ptr = p->row_ptr;
+  a->col = p->col;
+  a->aval = p->aval;
+  a->flag = p->flag;
+  a->flag_expect = p->step_id;
+  a->f = make_double2(f_re, f_im);
+  a->fixed_m = krylov_fixed;
+  a->maxorder = maxorder;
+  a->tol = tol;
+  a->kcap = kcap;
+  a->basis = ws;
+  a->wbuf = ws + (size_t)ncols * kcap * p->n;
+  a->gram = a->wbuf + (size_t)ncols * p->n;
+  a->cscr = a->gram + (size_t)ncols * kcap * kcap;
+  a->ell_col = p->ell_col;
+  a->ell_val = p->ell_val;
+  a->ell_w = p->ell_w;
+  a->ncols = (int32_t)ncols;
+  return RM_OK;
+}
+
 extern "C" int rm_step(rm_plan* p, const void* psi_in, void* psi_out, int64_t ld, int64_t ncols,
                        double f_re, double f_im, const void* phase, int32_t krylov_fixed,
                        int32_t maxorder, double tol, int32_t* kused, int32_t* status, void* stream) {
-  if (!p) return fail(RM_ERR_ARG, "rm_step: NULL plan");
-  if (ncols < 0 || ld < p->n) return fail(RM_ERR_ARG, "rm_step: bad ld=%lld or ncols=%lld", (long long)ld, (long long)ncols);
-  if (ncols > INT32_MAX) return fail(RM_ERR_ARG, "rm_step: ncols too large");
-  if (krylov_fixed < 0 || krylov_fixed > RM_MAX_KRYLOV)
-    return fail(RM_ERR_ARG, "rm_step: krylov_fixed=%d outside [0, %d]", krylov_fixed, RM_MAX_KRYLOV);
-  if (krylov_fixed == 0 && (maxorder < 1 || maxorder > RM_MAX_KRYLOV))
-    return fail(RM_ERR_ARG, "rm_step: maxorder=%d outside [1, %d]", maxorder, RM_MAX_KRYLOV);
-  if (krylov_fixed == 0 && !(tol > 0.0 && tol <= 1.0)) return fail(RM_ERR_ARG, "rm_step: tol must be in (0, 1]");
+  StepArgs a;
+  int rc = step_setup(p, "rm_step", ld, ncols, f_re, f_im, phase, krylov_fixed, maxorder, tol, &a);
+  if (rc) return rc;
   if (ncols == 0 || p->n == 0) return RM_OK;
   if (!psi_in || !psi_out) return fail(RM_ERR_ARG, "rm_step: NULL psi");
-  const int kcap = krylov_fixed > 0 ? krylov_fixed : maxorder;
-  const size_t per_col = (size_t)kcap * p->n + (size_t)p->n + (size_t)kcap * kcap;
-  int rc = ensure_ws(p, sizeof(double2) * per_col * (size_t)ncols);
-  if (rc) return rc;
-  double2* ws = (double2*)p->ws;
-  StepArgs a;
-  a.n = p->n;
-  a.ld = ld;
   a.psi_in = (const double2*)psi_in;
   a.psi_out = (double2*)psi_out;
-  a.phase = (const double2*)phase;
-  a.row_ptr = p->row_ptr;
-  a.col = p->col;
-  a.aval = p->aval;
-  a.flag = p->flag;
-  a.flag_expect = p->step_id;
-  a.f = make_double2(f_re, f_im);
-  a.fixed_m = krylov_fixed;
-  a.maxorder = maxorder;
-  a.tol = tol;
-  a.kcap = kcap;
-  a.basis = ws;
-  a.wbuf = ws + (size_t)ncols * kcap * p->n;
-  a.gram = a.wbuf + (size_t)ncols * p->n;
   a.kused = kused;
   a.status = status;
-  hipLaunchKernelGGL(k_lanczos_step, dim3((unsigned)ncols), dim3(NT), 0, (hipStream_t)stream, a);
+  HIPCHK(timed_launch_step(p, a, (unsigned)ncols, (hipStream_t)stream));
+  return RM_OK;
+}
+
+__global__ void k_fill_i32(int32_t* x, int64_t n, int32_t v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = v;
+}
+
+__global__ void k_copy_cols(int64_t n, int64_t ld, const double2* __restrict__ src,
+                            double2* __restrict__ dst) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t b = blockIdx.y;
+  if (i < n) dst[b * ld + i] = src[b * ld + i];
+}
+
+extern "C" int rm_propagate(rm_plan* p, void* psi, void* work, int64_t ld, int64_t ncols,
+                            int64_t nsteps, const double* fprod_dev, const double* thresh,
+                            double f_re, double f_im, const void* phase, int32_t krylov_fixed,
+                            int32_t maxorder, double tol, int32_t* kused, int32_t* status,
+                            void* stream) {
+  StepArgs a;
+  int rc = step_setup(p, "rm_propagate", ld, ncols, f_re, f_im, phase, krylov_fixed, maxorder, tol, &a);
+  if (rc) return rc;
+  if (nsteps < 0) return fail(RM_ERR_ARG, "rm_propagate: nsteps < 0");
+  if (ncols > 65535) return fail(RM_ERR_ARG, "rm_propagate: ncols > 65535");
+  if (ncols == 0 || p->n == 0 || nsteps == 0) return RM_OK;
+  if (!psi || !work) return fail(RM_ERR_ARG, "rm_propagate: NULL psi or work");
+  if (psi == work) return fail(RM_ERR_ARG, "rm_propagate: psi and work must differ");
+  if (p->nfield > 0 && !fprod_dev) return fail(RM_ERR_ARG, "rm_propagate: NULL field table");
+  hipStream_t s = (hipStream_t)stream;
+  FieldArgs fa;
+  memset(&fa, 0, sizeof(fa));
+  for (int i = 0; i < RM_MAX_LEAF; ++i) fa.thresh[i] = -1.0;
+  if (thresh)
+    for (int i = 0; i < p->nleaf; ++i) fa.thresh[i] = thresh[i];
+  if (status) hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(ncols, 256)), dim3(256), 0, s, status, ncols, 0);
+  a.status = status;
+  a.status_sticky = 1;
+  double2* cur = (double2*)psi;
+  double2* nxt = (double2*)work;
+  for (int64_t st = 0; st < nsteps; ++st) {
+    p->step_id = (p->step_id + 1) & 0x3fffffff;
+    if (p->nterm > 0)
+      hipLaunchKernelGGL(k_field_contract, dim3(grid_for(p->nterm, 256)), dim3(256), 0, s, p->nterm,
+                         p->term_ptr, p->term_field, p->term_mval, p->term_leaf, fa,
+                         (const double2*)fprod_dev + st * p->nfield, p->mf, p->flag, p->step_id);
+    if (p->nnz > 0)
+      hipLaunchKernelGGL(k_assemble, dim3(grid_for(p->nnz, 256)), dim3(256), 0, s, p->nnz,
+                         p->pair_ptr, p->pair_term, p->pair_kval, p->mf, p->aval, p->ell_pos,
+                         p->ell_val);
+    a.flag_expect = p->step_id;
+    a.psi_in = cur;
+    a.psi_out = nxt;
+    a.kused = kused ? kused + st * ncols : nullptr;
+    HIPCHK(timed_launch_step(p, a, (unsigned)ncols, s));
+    double2* t = cur;
+    cur = nxt;
+    nxt = t;
+  }
+  if (cur != (double2*)psi)
+    hipLaunchKernelGGL(k_copy_cols, dim3(grid_for(p->n, 256), (unsigned)ncols), dim3(256), 0, s,
+                       p->n, ld, (const double2*)cur, (double2*)psi);
   HIPCHK(hipGetLastError());
   return RM_OK;
 }

@@ -47,7 +47,9 @@ class Ensemble:
         self.maxorder = int(maxorder)
         self.tol = float(tol)
         self.stream = stream
-        self.plan = _plan.plan_for(H)
+        self.plan = None
+        self._active = None
+        self.kernel = "auto"
         dev = torch.device("cuda", torch.cuda.current_device())
         self.psi = torch.as_tensor(np.ascontiguousarray(psi0, dtype=np.complex128)).to(dev)
         self.out = torch.empty_like(self.psi)
@@ -61,11 +63,50 @@ class Ensemble:
             self.phase = torch.empty_like(d)
             _plan_phase(d, self.f, self.phase, stream)
 
+    def set_field(self, F, thresh=None):
+        """Field of the next step: products on the host, contraction on the
+        device; switches plan when the set of active field components changes."""
+        from . import plan as _plan
+        self.H.field(F, thresh)
+        active = _plan.active_carts(self.H)
+        if active != self._active:
+            self.plan = _plan.plan_for(self.H)
+            self.plan.set_kernel(self.kernel)
+            self._active = active
+        self.plan.set_field(self.H, self.stream)
+
+    def set_kernel(self, kernel):
+        self.kernel = kernel
+        if self.plan is not None:
+            self.plan.set_kernel(kernel)
+
     def step(self, F, thresh=None):
         """Set the field of the interaction and advance every row by one step."""
-        self.H.field(F, thresh)
-        self.plan.set_field(self.H, self.stream)
+        self.set_field(F, thresh)
         self.launch()
+
+    def prepare(self, fields, thresh=None):
+        """Device table of field products for a run of steps (one row per
+        step) and the plan that covers every component the run activates."""
+        import torch
+        from . import plan as _plan
+        active, cols, nst = _plan.field_table(self.H, np.asarray(fields, dtype=np.float64), thresh)
+        if active != self._active:
+            self.plan = _plan.plan_for(self.H, active)
+            self.plan.set_kernel(self.kernel)
+            self._active = active
+        tab = self.plan.table_array(cols, nst)
+        return torch.from_numpy(tab).to(self.psi.device)
+
+    def propagate(self, fields, thresh=None, table=None, kused=None):
+        """Advance every row through len(fields) steps with one device call
+        (rm_propagate).  ``table`` may hold a prepared field table."""
+        if table is None:
+            table = self.prepare(fields, thresh)
+        th = None if thresh is None else [thresh]
+        self.plan.propagate(self.psi, self.out, table, self.f, phase=self.phase, thresh=th,
+                            krylov_fixed=self.krylov_fixed, maxorder=self.maxorder, tol=self.tol,
+                            kused=kused, status=self.status, stream=self.stream)
 
     def launch(self):
         """The Krylov step of all rows with the field already on the device."""

@@ -48,12 +48,31 @@ def _basis_signature(T):
             tuple((J, tuple(T.symlist2[J]), tuple(T.dim2[J][s] for s in T.symlist2[J])) for J in T.Jlist2))
 
 
-def structure_key(T):
-    return (tuple(leaf._uid for leaf, _, _ in T.terms()), _basis_signature(T))
+def active_carts(T):
+    """Per term of T, the Cartesian components whose field product is non-zero
+    (after the field's threshold): only they can contribute to MF."""
+    out = []
+    for leaf, fp, _ in T.terms():
+        if fp is None:
+            out.append(())
+        else:
+            out.append(tuple(c for c in leaf_carts(leaf) if fp.get(c, 0) != 0))
+    return tuple(out)
 
 
-def flatten_operator(T):
-    """Host arrays of the plan (see module docstring)."""
+def structure_key(T, active=None):
+    if active is None:
+        active = active_carts(T)
+    return (tuple(leaf._uid for leaf, _, _ in T.terms()), _basis_signature(T), active)
+
+
+def flatten_operator(T, active=None):
+    """Host arrays of the plan (see module docstring).  ``active`` (per term,
+    the Cartesian components to include; default: those with non-zero field
+    products) restricts the sparsity pattern to the entries the field can
+    reach, e.g. Delta m = 0 only for a Z-polarised field."""
+    if active is None:
+        active = active_carts(T)
     off1, n1 = block_offsets(T, 1)
     off2, n2 = block_offsets(T, 2)
     R, C, PT, PK = [], [], [], []
@@ -63,7 +82,7 @@ def flatten_operator(T):
     slots = []
     terms = T.terms()
     for li, (leaf, _, _) in enumerate(terms):
-        carts = leaf_carts(leaf)
+        carts = [c for c in leaf_carts(leaf) if c in active[li]]
         cidx = {c: slot_base + i for i, c in enumerate(carts)}
         slots.append(carts)
         for Jp in sorted(set(leaf.kmat.keys()) & set(leaf.mmat.keys())):
@@ -83,6 +102,8 @@ def flatten_operator(T):
                         continue
                     mr, mc, ms, mv = [], [], [], []
                     for c in sorted(Mb[w].keys()):
+                        if c not in cidx:
+                            continue
                         M = Mb[w][c].tocoo()
                         nz = M.data != 0
                         mr.append(M.row[nz]); mc.append(M.col[nz])
@@ -139,12 +160,17 @@ def flatten_operator(T):
 
 
 class DevicePlan:
-    """rm_plan handle plus the slot layout of the field products."""
+    """rm_plan handle plus the slot layout of the field products.
 
-    def __init__(self, T):
+    A plan covers the field components active when it was built; set_field
+    refuses a field that activates any other component (plan_for() then
+    returns the plan for the new set)."""
+
+    def __init__(self, T, active=None):
         import ctypes
         self.lib = _lib.load()
-        arr = flatten_operator(T)
+        self.active = active_carts(T) if active is None else active
+        arr = flatten_operator(T, self.active)
         if arr["n"] != arr["n2"]:
             raise ValueError("bra and ket bases differ: the propagator needs a square operator")
         if arr["nfield"] > _lib.RM_MAX_FIELD:
@@ -178,15 +204,35 @@ class DevicePlan:
                 pass
             self.handle = None
 
+    KERNELS = {"auto": 0, "global": 1, "resident": 2, "ell": 3}
+    ELL_CFG0 = 16
+
+    def set_kernel(self, variant):
+        """Select the step kernel: 'auto', 'global', 'resident', 'ell', or
+        'ell:<i>' for ELL launch configuration i (see ell_configs())."""
+        if isinstance(variant, str) and variant.startswith("ell:"):
+            code = self.ELL_CFG0 + int(variant[4:])
+        else:
+            code = self.KERNELS[variant]
+        _lib.check(self.lib.rm_plan_set_kernel(self.handle, code), "rm_plan_set_kernel")
+        self.kernel = variant
+
     # ---------------------------------------------------------------- field
     def field_arrays(self, T):
         """Field products and thresholds of T's terms in slot order."""
         fprod = np.zeros(max(self.nfield, 1), dtype=np.complex128)
         thresh = np.full(max(self.nleaf, 1), -1.0)
         base = 0
-        for li, (leaf, fp, thr) in enumerate(T.terms()):
+        terms = T.terms()
+        if len(terms) != len(self.slots):
+            raise ValueError("operator does not match the plan's terms")
+        for li, (leaf, fp, thr) in enumerate(terms):
             carts = self.slots[li]
             if fp is not None:
+                for c, v in fp.items():
+                    if v != 0 and c not in carts and c in leaf_carts(leaf):
+                        raise ValueError(f"field component '{c}' is not in the plan "
+                                         f"(built for {self.active[li]})")
                 for i, c in enumerate(carts):
                     fprod[base + i] = complex(fp.get(c, 0.0))
             if thr is not None and thr > 0:
@@ -226,6 +272,53 @@ class DevicePlan:
             vp(status.data_ptr() if status is not None else None),
             vp(_stream_handle(stream))), "rm_step")
 
+    def timing(self, enable=True):
+        """Record HIP events around every Krylov-step launch of this plan."""
+        _lib.check(self.lib.rm_plan_timing(self.handle, 1 if enable else 0), "rm_plan_timing")
+
+    def kernel_time(self):
+        """(summed launch time in ms, number of launches) since the last call."""
+        import ctypes
+        ms, nl = ctypes.c_double(), ctypes.c_int64()
+        _lib.check(self.lib.rm_plan_kernel_time(self.handle, ctypes.byref(ms), ctypes.byref(nl)),
+                   "rm_plan_kernel_time")
+        return ms.value, nl.value
+
+    def table_array(self, cols, nsteps):
+        """(nsteps, nfield) complex table in slot order from field_table()."""
+        tab = np.zeros((nsteps, max(self.nfield, 1)), dtype=np.complex128)
+        base = 0
+        for li, carts in enumerate(self.slots):
+            for c, v in cols[li].items():
+                if c not in carts:
+                    if np.any(np.asarray(v) != 0):
+                        raise ValueError(f"field component '{c}' is not in the plan")
+                    continue
+                tab[:, base + carts.index(c)] = v
+            base += len(carts)
+        return tab
+
+    def propagate(self, psi, work, table, f, phase=None, thresh=None, krylov_fixed=0,
+                  maxorder=100, tol=1e-15, kused=None, status=None, stream=None):
+        """len(table) steps of every row of psi (in place); table: device
+        (nsteps, nfield) complex128 tensor of field products."""
+        import ctypes
+        _check_cols(psi, self.n); _check_cols(work, self.n)
+        if psi.stride(0) != work.stride(0):
+            raise ValueError("psi and work need the same row stride")
+        th = np.full(max(self.nleaf, 1), -1.0)
+        if thresh is not None:
+            th[: len(thresh)] = [t if (t is not None and t > 0) else -1.0 for t in thresh]
+        vp = ctypes.c_void_p
+        _lib.check(self.lib.rm_propagate(
+            self.handle, vp(psi.data_ptr()), vp(work.data_ptr()), psi.stride(0), psi.shape[0],
+            table.shape[0], vp(table.data_ptr()), _lib.ptr(th, ctypes.c_double),
+            float(f.real), float(f.imag), vp(phase.data_ptr() if phase is not None else None),
+            int(krylov_fixed), int(maxorder), float(tol),
+            vp(kused.data_ptr() if kused is not None else None),
+            vp(status.data_ptr() if status is not None else None),
+            vp(_stream_handle(stream))), "rm_propagate")
+
     def expect(self, psi, out, stream=None):
         import ctypes
         _check_cols(psi, self.n)
@@ -244,12 +337,65 @@ def _check_cols(t, n):
         raise ValueError(f"expected (ncols, {n}) rows with unit inner stride, got {tuple(t.shape)}")
 
 
-def plan_for(T):
-    """Cached DevicePlan for T's structure (leaf tensors + basis)."""
-    key = structure_key(T)
+def cart_products(cart, F):
+    """Field products of one Cartesian string for a (nsteps, 3) field array
+    (field.py:1094-1105): the product of the named components, "0" -> 1."""
+    F = np.asarray(F)
+    if cart == "0" or cart == "":
+        return np.ones(F.shape[0], dtype=np.result_type(F.dtype, np.float64))
+    out = F[:, "xyz".index(cart[0])].copy()
+    for ch in cart[1:]:
+        out = out * F[:, "xyz".index(ch)]
+    return out
+
+
+def field_table(T, fields, thresh=None, per_term=False):
+    """Per-step field products for every term of T.
+
+    fields: one (nsteps, 3) array for every term, or with per_term=True a
+    list with one entry per term (an array, or None to keep that term's
+    present field products).  thresh screens the products as
+    CarTens.field(F, thresh) does.  Returns (active, cols, nsteps): per term
+    the active components and {cart: products (array or scalar)}.
+    """
+    terms = T.terms()
+    if not per_term:
+        fields = [fields] * len(terms)
+    if len(fields) != len(terms):
+        raise ValueError(f"{len(fields)} field arrays for {len(terms)} terms")
+    cols, active, nst = [], [], None
+    for (leaf, fp, thr0), F in zip(terms, fields):
+        carts = leaf_carts(leaf)
+        cdict = {}
+        if F is None:
+            for c in carts:
+                v = 0.0 if fp is None else complex(fp.get(c, 0.0))
+                cdict[c] = v
+        else:
+            F = np.asarray(F)
+            nst = F.shape[0] if nst is None else nst
+            if F.shape[0] != nst:
+                raise ValueError("per-term field arrays differ in length")
+            thr = thresh
+            for c in carts:
+                v = cart_products(c, F)
+                if thr is not None:
+                    v = np.where(np.abs(v) >= thr, v, 0)
+                cdict[c] = v
+        cols.append(cdict)
+        active.append(tuple(c for c in carts if np.any(np.asarray(cdict[c]) != 0)))
+    return tuple(active), cols, nst
+
+
+def plan_for(T, active=None):
+    """Cached DevicePlan for T's structure (leaf tensors, basis, active field
+    components; by default the components T's present field activates)."""
+    if active is None:
+        active = active_carts(T)
+    key = structure_key(T, active)
     p = _CACHE.get(key)
     if p is None:
-        p = DevicePlan(T)
+        p = DevicePlan(T, active)
         _CACHE[key] = p
         while len(_CACHE) > _CACHE_MAX:
             _CACHE.popitem(last=False)
@@ -267,3 +413,17 @@ def apply(T, X):
     p.set_field(T)
     p.matvec(x, y)
     return y.cpu().numpy()
+
+
+def ell_configs():
+    """[(threads, rows per thread, wavepackets per workgroup, operator in LDS)]."""
+    import ctypes
+    lib = _lib.load()
+    out = []
+    i = 0
+    while True:
+        v = [ctypes.c_int32() for _ in range(4)]
+        if lib.rm_ell_config(i, *[ctypes.byref(x) for x in v]) != 0:
+            return out
+        out.append(tuple(x.value for x in v))
+        i += 1

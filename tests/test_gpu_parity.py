@@ -59,12 +59,16 @@ def test_matvec_sum_and_threshold(device):
     assert np.abs(Yd - Yr).max() <= 1e-13 * np.abs(Yr).max()
 
 
-def _run_pair(h0, H, fields, X, *, split=True, tol=1e-15, fixed=None, dt=10.0, t_units="fs"):
+def _run_pair(h0, H, fields, X, *, split=True, tol=1e-15, fixed=None, dt=10.0, t_units="fs",
+              kernel="auto"):
     """Propagate X with the device TDSE and the oracle; return max deviations."""
     import torch
+    from richmol_amd import plan as _plan
     from richmol_amd.tdse import TDSE
     Oh0 = OracleCarTens.from_attrs(h0)
     OH = OracleCarTens.from_attrs(H)
+    H.field(fields[0])
+    _plan.plan_for(H if split else h0 + H).set_kernel(kernel)
     nst = len(fields)
     td = TDSE(t_start=0.0, t_end=dt * nst, dt=dt, t_units=t_units)
     to = OracleTDSE(t_start=0.0, t_end=dt * nst, dt=dt, t_units=t_units)
@@ -93,36 +97,61 @@ def _run_pair(h0, H, fields, X, *, split=True, tol=1e-15, fixed=None, dt=10.0, t
     return dev, kdev, psi_d.cpu().numpy(), psi_o
 
 
+KERNELS = ["global", "resident", "ell", "ell:7"]
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("split", [True, False])
-def test_step_adaptive_xz_field(device, split):
+def test_step_adaptive_xz_field(device, split, kernel):
     """Adaptive Lanczos, X+Z field (Delta m = 0, +-1, +-2 couplings), 4 wavepackets."""
     h0, H, cos2, bas = linear_rotor(6)
     n = sum(len(bas.mlist[J]) for J in bas.Jlist)
     X = _rand_cols(np.random.default_rng(3), 4, n)
     amps = np.linspace(2e8, 1.6e9, 8)
     fields = [[0.6 * a, 0.0, a] for a in amps]
-    dev, kdev, _, _ = _run_pair(h0, H, fields, X, split=split)
+    dev, kdev, _, _ = _run_pair(h0, H, fields, X, split=split, kernel=kernel)
     assert dev <= TOL_PSI, dev
     assert kdev == 0
 
 
-def test_step_fixed_krylov(device):
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_step_fixed_krylov(device, kernel):
     h0, H, cos2, bas = linear_rotor(6)
     n = sum(len(bas.mlist[J]) for J in bas.Jlist)
     X = _rand_cols(np.random.default_rng(4), 3, n)
     fields = [[0.0, 0.0, a] for a in np.linspace(5e8, 1.9e9, 6)]
-    dev, kdev, _, _ = _run_pair(h0, H, fields, X, fixed=12)
+    dev, kdev, _, _ = _run_pair(h0, H, fields, X, fixed=12, kernel=kernel)
     assert dev <= TOL_PSI, dev
 
 
-def test_step_unnormalised_rows(device):
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_step_unnormalised_rows(device, kernel):
     """Boltzmann-weighted rows have norm < 1; the reference Lanczos does not
     normalise its first vector (tdse.py:443-447) and the device must agree."""
     h0, H, cos2, bas = linear_rotor(5)
     n = sum(len(bas.mlist[J]) for J in bas.Jlist)
     X = _rand_cols(np.random.default_rng(5), 3, n) * np.array([[0.2], [0.55], [1.7]])
     fields = [[0.0, 0.0, a] for a in np.linspace(8e8, 1.5e9, 5)]
-    dev, kdev, _, _ = _run_pair(h0, H, fields, X)
+    dev, kdev, _, _ = _run_pair(h0, H, fields, X, kernel=kernel)
+    assert dev <= TOL_PSI, dev
+    assert kdev == 0
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_step_exact_breakdown(device, kernel):
+    """A diagonal operator and basis-vector wavepackets: beta_1 == 0 exactly,
+    the Lanczos restarts from a vector of ones (tdse.py:459-465)."""
+    from richmol_amd import rot
+    bas = rot.linear_basis(0.2029, 3)
+    h0 = rot.hamiltonian(bas)
+    D = rot.hamiltonian(rot.linear_basis(7.3, 3))     # diagonal, distinct values
+    n = sum(len(bas.mlist[J]) for J in bas.Jlist)
+    X = np.zeros((3, n), dtype=np.complex128)
+    X[0, 1] = 1.0
+    X[1, 5] = 1.0
+    X[2] = _rand_cols(np.random.default_rng(8), 1, n)[0]
+    fields = [[0.0, 0.0, 1.0]] * 3
+    dev, kdev, psi_d, _ = _run_pair(h0, D, fields, X, kernel=kernel, dt=100.0)
     assert dev <= TOL_PSI, dev
     assert kdev == 0
 
@@ -137,21 +166,23 @@ def test_step_zero_field_skips_lanczos(device):
     assert dev <= TOL_PSI, dev
 
 
-def test_linear_rotor_jmax40_ensemble_slice(device):
-    """Headline basis (J<=40, all m, N=1681), 6 thermal-like columns, steps at
-    the pulse peak, adaptive FP64: wavepackets and <cos^2> against the oracle."""
+@pytest.mark.parametrize("kernel", ["auto", "ell:3", "ell:4", "ell:10"])
+def test_linear_rotor_jmax40_ensemble_slice(device, kernel):
+    """Headline basis (J<=40, all m, N=1681), 5 thermal-like columns (odd: the
+    last column group of a 2-column workgroup is half empty), steps at the
+    pulse peak, adaptive FP64: wavepackets and <cos^2> against the oracle."""
     import torch
     from richmol_amd import plan as _plan
     h0, H, cos2, bas = linear_rotor(40)
     n = sum(len(bas.mlist[J]) for J in bas.Jlist)
     assert n == 1681
     rng = np.random.default_rng(7)
-    idx = rng.choice(60, size=6, replace=False)
-    X = np.zeros((6, n), dtype=np.complex128)
-    X[np.arange(6), idx] = 1.0
-    t = np.arange(30, 36) * 10.0 + 5.0
+    idx = rng.choice(60, size=5, replace=False)
+    X = np.zeros((5, n), dtype=np.complex128)
+    X[np.arange(5), idx] = 1.0
+    t = np.arange(30, 35) * 10.0 + 5.0
     fields = [[0.0, 0.0, gaussian_pulse_field(tt)] for tt in t]
-    dev, kdev, psi_d, psi_o = _run_pair(h0, H, fields, X)
+    dev, kdev, psi_d, psi_o = _run_pair(h0, H, fields, X, kernel=kernel)
     assert dev <= TOL_PSI, dev
     Oc = OracleCarTens.from_attrs(cos2)
     C = Oc.tomat(form="full", cart="0")
@@ -159,7 +190,7 @@ def test_linear_rotor_jmax40_ensemble_slice(device):
     cos2.field([0, 0, 1])
     pc = _plan.plan_for(cos2)
     pc.set_field(cos2)
-    out = torch.zeros(6, dtype=torch.complex128, device=device)
+    out = torch.zeros(5, dtype=torch.complex128, device=device)
     pc.expect(torch.from_numpy(psi_d).to(device), out)
     c2_d = out.real.cpu().numpy() + 1 / 3
     assert np.abs(c2_d - c2_o).max() <= TOL_PSI
@@ -197,3 +228,31 @@ def test_ocs_golden_device(device):
             worst_g = max(worst_g, np.abs(np.round(pops, 4) - lan[i // 10]).max())
     assert worst_o <= TOL_PSI, worst_o
     assert worst_g <= 1.0001e-4, worst_g   # 4-decimal rounding of the golden file
+
+
+@pytest.mark.parametrize("kernel", ["auto", "global"])
+def test_propagate_equals_per_step(device, kernel):
+    """rm_propagate (device field table, one call for the run) reproduces the
+    per-step path bit for bit, and the oracle to 1e-10, including steps with
+    zero field and an odd number of steps (result lands in the work buffer)."""
+    import torch
+    from richmol_amd import ensemble
+    h0, H, cos2, bas = linear_rotor(12)
+    n = sum(len(bas.mlist[J]) for J in bas.Jlist)
+    X = _rand_cols(np.random.default_rng(11), 7, n)
+    amps = [0.0, 3e8, 9e8, 1.4e9, 1.1e9, 0.0, 6e8]
+    fields = [[0.0, 0.0, a] for a in amps]
+    from richmol_amd.tdse import TDSE
+    f = TDSE(t_end=100.0, dt=10.0, t_units="fs").exp_fac
+    e1 = ensemble.Ensemble(H, h0, f, X)
+    e1.set_kernel(kernel)
+    for F in fields:
+        e1.step(F)
+    e2 = ensemble.Ensemble(H, h0, f, X)
+    e2.set_kernel(kernel)
+    e2.propagate(fields)
+    e2.check()
+    a, b = e1.psi.cpu().numpy(), e2.psi.cpu().numpy()
+    assert np.array_equal(a, b)
+    dev, kdev, psi_d, psi_o = _run_pair(h0, H, fields, X)
+    assert np.abs(b - psi_o).max() <= TOL_PSI
