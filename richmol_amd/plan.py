@@ -169,6 +169,7 @@ class DevicePlan:
     def __init__(self, T, active=None):
         import ctypes
         self.lib = _lib.load()
+        self.kernel = "auto"
         self.active = active_carts(T) if active is None else active
         arr = flatten_operator(T, self.active)
         if arr["n"] != arr["n2"]:
