@@ -32,7 +32,7 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 # HBM bytes per k_lanczos_ell launch from rocprofv3 FETCH_SIZE/WRITE_SIZE passes,
 # scaled by a 1 GiB copy (FETCH x2.00, WRITE x1.00); key: (jmax, wavepackets, m, kernel)
-MEASURED_TRAFFIC = {(40, 512, 12, "auto"): 351.1e6}
+MEASURED_TRAFFIC = {(40, 512, 12, "auto"): 351.1e6, (40, 512, 12, "ell:4"): 351.1e6}
 MEASURED_TRAFFIC_SOURCE = "profiles/k_lanczos_ell_linear_j40_b512_m12_pmc.md"
 
 
@@ -88,8 +88,11 @@ def main():
     ap.add_argument("--cpu-cols", type=int, default=16)
     ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--kernel", default="auto",
-                    help="auto | global | resident | ell | ell:<i> (explicit ELL configuration)")
+    ap.add_argument("--kernel", default="tuned",
+                    help="tuned (autotuner cache, else auto) | auto | global | resident | ell | "
+                         "ell:<i> (explicit ELL configuration)")
+    ap.add_argument("--tune", action="store_true",
+                    help="time every kernel configuration on this workload first and use the fastest")
     ap.add_argument("--per-step", action="store_true",
                     help="one host call per step instead of one rm_propagate call for the run")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -132,6 +135,13 @@ def main():
 
     ens = ensemble.Ensemble(H, h0, f, psi0, krylov_fixed=args.krylov)
     ens.set_kernel(args.kernel)
+    tuning = None
+    if args.tune:
+        from richmol_amd import autotune
+        ens.prepare(fields[:1])
+        choice, res = autotune.tune(ens, fields, save=True)
+        tuning = {"choice": choice, "us_per_launch": {k: v * 1e3 for k, v in res.items()},
+                  "key": autotune.key(ens.plan, args.ncols, args.krylov)}
     if args.per_step:
         # one Python call per step (the drop-in TDSE.update pattern)
         for i in range(args.warmup):
@@ -213,6 +223,8 @@ def main():
                      "hbm_model_frac": hbm_alg / HBM_PEAK_GBS,
                      "traffic_source": MEASURED_TRAFFIC_SOURCE if measured else None},
         "cos2_ensemble": c2,
+        "kernel_selected": ens.plan.kernel,
+        "tuning": tuning,
     }
 
     if not args.no_cpu:

@@ -291,7 +291,60 @@ __global__ void k_assemble(int64_t nnz, const int64_t* __restrict__ pair_ptr,
   ell_val[ell_pos[e]] = acc;
 }
 
+// Contraction and assembly in one pass: each entry evaluates the MF terms its
+// pairs use (same arithmetic as k_field_contract + k_assemble, term by term).
+// Chosen when terms are used about once (linear rotors: dk = 1), where the
+// recomputation costs nothing and one launch per step is saved.
+__global__ void k_field_assemble(int64_t nnz, const int64_t* __restrict__ pair_ptr,
+                                 const int32_t* __restrict__ pair_term,
+                                 const double2* __restrict__ pair_kval,
+                                 const int64_t* __restrict__ term_ptr,
+                                 const int32_t* __restrict__ term_field,
+                                 const double2* __restrict__ term_mval,
+                                 const int32_t* __restrict__ term_leaf, FieldArgs fa,
+                                 const double2* __restrict__ fdev, double2* __restrict__ aval,
+                                 const int32_t* __restrict__ ell_pos, double2* __restrict__ ell_val,
+                                 int32_t* __restrict__ flag, int32_t id) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nnz) return;
+  double2 acc = make_double2(0.0, 0.0);
+  bool nz = false;
+  for (int64_t p = pair_ptr[e]; p < pair_ptr[e + 1]; ++p) {
+    const int32_t t = pair_term[p];
+    double2 mf = make_double2(0.0, 0.0);
+    for (int64_t q = term_ptr[t]; q < term_ptr[t + 1]; ++q)
+      mf = cfma(fdev ? fdev[term_field[q]] : fa.fprod[term_field[q]], term_mval[q], mf);
+    const double thr = fa.thresh[term_leaf[t]];
+    if (thr > 0.0 && hypot(mf.x, mf.y) < thr) mf = make_double2(0.0, 0.0);
+    nz |= (mf.x != 0.0 || mf.y != 0.0 || mf.x != mf.x || mf.y != mf.y);
+    acc = cfma(pair_kval[p], mf, acc);
+  }
+  aval[e] = acc;
+  ell_val[ell_pos[e]] = acc;
+  if (nz) *flag = id;
+}
+
 static inline unsigned grid_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// The field contraction of one step (fdev: device row of field products or
+// NULL for fa's copy): fused when every term is used about once.
+static void launch_field(rm_plan* p, const FieldArgs& fa, const double2* fdev, hipStream_t s) {
+  if (p->nnz > 0 && p->npair <= 2 * p->nterm) {
+    hipLaunchKernelGGL(k_field_assemble, dim3(grid_for(p->nnz, 256)), dim3(256), 0, s, p->nnz,
+                       p->pair_ptr, p->pair_term, p->pair_kval, p->term_ptr, p->term_field,
+                       p->term_mval, p->term_leaf, fa, fdev, p->aval, p->ell_pos, p->ell_val,
+                       p->flag, p->step_id);
+    return;
+  }
+  if (p->nterm > 0)
+    hipLaunchKernelGGL(k_field_contract, dim3(grid_for(p->nterm, 256)), dim3(256), 0, s, p->nterm,
+                       p->term_ptr, p->term_field, p->term_mval, p->term_leaf, fa, fdev, p->mf,
+                       p->flag, p->step_id);
+  if (p->nnz > 0)
+    hipLaunchKernelGGL(k_assemble, dim3(grid_for(p->nnz, 256)), dim3(256), 0, s, p->nnz,
+                       p->pair_ptr, p->pair_term, p->pair_kval, p->mf, p->aval, p->ell_pos,
+                       p->ell_val);
+}
 
 extern "C" int rm_plan_set_field(rm_plan* p, const double* fprod, const double* thresh, void* stream) {
   if (!p) return fail(RM_ERR_ARG, "rm_plan_set_field: NULL plan");
@@ -304,14 +357,7 @@ extern "C" int rm_plan_set_field(rm_plan* p, const double* fprod, const double* 
     for (int i = 0; i < p->nleaf; ++i) fa.thresh[i] = thresh[i];
   hipStream_t s = (hipStream_t)stream;
   p->step_id = (p->step_id + 1) & 0x3fffffff;
-  if (p->nterm > 0)
-    hipLaunchKernelGGL(k_field_contract, dim3(grid_for(p->nterm, 256)), dim3(256), 0, s, p->nterm,
-                       p->term_ptr, p->term_field, p->term_mval, p->term_leaf, fa,
-                       (const double2*)nullptr, p->mf, p->flag, p->step_id);
-  if (p->nnz > 0)
-    hipLaunchKernelGGL(k_assemble, dim3(grid_for(p->nnz, 256)), dim3(256), 0, s, p->nnz,
-                       p->pair_ptr, p->pair_term, p->pair_kval, p->mf, p->aval, p->ell_pos,
-                       p->ell_val);
+  launch_field(p, fa, nullptr, s);
   HIPCHK(hipGetLastError());
   return RM_OK;
 }
@@ -1701,14 +1747,7 @@ extern "C" int rm_propagate(rm_plan* p, void* psi, void* work, int64_t ld, int64
   double2* nxt = (double2*)work;
   for (int64_t st = 0; st < nsteps; ++st) {
     p->step_id = (p->step_id + 1) & 0x3fffffff;
-    if (p->nterm > 0)
-      hipLaunchKernelGGL(k_field_contract, dim3(grid_for(p->nterm, 256)), dim3(256), 0, s, p->nterm,
-                         p->term_ptr, p->term_field, p->term_mval, p->term_leaf, fa,
-                         (const double2*)fprod_dev + st * p->nfield, p->mf, p->flag, p->step_id);
-    if (p->nnz > 0)
-      hipLaunchKernelGGL(k_assemble, dim3(grid_for(p->nnz, 256)), dim3(256), 0, s, p->nnz,
-                         p->pair_ptr, p->pair_term, p->pair_kval, p->mf, p->aval, p->ell_pos,
-                         p->ell_val);
+    launch_field(p, fa, (const double2*)fprod_dev + st * p->nfield, s);
     a.flag_expect = p->step_id;
     a.psi_in = cur;
     a.psi_out = nxt;

@@ -71,14 +71,15 @@ class Ensemble:
         active = _plan.active_carts(self.H)
         if active != self._active:
             self.plan = _plan.plan_for(self.H)
-            self.plan.set_kernel(self.kernel)
+            self._apply_kernel()
             self._active = active
         self.plan.set_field(self.H, self.stream)
 
     def set_kernel(self, kernel):
+        """'auto', 'tuned', 'global', 'resident', 'ell' or 'ell:<i>'."""
         self.kernel = kernel
         if self.plan is not None:
-            self.plan.set_kernel(kernel)
+            self._apply_kernel()
 
     def step(self, F, thresh=None):
         """Set the field of the interaction and advance every row by one step."""
@@ -93,10 +94,19 @@ class Ensemble:
         active, cols, nst = _plan.field_table(self.H, np.asarray(fields, dtype=np.float64), thresh)
         if active != self._active:
             self.plan = _plan.plan_for(self.H, active)
-            self.plan.set_kernel(self.kernel)
+            self._apply_kernel()
             self._active = active
         tab = self.plan.table_array(cols, nst)
         return torch.from_numpy(tab).to(self.psi.device)
+
+    def _apply_kernel(self):
+        """Set the plan's kernel; 'tuned' takes the autotuner's cached choice
+        for this shape, or 'auto' when the shape was never tuned."""
+        k = self.kernel
+        if k == "tuned":
+            from . import autotune
+            k = autotune.lookup(self.plan, self.psi.shape[0], self.krylov_fixed) or "auto"
+        self.plan.set_kernel(k)
 
     def propagate(self, fields, thresh=None, table=None, kused=None):
         """Advance every row through len(fields) steps with one device call
