@@ -5,14 +5,19 @@ on): linear rigid rotor, B = 0.2029 cm^-1, alpha_perp = 26.0 au,
 Delta alpha = 27.6 au, J = 0..40 with all m (N = 1681 states); 512 initial
 |J, m> states drawn with Boltzmann weights at T = 1 K (seeded); Gaussian
 non-resonant Z pulse (cycle-averaged envelope, intensity FWHM 300 fs, peak
-5e11 W/cm^2, centre 375 fs), dt = 10 fs; split operator with the field-free
-phases; Lanczos with a fixed Krylov dimension 12; complex128 throughout.
+5e11 W/cm^2, centred at 4 FWHM and cut to zero beyond 4 FWHM of the centre),
+dt = 10 fs; split operator with the field-free phases; Lanczos with a fixed
+Krylov dimension 12; complex128.  The timed steps all lie inside the pulse
+(every step runs the Krylov expv).
 
-One step = field contraction on the device + one launch that advances all
-512 wavepackets.  ``value`` counts wavepacket-steps per second over the
-timed steps (inputs resident in HBM).  The CPU baseline is the FP64 oracle
-(numpy/scipy restatement of richmol's algorithm) on a bounded sample of the
-same workload, single thread.
+``value``: wavepacket-steps per second of the production path (kernel
+'auto': for this Z-polarised field every wavepacket stays in its (m, J-parity)
+invariant subspace, <= 21 states, and the whole run is one launch with one
+wave per wavepacket), inputs resident in HBM.  Beside it: the same workload
+through the full-vector ELL kernel (every wavepacket as a 1681-vector), the
+FP32-matvec precision mode with its observable errors against FP64, the
+plain single-wavepacket FP64 path (one TDSE.update call per step), the
+serving loop's latency, and the FP64 oracle on the host as CPU baseline.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
@@ -30,10 +35,14 @@ sys.path.insert(0, ROOT)
 METRIC = "wavepacket-steps/s at Jmax=40 (⟨cos²θ⟩ ≤1e-6 vs FP64) + p99 request latency"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
-# HBM bytes per k_lanczos_ell launch from rocprofv3 FETCH_SIZE/WRITE_SIZE passes,
-# scaled by a 1 GiB copy (FETCH x2.00, WRITE x1.00); key: (jmax, wavepackets, m, kernel)
-MEASURED_TRAFFIC = {(40, 512, 12, "auto"): 351.1e6, (40, 512, 12, "ell:4"): 351.1e6}
-MEASURED_TRAFFIC_SOURCE = "profiles/k_lanczos_ell_linear_j40_b512_m12_pmc.md"
+FWHM_FS = 300.0
+DT_FS = 10.0
+PULSE_STEPS = int(round(8 * FWHM_FS / DT_FS))   # steps with a non-zero field
+# HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE passes scaled by a
+# 1 GiB copy (FETCH x2.00, WRITE x1.00); key: (kernel, jmax, wavepackets, m, steps)
+MEASURED_TRAFFIC = {
+    ("ell", 40, 512, 12, 1): (351.1e6, "profiles/k_lanczos_ell_linear_j40_b512_m12_pmc.md"),
+}
 
 
 def workload(jmax=40, B=0.2029, alpha=(26.0, 26.0, 53.6)):
@@ -45,10 +54,14 @@ def workload(jmax=40, B=0.2029, alpha=(26.0, 26.0, 53.6)):
     return bas, h0, H, cos2
 
 
-def pulse(t_fs, fwhm=300.0, intensity=5e11):
+def pulse(t_fs, fwhm=FWHM_FS, intensity=5e11):
+    """Cycle-averaged Z field [V/m]: E0 exp(-2 ln2 (t-t0)^2/fwhm^2)/sqrt(2), t0 = 4 fwhm,
+    zero for |t - t0| > 4 fwhm (intensity below 2e-10 of the peak)."""
     from richmol_amd import units
     e0 = units.intensity_to_field(intensity)
-    t0 = 2.5 * fwhm / 2
+    t0 = 4.0 * fwhm
+    if abs(t_fs - t0) > 4.0 * fwhm:
+        return 0.0
     return e0 * np.exp(-2 * np.log(2) * (t_fs - t0) ** 2 / fwhm ** 2) / np.sqrt(2.0)
 
 
@@ -57,14 +70,14 @@ def exp_fac(dt_fs):
     return -1j * dt_fs * 1e-15 * units.enr_units()["invcm"] / units.HBAR
 
 
-def cpu_baseline(bas, h0, H, psi0, f, fields, krylov, ncols, nsteps):
+def cpu_baseline(h0, H, psi0, fields, krylov, ncols, nsteps):
     """Oracle (port of richmol's CPU path) timed on ncols x nsteps of the workload."""
     from threadpoolctl import threadpool_limits
     from oracle.cartens import OracleCarTens
     from oracle.tdse import OracleTDSE
     Oh0 = OracleCarTens.from_attrs(h0)
     OH = OracleCarTens.from_attrs(H)
-    td = OracleTDSE(t_end=10.0 * (len(fields) + 1), dt=10.0, t_units="fs")
+    td = OracleTDSE(t_end=DT_FS * (len(fields) + 1), dt=DT_FS, t_units="fs")
     td.time_grid()
     X = psi0[:ncols].copy()
     with threadpool_limits(limits=1):
@@ -74,6 +87,29 @@ def cpu_baseline(bas, h0, H, psi0, f, fields, krylov, ncols, nsteps):
             X, _ = td.update(OH, X, H0=Oh0, fixed_m=krylov)
         el = time.perf_counter() - t0
     return X, el
+
+
+def timed_run(ens, warm, timed, precision="fp64"):
+    """Warm-up run, then the timed run: (wall s, summed kernel ms, launches)."""
+    import torch
+    tw = ens.prepare(warm)
+    ens.plan.set_precision(precision)
+    ens.propagate(None, table=tw)
+    tt = ens.prepare(timed)
+    torch.cuda.synchronize()
+    ens.plan.timing(True)
+    t0 = time.perf_counter()
+    ens.propagate(None, table=tt)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms, nl = ens.plan.kernel_time()
+    ens.plan.timing(False)
+    ens.check()
+    return wall, ms, nl
+
+
+def cos2_of(ens, cplan):
+    return ens.expect(cplan).real.cpu().numpy() + 1.0 / 3.0
 
 
 def main():
@@ -88,13 +124,14 @@ def main():
     ap.add_argument("--cpu-cols", type=int, default=16)
     ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--kernel", default="tuned",
-                    help="tuned (autotuner cache, else auto) | auto | global | resident | ell | "
-                         "ell:<i> (explicit ELL configuration)")
-    ap.add_argument("--tune", action="store_true",
-                    help="time every kernel configuration on this workload first and use the fastest")
-    ap.add_argument("--per-step", action="store_true",
-                    help="one host call per step instead of one rm_propagate call for the run")
+    ap.add_argument("--kernel", default="auto",
+                    help="headline kernel: auto (subspace when it applies) | subspace | tuned | ell | "
+                         "ell:<i> | resident | global")
+    ap.add_argument("--full-kernel", default="tuned",
+                    help="kernel of the full-vector comparison run (tuned = autotuner cache)")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="headline run only (no full-vector, precision, single-wavepacket, serving legs)")
+    ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-sized device buffer 3 times (reference for the HBM "
                          "byte counters) and exit")
@@ -114,8 +151,9 @@ def main():
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
               "(wavepacket columns are independent: a multi-GPU run would shard them)", file=sys.stderr)
         sys.exit(2)
-    if args.warmup + args.steps > 480:
-        print("warmup + steps must stay within the pulse (<= 480 steps of 10 fs)", file=sys.stderr)
+    if args.warmup + args.steps > PULSE_STEPS:
+        print(f"warmup + steps must stay within the pulse (<= {PULSE_STEPS} steps of {DT_FS} fs)",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -124,71 +162,43 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
-    dt = 10.0
     bas, h0, H, cos2 = workload(args.jmax)
     n = sum(len(bas.mlist[J]) for J in bas.Jlist)
     idx = ensemble.thermal_sample(bas, 1.0, args.ncols, seed=args.seed)
     psi0 = ensemble.basis_rows(n, idx)
-    f = exp_fac(dt)
+    f = exp_fac(DT_FS)
     nst = args.warmup + args.steps
-    fields = [[0.0, 0.0, pulse((i + 0.5) * dt)] for i in range(nst)]
-
-    ens = ensemble.Ensemble(H, h0, f, psi0, krylov_fixed=args.krylov)
-    ens.set_kernel(args.kernel)
-    tuning = None
-    if args.tune:
-        from richmol_amd import autotune
-        ens.prepare(fields[:1])
-        choice, res = autotune.tune(ens, fields, save=True)
-        tuning = {"choice": choice, "us_per_launch": {k: v * 1e3 for k, v in res.items()},
-                  "key": autotune.key(ens.plan, args.ncols, args.krylov)}
-    if args.per_step:
-        # one Python call per step (the drop-in TDSE.update pattern)
-        for i in range(args.warmup):
-            ens.step(fields[i])
-        torch.cuda.synchronize()
-        ens.plan.timing(True)
-        t0 = time.perf_counter()
-        for k in range(args.steps):
-            ens.step(fields[args.warmup + k])
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
-    else:
-        # the whole run on the device: field products for all steps, then one
-        # rm_propagate call (field contraction + Krylov step per time step)
-        ens.propagate(fields[:args.warmup])
-        torch.cuda.synchronize()
-        ens.plan.timing(True)
-        t0 = time.perf_counter()
-        ens.propagate(fields[args.warmup:])
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
-    tot_ms, nlaunch = ens.plan.kernel_time()
-    ens.plan.timing(False)
-    assert nlaunch == args.steps, (nlaunch, args.steps)
-    kern_ms = tot_ms / nlaunch
-    ens.check()
-
-    value = args.ncols * args.steps / wall
+    fields = [[0.0, 0.0, pulse((i + 0.5) * DT_FS)] for i in range(nst)]
+    warm, timed = fields[:args.warmup], fields[args.warmup:]
     m = args.krylov
-    p = ens.plan
-    # algorithmic work per wavepacket-step (Krylov dimension m, nnz stored entries):
-    # matvecs 8 m nnz flops + 7 complex vector operations per iteration (dot,
-    # norm, 2 axpy, scale, final combination) 8 m N 7 flops; streaming-basis HBM
-    # model 16 N (4m+1) bytes + the operator once per launch
-    flops_wp = 8.0 * m * p.nnz + 8.0 * m * n * 7
-    flops_launch = args.ncols * flops_wp
-    bytes_op = p.nnz * (16 + 4) + (p.n + 1) * 8
-    bytes_launch = args.ncols * 16 * n * (4 * m + 1) + bytes_op
-    tflops = flops_launch / (kern_ms * 1e-3) / 1e12
-    hbm_alg = bytes_launch / (kern_ms * 1e-3) / 1e9
-    measured = MEASURED_TRAFFIC.get((args.jmax, args.ncols, m, ens.plan.kernel))
-
-    # <cos^2 theta> ensemble average after the timed steps (finite, physical)
     cos2.field([0, 0, 1])
     cplan = plan_for(cos2)
     cplan.set_field(cos2)
-    c2 = float(ens.expect(cplan).real.mean().item()) + 1.0 / 3.0
+
+    # ---- headline: production path
+    ens = ensemble.Ensemble(H, h0, f, psi0, krylov_fixed=m)
+    ens.set_kernel(args.kernel)
+    wall, kms, nl = timed_run(ens, warm, timed)
+    value = args.ncols * args.steps / wall
+    p = ens.plan
+    used = "subspace" if (nl == 1 and args.kernel in ("auto", "subspace")) else p.kernel
+    c2_fp64 = cos2_of(ens, cplan)
+    psi_fp64 = ens.psi.cpu().numpy()
+
+    # algorithmic FP64 work per wavepacket-step on the rows it spans (d rows,
+    # e stored entries): m matvecs of 8 e flops + 7 complex vector operations
+    # of 8 d flops per iteration
+    if used == "subspace":
+        d_rows, d_nnz = p.domain_sizes(psi0)
+        flops_launch = float(np.sum(8.0 * m * d_nnz + 8.0 * m * d_rows * 7)) * args.steps
+        kernel_name, launch_steps = "k_sub_prop", args.steps
+    else:
+        flops_launch = args.ncols * (8.0 * m * p.nnz + 8.0 * m * n * 7)
+        kernel_name, launch_steps = "k_lanczos_ell", 1
+    kern_ms = kms / nl
+    tflops = flops_launch / (kern_ms * 1e-3) / 1e12
+    meas = MEASURED_TRAFFIC.get(("subspace" if used == "subspace" else "ell", args.jmax, args.ncols, m,
+                                 launch_steps))
 
     result = {
         "metric": METRIC,
@@ -207,38 +217,89 @@ def main():
                                "512 wavepackets at 1 K, Z pulse 300 fs / 5e11 W/cm^2, dt=10 fs, "
                                "split operator + fixed Krylov m=12, FP64",
                    "jmax": args.jmax, "n_states": n, "wavepackets": args.ncols, "krylov": m,
-                   "dt_fs": dt, "kernel": args.kernel,
-                   "launch": "per-step host calls" if args.per_step else "one rm_propagate call",
-                   "parallelism": "one GPU; wavepackets split over persistent workgroups"},
-        # The recurrence vectors stay in registers/LDS and the basis is written and
-        # read once, so the streaming-basis HBM model overstates the traffic (the
-        # measured bytes are ~half of it); the FP64 arithmetic roofline is the
-        # one this kernel is reported against, with the measured traffic beside it.
+                   "dt_fs": DT_FS, "kernel": args.kernel, "kernel_used": used,
+                   "launch": "one rm_propagate call for the timed steps",
+                   "parallelism": "one GPU; one wave per wavepacket on its invariant subspace"
+                   if used == "subspace" else "one GPU; wavepackets split over persistent workgroups"},
         "roofline": {"bound": "mfma", "achieved": tflops, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": tflops / FP64_PEAK_TFLOPS,
-                     "traffic": measured, "precision": "fp64",
-                     "kernel": "k_lanczos_ell", "kernel_ms": kern_ms,
+                     "traffic": meas[0] if meas else None, "precision": "fp64",
+                     "kernel": kernel_name, "kernel_ms": kern_ms, "steps_per_launch": launch_steps,
                      "flops_per_launch": flops_launch,
-                     "hbm_model_bytes_per_launch": bytes_launch,
-                     "hbm_model_frac": hbm_alg / HBM_PEAK_GBS,
-                     "traffic_source": MEASURED_TRAFFIC_SOURCE if measured else None},
-        "cos2_ensemble": c2,
-        "kernel_selected": ens.plan.kernel,
-        "tuning": tuning,
+                     "traffic_source": meas[1] if meas else None},
+        "cos2_ensemble": float(c2_fp64.mean()),
     }
+    print(json.dumps({"progress": "headline", "value": value, "kernel_used": used}), file=sys.stderr,
+          flush=True)
+
+    if not args.no_extras:
+        # ---- full-vector kernel on the same workload
+        e2 = ensemble.Ensemble(H, h0, f, psi0, krylov_fixed=m)
+        e2.set_kernel(args.full_kernel)
+        w2, k2, n2 = timed_run(e2, warm, timed)
+        flops_full = args.ncols * (8.0 * m * e2.plan.nnz + 8.0 * m * n * 7)
+        k2l = k2 / max(n2, 1)
+        meas2 = MEASURED_TRAFFIC.get(("ell", args.jmax, args.ncols, m, 1))
+        result["full_vector"] = {
+            "kernel": e2.plan.kernel, "value": args.ncols * args.steps / w2,
+            "kernel_ms_per_step": k2l, "tflops": flops_full / (k2l * 1e-3) / 1e12,
+            "traffic_per_launch": meas2[0] if meas2 else None,
+            "max_abs_diff_vs_headline": float(np.abs(e2.psi.cpu().numpy() - psi_fp64).max())}
+        # ---- FP32 matvec precision mode: same steps, observables against FP64
+        e3 = ensemble.Ensemble(H, h0, f, psi0, krylov_fixed=m)
+        e3.set_kernel("subspace")
+        w3, k3, n3 = timed_run(e3, warm, timed, precision="fp32")
+        c2_32 = cos2_of(e3, cplan)
+        result["precision_fp32"] = {
+            "value": args.ncols * args.steps / w3,
+            "max_abs_psi_err": float(np.abs(e3.psi.cpu().numpy() - psi_fp64).max()),
+            "max_cos2_err_per_wavepacket": float(np.abs(c2_32 - c2_fp64).max()),
+            "cos2_ensemble_err": float(abs(c2_32.mean() - c2_fp64.mean())),
+            "norm_err": float(np.abs(np.linalg.norm(e3.psi.cpu().numpy(), axis=1) - 1).max()),
+            "gate": 1e-6,
+            "note": "complex64 operator and gathered vector, FP32 accumulation; Krylov recurrence, "
+                    "norms, exp(fT), phases and the wavepacket in FP64"}
+        # ---- plain single-wavepacket FP64 path: one TDSE.update call per step
+        from richmol_amd.tdse import TDSE
+        nsw = min(60, args.steps)
+        td = TDSE(t_end=DT_FS * (nst + 1), dt=DT_FS, t_units="fs")
+        td.time_grid()
+        v = torch.from_numpy(psi0[:1].copy()).to(dev)
+        for F in warm:
+            H.field(F)
+            v, _ = td.update(H, v, H0=h0, krylov_fixed=m)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for F in timed[:nsw]:
+            H.field(F)
+            v, _ = td.update(H, v, H0=h0, krylov_fixed=m)
+        torch.cuda.synchronize()
+        single = nsw / (time.perf_counter() - t0)
+        result["single_wavepacket_fp64"] = {
+            "value": single, "steps": nsw,
+            "path": "TDSE.update per step, one wavepacket, FP64 (host loop, rm_step)",
+            "speedup_of_value": value / single}
+        print(json.dumps({"progress": "extras"}), file=sys.stderr, flush=True)
+
+        # ---- serving: continuous batching of independent propagation requests
+        if args.serve_requests > 0:
+            from richmol_amd import serving
+            result["serving"] = serving.benchmark(n_requests=args.serve_requests, seed=args.seed)
 
     if not args.no_cpu:
-        Xc, el = cpu_baseline(bas, h0, H, psi0, f, fields, m, args.cpu_cols, args.cpu_steps)
+        # steps around the pulse peak (t0 = 4 FWHM), from the initial rows
+        s0 = int(4 * FWHM_FS / DT_FS) - args.cpu_steps // 2
+        peak = [[0.0, 0.0, pulse((s0 + i + 0.5) * DT_FS)] for i in range(args.cpu_steps)]
+        Xc, el = cpu_baseline(h0, H, psi0, peak, m, args.cpu_cols, args.cpu_steps)
         # the same sample on the device, from the same initial rows
         ens2 = ensemble.Ensemble(H, h0, f, psi0[:args.cpu_cols], krylov_fixed=m)
-        for F in fields[:args.cpu_steps]:
-            ens2.step(F)
+        ens2.propagate(peak)
         dev_err = float(np.abs(ens2.psi.cpu().numpy() - Xc).max())
         result["cpu_baseline"] = {
             "value": args.cpu_cols * args.cpu_steps / el, "unit": "wavepacket-steps/s", "cores": 1,
             "kind": "port",
-            "sample": f"{args.cpu_cols} wavepackets x {args.cpu_steps} steps of the same workload "
-                      f"({el:.1f} s), FP64 oracle (numpy/scipy restatement of richmol's "
+            "sample": f"{args.cpu_cols} wavepackets x {args.cpu_steps} steps of the same workload at the "
+                      f"pulse peak ({el:.1f} s), FP64 oracle (numpy/scipy restatement of richmol's "
                       "TDSE.update row loop and block matvec), 1 thread"}
         result["parity_vs_oracle_max_abs"] = dev_err
     print(json.dumps(result), flush=True)

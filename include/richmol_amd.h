@@ -46,7 +46,8 @@ typedef struct rm_plan rm_plan;
 /* Message of the last failed call on this thread ("" if none). */
 const char* rm_last_error(void);
 
-/* Library ABI version. */
+/* Library ABI version (RM_ABI_VERSION of the header it was built from). */
+#define RM_ABI_VERSION 2
 int rm_version(void);
 
 /*
@@ -137,14 +138,29 @@ int rm_propagate(rm_plan* plan, void* psi, void* work, int64_t ld, int64_t ncols
 int rm_plan_timing(rm_plan* plan, int32_t enable);
 int rm_plan_kernel_time(rm_plan* plan, double* total_ms, int64_t* launches);
 
-/* Kernel used by rm_step (no reference counterpart: a device tuning knob).
- * All variants compute the same step.  RM_KERNEL_AUTO picks the ELL kernel
- * (persistent workgroups, two wavepackets per workgroup, operator staged in
- * LDS when it fits) for n <= 4096, else the global-memory kernel. */
+/* Kernel used by rm_step / rm_propagate (no reference counterpart: a device
+ * tuning knob).  All variants compute the same step.
+ *
+ * RM_KERNEL_AUTO: rm_propagate takes the invariant-subspace kernel when it
+ * applies (below); otherwise, and in rm_step, the ELL kernel (persistent
+ * workgroups, operator staged in LDS when it fits) for n <= 4096, else the
+ * global-memory kernel.
+ *
+ * RM_KERNEL_SUBSPACE (rm_propagate only): the rows are split into the
+ * connected components of the operator's sparsity graph (invariant subspaces
+ * of every H(F) the plan can form).  A wavepacket's Krylov vectors stay in
+ * the union of the subspaces its initial support touches ("domain"), so each
+ * wavepacket is advanced by one wave on its domain only, through all nsteps
+ * steps in one launch, with the operator A(F) = sum_s F_s B_s assembled per
+ * step from the field table.  Applies when no MF threshold is given and every
+ * domain has at most RM_SUBSPACE_MAX_ROWS rows; results equal the full-vector
+ * kernels' to rounding (entries outside the domain stay exactly zero). */
 #define RM_KERNEL_AUTO      0
 #define RM_KERNEL_GLOBAL    1
 #define RM_KERNEL_RESIDENT  2
 #define RM_KERNEL_ELL       3
+#define RM_KERNEL_SUBSPACE  4
+#define RM_SUBSPACE_MAX_ROWS 1024
 /* RM_KERNEL_ELL_CFG0 + i: ELL kernel with explicit launch configuration i
  * (rm_ell_config() describes it); the autotuner's search space. */
 #define RM_KERNEL_ELL_CFG0  16
@@ -154,6 +170,69 @@ int rm_plan_set_kernel(rm_plan* plan, int32_t variant);
  * thread, wavepackets per workgroup, operator staged in LDS (1/0).
  * Returns RM_ERR_ARG when i is out of range. */
 int rm_ell_config(int32_t i, int32_t* threads, int32_t* rows, int32_t* cols, int32_t* op_lds);
+
+/* Invariant subspaces of the plan's operator (connected components of its
+ * sparsity graph, built on first call): count and largest size. */
+int rm_plan_subspaces(rm_plan* plan, int64_t* count, int64_t* max_rows);
+
+/* Arithmetic of the bulk matvec in the invariant-subspace kernel (no
+ * reference counterpart).  RM_PREC_FP64: complex128 throughout.
+ * RM_PREC_FP32: the per-step operator values and the gathered Krylov vector
+ * are rounded to complex64 and the matvec accumulates in FP32; Krylov
+ * recurrence, norms and dots, exp(fT), the split-operator phases and the
+ * wavepacket itself stay FP64. */
+#define RM_PREC_FP64 0
+#define RM_PREC_FP32 1
+int rm_plan_set_precision(rm_plan* plan, int32_t precision);
+
+/*
+ * Domain of each column for the subspace kernel: lohi[2b], lohi[2b+1] =
+ * [lo, hi) in the plan's permuted row order, the union of the invariant
+ * subspaces the column's non-zero entries touch (lo >= hi: zero column; a
+ * non-finite entry gives the whole basis).  psi: device; lohi: host, 2*ncols.
+ * Synchronises the stream.
+ */
+int rm_plan_domains(rm_plan* plan, const void* psi, int64_t ld, int64_t ncols,
+                    int32_t* lohi, void* stream);
+
+/* sflag[s] = 1 if step s (row s of fprod_dev, nsteps x nfield complex,
+ * device) makes any MF element non-zero, else 0: the `len(H.mfmat) > 0`
+ * test of tdse.py:377 without thresholds.  sflag: device int32[nsteps]. */
+int rm_plan_step_flags(rm_plan* plan, int64_t nsteps, const double* fprod_dev,
+                       int32_t* sflag, void* stream);
+
+/*
+ * Batches of independent propagation jobs (no reference counterpart: the
+ * serving loop's device call).  One rm_batch_propagate call advances every
+ * job by its own number of steps with the subspace kernel: one wave per job,
+ * all jobs in as many launches as there are launch classes (domain size and
+ * operator width; a job's result does not depend on the other jobs of the
+ * call, bit for bit).  Jobs may share plans; each has its own wavepacket,
+ * phases and field table.  Calls on one batch object must use one stream
+ * (descriptor staging is double-buffered and reused in call order).
+ */
+typedef struct rm_batch rm_batch;
+typedef struct {
+  rm_plan* plan;             /* operator (only its subspace arrays are read) */
+  void* psi;                 /* device: the wavepacket is column `col` at psi + col*ld */
+  int64_t ld;
+  int32_t col;
+  int32_t lo, len;           /* its domain, from rm_plan_domains (len <= RM_SUBSPACE_MAX_ROWS) */
+  const double* fprod_dev;   /* device: field products of this call's steps, nsteps x nfield complex */
+  const int32_t* sflag_dev;  /* device: rm_plan_step_flags of those rows */
+  int32_t nsteps;
+  double f_re, f_im;         /* exponent factor of the step (TDSE.update, tdse.py:336-337) */
+  const void* phase;         /* device: n complex split-operator phases (rm_phase_init) or NULL */
+  int32_t* kused;            /* device int32[nsteps] or NULL: Krylov dimension per step */
+  int32_t* status;           /* device int32[1] or NULL: set to RM_NOT_CONVERGED on failure */
+} rm_job;
+int rm_batch_create(rm_batch** out);
+int rm_batch_destroy(rm_batch* batch);
+/* job_bytes: sizeof(rm_job) as the caller laid the array out; any other
+ * value is rejected (RM_ERR_ARG) before a job is read. */
+int rm_batch_propagate(rm_batch* batch, const rm_job* jobs, int32_t njobs, int32_t job_bytes,
+                       int32_t krylov_fixed, int32_t maxorder, double tol, int32_t precision,
+                       void* stream);
 
 /* psi_out[:, b] = (psi_in[:, b] * phase) * phase: the step when the operator
  * carries no field at all (tdse.py:375, 394 with the Lanczos skipped). */

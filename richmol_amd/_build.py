@@ -6,7 +6,8 @@ PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 REPO_DIR = os.path.dirname(PKG_DIR)
 LIB_NAME = "librichmol_amd.so"
 LIB_PATH = os.path.join(PKG_DIR, LIB_NAME)
-SOURCES = [os.path.join(PKG_DIR, "csrc", "rm_tdse.hip")]
+SOURCES = [os.path.join(PKG_DIR, "csrc", "rm_tdse.hip"),
+           os.path.join(PKG_DIR, "csrc", "rm_subspace.hip")]
 
 HIPCC_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared",

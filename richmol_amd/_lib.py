@@ -44,7 +44,26 @@ SIGNATURES = {
     "rm_ell_config": (ctypes.c_int, [ctypes.c_int32, _c_i32p, _c_i32p, _c_i32p, _c_i32p]),
     "rm_phase_apply": (ctypes.c_int, [ctypes.c_int64, _vp, _vp, ctypes.c_int64, ctypes.c_int64, _vp, _vp]),
     "rm_expect": (ctypes.c_int, [_vp, _vp, ctypes.c_int64, ctypes.c_int64, _vp, _vp]),
+    "rm_plan_subspaces": (ctypes.c_int, [_vp, _c_i64p, _c_i64p]),
+    "rm_plan_set_precision": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "rm_plan_domains": (ctypes.c_int, [_vp, _vp, ctypes.c_int64, ctypes.c_int64, _c_i32p, _vp]),
+    "rm_plan_step_flags": (ctypes.c_int, [_vp, ctypes.c_int64, _vp, _vp, _vp]),
+    "rm_batch_create": (ctypes.c_int, [ctypes.POINTER(_vp)]),
+    "rm_batch_destroy": (ctypes.c_int, [_vp]),
+    "rm_batch_propagate": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                          ctypes.c_int32, ctypes.c_double, ctypes.c_int32, _vp]),
 }
+
+
+class RmJob(ctypes.Structure):
+    """rm_job of include/richmol_amd.h (one job of rm_batch_propagate)."""
+    _fields_ = [("plan", _vp), ("psi", _vp), ("ld", ctypes.c_int64), ("col", ctypes.c_int32),
+                ("lo", ctypes.c_int32), ("len", ctypes.c_int32), ("fprod_dev", _vp),
+                ("sflag_dev", _vp), ("nsteps", ctypes.c_int32), ("f_re", ctypes.c_double),
+                ("f_im", ctypes.c_double), ("phase", _vp), ("kused", _vp), ("status", _vp)]
+
+RM_SUBSPACE_MAX_ROWS = 1024
+PRECISIONS = {"fp64": 0, "fp32": 1}
 
 _LIB = None
 

@@ -1,0 +1,408 @@
+// richmol_amd: declarations shared by the library's translation units -- error
+// reporting, complex arithmetic, the plan object, wave-level reductions and the
+// small dense Krylov pieces (exp(fT) e0, adaptive stopping metric).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "../../include/richmol_amd.h"
+
+// ---------------------------------------------------------------------------
+// errors (message per calling thread, returned by rm_last_error)
+
+int fail(int code, const char* fmt, ...);
+
+#define HIPCHK(expr)                                                        \
+  do {                                                                      \
+    hipError_t e_ = (expr);                                                 \
+    if (e_ != hipSuccess)                                                   \
+      return fail(RM_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+static inline unsigned grid_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// ---------------------------------------------------------------------------
+// complex helpers (double2 = re, im)
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) {
+  return make_double2(a.x + b.x, a.y + b.y);
+}
+__device__ __forceinline__ double2 csub(double2 a, double2 b) {
+  return make_double2(a.x - b.x, a.y - b.y);
+}
+__device__ __forceinline__ double2 cscale(double s, double2 a) {
+  return make_double2(s * a.x, s * a.y);
+}
+__device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+// acc + a*b
+__device__ __forceinline__ double2 cfma(double2 a, double2 b, double2 acc) {
+  return make_double2(acc.x + (a.x * b.x - a.y * b.y), acc.y + (a.x * b.y + a.y * b.x));
+}
+__device__ __forceinline__ float2 cfmaf(float2 a, float2 b, float2 acc) {
+  return make_float2(acc.x + (a.x * b.x - a.y * b.y), acc.y + (a.x * b.y + a.y * b.x));
+}
+
+// ---------------------------------------------------------------------------
+// plan
+
+struct rm_plan {
+  int64_t n = 0, nnz = 0, npair = 0, nterm = 0, nq = 0;
+  int32_t nfield = 0, nleaf = 0;
+  int device = 0;
+  // host copies of the structure (the invariant-subspace path is built from them)
+  std::vector<int64_t> h_row_ptr;
+  std::vector<int32_t> h_col;
+  std::vector<int64_t> h_pair_ptr;
+  std::vector<int32_t> h_pair_term;
+  std::vector<double2> h_pair_kval;
+  std::vector<int64_t> h_term_ptr;
+  std::vector<int32_t> h_term_field;
+  std::vector<double2> h_term_mval;
+  int64_t* row_ptr = nullptr;
+  int32_t* col = nullptr;
+  int64_t* pair_ptr = nullptr;
+  int32_t* pair_term = nullptr;
+  double2* pair_kval = nullptr;
+  int64_t* term_ptr = nullptr;
+  int32_t* term_field = nullptr;
+  double2* term_mval = nullptr;
+  int32_t* term_leaf = nullptr;
+  double2* mf = nullptr;     // nterm
+  double2* aval = nullptr;   // nnz
+  int32_t* flag = nullptr;   // id of the last field step with a non-zero MF
+  int32_t step_id = 0;
+  void* ws = nullptr;        // Krylov workspace
+  size_t ws_bytes = 0;
+  int32_t variant = RM_KERNEL_AUTO;
+  int32_t precision = RM_PREC_FP64;
+  // ELL copy of the matrix: width = max entries per row, column-major
+  // (entry j of row i at j*n + i), padding entries (value 0, column i)
+  int32_t ell_w = 0;
+  int32_t* ell_col = nullptr;
+  double2* ell_val = nullptr;
+  int32_t* ell_pos = nullptr;  // CSR entry -> ELL slot
+  int n_cu = 0;
+  // invariant subspaces (connected components of the sparsity graph), built
+  // on first use: rows permuted so that each subspace is contiguous, and the
+  // operator split per field slot, A(F) = sum_s F_s B_s (no thresholds)
+  int sub_ready = 0;             // 0 not built, 1 built
+  int32_t nsub = 0, max_sub = 0;
+  int32_t sub_w = 0;             // ELL width in permuted order
+  int32_t* sub_perm = nullptr;   // permuted row -> original row
+  int32_t* sub_label = nullptr;  // original row -> subspace
+  int32_t* sub_lo = nullptr;     // subspace -> first permuted row
+  int32_t* sub_hi = nullptr;     // subspace -> one past its last permuted row
+  int32_t* sub_col = nullptr;    // [sub_w][n] permuted column of each entry
+  double2* sub_val = nullptr;    // [nfield][sub_w][n] B_s values
+  std::vector<int32_t> h_sub_start, h_sub_end;   // per subspace: first / one past last permuted row
+  std::vector<int32_t> h_sub_label_perm;         // permuted row -> subspace
+  // per-call scratch of the subspace path
+  int32_t* dom = nullptr;        // 2 per column: domain lo, hi
+  int64_t dom_cap = 0;
+  int32_t* sflag = nullptr;      // per step: 1 if any MF element is non-zero
+  int64_t sflag_cap = 0;
+  std::vector<int32_t> h_dom;
+  void* sub_scratch = nullptr;   // launch staging and workspace (rm_subspace.hip)
+  // optional HIP-event timing of the Krylov step launches
+  int timing = 0;
+  int nrec = 0;
+  std::vector<hipEvent_t> ev0, ev1;
+};
+
+template <typename T>
+static int upload(T** dst, const T* src, int64_t count) {
+  *dst = nullptr;
+  size_t bytes = sizeof(T) * (size_t)(count > 0 ? count : 1);
+  HIPCHK(hipMalloc((void**)dst, bytes));
+  if (count > 0 && src) HIPCHK(hipMemcpy(*dst, src, sizeof(T) * (size_t)count, hipMemcpyHostToDevice));
+  return RM_OK;
+}
+
+// Grow a device scratch array to hold at least count elements (synchronises
+// the device before freeing a smaller one).
+template <typename T>
+static int grow(T** buf, int64_t* cap, int64_t count) {
+  if (*cap >= count && *buf) return RM_OK;
+  if (*buf) {
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipFree(*buf));
+    *buf = nullptr;
+    *cap = 0;
+  }
+  HIPCHK(hipMalloc((void**)buf, sizeof(T) * (size_t)(count > 0 ? count : 1)));
+  *cap = count;
+  return RM_OK;
+}
+
+// Krylov workspace of the plan, at least `bytes` (defined with the step kernels).
+int ensure_ws(rm_plan* p, size_t bytes);
+// Free a plan's subspace-launch scratch (rm_subspace.hip).
+void sub_scratch_release(void* sc);
+// Event-bracketed launch helpers for the plan's optional timing.
+hipError_t timing_begin(rm_plan* p, hipStream_t s);
+hipError_t timing_end(rm_plan* p, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// wave-level reductions (64-lane wavefronts)
+
+constexpr int KMAX = RM_MAX_KRYLOV;
+constexpr int TAYLOR_TERMS = 18;   // |h f T|_1 <= 1/2: truncation < 0.5^19/19! ~ 1.6e-23
+
+// DPP row rotation of a double (both 32-bit halves); CTRL is the DPP control.
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+
+// Sum over the 64 lanes of a wave (all lanes active), bitwise identical on
+// every lane: rotations by 8, 4, 2, 1 inside each 16-lane row pair every lane
+// with partners whose partial sums are the same two operands (addition is
+// commutative), then the four row sums are combined in a fixed order.
+__device__ __forceinline__ double wave_allsum(double v) {
+  v += dpp_mov<0x128>(v);   // row_ror:8
+  v += dpp_mov<0x124>(v);   // row_ror:4
+  v += dpp_mov<0x122>(v);   // row_ror:2
+  v += dpp_mov<0x121>(v);   // row_ror:1
+  return (readlane_d(v, 0) + readlane_d(v, 16)) + (readlane_d(v, 32) + readlane_d(v, 48));
+}
+
+// wave_allsum for values that are zero on lanes 32..63: the same sum (adding
+// the two zero rows changes nothing), without reading the upper half.
+__device__ __forceinline__ double wave_allsum32(double v) {
+  v += dpp_mov<0x128>(v);
+  v += dpp_mov<0x124>(v);
+  v += dpp_mov<0x122>(v);
+  v += dpp_mov<0x121>(v);
+  return readlane_d(v, 0) + readlane_d(v, 16);
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+  return v;
+}
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+__device__ __forceinline__ double2 shfl2(double2 v, int src) {
+  return make_double2(__shfl(v.x, src), __shfl(v.y, src));
+}
+
+// Order LDS and memory operations of the lanes of one wave: every lane's
+// earlier writes are visible to every lane's later reads (no s_barrier, so it
+// is safe where the waves of a workgroup follow different control flow).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ---------------------------------------------------------------------------
+// small dense Krylov pieces, run by the 64 lanes of one wave
+
+// c = exp(f*T) e_0 for the s x s tridiagonal T (diag alpha[0..s-1], off-diagonal
+// beta[1..s-1], T[i-1,i] = T[i,i-1] = beta[i]).  Lane l owns rows l and l+64
+// (s <= 128).  Scaling and stepping: exp(A)e0 = (exp(A/nsub))^nsub e0 with
+// |A/nsub|_1 <= 1/2, each factor a Taylor sum.
+static __device__ void expm_e0(int s, const double2* alpha, const double* beta, double2 f, double2* c) {
+  const int l = threadIdx.x & 63;
+  double2 a[2], bl[2], bu[2];   // A_ii, A_{i,i-1}, A_{i,i+1}
+  double col1 = 0.0;
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    int i = l + 64 * r;
+    a[r] = make_double2(0.0, 0.0);
+    bl[r] = make_double2(0.0, 0.0);
+    bu[r] = make_double2(0.0, 0.0);
+    if (i < s) {
+      a[r] = cmul(f, alpha[i]);
+      if (i >= 1) bl[r] = cscale(beta[i], f);
+      if (i + 1 < s) bu[r] = cscale(beta[i + 1], f);
+      // column i of |A|: |A_ii| + |A_{i-1,i}| + |A_{i+1,i}|, A symmetric off-diagonal
+      col1 = fmax(col1, hypot(a[r].x, a[r].y) + hypot(bl[r].x, bl[r].y) + hypot(bu[r].x, bu[r].y));
+    }
+  }
+  double norm1 = wave_max(col1);
+  int nsub = 1;
+  if (norm1 > 0.5) nsub = (int)ceil(norm1 / 0.5);
+  if (!(nsub >= 1) || nsub > (1 << 20)) nsub = 1;   // NaN / overflow: result becomes NaN anyway
+  const double h = 1.0 / (double)nsub;
+  double2 y[2];
+  y[0] = make_double2(l == 0 ? 1.0 : 0.0, 0.0);
+  y[1] = make_double2(0.0, 0.0);
+  for (int sub = 0; sub < nsub; ++sub) {
+    double2 term[2] = {y[0], y[1]};
+    double2 acc[2] = {y[0], y[1]};
+    for (int j = 1; j <= TAYLOR_TERMS; ++j) {
+      // neighbours: row i-1 and i+1
+      double2 t0_dn = shfl2(term[0], (l + 63) & 63);   // lane l-1, row l-1 (wraps at l=0)
+      double2 t1_dn = shfl2(term[1], (l + 63) & 63);
+      double2 t0_up = shfl2(term[0], (l + 1) & 63);
+      double2 t1_up = shfl2(term[1], (l + 1) & 63);
+      double2 prev0 = (l == 0) ? make_double2(0.0, 0.0) : t0_dn;
+      double2 prev1 = (l == 0) ? t0_dn /* lane 63 row 63 */ : t1_dn;
+      double2 next0 = (l == 63) ? t1_up /* lane 0 row 64 */ : t0_up;
+      double2 next1 = (l == 63) ? make_double2(0.0, 0.0) : t1_up;
+      const double sc = h / (double)j;
+      double2 n0 = cmul(a[0], term[0]);
+      n0 = cfma(bl[0], prev0, n0);
+      n0 = cfma(bu[0], next0, n0);
+      double2 n1 = cmul(a[1], term[1]);
+      n1 = cfma(bl[1], prev1, n1);
+      n1 = cfma(bu[1], next1, n1);
+      term[0] = cscale(sc, n0);
+      term[1] = cscale(sc, n1);
+      acc[0] = cadd(acc[0], term[0]);
+      acc[1] = cadd(acc[1], term[1]);
+    }
+    y[0] = acc[0];
+    y[1] = acc[1];
+  }
+  if (l < s) c[l] = y[0];
+  if (l + 64 < s) c[l + 64] = y[1];
+}
+
+// DPP move of a double within 16-lane rows; lanes whose source falls outside
+// the row receive 0.  CTRL 0x111 (row_shr:1): lane l gets lane l-1;
+// CTRL 0x101 (row_shl:1): lane l gets lane l+1.
+template <int CTRL>
+__device__ __forceinline__ double dpp_row_shift0(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Maximum over each 16-lane row (every lane of the row receives it).
+__device__ __forceinline__ double row_max16(double v) {
+  v = fmax(v, dpp_mov<0x128>(v));   // row_ror:8
+  v = fmax(v, dpp_mov<0x124>(v));   // row_ror:4
+  v = fmax(v, dpp_mov<0x122>(v));   // row_ror:2
+  v = fmax(v, dpp_mov<0x121>(v));   // row_ror:1
+  return v;
+}
+
+// exp(f*T) e_0 as expm_e0, for s <= 16: lane i (< s) of each 16-lane row owns
+// row i and its neighbours arrive by DPP row shifts (no LDS traffic); the
+// Taylor sum of each factor stops at the first term below 2^-60 of the
+// bound (h |f T|_1)^j / j!, at most TAYLOR_TERMS terms.  Lanes 0..s-1 write c.
+static __device__ void expm_e0_16(int s, const double2* alpha, const double* beta, double2 f, double2* c) {
+  const int l = threadIdx.x & 63;
+  const int i = l & 15;
+  double2 a = make_double2(0.0, 0.0), bl = a, bu = a;
+  double col1 = 0.0;
+  if (i < s) {
+    a = cmul(f, alpha[i]);
+    if (i >= 1) bl = cscale(beta[i], f);
+    if (i + 1 < s) bu = cscale(beta[i + 1], f);
+    // |re| + |im| >= |z|: a (slightly larger) bound for the scaling choice
+    col1 = fabs(a.x) + fabs(a.y) + fabs(bl.x) + fabs(bl.y) + fabs(bu.x) + fabs(bu.y);
+  }
+  const double norm1 = row_max16(col1);
+  int nsub = 1;
+  if (norm1 > 0.5) nsub = (int)ceil(norm1 / 0.5);
+  if (!(nsub >= 1) || nsub > (1 << 20)) nsub = 1;
+  const double h = 1.0 / (double)nsub;
+  int nterm = TAYLOR_TERMS;
+  {
+    const double x = h * norm1;
+    double t = 1.0;
+    for (int j = 1; j <= TAYLOR_TERMS; ++j) {
+      t *= x / (double)j;
+      if (t < 0x1p-60) {
+        nterm = j;
+        break;
+      }
+    }
+  }
+  double2 y = make_double2(i == 0 ? 1.0 : 0.0, 0.0);
+  for (int sub = 0; sub < nsub; ++sub) {
+    double2 term = y, acc = y;
+    for (int j = 1; j <= nterm; ++j) {
+      const double2 prev = make_double2(dpp_row_shift0<0x111>(term.x), dpp_row_shift0<0x111>(term.y));
+      const double2 next = make_double2(dpp_row_shift0<0x101>(term.x), dpp_row_shift0<0x101>(term.y));
+      double2 nt = cmul(a, term);
+      nt = cfma(bl, prev, nt);
+      nt = cfma(bu, next, nt);
+      term = cscale(h / (double)j, nt);
+      acc = cadd(acc, term);
+    }
+    y = acc;
+  }
+  if (l < s) c[l] = y;
+}
+
+// Adaptive stopping metric after iteration k (one wave; tdse.py:473-476).
+// Gram column k of the Krylov basis from the three-term relation
+//   <V_i|V_k> = (<H V_i|V_{k-1}> - a_{k-1} G_{i,k-1} - b_{k-1} G_{i,k-2}) / b_k,
+//   H V_i = b_{i+1} V_{i+1} + a_i V_i + b_i V_{i-1}
+// (skipped after an exact breakdown, whose column the caller computed
+// explicitly), then c = exp(f T_{k+1}) e0 and
+//   conv = |u_k - u_{k-1}|^2 = d^H G d,  d = c - [c_prev, 0];
+// finally c_prev = c.  Returns conv on every lane of the wave.
+// FAST: exp(fT) e0 by expm_e0_16 while the Krylov dimension is <= 16.
+template <bool FAST = false>
+static __device__ double adaptive_metric(int k, bool breakdown, double2* G, int kc, const double2* s_alpha,
+                                         const double* s_beta, double2 f, double2* s_c, double2* s_cprev) {
+  const int tid = threadIdx.x & 63;
+  const double beta = s_beta[k];
+  if (!breakdown) {
+    for (int i = tid; i < k; i += 64) {
+      double2 hv;
+      if (i == k - 1) {
+        hv = cconj(s_alpha[k - 1]);
+      } else {
+        hv = cscale(s_beta[i + 1], G[(i + 1) * kc + (k - 1)]);
+        hv = cfma(cconj(s_alpha[i]), G[i * kc + (k - 1)], hv);
+        if (i > 0) hv = cadd(hv, cscale(s_beta[i], G[(i - 1) * kc + (k - 1)]));
+      }
+      double2 t = csub(hv, cmul(s_alpha[k - 1], G[i * kc + (k - 1)]));
+      if (k >= 2) t = csub(t, cscale(s_beta[k - 1], G[i * kc + (k - 2)]));
+      t = make_double2(t.x / beta, t.y / beta);
+      G[i * kc + k] = t;
+      G[k * kc + i] = cconj(t);
+    }
+  }
+  if (tid == 0) G[k * kc + k] = make_double2(1.0, 0.0);
+  __threadfence_block();
+  if (FAST && k + 1 <= 16)
+    expm_e0_16(k + 1, s_alpha, s_beta, f, s_c);
+  else
+    expm_e0(k + 1, s_alpha, s_beta, f, s_c);
+  __threadfence_block();
+  double part = 0.0;
+  const int s = k + 1;
+  for (int i = tid; i < s; i += 64) {
+    double2 di = csub(s_c[i], i < k ? s_cprev[i] : make_double2(0.0, 0.0));
+    double2 r = make_double2(0.0, 0.0);
+    for (int j = 0; j < s; ++j) {
+      double2 dj = csub(s_c[j], j < k ? s_cprev[j] : make_double2(0.0, 0.0));
+      r = cfma(G[i * kc + j], dj, r);
+    }
+    part += di.x * r.x + di.y * r.y;
+  }
+  part = wave_sum(part);
+  __threadfence_block();
+  for (int i = tid; i < s; i += 64) s_cprev[i] = s_c[i];
+  return part;
+}

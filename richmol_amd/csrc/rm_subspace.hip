@@ -1,0 +1,935 @@
+// richmol_amd: invariant-subspace propagation.
+//
+// The sparsity graph of a plan's operator (union over every term and field
+// component the plan holds; H0 is diagonal) splits the basis into connected
+// components.  Each component is invariant under every H(F) the plan can
+// form, so the Krylov vectors of a wavepacket never leave the union of the
+// components its initial support touches -- its "domain".  Outside the domain
+// the reference's vectors stay exactly zero (sums of products with zeros), and
+// after an exact breakdown the restart vector only enters with a coefficient
+// that is exactly zero, so the restricted computation gives the full-vector
+// result up to the order of floating-point sums.
+//
+// Rows are permuted so every component is contiguous; a domain is then the
+// row range [lo, hi) from its first to its last component.  One wave owns one
+// wavepacket: it keeps the wavepacket, its split-operator phases and (up to
+// two field slots) its operator rows in registers through every step of the
+// run, assembles A(F_step) = sum_s F_s B_s per step, and runs the Lanczos
+// expv (tdse.py:417-486) with wave-level reductions only (no workgroup
+// barriers).  The gathered vector of the matvec lives in LDS, and so does the
+// Krylov basis when it fits; otherwise the basis is in global memory.
+
+#include <algorithm>
+#include <climits>
+#include <numeric>
+
+#include "rm_common.h"
+
+namespace {
+
+constexpr int NSR = 2;                         // field slots held in registers
+constexpr size_t SUB_LDS_BASIS = 32 * 1024;    // LDS budget of a wave's Krylov basis
+
+// One propagation job of a launch: an operator (a plan's subspace arrays),
+// the split-operator phases and field table of the job, and its wavepackets.
+struct SubOp {
+  const int32_t* col;      // [w][n] permuted column of each ELL entry
+  const double2* val;      // [nslot][w][n] B_s
+  const int32_t* perm;     // permuted row -> original row
+  const double2* phase;    // original order, or NULL
+  const double2* table;    // [nsteps][nslot] field products of the launch's steps
+  const int32_t* sflag;    // [nsteps] 1: some MF element non-zero
+  double2* psi;            // column b at psi + b*ld (original order)
+  int64_t ld;
+  int32_t* kused;          // kused[step*kused_ld + b - kcol0] or NULL
+  int64_t kused_ld;
+  int32_t* status;         // status[b - kcol0] or NULL: failures only
+  double2 f;
+  int32_t n, w, nslot, nsteps, kcol0;
+};
+
+struct SubArgs {
+  const SubOp* ops;
+  const int4* items;       // (column, lo, len, op), one wavepacket each
+  int32_t nitems;
+  double2* basis;          // per wave kcap * 64R (global-basis variant)
+  double2* gram;           // per wave kcap * kcap (adaptive)
+  double2* cscr;           // per wave 2 * kcap (adaptive)
+  int32_t kcap, fixed_m, maxorder;
+  double tol;
+};
+
+template <bool F32>
+struct XType { using T = double2; };
+template <>
+struct XType<true> { using T = float2; };
+
+__device__ __forceinline__ double2 to_d2(double2 v) { return v; }
+__device__ __forceinline__ double2 to_d2(float2 v) { return make_double2((double)v.x, (double)v.y); }
+template <bool F32>
+__device__ __forceinline__ typename XType<F32>::T to_x(double2 v);
+template <>
+__device__ __forceinline__ double2 to_x<false>(double2 v) { return v; }
+template <>
+__device__ __forceinline__ float2 to_x<true>(double2 v) {
+  return make_float2(__double2float_rn(v.x), __double2float_rn(v.y));
+}
+__device__ __forceinline__ double2 xfma(double2 a, double2 b, double2 acc) { return cfma(a, b, acc); }
+__device__ __forceinline__ float2 xfma(float2 a, float2 b, float2 acc) { return cfmaf(a, b, acc); }
+
+// exp(f T_s) e0: the DPP-row variant for s <= 16, the general one beyond.
+__device__ __forceinline__ void expm_e0_any(int s, const double2* alpha, const double* beta, double2 f,
+                                            double2* c) {
+  if (s <= 16)
+    expm_e0_16(s, alpha, beta, f, c);
+  else
+    expm_e0(s, alpha, beta, f, c);
+}
+
+// One wave per wavepacket, R rows per lane (domain <= 64 R rows).  WM > 0:
+// the ELL entries of the lane's rows (width <= WM) are held in registers
+// (their B_s too when nslot <= NSR); WM == 0: they are read (L2) at every
+// matvec.  FIXED: fixed Krylov dimension.  F32: FP32 matvec (operator and
+// gathered vector rounded, FP32 accumulation), everything else FP64.  BL: the
+// Krylov basis in LDS (kcap * 64R * 16 B <= SUB_LDS_BASIS).
+template <int R, int WM, bool FIXED, bool F32, bool BL>
+__global__ __launch_bounds__(64) void k_sub_prop(SubArgs a) {
+  using XT = typename XType<F32>::T;
+  constexpr int NR = 64 * R;
+  constexpr int WMR = WM > 0 ? WM : 1;
+  extern __shared__ char s_raw[];
+  XT* s_x = (XT*)s_raw;
+  __shared__ double2 s_alpha[KMAX];
+  __shared__ double s_beta[KMAX + 1];
+  __shared__ double2 s_cf[KMAX];
+  const int l = threadIdx.x;
+  const int gw = blockIdx.x;
+  const int nw = gridDim.x;
+  const int kc = a.kcap;
+  double2* V;
+  if constexpr (BL)
+    V = (double2*)(s_raw + NR * sizeof(XT));
+  else
+    V = a.basis + (int64_t)gw * kc * NR;
+  double2* G = a.gram + (int64_t)gw * kc * kc;
+  double2* cv = a.cscr + (int64_t)gw * 2 * kc;
+  const double2 zero = make_double2(0.0, 0.0);
+
+  for (int itm = gw; itm < a.nitems; itm += nw) {
+    const int4 item = a.items[itm];
+    const int b = item.x, lo = item.y, len = item.z;
+    const SubOp op = a.ops[item.w];
+    const int n = op.n, W = op.w;
+    const bool breg = WM > 0 && op.nslot <= NSR;
+    // domain within lanes 0..31: the upper half of every sum is zero
+    const bool half = R == 1 && len <= 32;
+    auto rsum = [half](double v) __attribute__((always_inline)) {
+      return half ? wave_allsum32(v) : wave_allsum(v);
+    };
+    int gq[R], orig[R];
+    bool ok[R];
+    double2 st[R], ph[R];   // the wavepacket and its phases on my rows, across steps
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int i = l + 64 * r;
+      ok[r] = i < len;
+      gq[r] = lo + min(i, len - 1);
+      orig[r] = op.perm[gq[r]];
+      // (rvalue alternatives throughout: `c ? mem : lvalue` would select pointers into scratch)
+      st[r] = ok[r] ? op.psi[(int64_t)b * op.ld + orig[r]] : make_double2(0.0, 0.0);
+      ph[r] = op.phase ? op.phase[orig[r]] : make_double2(1.0, 0.0);
+    }
+    int cidx[WMR][R];
+    double2 Bv[NSR][WMR][R];
+#pragma unroll
+    for (int j = 0; j < WMR; ++j)
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        cidx[j][r] = (WM > 0 && j < W) ? op.col[(int64_t)j * n + gq[r]] - lo : gq[r] - lo;
+#pragma unroll
+        for (int s = 0; s < NSR; ++s)
+          Bv[s][j][r] = (breg && j < W && s < op.nslot) ? op.val[((int64_t)s * W + j) * n + gq[r]]
+                                                       : make_double2(0.0, 0.0);
+      }
+
+    for (int step = 0; step < op.nsteps; ++step) {
+      double2 vk[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) vk[r] = op.phase ? cmul(st[r], ph[r]) : st[r];
+      if (op.sflag[step] == 0) {
+        // no field: the expv is skipped (tdse.py:377), phases only
+#pragma unroll
+        for (int r = 0; r < R; ++r) st[r] = op.phase ? cmul(vk[r], ph[r]) : vk[r];
+        if (l == 0 && op.kused) op.kused[(int64_t)step * op.kused_ld + b - op.kcol0] = 0;
+        continue;
+      }
+      const double2* fp = op.table + (int64_t)step * op.nslot;
+      // operator entries of my rows for this step
+      XT A[WMR][R];
+      if (WM > 0) {
+        double2 f0 = zero, f1 = zero;
+        if (breg) {
+          f0 = fp[0];
+          if (op.nslot > 1) f1 = fp[1];
+        }
+        // slots j >= W hold B = 0 (column = own row): no branch on W below
+#pragma unroll
+        for (int j = 0; j < WMR; ++j)
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            double2 v;
+            if (breg) {
+              v = cmul(f0, Bv[0][j][r]);
+              if (NSR > 1) v = cfma(f1, Bv[1][j][r], v);
+            } else {
+              v = make_double2(0.0, 0.0);
+              if (j < W)
+                for (int s = 0; s < op.nslot; ++s)
+                  v = cfma(fp[s], op.val[((int64_t)s * W + j) * n + gq[r]], v);
+            }
+            A[j][r] = to_x<F32>(v);
+          }
+      }
+
+      // y = A x for x in LDS (rows past the domain computed on a clamped row, then zeroed)
+      auto matvec = [&](double2 (&Y)[R]) __attribute__((always_inline)) {
+        const double2 z0 = make_double2(0.0, 0.0);   // a local: a captured constant would live in scratch
+        XT acc[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) acc[r] = to_x<F32>(z0);
+        if (WM > 0) {
+#pragma unroll
+          for (int j = 0; j < WMR; ++j)
+#pragma unroll
+            for (int r = 0; r < R; ++r) acc[r] = xfma(A[j][r], s_x[cidx[j][r]], acc[r]);
+        } else {
+          for (int j = 0; j < W; ++j) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+              const int64_t q = (int64_t)j * n + gq[r];
+              const int c = op.col[q] - lo;
+              double2 v = z0;
+              for (int s = 0; s < op.nslot; ++s) v = cfma(fp[s], op.val[(int64_t)s * W * n + q], v);
+              acc[r] = xfma(to_x<F32>(v), s_x[c], acc[r]);
+            }
+          }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) Y[r] = ok[r] ? to_d2(acc[r]) : z0;
+      };
+
+      // V_0 = phi (unnormalised, tdse.py:443-446).  Rows past the domain hold
+      // zeros and are written unguarded (no valid row gathers them).
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        V[l + 64 * r] = vk[r];
+        s_x[l + 64 * r] = to_x<F32>(vk[r]);
+      }
+      wave_sync();
+      double2 Wr[R], vp[R];
+      matvec(Wr);
+      double acc3[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const double2 w = Wr[r], v = vk[r];
+        acc3[0] += w.x * v.x + w.y * v.y;
+        acc3[1] += w.x * v.y - w.y * v.x;
+        acc3[2] += v.x * v.x + v.y * v.y;
+        vp[r] = zero;
+      }
+#pragma unroll
+      for (int q = 0; q < 3; ++q) acc3[q] = rsum(acc3[q]);
+      double2 al = make_double2(acc3[0], acc3[1]);
+#pragma unroll
+      for (int r = 0; r < R; ++r) Wr[r] = csub(Wr[r], cmul(al, vk[r]));
+      s_alpha[0] = al;   // uniform value: every lane stores it (no branch)
+      if (!FIXED && l == 0) {
+        G[0] = make_double2(acc3[2], 0.0);
+        cv[kc] = make_double2(1.0, 0.0);   // c_prev = [1]
+      }
+      int k = 1;
+      double conv = 1.0;
+      for (int it = 1;; ++it) {
+        const bool live = FIXED ? it < a.fixed_m : (it < a.maxorder && conv > a.tol);
+        if (!live) break;
+        // beta_it = |W| (tdse.py:453)
+        double bs = 0.0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) bs += Wr[r].x * Wr[r].x + Wr[r].y * Wr[r].y;
+        const double beta = sqrt(rsum(bs));
+        const bool brk = !(beta != 0.0);
+        if (!brk) {
+          // 1/beta: hardware reciprocal and two Newton steps (within an ulp of the quotient)
+          double ib = __builtin_amdgcn_rcp(beta);
+          ib = fma(fma(-beta, ib, 1.0), ib, ib);
+          ib = fma(fma(-beta, ib, 1.0), ib, ib);
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            vp[r] = vk[r];
+            vk[r] = cscale(ib, Wr[r]);
+          }
+        } else {
+          // exact breakdown: Gram-Schmidt of ones against the basis (tdse.py:459-465)
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            vp[r] = vk[r];
+            Wr[r] = ok[r] ? make_double2(1.0, 0.0) : zero;
+          }
+          for (int j = 0; j < it; ++j) {
+            const double2* Vj = V + (int64_t)j * NR;
+            double d0 = 0.0, d1 = 0.0;
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+              if (ok[r]) {
+                const double2 x = Vj[l + 64 * r];
+                d0 += x.x * Wr[r].x + x.y * Wr[r].y;
+                d1 += x.x * Wr[r].y - x.y * Wr[r].x;
+              }
+            const double2 proj = make_double2(rsum(d0), rsum(d1));
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+              if (ok[r]) Wr[r] = csub(Wr[r], cmul(proj, Vj[l + 64 * r]));
+          }
+          double nv = 0.0;
+#pragma unroll
+          for (int r = 0; r < R; ++r) nv += Wr[r].x * Wr[r].x + Wr[r].y * Wr[r].y;
+          nv = sqrt(rsum(nv));
+          // nv == 0: the basis already spans the domain, so the reference's
+          // restart vector lies entirely outside it.  Its part on the domain
+          // is zero; it enters the result with coefficient exactly zero
+          // (T is block diagonal after beta == 0), so zero stands for it.
+#pragma unroll
+          for (int r = 0; r < R; ++r)
+            vk[r] = nv > 0.0 ? make_double2(Wr[r].x / nv, Wr[r].y / nv) : zero;
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          V[(int64_t)it * NR + l + 64 * r] = vk[r];
+          s_x[l + 64 * r] = to_x<F32>(vk[r]);
+        }
+        s_beta[it] = beta;
+        wave_sync();
+        if (!FIXED && brk) {
+          // Gram column of the restart vector, explicit dots
+          for (int j = 0; j < it; ++j) {
+            const double2* Vj = V + (int64_t)j * NR;
+            double d0 = 0.0, d1 = 0.0;
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+              if (ok[r]) {
+                const double2 x = Vj[l + 64 * r];
+                d0 += x.x * vk[r].x + x.y * vk[r].y;
+                d1 += x.x * vk[r].y - x.y * vk[r].x;
+              }
+            d0 = rsum(d0);
+            d1 = rsum(d1);
+            if (l == 0) {
+              G[j * kc + it] = make_double2(d0, d1);
+              G[it * kc + j] = make_double2(d0, -d1);
+            }
+          }
+        }
+        // w = H V_it; alpha = vdot(w, V_it); W = w - alpha V_it - beta V_{it-1}
+        matvec(Wr);
+        double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          a0 += Wr[r].x * vk[r].x + Wr[r].y * vk[r].y;
+          a1 += Wr[r].x * vk[r].y - Wr[r].y * vk[r].x;
+        }
+        al = make_double2(rsum(a0), rsum(a1));
+#pragma unroll
+        for (int r = 0; r < R; ++r) Wr[r] = csub(csub(Wr[r], cmul(al, vk[r])), cscale(beta, vp[r]));
+        s_alpha[it] = al;
+        k = it + 1;
+        if (!FIXED) {
+          wave_sync();
+          conv = adaptive_metric<true>(it, brk, G, kc, s_alpha, s_beta, op.f, cv, cv + kc);
+        }
+      }
+
+      // coefficients of the final combination
+      wave_sync();
+      const double2* cf = s_cf;
+      if (FIXED) {
+        if (k > 1) expm_e0_any(k, s_alpha, s_beta, op.f, s_cf);
+        else if (l == 0) s_cf[0] = make_double2(1.0, 0.0);
+      } else if (k == 1) {
+        if (l == 0) s_cf[0] = make_double2(1.0, 0.0);
+      } else {
+        cf = cv;   // exp(f T) e0 of the last metric evaluation
+      }
+      __threadfence_block();
+      wave_sync();
+      double2 u[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) u[r] = zero;
+      for (int j = 0; j < k - 1; ++j) {
+        const double2 cj = cf[j];
+        const double2* Vj = V + (int64_t)j * NR;
+#pragma unroll
+        for (int r = 0; r < R; ++r) u[r] = cfma(cj, Vj[min(l + 64 * r, len - 1)], u[r]);
+      }
+      const double2 clast = cf[k - 1];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const double2 uu = cfma(clast, vk[r], u[r]);
+        st[r] = ok[r] ? (op.phase ? cmul(uu, ph[r]) : uu) : zero;
+      }
+      if (l == 0) {
+        if (op.kused) op.kused[(int64_t)step * op.kused_ld + b - op.kcol0] = k;
+        if (!FIXED && k >= a.maxorder && op.status) op.status[b - op.kcol0] = RM_NOT_CONVERGED;
+      }
+      // the next step rewrites s_cf / s_x / the basis only after this wave's reads above
+      wave_sync();
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      if (ok[r]) op.psi[(int64_t)b * op.ld + orig[r]] = st[r];
+  }
+}
+
+__global__ void k_sub_fill(int32_t* x, int64_t n, int32_t v0, int32_t v1) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = (i & 1) ? v1 : v0;
+}
+
+// Domain of every column: [min lo, max hi) over the components of its
+// non-zero rows; a non-finite entry makes the domain the whole basis.
+__global__ void k_col_domain(int64_t n, int64_t ld, const double2* __restrict__ psi,
+                             const int32_t* __restrict__ label, const int32_t* __restrict__ lo,
+                             const int32_t* __restrict__ hi, int32_t* __restrict__ dom) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t b = blockIdx.y;
+  if (i >= n) return;
+  const double2 v = psi[b * ld + i];
+  if (!(isfinite(v.x) && isfinite(v.y))) {
+    atomicMin(&dom[2 * b], 0);
+    atomicMax(&dom[2 * b + 1], (int32_t)n);
+  } else if (v.x != 0.0 || v.y != 0.0) {
+    const int32_t c = label[i];
+    atomicMin(&dom[2 * b], lo[c]);
+    atomicMax(&dom[2 * b + 1], hi[c]);
+  }
+}
+
+// sflag[st] = 1 if any MF element of step st is non-zero (or not a number):
+// the `len(H.mfmat) > 0` test of tdse.py:377 without thresholds.
+__global__ void k_step_flags(int64_t nterm, int64_t nsteps, int32_t nfield,
+                             const int64_t* __restrict__ term_ptr, const int32_t* __restrict__ term_field,
+                             const double2* __restrict__ term_mval, const double2* __restrict__ table,
+                             int32_t* __restrict__ sflag) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nterm) return;
+  for (int64_t st = blockIdx.y; st < nsteps; st += gridDim.y) {
+    const double2* fp = table + st * nfield;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int64_t q = term_ptr[t]; q < term_ptr[t + 1]; ++q) acc = cfma(fp[term_field[q]], term_mval[q], acc);
+    if (acc.x != 0.0 || acc.y != 0.0 || acc.x != acc.x || acc.y != acc.y) sflag[st] = 1;
+  }
+}
+
+inline double2 h_cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// Connected components of the sparsity graph (host union-find), the row
+// permutation that makes them contiguous, and the per-slot operator B_s in
+// permuted ELL order; uploaded once per plan.
+int sub_build(rm_plan* p) {
+  if (p->sub_ready) return RM_OK;
+  const int64_t n = p->n;
+  std::vector<int32_t> parent((size_t)n);
+  std::iota(parent.begin(), parent.end(), 0);
+  auto find = [&](int32_t x) {
+    while (parent[x] != x) {
+      parent[x] = parent[parent[x]];
+      x = parent[x];
+    }
+    return x;
+  };
+  for (int64_t i = 0; i < n; ++i)
+    for (int64_t e = p->h_row_ptr[i]; e < p->h_row_ptr[i + 1]; ++e) {
+      int32_t a = find((int32_t)i), b = find(p->h_col[e]);
+      if (a != b) parent[std::max(a, b)] = std::min(a, b);
+    }
+  std::vector<int32_t> label((size_t)n), cid((size_t)n, -1);
+  int32_t nc = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    const int32_t r = find((int32_t)i);
+    if (cid[r] < 0) cid[r] = nc++;
+    label[i] = cid[r];
+  }
+  std::vector<int32_t> lo((size_t)nc + 1, 0), hi((size_t)nc, 0), perm((size_t)n), iperm((size_t)n);
+  for (int64_t i = 0; i < n; ++i) lo[label[i] + 1]++;
+  for (int32_t c = 0; c < nc; ++c) lo[c + 1] += lo[c];
+  std::vector<int32_t> pos(lo.begin(), lo.end() - 1);
+  for (int64_t i = 0; i < n; ++i) {
+    perm[pos[label[i]]] = (int32_t)i;
+    iperm[i] = pos[label[i]]++;
+  }
+  int32_t mx = 0;
+  for (int32_t c = 0; c < nc; ++c) {
+    hi[c] = lo[c + 1];
+    mx = std::max(mx, hi[c] - lo[c]);
+  }
+  p->nsub = nc;
+  p->max_sub = mx;
+  // B_s per CSR entry: sum over pairs p of K_p * sum_{q in term_p, field_q = s} M_q
+  const int64_t nnz = p->nnz;
+  const int32_t nf = std::max(p->nfield, 1);
+  std::vector<double2> B((size_t)nf * (size_t)nnz, make_double2(0.0, 0.0));
+  for (int64_t e = 0; e < nnz; ++e)
+    for (int64_t q = p->h_pair_ptr[e]; q < p->h_pair_ptr[e + 1]; ++q) {
+      const int32_t t = p->h_pair_term[q];
+      const double2 kv = p->h_pair_kval[q];
+      for (int64_t m = p->h_term_ptr[t]; m < p->h_term_ptr[t + 1]; ++m) {
+        double2& d = B[(size_t)p->h_term_field[m] * nnz + e];
+        const double2 pr = h_cmul(kv, p->h_term_mval[m]);
+        d.x += pr.x;
+        d.y += pr.y;
+      }
+    }
+  const int32_t W = p->ell_w;
+  const size_t ne = (size_t)W * (size_t)n;
+  std::vector<int32_t> hcol(ne);
+  std::vector<double2> hval((size_t)nf * ne, make_double2(0.0, 0.0));
+  for (int64_t ip = 0; ip < n; ++ip) {
+    const int32_t i = perm[ip];
+    const int64_t r0 = p->h_row_ptr[i], len = p->h_row_ptr[i + 1] - r0;
+    for (int32_t j = 0; j < W; ++j) {
+      const size_t q = (size_t)j * n + ip;
+      if (j < len) {
+        hcol[q] = iperm[p->h_col[r0 + j]];
+        for (int32_t s = 0; s < nf; ++s) hval[(size_t)s * ne + q] = B[(size_t)s * nnz + r0 + j];
+      } else {
+        hcol[q] = (int32_t)ip;
+      }
+    }
+  }
+  p->h_sub_start.assign(lo.begin(), lo.end() - 1);
+  p->h_sub_end = hi;
+  p->h_sub_label_perm.resize((size_t)n);
+  for (int64_t ip = 0; ip < n; ++ip) p->h_sub_label_perm[ip] = label[perm[ip]];
+  int rc = RM_OK;
+  if ((rc = upload(&p->sub_perm, perm.data(), n)) || (rc = upload(&p->sub_label, label.data(), n)) ||
+      (rc = upload(&p->sub_lo, lo.data(), nc)) || (rc = upload(&p->sub_hi, hi.data(), nc)) ||
+      (rc = upload(&p->sub_col, hcol.data(), (int64_t)ne)) ||
+      (rc = upload(&p->sub_val, hval.data(), (int64_t)((size_t)nf * ne))))
+    return rc;
+  p->sub_w = W;
+  p->sub_ready = 1;
+  return RM_OK;
+}
+
+// Launch classes (one kernel instantiation each): rows per lane R and where
+// the operator entries live.  A wavepacket's class depends only on its own
+// domain and operator width, so its arithmetic -- and its result, bit for bit
+// -- does not depend on the other wavepackets of the launch.
+//   0: R=1, 1: R=2, 2: R=4 with the operator in registers (ELL width <= 4)
+//   3: R=4, 4: R=8, 5: R=16 reading the operator at every matvec
+constexpr int NCLASS = 6;
+constexpr int CLASS_R[NCLASS] = {1, 2, 4, 4, 8, 16};
+
+int sub_class(int w, int len) {
+  if (w <= 4) {
+    if (len <= 64) return 0;
+    if (len <= 128) return 1;
+    if (len <= 256) return 2;
+  } else if (len <= 256) {
+    return 3;
+  }
+  return len <= 512 ? 4 : 5;
+}
+
+// Krylov basis in LDS: register classes whose basis fits.
+bool class_basis_lds(int c, int kcap) {
+  return c <= 2 && (size_t)kcap * 64 * CLASS_R[c] * sizeof(double2) <= SUB_LDS_BASIS;
+}
+
+template <int R, int WM, bool FIXED, bool F32, bool BL>
+hipError_t launch_sub_k(const SubArgs& a0, int n_cu, hipStream_t s, int* waves_out, bool dry) {
+  auto kern = k_sub_prop<R, WM, FIXED, F32, BL>;
+  const size_t lds = (size_t)64 * R * (F32 ? sizeof(float2) : sizeof(double2)) +
+                     (BL ? (size_t)a0.kcap * 64 * R * sizeof(double2) : 0);
+  static bool attr = false;
+  if (!attr) {
+    hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)(64 * 1024));
+    if (e != hipSuccess) return e;
+    attr = true;
+  }
+  int occ = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)kern, 64, lds);
+  if (e != hipSuccess) return e;
+  if (occ < 1) occ = 1;
+  int waves = std::min<int64_t>(a0.nitems, (int64_t)occ * n_cu);
+  if (waves < 1) waves = 1;
+  *waves_out = waves;
+  if (dry) return hipSuccess;
+  hipLaunchKernelGGL(kern, dim3(waves), dim3(64), lds, s, a0);
+  return hipGetLastError();
+}
+
+template <int R, int WM, bool FIXED, bool F32>
+hipError_t launch_sub_b(int c, const SubArgs& a, int n_cu, hipStream_t s, int* waves, bool dry) {
+  if (class_basis_lds(c, a.kcap)) return launch_sub_k<R, WM, FIXED, F32, true>(a, n_cu, s, waves, dry);
+  return launch_sub_k<R, WM, FIXED, F32, false>(a, n_cu, s, waves, dry);
+}
+
+template <bool FIXED, bool F32>
+hipError_t launch_class_p(int c, const SubArgs& a, int n_cu, hipStream_t s, int* waves, bool dry) {
+  switch (c) {
+    case 0: return launch_sub_b<1, 4, FIXED, F32>(c, a, n_cu, s, waves, dry);
+    case 1: return launch_sub_b<2, 4, FIXED, F32>(c, a, n_cu, s, waves, dry);
+    case 2: return launch_sub_b<4, 4, FIXED, F32>(c, a, n_cu, s, waves, dry);
+    case 3: return launch_sub_k<4, 0, FIXED, F32, false>(a, n_cu, s, waves, dry);
+    case 4: return launch_sub_k<8, 0, FIXED, F32, false>(a, n_cu, s, waves, dry);
+    default: return launch_sub_k<16, 0, FIXED, F32, false>(a, n_cu, s, waves, dry);
+  }
+}
+
+hipError_t launch_class(int c, const SubArgs& a, bool f32, int n_cu, hipStream_t s, int* waves, bool dry) {
+  const bool fixed = a.fixed_m > 0;
+  if (fixed && f32) return launch_class_p<true, true>(c, a, n_cu, s, waves, dry);
+  if (fixed) return launch_class_p<true, false>(c, a, n_cu, s, waves, dry);
+  if (f32) return launch_class_p<false, true>(c, a, n_cu, s, waves, dry);
+  return launch_class_p<false, false>(c, a, n_cu, s, waves, dry);
+}
+
+// Job descriptors and work items staged to the device through two pinned
+// buffers used in turn; a buffer is refilled only after the launches that
+// read it have completed (its event), so back-to-back calls never overwrite
+// descriptors in use.  ws: the Krylov workspace of the waves.
+struct SubScratch {
+  char* dev[2] = {nullptr, nullptr};
+  char* host[2] = {nullptr, nullptr};
+  size_t cap[2] = {0, 0};
+  hipEvent_t done[2] = {nullptr, nullptr};
+  int cur = 0;
+  void* ws = nullptr;
+  size_t ws_bytes = 0;
+};
+
+void scratch_release(SubScratch* sc) {
+  if (!sc) return;
+  for (int i = 0; i < 2; ++i) {
+    if (sc->done[i]) (void)hipEventSynchronize(sc->done[i]);
+    if (sc->dev[i]) (void)hipFree(sc->dev[i]);
+    if (sc->host[i]) (void)hipHostFree(sc->host[i]);
+    if (sc->done[i]) (void)hipEventDestroy(sc->done[i]);
+  }
+  if (sc->ws) (void)hipFree(sc->ws);
+  delete sc;
+}
+
+// Stage ops and items (items grouped by class) and launch one kernel per
+// non-empty class on s; with timing on, the launches are bracketed as one.
+int sub_run(SubScratch* sc, const std::vector<SubOp>& ops, std::vector<int4> (&cls)[NCLASS],
+            int32_t kcap, int32_t fixed_m, int32_t maxorder, double tol, bool f32, int n_cu,
+            hipStream_t s, rm_plan* timed) {
+  size_t nitems = 0;
+  for (int c = 0; c < NCLASS; ++c) nitems += cls[c].size();
+  if (nitems == 0) return RM_OK;
+  const size_t ops_bytes = (ops.size() * sizeof(SubOp) + 255) & ~(size_t)255;
+  const size_t bytes = ops_bytes + nitems * sizeof(int4);
+  const int bi = sc->cur;
+  sc->cur ^= 1;
+  if (sc->done[bi]) HIPCHK(hipEventSynchronize(sc->done[bi]));
+  if (sc->cap[bi] < bytes) {
+    if (sc->dev[bi]) HIPCHK(hipFree(sc->dev[bi]));
+    if (sc->host[bi]) HIPCHK(hipHostFree(sc->host[bi]));
+    sc->dev[bi] = nullptr;
+    sc->host[bi] = nullptr;
+    sc->cap[bi] = 0;
+    const size_t cap = bytes * 2;
+    HIPCHK(hipMalloc((void**)&sc->dev[bi], cap));
+    HIPCHK(hipHostMalloc((void**)&sc->host[bi], cap, hipHostMallocDefault));
+    sc->cap[bi] = cap;
+  }
+  memcpy(sc->host[bi], ops.data(), ops.size() * sizeof(SubOp));
+  size_t off[NCLASS];
+  size_t pos = ops_bytes;
+  for (int c = 0; c < NCLASS; ++c) {
+    off[c] = pos;
+    if (!cls[c].empty()) memcpy(sc->host[bi] + pos, cls[c].data(), cls[c].size() * sizeof(int4));
+    pos += cls[c].size() * sizeof(int4);
+  }
+  HIPCHK(hipMemcpyAsync(sc->dev[bi], sc->host[bi], bytes, hipMemcpyHostToDevice, s));
+
+  SubArgs a;
+  memset(&a, 0, sizeof(a));
+  a.ops = (const SubOp*)sc->dev[bi];
+  a.kcap = kcap;
+  a.fixed_m = fixed_m;
+  a.maxorder = maxorder;
+  a.tol = tol;
+  size_t need = 0;
+  for (int c = 0; c < NCLASS; ++c) {
+    if (cls[c].empty()) continue;
+    a.nitems = (int32_t)cls[c].size();
+    int waves = 0;
+    HIPCHK(launch_class(c, a, f32, n_cu, s, &waves, true));
+    const size_t per_wave = (class_basis_lds(c, kcap) ? 0 : (size_t)kcap * 64 * CLASS_R[c]) +
+                            (fixed_m > 0 ? 0 : (size_t)kcap * kcap + 2 * (size_t)kcap);
+    need = std::max(need, sizeof(double2) * std::max<size_t>(per_wave, 1) * (size_t)waves);
+  }
+  if (sc->ws_bytes < need) {
+    if (sc->ws) {
+      HIPCHK(hipDeviceSynchronize());
+      HIPCHK(hipFree(sc->ws));
+      sc->ws = nullptr;
+      sc->ws_bytes = 0;
+    }
+    HIPCHK(hipMalloc(&sc->ws, need));
+    sc->ws_bytes = need;
+  }
+  if (timed) HIPCHK(timing_begin(timed, s));
+  for (int c = 0; c < NCLASS; ++c) {
+    if (cls[c].empty()) continue;
+    a.items = (const int4*)(sc->dev[bi] + off[c]);
+    a.nitems = (int32_t)cls[c].size();
+    int waves = 0;
+    HIPCHK(launch_class(c, a, f32, n_cu, s, &waves, true));
+    const bool bl = class_basis_lds(c, kcap);
+    a.basis = (double2*)sc->ws;
+    a.gram = a.basis + (bl ? 0 : (size_t)waves * kcap * 64 * CLASS_R[c]);
+    a.cscr = a.gram + (fixed_m > 0 ? 0 : (size_t)waves * kcap * kcap);
+    HIPCHK(launch_class(c, a, f32, n_cu, s, &waves, false));
+  }
+  if (timed) HIPCHK(timing_end(timed, s));
+  if (!sc->done[bi]) HIPCHK(hipEventCreateWithFlags(&sc->done[bi], hipEventDisableTiming));
+  HIPCHK(hipEventRecord(sc->done[bi], s));
+  return RM_OK;
+}
+
+// Domains [lo, hi) of ncols columns into p->h_dom (synchronises s).
+int sub_domains(rm_plan* p, const void* psi, int64_t ld, int64_t ncols, hipStream_t s) {
+  const int64_t n = p->n;
+  int rc = grow(&p->dom, &p->dom_cap, 2 * ncols);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_sub_fill, dim3(grid_for(2 * ncols, 256)), dim3(256), 0, s, p->dom, 2 * ncols,
+                     (int32_t)n, 0);
+  hipLaunchKernelGGL(k_col_domain, dim3(grid_for(n, 256), (unsigned)ncols), dim3(256), 0, s, n, ld,
+                     (const double2*)psi, p->sub_label, p->sub_lo, p->sub_hi, p->dom);
+  HIPCHK(hipGetLastError());
+  p->h_dom.resize((size_t)(2 * ncols));
+  HIPCHK(hipMemcpyAsync(p->h_dom.data(), p->dom, sizeof(int32_t) * 2 * ncols, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return RM_OK;
+}
+
+void step_flags(rm_plan* p, int64_t nsteps, const double* fprod_dev, int32_t* sflag, hipStream_t s) {
+  hipLaunchKernelGGL(k_sub_fill, dim3(grid_for(nsteps, 256)), dim3(256), 0, s, sflag, nsteps, 0, 0);
+  if (p->nterm > 0)
+    hipLaunchKernelGGL(k_step_flags, dim3(grid_for(p->nterm, 256), (unsigned)std::min<int64_t>(nsteps, 65535)),
+                       dim3(256), 0, s, p->nterm, nsteps, p->nfield, p->term_ptr, p->term_field,
+                       p->term_mval, (const double2*)fprod_dev, sflag);
+}
+
+SubOp plan_op(const rm_plan* p) {
+  SubOp op;
+  memset(&op, 0, sizeof(op));
+  op.col = p->sub_col;
+  op.val = p->sub_val;
+  op.perm = p->sub_perm;
+  op.n = (int32_t)p->n;
+  op.w = p->sub_w;
+  op.nslot = p->nfield;
+  return op;
+}
+
+}  // namespace
+
+void sub_scratch_release(void* sc) { scratch_release((SubScratch*)sc); }
+
+extern "C" int rm_plan_subspaces(rm_plan* p, int64_t* count, int64_t* max_rows) {
+  if (!p) return fail(RM_ERR_ARG, "rm_plan_subspaces: NULL plan");
+  int rc = sub_build(p);
+  if (rc) return rc;
+  if (count) *count = p->nsub;
+  if (max_rows) *max_rows = p->max_sub;
+  return RM_OK;
+}
+
+extern "C" int rm_plan_set_precision(rm_plan* p, int32_t precision) {
+  if (!p) return fail(RM_ERR_ARG, "rm_plan_set_precision: NULL plan");
+  if (precision != RM_PREC_FP64 && precision != RM_PREC_FP32)
+    return fail(RM_ERR_ARG, "rm_plan_set_precision: unknown precision %d", precision);
+  p->precision = precision;
+  return RM_OK;
+}
+
+// The subspace path of rm_propagate.  Arguments are validated by the caller
+// (ld, ncols, Krylov parameters, pointers).  *done = false when the path does
+// not apply to these wavepackets (a domain longer than RM_SUBSPACE_MAX_ROWS);
+// nothing has then been written to psi.
+int sub_propagate(rm_plan* p, void* psi, int64_t ld, int64_t ncols, int64_t nsteps,
+                  const double* fprod_dev, double f_re, double f_im, const void* phase,
+                  int32_t krylov_fixed, int32_t maxorder, double tol, int32_t* kused,
+                  int32_t* status, hipStream_t s, bool* done) {
+  *done = false;
+  if (nsteps > INT32_MAX) return RM_OK;
+  int rc = sub_build(p);
+  if (rc) return rc;
+  if ((rc = sub_domains(p, psi, ld, ncols, s))) return rc;
+  std::vector<int4> cls[NCLASS];
+  for (int64_t b = 0; b < ncols; ++b) {
+    const int32_t lo = p->h_dom[2 * b], hi = p->h_dom[2 * b + 1];
+    if (hi <= lo) continue;   // zero column: stays zero (kused 0)
+    if (hi - lo > RM_SUBSPACE_MAX_ROWS) return RM_OK;
+    cls[sub_class(p->sub_w, hi - lo)].push_back(make_int4((int)b, lo, hi - lo, 0));
+  }
+  for (auto& v : cls)
+    std::stable_sort(v.begin(), v.end(), [](const int4& x, const int4& y) { return x.z > y.z; });
+  if ((rc = grow(&p->sflag, &p->sflag_cap, nsteps))) return rc;
+  step_flags(p, nsteps, fprod_dev, p->sflag, s);
+  if (kused) {
+    const int64_t nk = nsteps * ncols;
+    hipLaunchKernelGGL(k_sub_fill, dim3(grid_for(nk, 256)), dim3(256), 0, s, kused, nk, 0, 0);
+  }
+  if (status) hipLaunchKernelGGL(k_sub_fill, dim3(grid_for(ncols, 256)), dim3(256), 0, s, status, ncols, 0, 0);
+  HIPCHK(hipGetLastError());
+  *done = true;
+
+  std::vector<SubOp> ops(1, plan_op(p));
+  SubOp& op = ops[0];
+  op.phase = (const double2*)phase;
+  op.table = (const double2*)fprod_dev;
+  op.sflag = p->sflag;
+  op.psi = (double2*)psi;
+  op.ld = ld;
+  op.kused = kused;
+  op.kused_ld = ncols;
+  op.status = status;
+  op.f = make_double2(f_re, f_im);
+  op.nsteps = (int32_t)nsteps;
+  if (!p->sub_scratch) p->sub_scratch = new (std::nothrow) SubScratch();
+  if (!p->sub_scratch) return fail(RM_ERR_ALLOC, "rm_propagate: host allocation");
+  return sub_run((SubScratch*)p->sub_scratch, ops, cls, krylov_fixed > 0 ? krylov_fixed : maxorder,
+                 krylov_fixed, maxorder, tol, p->precision == RM_PREC_FP32, p->n_cu, s, p);
+}
+
+// ---------------------------------------------------------------------------
+// batches of independent jobs (continuous batching)
+
+struct rm_batch {
+  SubScratch* sc = nullptr;
+  int device = 0;
+  int n_cu = 256;
+};
+
+extern "C" int rm_batch_create(rm_batch** out) {
+  if (!out) return fail(RM_ERR_ARG, "rm_batch_create: out is NULL");
+  *out = nullptr;
+  rm_batch* b = new (std::nothrow) rm_batch();
+  if (!b) return fail(RM_ERR_ALLOC, "rm_batch_create: host allocation");
+  b->sc = new (std::nothrow) SubScratch();
+  if (!b->sc) {
+    delete b;
+    return fail(RM_ERR_ALLOC, "rm_batch_create: host allocation");
+  }
+  if (hipGetDevice(&b->device) != hipSuccess ||
+      hipDeviceGetAttribute(&b->n_cu, hipDeviceAttributeMultiprocessorCount, b->device) != hipSuccess ||
+      b->n_cu <= 0)
+    b->n_cu = 256;
+  *out = b;
+  return RM_OK;
+}
+
+extern "C" int rm_batch_destroy(rm_batch* b) {
+  if (!b) return RM_OK;
+  scratch_release(b->sc);
+  delete b;
+  return RM_OK;
+}
+
+extern "C" int rm_plan_domains(rm_plan* p, const void* psi, int64_t ld, int64_t ncols, int32_t* lohi,
+                               void* stream) {
+  if (!p) return fail(RM_ERR_ARG, "rm_plan_domains: NULL plan");
+  if (ncols < 0 || ncols > 65535 || ld < p->n) return fail(RM_ERR_ARG, "rm_plan_domains: bad ld/ncols");
+  if (ncols == 0 || p->n == 0) return RM_OK;
+  if (!psi || !lohi) return fail(RM_ERR_ARG, "rm_plan_domains: NULL pointer");
+  int rc = sub_build(p);
+  if (rc) return rc;
+  if ((rc = sub_domains(p, psi, ld, ncols, (hipStream_t)stream))) return rc;
+  memcpy(lohi, p->h_dom.data(), sizeof(int32_t) * 2 * ncols);
+  return RM_OK;
+}
+
+extern "C" int rm_plan_step_flags(rm_plan* p, int64_t nsteps, const double* fprod_dev, int32_t* sflag,
+                                  void* stream) {
+  if (!p) return fail(RM_ERR_ARG, "rm_plan_step_flags: NULL plan");
+  if (nsteps < 0) return fail(RM_ERR_ARG, "rm_plan_step_flags: nsteps < 0");
+  if (nsteps == 0) return RM_OK;
+  if (!sflag || (p->nfield > 0 && !fprod_dev)) return fail(RM_ERR_ARG, "rm_plan_step_flags: NULL pointer");
+  step_flags(p, nsteps, fprod_dev, sflag, (hipStream_t)stream);
+  HIPCHK(hipGetLastError());
+  return RM_OK;
+}
+
+extern "C" int rm_batch_propagate(rm_batch* bt, const rm_job* jobs, int32_t njobs, int32_t job_bytes,
+                                  int32_t krylov_fixed, int32_t maxorder, double tol, int32_t precision,
+                                  void* stream) {
+  if (!bt) return fail(RM_ERR_ARG, "rm_batch_propagate: NULL batch");
+  if (job_bytes != (int32_t)sizeof(rm_job))
+    return fail(RM_ERR_ARG, "rm_batch_propagate: job records of %d bytes, the ABI's rm_job has %d",
+                job_bytes, (int)sizeof(rm_job));
+  if (njobs < 0 || (njobs > 0 && !jobs)) return fail(RM_ERR_ARG, "rm_batch_propagate: bad job list");
+  if (krylov_fixed < 0 || krylov_fixed > RM_MAX_KRYLOV)
+    return fail(RM_ERR_ARG, "rm_batch_propagate: krylov_fixed=%d outside [0, %d]", krylov_fixed, RM_MAX_KRYLOV);
+  if (krylov_fixed == 0 && (maxorder < 1 || maxorder > RM_MAX_KRYLOV))
+    return fail(RM_ERR_ARG, "rm_batch_propagate: maxorder=%d outside [1, %d]", maxorder, RM_MAX_KRYLOV);
+  if (krylov_fixed == 0 && !(tol > 0.0 && tol <= 1.0)) return fail(RM_ERR_ARG, "rm_batch_propagate: tol must be in (0, 1]");
+  if (precision != RM_PREC_FP64 && precision != RM_PREC_FP32)
+    return fail(RM_ERR_ARG, "rm_batch_propagate: unknown precision %d", precision);
+  std::vector<SubOp> ops;
+  ops.reserve((size_t)njobs);
+  std::vector<int4> cls[NCLASS];
+  std::vector<int64_t> work[NCLASS];
+  for (int32_t j = 0; j < njobs; ++j) {
+    const rm_job& jb = jobs[j];
+    rm_plan* p = jb.plan;
+    if (!p) return fail(RM_ERR_ARG, "rm_batch_propagate: job %d has no plan", j);
+    if (jb.nsteps < 0) return fail(RM_ERR_ARG, "rm_batch_propagate: job %d nsteps < 0", j);
+    if (jb.nsteps == 0) continue;
+    int rc = sub_build(p);
+    if (rc) return rc;
+    if (!jb.psi || jb.ld < p->n || jb.col < 0 || !jb.sflag_dev || (p->nfield > 0 && !jb.fprod_dev))
+      return fail(RM_ERR_ARG, "rm_batch_propagate: job %d: bad wavepacket, field table or flags", j);
+    // the domain must be a union of whole subspaces, or gathers would leave it
+    const int32_t lo = jb.lo, len = jb.len;
+    if (len < 1 || len > RM_SUBSPACE_MAX_ROWS || lo < 0 || (int64_t)lo + len > p->n ||
+        p->h_sub_start[p->h_sub_label_perm[lo]] != lo ||
+        p->h_sub_end[p->h_sub_label_perm[lo + len - 1]] != lo + len)
+      return fail(RM_ERR_ARG, "rm_batch_propagate: job %d: [%d, %d) is not a domain of %d rows at most "
+                  "(use rm_plan_domains)", j, lo, lo + len, RM_SUBSPACE_MAX_ROWS);
+    SubOp op = plan_op(p);
+    op.phase = (const double2*)jb.phase;
+    op.table = (const double2*)jb.fprod_dev;
+    op.sflag = jb.sflag_dev;
+    op.psi = (double2*)jb.psi;
+    op.ld = jb.ld;
+    op.kused = jb.kused;
+    op.kused_ld = 1;
+    op.kcol0 = jb.col;
+    op.status = jb.status;
+    op.f = make_double2(jb.f_re, jb.f_im);
+    op.nsteps = jb.nsteps;
+    const int c = sub_class(p->sub_w, len);
+    cls[c].push_back(make_int4(jb.col, lo, len, (int)ops.size()));
+    work[c].push_back((int64_t)jb.nsteps * len);
+    ops.push_back(op);
+  }
+  // longest jobs first within a class (the waves take items in order)
+  for (int c = 0; c < NCLASS; ++c) {
+    std::vector<size_t> ix(cls[c].size());
+    std::iota(ix.begin(), ix.end(), 0);
+    std::stable_sort(ix.begin(), ix.end(), [&](size_t x, size_t y) { return work[c][x] > work[c][y]; });
+    std::vector<int4> sorted(ix.size());
+    for (size_t t = 0; t < ix.size(); ++t) sorted[t] = cls[c][ix[t]];
+    cls[c].swap(sorted);
+  }
+  return sub_run(bt->sc, ops, cls, krylov_fixed > 0 ? krylov_fixed : maxorder, krylov_fixed, maxorder,
+                 tol, precision == RM_PREC_FP32, bt->n_cu, (hipStream_t)stream, nullptr);
+}

@@ -11,25 +11,16 @@
 // waves in index order) so a column's result does not depend on how many
 // other columns share the launch.
 
-#include <hip/hip_runtime.h>
-
-#include <cmath>
-#include <cstdarg>
-#include <cstdint>
-#include <cstdio>
-#include <cstdlib>
-#include <cstring>
 #include <string>
-#include <vector>
 
-#include "../../include/richmol_amd.h"
+#include "rm_common.h"
 
 // ---------------------------------------------------------------------------
 // errors
 
 static thread_local char g_err[512] = "";
 
-static int fail(int code, const char* fmt, ...) {
+int fail(int code, const char* fmt, ...) {
   va_list ap;
   va_start(ap, fmt);
   vsnprintf(g_err, sizeof(g_err), fmt, ap);
@@ -37,86 +28,18 @@ static int fail(int code, const char* fmt, ...) {
   return code;
 }
 
-#define HIPCHK(expr)                                                        \
-  do {                                                                      \
-    hipError_t e_ = (expr);                                                 \
-    if (e_ != hipSuccess)                                                   \
-      return fail(RM_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
-  } while (0)
-
 extern "C" const char* rm_last_error(void) { return g_err; }
-extern "C" int rm_version(void) { return 1; }
-
-// ---------------------------------------------------------------------------
-// complex helpers (double2 = re, im)
-
-__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
-  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-__device__ __forceinline__ double2 cadd(double2 a, double2 b) {
-  return make_double2(a.x + b.x, a.y + b.y);
-}
-__device__ __forceinline__ double2 csub(double2 a, double2 b) {
-  return make_double2(a.x - b.x, a.y - b.y);
-}
-__device__ __forceinline__ double2 cscale(double s, double2 a) {
-  return make_double2(s * a.x, s * a.y);
-}
-__device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
-// acc + a*b
-__device__ __forceinline__ double2 cfma(double2 a, double2 b, double2 acc) {
-  return make_double2(acc.x + (a.x * b.x - a.y * b.y), acc.y + (a.x * b.y + a.y * b.x));
-}
+extern "C" int rm_version(void) { return RM_ABI_VERSION; }
 
 // ---------------------------------------------------------------------------
 // plan
 
-struct rm_plan {
-  int64_t n = 0, nnz = 0, npair = 0, nterm = 0, nq = 0;
-  int32_t nfield = 0, nleaf = 0;
-  int device = 0;
-  int64_t* row_ptr = nullptr;
-  int32_t* col = nullptr;
-  int64_t* pair_ptr = nullptr;
-  int32_t* pair_term = nullptr;
-  double2* pair_kval = nullptr;
-  int64_t* term_ptr = nullptr;
-  int32_t* term_field = nullptr;
-  double2* term_mval = nullptr;
-  int32_t* term_leaf = nullptr;
-  double2* mf = nullptr;     // nterm
-  double2* aval = nullptr;   // nnz
-  int32_t* flag = nullptr;   // id of the last field step with a non-zero MF
-  int32_t step_id = 0;
-  void* ws = nullptr;        // Krylov workspace
-  size_t ws_bytes = 0;
-  int32_t variant = RM_KERNEL_AUTO;
-  // ELL copy of the matrix: width = max entries per row, column-major
-  // (entry j of row i at j*n + i), padding entries (value 0, column i)
-  int32_t ell_w = 0;
-  int32_t* ell_col = nullptr;
-  double2* ell_val = nullptr;
-  int32_t* ell_pos = nullptr;  // CSR entry -> ELL slot
-  int n_cu = 0;
-  // optional HIP-event timing of the Krylov step launches
-  int timing = 0;
-  int nrec = 0;
-  std::vector<hipEvent_t> ev0, ev1;
-};
-
-template <typename T>
-static int upload(T** dst, const T* src, int64_t count) {
-  *dst = nullptr;
-  size_t bytes = sizeof(T) * (size_t)(count > 0 ? count : 1);
-  HIPCHK(hipMalloc((void**)dst, bytes));
-  if (count > 0 && src) HIPCHK(hipMemcpy(*dst, src, sizeof(T) * (size_t)count, hipMemcpyHostToDevice));
-  return RM_OK;
-}
-
 static void plan_free(rm_plan* p) {
   void* bufs[] = {p->row_ptr, p->col, p->pair_ptr, p->pair_term, p->pair_kval, p->term_ptr,
                   p->term_field, p->term_mval, p->term_leaf, p->mf, p->aval, p->flag, p->ws,
-                  p->ell_col, p->ell_val, p->ell_pos};
+                  p->ell_col, p->ell_val, p->ell_pos, p->sub_perm, p->sub_label, p->sub_lo,
+                  p->sub_hi, p->sub_col, p->sub_val, p->dom, p->sflag};
+  sub_scratch_release(p->sub_scratch);
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (hipEvent_t e : p->ev0) (void)hipEventDestroy(e);
@@ -165,6 +88,14 @@ extern "C" int rm_plan_create(int64_t n, int64_t nnz, const int64_t* row_ptr, co
   if (!p) return fail(RM_ERR_ALLOC, "rm_plan_create: host allocation");
   p->n = n; p->nnz = nnz; p->npair = npair; p->nterm = nterm; p->nq = nq;
   p->nfield = nfield; p->nleaf = nleaf;
+  p->h_row_ptr.assign(row_ptr, row_ptr + n + 1);
+  p->h_col.assign(col_idx, col_idx + nnz);
+  p->h_pair_ptr.assign(pair_ptr, pair_ptr + (nnz > 0 ? nnz + 1 : 0));
+  p->h_pair_term.assign(pair_term, pair_term + npair);
+  p->h_pair_kval.assign((const double2*)pair_kval, (const double2*)pair_kval + npair);
+  p->h_term_ptr.assign(term_ptr, term_ptr + (nterm > 0 ? nterm + 1 : 0));
+  p->h_term_field.assign(term_field, term_field + nq);
+  p->h_term_mval.assign((const double2*)term_mval, (const double2*)term_mval + nq);
   if (hipGetDevice(&p->device) != hipSuccess) { plan_free(p); return fail(RM_ERR_HIP, "hipGetDevice failed"); }
   int rc = RM_OK;
   if ((rc = upload(&p->row_ptr, row_ptr, n + 1)) ||
@@ -324,8 +255,6 @@ __global__ void k_field_assemble(int64_t nnz, const int64_t* __restrict__ pair_p
   if (nz) *flag = id;
 }
 
-static inline unsigned grid_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
-
 // The field contraction of one step (fdev: device row of field products or
 // NULL for fa's copy): fused when every term is used about once.
 static void launch_field(rm_plan* p, const FieldArgs& fa, const double2* fdev, hipStream_t s) {
@@ -451,8 +380,6 @@ namespace {
 
 constexpr int NT = 256;
 constexpr int NWAVE = NT / 64;
-constexpr int KMAX = RM_MAX_KRYLOV;
-constexpr int TAYLOR_TERMS = 18;   // |h f T|_1 <= 1/2: truncation < 0.5^19/19! ~ 1.6e-23
 
 struct StepArgs {
   int64_t n, ld;
@@ -486,32 +413,6 @@ struct StepArgs {
 // maxorder; in sticky mode a success does not overwrite an earlier failure.
 __device__ __forceinline__ void put_status(const StepArgs& a, int b, bool bad) {
   if (a.status && (bad || !a.status_sticky)) a.status[b] = bad ? RM_NOT_CONVERGED : 0;
-}
-
-// DPP row rotation of a double (both 32-bit halves); CTRL is the DPP control.
-template <int CTRL>
-__device__ __forceinline__ double dpp_mov(double v) {
-  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xf, 0xf, false);
-  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xf, 0xf, false);
-  return __hiloint2double(hi, lo);
-}
-
-__device__ __forceinline__ double readlane_d(double v, int lane) {
-  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
-  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
-  return __hiloint2double(hi, lo);
-}
-
-// Sum over the 64 lanes of a wave (all lanes active), bitwise identical on
-// every lane: rotations by 8, 4, 2, 1 inside each 16-lane row pair every lane
-// with partners whose partial sums are the same two operands (addition is
-// commutative), then the four row sums are combined in a fixed order.
-__device__ __forceinline__ double wave_allsum(double v) {
-  v += dpp_mov<0x128>(v);   // row_ror:8
-  v += dpp_mov<0x124>(v);   // row_ror:4
-  v += dpp_mov<0x122>(v);   // row_ror:2
-  v += dpp_mov<0x121>(v);   // row_ror:1
-  return (readlane_d(v, 0) + readlane_d(v, 16)) + (readlane_d(v, 32) + readlane_d(v, 48));
 }
 
 // Workgroup sum of NV doubles with DPP wave sums (red: [2][NW][16]).
@@ -556,132 +457,6 @@ __device__ __forceinline__ void block_sum(double* v, double (*red)[NW][4], int& 
     v[i] = s;
   }
   buf ^= 1;
-}
-
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
-  return v;
-}
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-  return v;
-}
-__device__ __forceinline__ double2 shfl2(double2 v, int src) {
-  return make_double2(__shfl(v.x, src), __shfl(v.y, src));
-}
-
-// c = exp(f*T) e_0 for the s x s tridiagonal T (diag alpha[0..s-1], off-diagonal
-// beta[1..s-1], T[i-1,i] = T[i,i-1] = beta[i]).  Called by the 64 lanes of wave 0;
-// lane l owns rows l and l+64 (s <= 128).  Scaling and stepping: exp(A)e0 =
-// (exp(A/nsub))^nsub e0 with |A/nsub|_1 <= 1/2, each factor a Taylor sum.
-__device__ void expm_e0(int s, const double2* alpha, const double* beta, double2 f, double2* c) {
-  const int l = threadIdx.x & 63;
-  double2 a[2], bl[2], bu[2];   // A_ii, A_{i,i-1}, A_{i,i+1}
-  double col1 = 0.0;
-#pragma unroll
-  for (int r = 0; r < 2; ++r) {
-    int i = l + 64 * r;
-    a[r] = make_double2(0.0, 0.0);
-    bl[r] = make_double2(0.0, 0.0);
-    bu[r] = make_double2(0.0, 0.0);
-    if (i < s) {
-      a[r] = cmul(f, alpha[i]);
-      if (i >= 1) bl[r] = cscale(beta[i], f);
-      if (i + 1 < s) bu[r] = cscale(beta[i + 1], f);
-      // column i of |A|: |A_ii| + |A_{i-1,i}| + |A_{i+1,i}|, A symmetric off-diagonal
-      col1 = fmax(col1, hypot(a[r].x, a[r].y) + hypot(bl[r].x, bl[r].y) + hypot(bu[r].x, bu[r].y));
-    }
-  }
-  double norm1 = wave_max(col1);
-  int nsub = 1;
-  if (norm1 > 0.5) nsub = (int)ceil(norm1 / 0.5);
-  if (!(nsub >= 1) || nsub > (1 << 20)) nsub = 1;   // NaN / overflow: result becomes NaN anyway
-  const double h = 1.0 / (double)nsub;
-  double2 y[2];
-  y[0] = make_double2(l == 0 ? 1.0 : 0.0, 0.0);
-  y[1] = make_double2(0.0, 0.0);
-  for (int sub = 0; sub < nsub; ++sub) {
-    double2 term[2] = {y[0], y[1]};
-    double2 acc[2] = {y[0], y[1]};
-    for (int j = 1; j <= TAYLOR_TERMS; ++j) {
-      // neighbours: row i-1 and i+1
-      double2 t0_dn = shfl2(term[0], (l + 63) & 63);   // lane l-1, row l-1 (wraps at l=0)
-      double2 t1_dn = shfl2(term[1], (l + 63) & 63);
-      double2 t0_up = shfl2(term[0], (l + 1) & 63);
-      double2 t1_up = shfl2(term[1], (l + 1) & 63);
-      double2 prev0 = (l == 0) ? make_double2(0.0, 0.0) : t0_dn;
-      double2 prev1 = (l == 0) ? t0_dn /* lane 63 row 63 */ : t1_dn;
-      double2 next0 = (l == 63) ? t1_up /* lane 0 row 64 */ : t0_up;
-      double2 next1 = (l == 63) ? make_double2(0.0, 0.0) : t1_up;
-      const double sc = h / (double)j;
-      double2 n0 = cmul(a[0], term[0]);
-      n0 = cfma(bl[0], prev0, n0);
-      n0 = cfma(bu[0], next0, n0);
-      double2 n1 = cmul(a[1], term[1]);
-      n1 = cfma(bl[1], prev1, n1);
-      n1 = cfma(bu[1], next1, n1);
-      term[0] = cscale(sc, n0);
-      term[1] = cscale(sc, n1);
-      acc[0] = cadd(acc[0], term[0]);
-      acc[1] = cadd(acc[1], term[1]);
-    }
-    y[0] = acc[0];
-    y[1] = acc[1];
-  }
-  if (l < s) c[l] = y[0];
-  if (l + 64 < s) c[l + 64] = y[1];
-}
-
-// Adaptive stopping metric after iteration k (wave 0 only; tdse.py:473-476).
-// Gram column k of the Krylov basis from the three-term relation
-//   <V_i|V_k> = (<H V_i|V_{k-1}> - a_{k-1} G_{i,k-1} - b_{k-1} G_{i,k-2}) / b_k,
-//   H V_i = b_{i+1} V_{i+1} + a_i V_i + b_i V_{i-1}
-// (skipped after an exact breakdown, whose column the caller computed
-// explicitly), then c = exp(f T_{k+1}) e0 and
-//   conv = |u_k - u_{k-1}|^2 = d^H G d,  d = c - [c_prev, 0];
-// finally c_prev = c.  Returns conv on every lane of wave 0.
-__device__ double adaptive_metric(int k, bool breakdown, double2* G, int kc, const double2* s_alpha,
-                                  const double* s_beta, double2 f, double2* s_c, double2* s_cprev) {
-  const int tid = threadIdx.x & 63;   // lane: any single wave may run this
-  const double beta = s_beta[k];
-  if (!breakdown) {
-    for (int i = tid; i < k; i += 64) {
-      double2 hv;
-      if (i == k - 1) {
-        hv = cconj(s_alpha[k - 1]);
-      } else {
-        hv = cscale(s_beta[i + 1], G[(i + 1) * kc + (k - 1)]);
-        hv = cfma(cconj(s_alpha[i]), G[i * kc + (k - 1)], hv);
-        if (i > 0) hv = cadd(hv, cscale(s_beta[i], G[(i - 1) * kc + (k - 1)]));
-      }
-      double2 t = csub(hv, cmul(s_alpha[k - 1], G[i * kc + (k - 1)]));
-      if (k >= 2) t = csub(t, cscale(s_beta[k - 1], G[i * kc + (k - 2)]));
-      t = make_double2(t.x / beta, t.y / beta);
-      G[i * kc + k] = t;
-      G[k * kc + i] = cconj(t);
-    }
-  }
-  if (tid == 0) G[k * kc + k] = make_double2(1.0, 0.0);
-  __threadfence_block();
-  expm_e0(k + 1, s_alpha, s_beta, f, s_c);
-  __threadfence_block();
-  double part = 0.0;
-  const int s = k + 1;
-  for (int i = tid; i < s; i += 64) {
-    double2 di = csub(s_c[i], i < k ? s_cprev[i] : make_double2(0.0, 0.0));
-    double2 r = make_double2(0.0, 0.0);
-    for (int j = 0; j < s; ++j) {
-      double2 dj = csub(s_c[j], j < k ? s_cprev[j] : make_double2(0.0, 0.0));
-      r = cfma(G[i * kc + j], dj, r);
-    }
-    part += di.x * r.x + di.y * r.y;
-  }
-  part = wave_sum(part);
-  __threadfence_block();
-  for (int i = tid; i < s; i += 64) s_cprev[i] = s_c[i];
-  return part;
 }
 
 // Resident variant: the current Krylov vector lives in LDS (gathered by the
@@ -1573,7 +1348,7 @@ extern "C" int rm_ell_config(int32_t i, int32_t* threads, int32_t* rows, int32_t
 extern "C" int rm_plan_set_kernel(rm_plan* p, int32_t variant) {
   if (!p) return fail(RM_ERR_ARG, "rm_plan_set_kernel: NULL plan");
   const bool cfg = variant >= RM_KERNEL_ELL_CFG0 && variant < RM_KERNEL_ELL_CFG0 + N_ELL_CFGS;
-  if (!cfg && (variant < RM_KERNEL_AUTO || variant > RM_KERNEL_ELL))
+  if (!cfg && (variant < RM_KERNEL_AUTO || variant > RM_KERNEL_SUBSPACE))
     return fail(RM_ERR_ARG, "rm_plan_set_kernel: unknown variant %d", variant);
   if (cfg && p->n > (int64_t)ELL_CFGS[variant - RM_KERNEL_ELL_CFG0].ntr * ELL_CFGS[variant - RM_KERNEL_ELL_CFG0].r)
     return fail(RM_ERR_ARG, "rm_plan_set_kernel: n=%lld exceeds ELL configuration %d",
@@ -1588,7 +1363,7 @@ extern "C" int rm_plan_set_kernel(rm_plan* p, int32_t variant) {
   return RM_OK;
 }
 
-static int ensure_ws(rm_plan* p, size_t bytes) {
+int ensure_ws(rm_plan* p, size_t bytes) {
   if (p->ws_bytes >= bytes) return RM_OK;
   if (p->ws) {
     HIPCHK(hipDeviceSynchronize());
@@ -1601,9 +1376,9 @@ static int ensure_ws(rm_plan* p, size_t bytes) {
   return RM_OK;
 }
 
-// launch_step, bracketed by HIP events on the same stream when timing is on.
-static hipError_t timed_launch_step(rm_plan* p, const StepArgs& a, unsigned ncols, hipStream_t s) {
-  if (!p->timing) return launch_step(a, ncols, s, p->variant, p->n_cu);
+// Event pair around one timed launch (no-ops unless the plan's timing is on).
+hipError_t timing_begin(rm_plan* p, hipStream_t s) {
+  if (!p->timing) return hipSuccess;
   if (p->nrec >= (int)p->ev0.size()) {
     hipEvent_t e0, e1;
     hipError_t e = hipEventCreate(&e0);
@@ -1616,13 +1391,23 @@ static hipError_t timed_launch_step(rm_plan* p, const StepArgs& a, unsigned ncol
     p->ev0.push_back(e0);
     p->ev1.push_back(e1);
   }
-  hipError_t e = hipEventRecord(p->ev0[p->nrec], s);
-  if (e != hipSuccess) return e;
-  e = launch_step(a, ncols, s, p->variant, p->n_cu);
-  if (e != hipSuccess) return e;
-  e = hipEventRecord(p->ev1[p->nrec], s);
+  return hipEventRecord(p->ev0[p->nrec], s);
+}
+
+hipError_t timing_end(rm_plan* p, hipStream_t s) {
+  if (!p->timing) return hipSuccess;
+  hipError_t e = hipEventRecord(p->ev1[p->nrec], s);
   p->nrec++;
   return e;
+}
+
+// launch_step, bracketed by HIP events on the same stream when timing is on.
+static hipError_t timed_launch_step(rm_plan* p, const StepArgs& a, unsigned ncols, hipStream_t s) {
+  hipError_t e = timing_begin(p, s);
+  if (e != hipSuccess) return e;
+  e = launch_step(a, ncols, s, p->variant == RM_KERNEL_SUBSPACE ? RM_KERNEL_AUTO : p->variant, p->n_cu);
+  if (e != hipSuccess) return e;
+  return timing_end(p, s);
 }
 
 extern "C" int rm_plan_timing(rm_plan* p, int32_t enable) {
@@ -1720,6 +1505,12 @@ __global__ void k_copy_cols(int64_t n, int64_t ld, const double2* __restrict__ s
   if (i < n) dst[b * ld + i] = src[b * ld + i];
 }
 
+// invariant-subspace path (rm_subspace.hip)
+int sub_propagate(rm_plan* p, void* psi, int64_t ld, int64_t ncols, int64_t nsteps,
+                  const double* fprod_dev, double f_re, double f_im, const void* phase,
+                  int32_t krylov_fixed, int32_t maxorder, double tol, int32_t* kused,
+                  int32_t* status, hipStream_t s, bool* done);
+
 extern "C" int rm_propagate(rm_plan* p, void* psi, void* work, int64_t ld, int64_t ncols,
                             int64_t nsteps, const double* fprod_dev, const double* thresh,
                             double f_re, double f_im, const void* phase, int32_t krylov_fixed,
@@ -1737,9 +1528,24 @@ extern "C" int rm_propagate(rm_plan* p, void* psi, void* work, int64_t ld, int64
   hipStream_t s = (hipStream_t)stream;
   FieldArgs fa;
   memset(&fa, 0, sizeof(fa));
+  bool any_thresh = false;
   for (int i = 0; i < RM_MAX_LEAF; ++i) fa.thresh[i] = -1.0;
   if (thresh)
-    for (int i = 0; i < p->nleaf; ++i) fa.thresh[i] = thresh[i];
+    for (int i = 0; i < p->nleaf; ++i) {
+      fa.thresh[i] = thresh[i];
+      any_thresh |= thresh[i] > 0.0;
+    }
+  if ((p->variant == RM_KERNEL_AUTO || p->variant == RM_KERNEL_SUBSPACE) && !any_thresh) {
+    bool done = false;
+    rc = sub_propagate(p, psi, ld, ncols, nsteps, fprod_dev, f_re, f_im, phase, krylov_fixed,
+                       maxorder, tol, kused, status, s, &done);
+    if (rc) return rc;
+    if (done) return RM_OK;
+  }
+  if (p->variant == RM_KERNEL_SUBSPACE)
+    return fail(RM_ERR_ARG,
+                "rm_propagate: the subspace kernel does not apply (%s)",
+                any_thresh ? "MF thresholds given" : "a wavepacket's domain exceeds RM_SUBSPACE_MAX_ROWS");
   if (status) hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(ncols, 256)), dim3(256), 0, s, status, ncols, 0);
   a.status = status;
   a.status_sticky = 1;

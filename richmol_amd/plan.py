@@ -170,6 +170,7 @@ class DevicePlan:
         import ctypes
         self.lib = _lib.load()
         self.kernel = "auto"
+        self.precision = "fp64"
         self.active = active_carts(T) if active is None else active
         arr = flatten_operator(T, self.active)
         if arr["n"] != arr["n2"]:
@@ -205,18 +206,54 @@ class DevicePlan:
                 pass
             self.handle = None
 
-    KERNELS = {"auto": 0, "global": 1, "resident": 2, "ell": 3}
+    KERNELS = {"auto": 0, "global": 1, "resident": 2, "ell": 3, "subspace": 4}
     ELL_CFG0 = 16
 
     def set_kernel(self, variant):
-        """Select the step kernel: 'auto', 'global', 'resident', 'ell', or
-        'ell:<i>' for ELL launch configuration i (see ell_configs())."""
+        """Select the step kernel: 'auto', 'global', 'resident', 'ell',
+        'ell:<i>' for ELL launch configuration i (see ell_configs()), or
+        'subspace' (propagate only: one wave per wavepacket on its invariant
+        subspace; an error when it does not apply)."""
         if isinstance(variant, str) and variant.startswith("ell:"):
             code = self.ELL_CFG0 + int(variant[4:])
         else:
             code = self.KERNELS[variant]
         _lib.check(self.lib.rm_plan_set_kernel(self.handle, code), "rm_plan_set_kernel")
         self.kernel = variant
+
+    def set_precision(self, precision):
+        """'fp64' (default) or 'fp32': complex64 bulk matvec in the subspace
+        kernel with FP64 Krylov recurrence, norms, exp(fT) and phases."""
+        _lib.check(self.lib.rm_plan_set_precision(self.handle, _lib.PRECISIONS[precision]),
+                   "rm_plan_set_precision")
+        self.precision = precision
+
+    def domain_sizes(self, X):
+        """Rows and stored entries of each column's domain (host, from the
+        flattened pattern): the union of the connected components that the
+        column's non-zero entries touch -- the work of the subspace kernel."""
+        from scipy.sparse import csr_matrix
+        from scipy.sparse.csgraph import connected_components
+        arr = self._arrays
+        n = arr["n"]
+        A = csr_matrix((np.ones(arr["nnz"]), arr["col_idx"], arr["row_ptr"]), shape=(n, n))
+        nc, lab = connected_components(A, directed=False)
+        size = np.bincount(lab, minlength=nc)
+        nnz = np.bincount(lab, weights=np.diff(arr["row_ptr"]), minlength=nc)
+        rows, ents = [], []
+        for x in np.asarray(X):
+            comps = np.unique(lab[np.nonzero(x)[0]])
+            rows.append(int(size[comps].sum()))
+            ents.append(int(nnz[comps].sum()))
+        return np.array(rows), np.array(ents)
+
+    def subspaces(self):
+        """(number of invariant subspaces, rows of the largest)."""
+        import ctypes
+        c, m = ctypes.c_int64(), ctypes.c_int64()
+        _lib.check(self.lib.rm_plan_subspaces(self.handle, ctypes.byref(c), ctypes.byref(m)),
+                   "rm_plan_subspaces")
+        return c.value, m.value
 
     # ---------------------------------------------------------------- field
     def field_arrays(self, T):

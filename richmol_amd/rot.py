@@ -187,19 +187,37 @@ def _m_block(ucols_w, w, J1, ms1, J2, ms2, carts):
     """M_w,cart[m1, m2] for every cart (labtens.py:501-523 + overlap_m)."""
     out = {c: np.zeros((len(ms1), len(ms2)), dtype=np.complex128) for c in carts}
     fac = math.sqrt((2 * J1 + 1) * (2 * J2 + 1))
+    index2 = {m: b for b, m in enumerate(ms2)}
     for a, m1 in enumerate(ms1):
-        for b, m2 in enumerate(ms2):
-            s = m1 - m2
-            if s not in ucols_w:
+        for s, col in ucols_w.items():          # only m2 = m1 - s couples (|s| <= w)
+            b = index2.get(m1 - s)
+            if b is None:
                 continue
-            t = wigner3j(J2, w, J1, m2, s, -m1)
+            t = wigner3j(J2, w, J1, m1 - s, s, -m1)
             if t == 0.0:
                 continue
-            col = ucols_w[s]
             for i, c in enumerate(carts):
                 if col[i] != 0:
                     out[c][a, b] += col[i] * t * fac * (-1) ** abs(m1)
     return out
+
+
+def restrict(T, Jmax):
+    """Copy of a built operator on the sub-basis J <= Jmax (its blocks are the
+    same matrices: matrix elements of a tensor between (J1, J2) do not depend
+    on the rest of the basis)."""
+    from .field import CarTens
+    R = CarTens()
+    keep = [J for J in T.Jlist1 if J <= Jmax]
+    R.Jlist1 = list(keep); R.Jlist2 = list(keep)
+    for name in ("symlist1", "symlist2", "dim1", "dim2", "dim_m1", "dim_m2", "dim_k1", "dim_k2",
+                 "quanta_m1", "quanta_m2", "quanta_k1", "quanta_k2"):
+        if hasattr(T, name):
+            setattr(R, name, {J: v for J, v in getattr(T, name).items() if J <= Jmax})
+    R.rank, R.cart, R.os = T.rank, list(T.cart), list(getattr(T, "os", []))
+    R.kmat = {Jp: v for Jp, v in T.kmat.items() if Jp[0] <= Jmax and Jp[1] <= Jmax}
+    R.mmat = {Jp: v for Jp, v in T.mmat.items() if Jp[0] <= Jmax and Jp[1] <= Jmax}
+    return R
 
 
 def _build(basis, rank, carts, comps, ucols):

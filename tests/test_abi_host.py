@@ -34,7 +34,8 @@ def test_library_exports_every_header_symbol(lib):
     for name in names:
         assert hasattr(lib, name), name
         assert name in _lib.SIGNATURES, f"{name} has no ctypes signature"
-    assert lib.rm_version() == 1
+    src = open(os.path.join(ROOT, "include", "richmol_amd.h")).read()
+    assert lib.rm_version() == int(re.search(r"#define RM_ABI_VERSION (\d+)", src).group(1))
 
 
 def test_plan_create_rejects_bad_indices(lib):
@@ -127,3 +128,51 @@ def test_flattened_plan_ocs_fixture():
     O.field([0, 0, 3e9])
     ref = O.tomat(form="full", repres="dense")
     assert np.abs(A - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+def test_job_arrays_keep_the_rm_job_layout(lib):
+    """Job records concatenated for rm_batch_propagate keep rm_job's padding
+    (numpy's structured promotion packs them to 96 bytes, which shifts every
+    pointer after `len` -- the C side would read garbage device addresses);
+    a packed array is refused in Python, and the library refuses any record
+    size other than sizeof(rm_job) before reading a job."""
+    from richmol_amd import serving
+    recs = [np.zeros(1, dtype=serving.JOB) for _ in range(3)]
+    for i, r in enumerate(recs):
+        r["fprod_dev"] = 0x1000 * (i + 1)
+        r["nsteps"] = 7 + i
+    jobs = serving.job_array(recs)
+    assert jobs.dtype == serving.JOB and jobs.dtype.itemsize == ctypes.sizeof(serving._lib.RmJob)
+    raw = jobs.tobytes()
+    for i in range(3):
+        assert int.from_bytes(raw[104 * i + 40:104 * i + 48], "little") == 0x1000 * (i + 1)
+        assert int.from_bytes(raw[104 * i + 56:104 * i + 60], "little") == 7 + i
+    packed = np.concatenate(recs)
+    with pytest.raises(TypeError):
+        serving.batch_propagate(lib, None, packed, 12, 100, 1e-15, 0, 0)
+    h = ctypes.c_void_p()
+    assert lib.rm_batch_create(ctypes.byref(h)) == 0
+    assert lib.rm_batch_propagate(h, jobs.ctypes.data, 3, 96, 12, 100, 1e-15, 0, None) == -1
+    assert b"bytes" in lib.rm_last_error()
+    assert lib.rm_batch_destroy(h) == 0
+
+
+def test_rm_job_layout_matches_header(tmp_path):
+    """The ctypes mirror of rm_job (used to fill job arrays from numpy) has the
+    C struct's field offsets and size (compiled with gcc from the header)."""
+    import subprocess
+    from richmol_amd._lib import RmJob
+    names = [n for n, _ in RmJob._fields_]
+    src = tmp_path / "layout.c"
+    body = "".join(f'  printf("{n} %zu\\n", offsetof(rm_job, {n}));\n' for n in names)
+    src.write_text('#include <stddef.h>\n#include <stdio.h>\n#include "richmol_amd.h"\n'
+                   'int main(void) {\n' + body + '  printf("sizeof %zu\\n", sizeof(rm_job));\n'
+                   '  return 0;\n}\n')
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                   check=True)
+    out = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                       check=True).stdout.splitlines())
+    for n in names:
+        assert int(out[n]) == getattr(RmJob, n).offset, n
+    assert int(out["sizeof"]) == ctypes.sizeof(RmJob)

@@ -1,0 +1,208 @@
+"""Invariant-subspace kernel (rm_propagate, one wave per wavepacket on the
+union of the operator's connected components its initial state touches)
+against the FP64 oracle on the same inputs.
+
+Tolerances: 1e-10 absolute in every amplitude of the full wavepacket (the
+entries outside the domain must come back exactly zero where the oracle's are)
+and identical Krylov dimensions per step; the FP32-matvec precision mode is
+held to the error bounds stated in its test.
+"""
+import numpy as np
+import pytest
+
+from _fixtures import gaussian_pulse_field, linear_rotor
+from oracle.cartens import OracleCarTens
+from oracle.tdse import OracleTDSE
+
+pytestmark = pytest.mark.gpu
+
+TOL_PSI = 1e-10
+
+
+def _oracle_run(h0, H, fields, X, *, fixed=None, tol=1e-15, dt=10.0, split=True):
+    Oh0 = OracleCarTens.from_attrs(h0)
+    OH = OracleCarTens.from_attrs(H)
+    nst = len(fields)
+    to = OracleTDSE(t_start=0.0, t_end=dt * nst, dt=dt, t_units="fs")
+    to.time_grid()
+    psi = X.copy()
+    ks = []
+    for F in fields:
+        OH.field(F)
+        stats = []
+        if split:
+            psi, _ = to.update(OH, psi, H0=Oh0, tol=tol, fixed_m=fixed, stats=stats)
+        else:
+            psi, _ = to.update(Oh0 + OH, psi, tol=tol, fixed_m=fixed, stats=stats)
+        ks.append(stats if stats else [0] * len(X))
+    return psi, np.array(ks)
+
+
+def _device_run(h0, H, fields, X, *, kernel="subspace", fixed=0, tol=1e-15, dt=10.0,
+                precision="fp64"):
+    import torch
+    from richmol_amd import ensemble
+    from richmol_amd.tdse import TDSE
+    f = TDSE(t_end=dt * len(fields), dt=dt, t_units="fs").exp_fac
+    e = ensemble.Ensemble(H, h0, f, X, krylov_fixed=fixed, tol=tol)
+    e.set_kernel(kernel)
+    table = e.prepare(fields)
+    e.plan.set_precision(precision)
+    kused = torch.zeros((len(fields), len(X)), dtype=torch.int32, device=e.psi.device)
+    e.propagate(None, table=table, kused=kused)
+    e.check()
+    return e.psi.cpu().numpy(), kused.cpu().numpy(), e
+
+
+def _basis_columns(n, idx):
+    X = np.zeros((len(idx), n), dtype=np.complex128)
+    X[np.arange(len(idx)), idx] = 1.0
+    return X
+
+
+def test_subspaces_match_connected_components(device):
+    """The plan's subspace count and size equal scipy's connected components
+    of the flattened operator's pattern (Z field: (m, J parity) blocks)."""
+    from scipy.sparse.csgraph import connected_components
+    from richmol_amd import plan as _plan
+    h0, H, cos2, bas = linear_rotor(10)
+    H.field([0.0, 0.0, 1e9])
+    p = _plan.plan_for(H)
+    arr = _plan.flatten_operator(H, p.active)
+    from scipy.sparse import csr_matrix
+    A = csr_matrix((np.ones(arr["nnz"]), arr["col_idx"], arr["row_ptr"]), shape=(arr["n"], arr["n"]))
+    nc, lab = connected_components(A, directed=False)
+    count, mx = p.subspaces()
+    assert count == nc
+    assert mx == np.bincount(lab).max()
+    assert count == 2 * 21 - 2   # m = -10..10, two J parities each, |m| = 10 has one
+
+
+@pytest.mark.parametrize("fixed", [0, 12])
+def test_thermal_columns_jmax40(device, fixed):
+    """Headline basis (J<=40, N=1681), basis-state columns spread over m, a
+    pulse window around the peak: every column's domain is one (m, parity)
+    block of <= 21 states.  The last three columns start in one-state blocks
+    (|J=40, m=-40>, |40, 40>, |39, -39>), so the fixed 12-dimensional Krylov
+    space exhausts the domain and restarts (tdse.py:459-465) on it."""
+    h0, H, cos2, bas = linear_rotor(40)
+    n = 1681
+    rng = np.random.default_rng(21)
+    idx = list(rng.choice(1500, size=6, replace=False)) + [1600, 1680, 1521]
+    X = _basis_columns(n, idx)
+    t = np.arange(28, 36) * 10.0 + 5.0
+    fields = [[0.0, 0.0, gaussian_pulse_field(tt)] for tt in t]
+    psi_d, k_d, _ = _device_run(h0, H, fields, X, fixed=fixed)
+    psi_o, k_o = _oracle_run(h0, H, fields, X, fixed=fixed or None)
+    assert np.isfinite(psi_d).all()
+    assert np.abs(psi_d - psi_o).max() <= TOL_PSI
+    assert np.all(psi_d[psi_o == 0] == 0)
+    if not fixed:
+        assert np.array_equal(k_d, k_o)
+
+
+@pytest.mark.parametrize("precision", ["fp64"])
+def test_dense_columns_full_domain(device, precision):
+    """Random dense columns touch every component: the domain is the whole
+    basis (N=169 <= 256 rows, 4 rows per lane)."""
+    h0, H, cos2, bas = linear_rotor(12)
+    n = 169
+    rng = np.random.default_rng(22)
+    X = rng.normal(size=(5, n)) + 1j * rng.normal(size=(5, n))
+    X /= np.linalg.norm(X, axis=1)[:, None]
+    fields = [[0.0, 0.0, a] for a in [3e8, 9e8, 1.4e9, 0.0, 1.1e9]]
+    psi_d, k_d, _ = _device_run(h0, H, fields, X, precision=precision)
+    psi_o, k_o = _oracle_run(h0, H, fields, X)
+    assert np.abs(psi_d - psi_o).max() <= TOL_PSI
+    assert np.array_equal(k_d, k_o)
+
+
+def test_xz_field_mixed_domains(device):
+    """X+Z field couples m: the components are the two J parities; columns are
+    a mix of basis states, a two-component superposition and zero rows
+    (which stay zero, with Krylov dimension 0)."""
+    h0, H, cos2, bas = linear_rotor(8)
+    n = 81
+    X = np.zeros((4, n), dtype=np.complex128)
+    X[0, 0] = 1.0
+    X[1, 5] = 0.6
+    X[1, 10] = 0.8j
+    X[3, 40] = 1.0
+    amps = np.linspace(2e8, 1.6e9, 6)
+    fields = [[0.6 * a, 0.0, a] for a in amps]
+    psi_d, k_d, _ = _device_run(h0, H, fields, X)
+    psi_o, k_o = _oracle_run(h0, H, fields, X)
+    assert np.abs(psi_d - psi_o).max() <= TOL_PSI
+    assert np.all(psi_d[2] == 0)
+    assert np.array_equal(k_d[:, [0, 1, 3]], k_o[:, [0, 1, 3]])
+
+
+def test_exact_breakdown_single_state_domains(device):
+    """A diagonal operator makes every state its own subspace: beta_1 == 0,
+    the restart vector is built on a one-row domain, and the result and the
+    Krylov dimensions still equal the reference's full-vector restart."""
+    from richmol_amd import rot
+    bas = rot.linear_basis(0.2029, 3)
+    h0 = rot.hamiltonian(bas)
+    D = rot.hamiltonian(rot.linear_basis(7.3, 3))
+    n = 16
+    X = np.zeros((3, n), dtype=np.complex128)
+    X[0, 1] = 1.0
+    X[1, 5] = 1.0
+    rng = np.random.default_rng(8)
+    X[2] = rng.normal(size=n) + 1j * rng.normal(size=n)
+    X[2] /= np.linalg.norm(X[2])
+    fields = [[0.0, 0.0, 1.0]] * 3
+    psi_d, k_d, _ = _device_run(h0, D, fields, X, dt=100.0)
+    psi_o, k_o = _oracle_run(h0, D, fields, X, dt=100.0)
+    assert np.abs(psi_d - psi_o).max() <= TOL_PSI
+    assert np.array_equal(k_d, k_o)
+
+
+def test_unnormalised_and_zero_field_steps(device):
+    """Rows of norm != 1 (unnormalised first Krylov vector) and zero-field
+    steps (Lanczos skipped, phases only, Krylov dimension 0)."""
+    h0, H, cos2, bas = linear_rotor(6)
+    n = 49
+    rng = np.random.default_rng(5)
+    X = rng.normal(size=(3, n)) + 1j * rng.normal(size=(3, n))
+    X *= (np.array([0.2, 0.55, 1.7]) / np.linalg.norm(X, axis=1))[:, None]
+    fields = [[0.0, 0.0, 0.0]] * 2 + [[0.0, 0.0, a] for a in np.linspace(8e8, 1.5e9, 4)] + [[0.0, 0.0, 0.0]]
+    psi_d, k_d, _ = _device_run(h0, H, fields, X)
+    psi_o, k_o = _oracle_run(h0, H, fields, X)
+    assert np.abs(psi_d - psi_o).max() <= TOL_PSI
+    assert np.array_equal(k_d, k_o)
+    assert np.all(k_d[:2] == 0) and np.all(k_d[-1] == 0)
+
+
+def test_auto_equals_subspace_and_ell_agrees(device):
+    """'auto' takes the subspace kernel for rm_propagate (bitwise the same
+    run) and agrees with the full-vector ELL kernel to rounding."""
+    h0, H, cos2, bas = linear_rotor(20)
+    n = 441
+    rng = np.random.default_rng(3)
+    X = _basis_columns(n, rng.choice(n, size=40, replace=False))
+    t = np.arange(20, 40) * 10.0 + 5.0
+    fields = [[0.0, 0.0, gaussian_pulse_field(tt)] for tt in t]
+    a, ka, _ = _device_run(h0, H, fields, X, kernel="subspace", fixed=12)
+    b, kb, _ = _device_run(h0, H, fields, X, kernel="auto", fixed=12)
+    c, kc, _ = _device_run(h0, H, fields, X, kernel="ell", fixed=12)
+    assert np.array_equal(a, b)
+    assert np.abs(a - c).max() <= 1e-12
+
+
+def test_subspace_refuses_thresholds_and_long_domains(device):
+    """Forced 'subspace' raises when the path does not apply; 'auto' falls
+    back to the full-vector kernels with the same result as the oracle."""
+    from richmol_amd._lib import RichmolAmdError
+    h0, H, cos2, bas = linear_rotor(40)
+    n = 1681
+    rng = np.random.default_rng(9)
+    X = rng.normal(size=(2, n)) + 1j * rng.normal(size=(2, n))
+    X /= np.linalg.norm(X, axis=1)[:, None]
+    fields = [[0.3e9, 0.0, 1e9]] * 2     # X+Z: one parity block holds ~840 states, domain = all 1681
+    with pytest.raises(RichmolAmdError):
+        _device_run(h0, H, fields, X, kernel="subspace")
+    psi_d, k_d, _ = _device_run(h0, H, fields, X, kernel="auto")
+    psi_o, k_o = _oracle_run(h0, H, fields, X)
+    assert np.abs(psi_d - psi_o).max() <= TOL_PSI
