@@ -113,6 +113,69 @@ def cos2_of(ens, cplan):
     return ens.expect(cplan).real.cpu().numpy() + 1.0 / 3.0
 
 
+def asymtop_leg(nsteps, cpu_steps=3):
+    """The asymmetric-top configuration: water-like rotor (A, B, C = 27.88,
+    14.51, 9.28 cm^-1, D2), J <= 20 with all m (N = 12 341), dipole
+    (0, 0, -0.7288) au and polarizability diag(9.14, 9.87, 9.45) au, static
+    50 kV/cm Z field plus a cycle-averaged X pulse (FWHM 100 fs, 5e12 W/cm^2,
+    centred at 400 fs), psi0 = ground state, dt = 5 fs, adaptive Lanczos,
+    FP64: the whole run as one rm_propagate call."""
+    import torch
+    from richmol_amd import ensemble
+    from tests._fixtures import water, water_fields
+    dt = 5.0
+    t = (np.arange(nsteps) + 0.5) * dt
+    h0, leaves, bas = water(20)
+    fields = water_fields(t)
+    for T, F in zip(leaves, fields):
+        T.field(list(F[0]))
+    H = leaves[0] + leaves[1] + leaves[2]
+    f = -1j * dt * 1e-15 * __import__("richmol_amd").units.enr_units()["invcm"] / \
+        __import__("richmol_amd").units.HBAR
+    n = 12341
+    X = np.zeros((1, n), dtype=np.complex128)
+    X[0, 0] = 1.0
+    ens = ensemble.Ensemble(H, h0, f, X)
+    warm = ens.prepare([F[:10] for F in fields], per_term=True)
+    ens.propagate(None, table=warm)
+    ens.psi.copy_(torch.from_numpy(X).to(ens.psi.device))
+    table = ens.prepare(fields, per_term=True)
+    kused = torch.zeros((nsteps, 1), dtype=torch.int32, device=ens.psi.device)
+    torch.cuda.synchronize()
+    ens.plan.timing(True)
+    t0 = time.perf_counter()
+    ens.propagate(None, table=table, kused=kused)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kms, nl = ens.plan.kernel_time()
+    ens.plan.timing(False)
+    ens.check()
+    out = {"n_states": n, "nnz": ens.plan.nnz, "steps": nsteps, "dt_fs": dt,
+           "wavepacket_steps_per_s": nsteps / wall, "ms_per_step": wall / nsteps * 1e3,
+           "krylov_kernel_ms_per_step": kms / max(nl, 1), "kernel": ens.plan.kernel,
+           "mean_krylov_dim": float(kused.float().mean()),
+           "norm_err": float(abs(np.linalg.norm(ens.psi.cpu().numpy()) - 1)),
+           "ground_population_end": float(abs(ens.psi[0, 0].item()) ** 2)}
+    if cpu_steps:
+        from oracle.cartens import OracleCarTens
+        from oracle.tdse import OracleTDSE
+        from threadpoolctl import threadpool_limits
+        Oh0 = OracleCarTens.from_attrs(h0)
+        OL = [OracleCarTens.from_attrs(T) for T in leaves]
+        to = OracleTDSE(t_end=dt * (cpu_steps + 1), dt=dt, t_units="fs")
+        to.time_grid()
+        v = X.copy()
+        with threadpool_limits(limits=1):
+            t0 = time.perf_counter()
+            for i in range(cpu_steps):
+                for O, F in zip(OL, fields):
+                    O.field(list(F[i]))
+                v, _ = to.update(OL[0] + OL[1] + OL[2], v, H0=Oh0)
+            el = time.perf_counter() - t0
+        out["cpu_oracle_steps_per_s"] = cpu_steps / el
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -133,6 +196,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true",
                     help="headline run only (no full-vector, precision, single-wavepacket, serving legs)")
     ap.add_argument("--serve-requests", type=int, default=200)
+    ap.add_argument("--asym-steps", type=int, default=2000,
+                    help="steps of the asymmetric-top leg (0: skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-sized device buffer 3 times (reference for the HBM "
                          "byte counters) and exit")
@@ -281,6 +346,11 @@ def main():
             "path": "TDSE.update per step, one wavepacket, FP64 (host loop, rm_step)",
             "speedup_of_value": value / single}
         print(json.dumps({"progress": "extras"}), file=sys.stderr, flush=True)
+
+        # ---- the asymmetric-top configuration (single wavepacket, N = 12 341)
+        if args.asym_steps > 0:
+            result["asymtop_water_j20"] = asymtop_leg(args.asym_steps)
+            print(json.dumps({"progress": "asymtop"}), file=sys.stderr, flush=True)
 
         # ---- serving: continuous batching of independent propagation requests
         if args.serve_requests > 0:

@@ -86,12 +86,18 @@ class Ensemble:
         self.set_field(F, thresh)
         self.launch()
 
-    def prepare(self, fields, thresh=None):
+    def prepare(self, fields, thresh=None, per_term=False):
         """Device table of field products for a run of steps (one row per
-        step) and the plan that covers every component the run activates."""
+        step) and the plan that covers every component the run activates.
+        per_term: ``fields`` holds one (nsteps, 3) array per summed tensor
+        (e.g. a static field on the dipole and a pulse on the polarizability)."""
         import torch
         from . import plan as _plan
-        active, cols, nst = _plan.field_table(self.H, np.asarray(fields, dtype=np.float64), thresh)
+        if per_term:
+            fields = [None if F is None else np.asarray(F, dtype=np.float64) for F in fields]
+        else:
+            fields = np.asarray(fields, dtype=np.float64)
+        active, cols, nst = _plan.field_table(self.H, fields, thresh, per_term=per_term)
         if active != self._active:
             self.plan = _plan.plan_for(self.H, active)
             self._apply_kernel()

@@ -152,6 +152,67 @@ def linear_basis(B, Jmax, Jmin=0, m_filter=None, J_filter=None):
     return RotorBasis(jl, ml, kf, en)
 
 
+# D2 species of a Wang function |J K ±> with z the quantization axis, from its
+# behaviour under C2(z) (K parity) and C2(x) (|J k> -> (-1)^J |J -k>).
+_D2 = {(0, 1): "A", (0, -1): "B1", (1, -1): "B2", (1, 1): "B3"}
+
+
+def rigid_rotor_matrix(J, Bx, By, Bz):
+    """<J k'|Bx Jx^2 + By Jy^2 + Bz Jz^2|J k> in the symmetric-top basis
+    k = -J..J (molecule-fixed angular momentum, z the quantization axis):
+    diagonal (Bx+By)/2 (J(J+1) - k^2) + Bz k^2, and (Bx-By)/4 times the
+    Delta k = +-2 ladder elements."""
+    ks = np.arange(-J, J + 1)
+    jj = J * (J + 1.0)
+    H = np.diag((Bx + By) / 2.0 * (jj - ks ** 2) + Bz * ks ** 2).astype(np.float64)
+    for i, k in enumerate(ks[:-2]):
+        v = (Bx - By) / 4.0 * math.sqrt((jj - k * (k + 1)) * (jj - (k + 1) * (k + 2)))
+        H[i + 2, i] = H[i, i + 2] = v
+    return H
+
+
+def asymtop_basis(Bx, By, Bz, Jmax, Jmin=0, m_filter=None, J_filter=None):
+    """Rigid asymmetric-top basis (rot/solution.py:160-391, rot/basis.py:702-801):
+    per J the eigenstates of Bx Jx^2 + By Jy^2 + Bz Jz^2 [cm^-1], with the
+    rotational constants along the molecule-frame axes of the tensors that
+    will be built on it.  The Wang functions split each J into the four D2
+    species (the Hamiltonian couples Delta k = 0, +-2 and commutes with
+    k -> -k); a species block is diagonalised on its own and its states are
+    stored as coefficient columns over primitive k = -J..J, ordered by energy."""
+    jl, ml, kf, en = [], {}, {}, {}
+    for J in range(Jmin, Jmax + 1):
+        if J_filter is not None and not J_filter(J):
+            continue
+        ms = [m for m in range(-J, J + 1) if m_filter is None or m_filter(J, m)]
+        if not ms:
+            continue
+        ks = list(range(-J, J + 1))
+        H = rigid_rotor_matrix(J, Bx, By, Bz)
+        # Wang functions (columns over primitive k) and their D2 species
+        cols = {}
+        for K in range(0, J + 1):
+            for sgn in ((1,) if K == 0 else (1, -1)):
+                w = np.zeros(2 * J + 1)
+                if K == 0:
+                    w[J] = 1.0
+                else:
+                    w[J + K] = 1.0 / math.sqrt(2.0)
+                    w[J - K] = sgn / math.sqrt(2.0)
+                c2x = sgn * (-1) ** J
+                cols.setdefault(_D2[(K % 2, c2x)], []).append(w)
+        jl.append(J)
+        ml[J] = ms
+        kf[J], en[J] = {}, {}
+        for sym in ("A", "B1", "B2", "B3"):
+            if sym not in cols:
+                continue
+            Wm = np.array(cols[sym]).T
+            e, v = np.linalg.eigh(Wm.T @ H @ Wm)
+            kf[J][sym] = (ks, (Wm @ v).astype(np.complex128))
+            en[J][sym] = e
+    return RotorBasis(jl, ml, kf, en)
+
+
 def hamiltonian(basis):
     """Field-free Hamiltonian as a rank-0 tensor: K = diag(E), M = identity
     (labtens.py:285-308 with the eigenbasis of H0)."""
@@ -167,19 +228,33 @@ def hamiltonian(basis):
     return T
 
 
-def _k_block(comps_w, w, J1, kf1, J2, kf2):
-    """K_w[bra J1 states, ket J2 states] (labtens.py:482-496 + overlap_k)."""
+def _k_prim(comps_w, w, J1, kp1, J2, kp2):
+    """Primitive-k matrix of K_w: sum_s 3j(J2 w J1; k2 s -k1) UsT_s (-1)^|k1|;
+    only k1 = k2 + s couples."""
+    prim = np.zeros((len(kp1), len(kp2)), dtype=np.complex128)
+    index1 = {k: a for a, k in enumerate(kp1)}
+    for b, k2 in enumerate(kp2):
+        for s, u in comps_w.items():
+            a = index1.get(k2 + s)
+            if a is None:
+                continue
+            t = wigner3j(J2, w, J1, k2, s, -(k2 + s))
+            if t != 0.0:
+                prim[a, b] += t * u * (-1) ** abs(k2 + s)
+    return prim
+
+
+def _k_block(comps_w, w, J1, kf1, J2, kf2, cache=None):
+    """K_w[bra J1 states, ket J2 states] (labtens.py:482-496 + overlap_k).  The
+    primitive matrix is shared by every species pair of (J1, J2) via cache."""
     kp1, c1 = kf1
     kp2, c2 = kf2
-    prim = np.zeros((len(kp1), len(kp2)), dtype=np.complex128)
-    for a, k1 in enumerate(kp1):
-        for b, k2 in enumerate(kp2):
-            me = 0.0
-            for s, u in comps_w.items():
-                t = wigner3j(J2, w, J1, k2, s, -k1)
-                if t != 0.0:
-                    me += t * u
-            prim[a, b] = me * (-1) ** abs(k1)
+    key = (w, J1, tuple(kp1), J2, tuple(kp2))
+    prim = None if cache is None else cache.get(key)
+    if prim is None:
+        prim = _k_prim(comps_w, w, J1, kp1, J2, kp2)
+        if cache is not None:
+            cache[key] = prim
     return c1.conj().T @ prim @ c2
 
 
@@ -227,6 +302,7 @@ def _build(basis, rank, carts, comps, ucols):
     T.os = [(o, s) for o in sorted(comps) for s in sorted(comps[o])]
     T.kmat, T.mmat = {}, {}
     dJ = max(comps.keys()) if comps else 0
+    kcache, mcache = {}, {}
     for J2 in basis.Jlist:
         for J1 in basis.Jlist:
             if abs(J1 - J2) > dJ:
@@ -235,11 +311,14 @@ def _build(basis, rank, carts, comps, ucols):
                 for s2 in basis.symlist[J2]:
                     kd, md = {}, {}
                     for w in sorted(comps):
-                        K = _k_block(comps[w], w, J1, basis.kfun[J1][s1], J2, basis.kfun[J2][s2])
+                        K = _k_block(comps[w], w, J1, basis.kfun[J1][s1], J2, basis.kfun[J2][s2], kcache)
                         K[np.abs(K) < _EPS] = 0
                         if not np.any(K):
                             continue
-                        Ms = _m_block(ucols[w], w, J1, basis.mlist[J1], J2, basis.mlist[J2], carts)
+                        mkey = (w, J1, J2)
+                        if mkey not in mcache:
+                            mcache[mkey] = _m_block(ucols[w], w, J1, basis.mlist[J1], J2, basis.mlist[J2], carts)
+                        Ms = {c: M.copy() for c, M in mcache[mkey].items()}
                         mm = {}
                         for c, M in Ms.items():
                             M[np.abs(M) < _EPS] = 0

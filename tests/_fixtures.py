@@ -52,6 +52,46 @@ def linear_rotor(Jmax, B=0.2029, alpha=(26.0, 26.0, 53.6), m_filter=None):
     return h0, H, cos2, bas
 
 
+# Water-like asymmetric top (the asymmetric-top configuration): A, B, C =
+# 27.88, 14.51, 9.28 cm^-1, frame x = c, y = a, z = b (the C2 axis, along
+# which the dipole lies), dipole (0, 0, -0.7288) au, polarizability
+# diag(9.14, 9.87, 9.45) au.
+WATER_B = (9.28, 27.88, 14.51)          # (Bx, By, Bz)
+WATER_MU = (0.0, 0.0, -0.7288)
+WATER_ALPHA = (9.14, 9.87, 9.45)
+WATER_STATIC = 5e6                      # 50 kV/cm along Z [V/m]
+
+
+def water(jmax, m_filter=None):
+    """(H0, [dipole, polarizability (static), polarizability (pulse)], basis):
+    the interaction -mu.F_s - 1/2 F_s.alpha.F_s - 1/2 <F_p.alpha.F_p> with a
+    static Z field F_s and a cycle-averaged X pulse F_p (the cross term of the
+    static and the oscillating field averages to zero, so the two act through
+    separate tensors)."""
+    bas = rot.asymtop_basis(*WATER_B, jmax, m_filter=m_filter)
+    h0 = rot.hamiltonian(bas)
+    mu = rot.tensor(np.array(WATER_MU), bas) * (-units.AUdip_x_Vm_to_invcm())
+    als = rot.tensor(np.diag(WATER_ALPHA), bas) * (-0.5 * units.AUpol_x_Vm_to_invcm())
+    alp = rot.tensor(np.diag(WATER_ALPHA), bas) * (-0.5 * units.AUpol_x_Vm_to_invcm())
+    return h0, [mu, als, alp], bas
+
+
+def water_fields(t_fs, fwhm_fs=100.0, intensity=5e12):
+    """Per-tensor field arrays at times t_fs: static Z on the dipole and the
+    static polarizability term, the X pulse envelope (centred at 4 FWHM, cut
+    beyond 4 FWHM) on the pulse polarizability term."""
+    t = np.asarray(t_fs, dtype=np.float64)
+    e0 = units.intensity_to_field(intensity)
+    t0 = 4.0 * fwhm_fs
+    ex = e0 * np.exp(-2 * np.log(2) * (t - t0) ** 2 / fwhm_fs ** 2) / np.sqrt(2.0)
+    ex[np.abs(t - t0) > 4.0 * fwhm_fs] = 0.0
+    stat = np.zeros((len(t), 3))
+    stat[:, 2] = WATER_STATIC
+    pul = np.zeros((len(t), 3))
+    pul[:, 0] = ex
+    return [stat, stat, pul]
+
+
 def gaussian_pulse_field(t_fs, fwhm_fs=300.0, intensity=5e11):
     """Cycle-averaged Z field of a Gaussian pulse: E(t) = E0 exp(-2 ln2 (t-t0)^2/tau^2)/sqrt(2),
     tau = intensity FWHM, t0 = 2.5 tau / 2, E0 from the peak intensity [W/cm^2]."""
