@@ -94,7 +94,7 @@ def timed_run(ens, warm, timed, precision="fp64"):
     """Warm-up run, then the timed run: (wall s, summed kernel ms, launches)."""
     import torch
     tw = ens.prepare(warm)
-    ens.plan.set_precision(precision)
+    ens.set_precision(precision)
     ens.propagate(None, table=tw)
     tt = ens.prepare(timed)
     torch.cuda.synchronize()

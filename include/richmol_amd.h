@@ -33,6 +33,7 @@ extern "C" {
 
 #define RM_OK              0
 #define RM_NOT_CONVERGED   1    /* a column hit maxorder (tdse.py:480-484) */
+#define RM_DEVICE_TIMEOUT  2    /* status bit: a cooperative launch's grid barrier timed out */
 #define RM_ERR_ARG        -1
 #define RM_ERR_HIP        -2
 #define RM_ERR_ALLOC      -3
@@ -152,14 +153,22 @@ int rm_plan_kernel_time(rm_plan* plan, double* total_ms, int64_t* launches);
  * the union of the subspaces its initial support touches ("domain"), so each
  * wavepacket is advanced by one wave on its domain only, through all nsteps
  * steps in one launch, with the operator A(F) = sum_s F_s B_s assembled per
- * step from the field table.  Applies when no MF threshold is given and every
- * domain has at most RM_SUBSPACE_MAX_ROWS rows; results equal the full-vector
- * kernels' to rounding (entries outside the domain stay exactly zero). */
+ * step from the field table.  Applies when no MF threshold is given; results
+ * equal the full-vector kernels' to rounding (entries outside the domain stay
+ * exactly zero).  Domains of more than RM_SUBSPACE_MAX_ROWS rows are advanced
+ * by a cooperative launch (every workgroup of the device on one wavepacket,
+ * grid barriers between the Lanczos reductions), one wavepacket at a time;
+ * with more than 32 such wavepackets RM_KERNEL_AUTO takes the full-vector
+ * kernels instead.
+ *
+ * RM_KERNEL_COOP (rm_propagate only): every wavepacket through the
+ * cooperative kernel, whatever its domain (no MF thresholds). */
 #define RM_KERNEL_AUTO      0
 #define RM_KERNEL_GLOBAL    1
 #define RM_KERNEL_RESIDENT  2
 #define RM_KERNEL_ELL       3
 #define RM_KERNEL_SUBSPACE  4
+#define RM_KERNEL_COOP      5
 #define RM_SUBSPACE_MAX_ROWS 1024
 /* RM_KERNEL_ELL_CFG0 + i: ELL kernel with explicit launch configuration i
  * (rm_ell_config() describes it); the autotuner's search space. */

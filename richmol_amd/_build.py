@@ -7,7 +7,8 @@ REPO_DIR = os.path.dirname(PKG_DIR)
 LIB_NAME = "librichmol_amd.so"
 LIB_PATH = os.path.join(PKG_DIR, LIB_NAME)
 SOURCES = [os.path.join(PKG_DIR, "csrc", "rm_tdse.hip"),
-           os.path.join(PKG_DIR, "csrc", "rm_subspace.hip")]
+           os.path.join(PKG_DIR, "csrc", "rm_subspace.hip"),
+           os.path.join(PKG_DIR, "csrc", "rm_coop.hip")]
 
 HIPCC_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared",

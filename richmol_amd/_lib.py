@@ -10,6 +10,7 @@ from ._build import LIB_PATH
 
 RM_OK = 0
 RM_NOT_CONVERGED = 1
+RM_DEVICE_TIMEOUT = 2
 RM_MAX_FIELD = 32
 RM_MAX_LEAF = 8
 RM_MAX_KRYLOV = 128

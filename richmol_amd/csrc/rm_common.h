@@ -53,6 +53,27 @@ __device__ __forceinline__ float2 cfmaf(float2 a, float2 b, float2 acc) {
   return make_float2(acc.x + (a.x * b.x - a.y * b.y), acc.y + (a.x * b.y + a.y * b.x));
 }
 
+// matvec operand type of the precision modes: FP64, or FP32 (complex64
+// operator and gathered vector, FP32 accumulation)
+template <bool F32>
+struct XType { using T = double2; };
+template <>
+struct XType<true> { using T = float2; };
+
+__device__ __forceinline__ double2 to_d2(double2 v) { return v; }
+__device__ __forceinline__ double2 to_d2(float2 v) { return make_double2((double)v.x, (double)v.y); }
+template <bool F32>
+__device__ __forceinline__ typename XType<F32>::T to_x(double2 v);
+template <>
+__device__ __forceinline__ double2 to_x<false>(double2 v) { return v; }
+template <>
+__device__ __forceinline__ float2 to_x<true>(double2 v) {
+  return make_float2(__double2float_rn(v.x), __double2float_rn(v.y));
+}
+__device__ __forceinline__ double2 xfma(double2 a, double2 b, double2 acc) { return cfma(a, b, acc); }
+__device__ __forceinline__ float2 xfma(float2 a, float2 b, float2 acc) { return cfmaf(a, b, acc); }
+
+
 // ---------------------------------------------------------------------------
 // plan
 
@@ -107,6 +128,13 @@ struct rm_plan {
   double2* sub_val = nullptr;    // [nfield][sub_w][n] B_s values
   std::vector<int32_t> h_sub_start, h_sub_end;   // per subspace: first / one past last permuted row
   std::vector<int32_t> h_sub_label_perm;         // permuted row -> subspace
+  std::vector<int32_t> h_sub_perm;               // permuted row -> original row
+  // the B_s as CSR in permuted order, for domains longer than a wave's
+  // (built when some subspace has more than RM_SUBSPACE_MAX_ROWS rows)
+  int64_t* sub_rp = nullptr;     // [n+1]
+  int32_t* sub_ccol = nullptr;   // [nnz] permuted column
+  double2* sub_cval = nullptr;   // [nfield][nnz]
+  std::vector<int64_t> h_sub_rp;
   // per-call scratch of the subspace path
   int32_t* dom = nullptr;        // 2 per column: domain lo, hi
   int64_t dom_cap = 0;
@@ -114,10 +142,33 @@ struct rm_plan {
   int64_t sflag_cap = 0;
   std::vector<int32_t> h_dom;
   void* sub_scratch = nullptr;   // launch staging and workspace (rm_subspace.hip)
+  void* coop_scratch = nullptr;  // workspace of the cooperative kernel (rm_coop.hip)
   // optional HIP-event timing of the Krylov step launches
   int timing = 0;
   int nrec = 0;
   std::vector<hipEvent_t> ev0, ev1;
+};
+
+// One propagation job of a launch: an operator (a plan's subspace arrays),
+// the split-operator phases and field table of the job, and its wavepackets.
+struct SubOp {
+  const int32_t* col;      // [w][n] permuted column of each ELL entry
+  const double2* val;      // [nslot][w][n] B_s
+  const int64_t* rp;       // the same B_s as permuted CSR: row pointers [n+1],
+  const int32_t* ccol;     //   columns [nnz] and
+  const double2* cval;     //   values [nslot][nnz]
+  int64_t nnz;
+  const int32_t* perm;     // permuted row -> original row
+  const double2* phase;    // original order, or NULL
+  const double2* table;    // [nsteps][nslot] field products of the launch's steps
+  const int32_t* sflag;    // [nsteps] 1: some MF element non-zero
+  double2* psi;            // column b at psi + b*ld (original order)
+  int64_t ld;
+  int32_t* kused;          // kused[step*kused_ld + b - kcol0] or NULL
+  int64_t kused_ld;
+  int32_t* status;         // status[b - kcol0] or NULL: failures only
+  double2 f;
+  int32_t n, w, nslot, nsteps, kcol0;
 };
 
 template <typename T>
@@ -149,6 +200,12 @@ static int grow(T** buf, int64_t* cap, int64_t count) {
 int ensure_ws(rm_plan* p, size_t bytes);
 // Free a plan's subspace-launch scratch (rm_subspace.hip).
 void sub_scratch_release(void* sc);
+// Free a plan's cooperative-kernel workspace (rm_coop.hip).
+void coop_scratch_release(void* sc);
+// Propagate wavepackets (column, lo, len, -) of one job with the cooperative
+// kernel, one launch each (rm_coop.hip).
+int coop_run(rm_plan* p, void** scratch, const SubOp& op, const std::vector<int4>& items, int32_t kcap,
+             int32_t fixed_m, int32_t maxorder, double tol, bool f32, int n_cu, hipStream_t s);
 // Event-bracketed launch helpers for the plan's optional timing.
 hipError_t timing_begin(rm_plan* p, hipStream_t s);
 hipError_t timing_end(rm_plan* p, hipStream_t s);
@@ -361,6 +418,15 @@ static __device__ void expm_e0_16(int s, const double2* alpha, const double* bet
 //   conv = |u_k - u_{k-1}|^2 = d^H G d,  d = c - [c_prev, 0];
 // finally c_prev = c.  Returns conv on every lane of the wave.
 // FAST: exp(fT) e0 by expm_e0_16 while the Krylov dimension is <= 16.
+// exp(f T_s) e0: the DPP-row variant for s <= 16, the general one beyond.
+__device__ __forceinline__ void expm_e0_any(int s, const double2* alpha, const double* beta, double2 f,
+                                            double2* c) {
+  if (s <= 16)
+    expm_e0_16(s, alpha, beta, f, c);
+  else
+    expm_e0(s, alpha, beta, f, c);
+}
+
 template <bool FAST = false>
 static __device__ double adaptive_metric(int k, bool breakdown, double2* G, int kc, const double2* s_alpha,
                                          const double* s_beta, double2 f, double2* s_c, double2* s_cprev) {

@@ -29,24 +29,7 @@ namespace {
 
 constexpr int NSR = 2;                         // field slots held in registers
 constexpr size_t SUB_LDS_BASIS = 32 * 1024;    // LDS budget of a wave's Krylov basis
-
-// One propagation job of a launch: an operator (a plan's subspace arrays),
-// the split-operator phases and field table of the job, and its wavepackets.
-struct SubOp {
-  const int32_t* col;      // [w][n] permuted column of each ELL entry
-  const double2* val;      // [nslot][w][n] B_s
-  const int32_t* perm;     // permuted row -> original row
-  const double2* phase;    // original order, or NULL
-  const double2* table;    // [nsteps][nslot] field products of the launch's steps
-  const int32_t* sflag;    // [nsteps] 1: some MF element non-zero
-  double2* psi;            // column b at psi + b*ld (original order)
-  int64_t ld;
-  int32_t* kused;          // kused[step*kused_ld + b - kcol0] or NULL
-  int64_t kused_ld;
-  int32_t* status;         // status[b - kcol0] or NULL: failures only
-  double2 f;
-  int32_t n, w, nslot, nsteps, kcol0;
-};
+constexpr int64_t COOP_MAX_COLS = 32;          // long-domain wavepackets per call for the cooperative kernel
 
 struct SubArgs {
   const SubOp* ops;
@@ -58,33 +41,6 @@ struct SubArgs {
   int32_t kcap, fixed_m, maxorder;
   double tol;
 };
-
-template <bool F32>
-struct XType { using T = double2; };
-template <>
-struct XType<true> { using T = float2; };
-
-__device__ __forceinline__ double2 to_d2(double2 v) { return v; }
-__device__ __forceinline__ double2 to_d2(float2 v) { return make_double2((double)v.x, (double)v.y); }
-template <bool F32>
-__device__ __forceinline__ typename XType<F32>::T to_x(double2 v);
-template <>
-__device__ __forceinline__ double2 to_x<false>(double2 v) { return v; }
-template <>
-__device__ __forceinline__ float2 to_x<true>(double2 v) {
-  return make_float2(__double2float_rn(v.x), __double2float_rn(v.y));
-}
-__device__ __forceinline__ double2 xfma(double2 a, double2 b, double2 acc) { return cfma(a, b, acc); }
-__device__ __forceinline__ float2 xfma(float2 a, float2 b, float2 acc) { return cfmaf(a, b, acc); }
-
-// exp(f T_s) e0: the DPP-row variant for s <= 16, the general one beyond.
-__device__ __forceinline__ void expm_e0_any(int s, const double2* alpha, const double* beta, double2 f,
-                                            double2* c) {
-  if (s <= 16)
-    expm_e0_16(s, alpha, beta, f, c);
-  else
-    expm_e0(s, alpha, beta, f, c);
-}
 
 // One wave per wavepacket, R rows per lane (domain <= 64 R rows).  WM > 0:
 // the ELL entries of the lane's rows (width <= WM) are held in registers
@@ -433,6 +389,58 @@ inline double2 h_cmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
+// B_s per CSR entry: sum over pairs p of K_p * sum_{q in term_p, field_q = s} M_q
+std::vector<double2> entry_values(const rm_plan* p) {
+  const int64_t nnz = p->nnz;
+  const int32_t nf = std::max(p->nfield, 1);
+  std::vector<double2> B((size_t)nf * (size_t)nnz, make_double2(0.0, 0.0));
+  for (int64_t e = 0; e < nnz; ++e)
+    for (int64_t q = p->h_pair_ptr[e]; q < p->h_pair_ptr[e + 1]; ++q) {
+      const int32_t t = p->h_pair_term[q];
+      const double2 kv = p->h_pair_kval[q];
+      for (int64_t m = p->h_term_ptr[t]; m < p->h_term_ptr[t + 1]; ++m) {
+        double2& d = B[(size_t)p->h_term_field[m] * nnz + e];
+        const double2 pr = h_cmul(kv, p->h_term_mval[m]);
+        d.x += pr.x;
+        d.y += pr.y;
+      }
+    }
+  return B;
+}
+
+// The B_s as CSR in permuted row and column order (the cooperative kernel's
+// operator); rows keep the order of their entries.
+int csr_build(rm_plan* p, const std::vector<double2>& B) {
+  if (p->sub_rp) return RM_OK;
+  const int64_t n = p->n, nnz = p->nnz;
+  const int32_t nf = std::max(p->nfield, 1);
+  const std::vector<int32_t>& perm = p->h_sub_perm;
+  std::vector<int32_t> iperm((size_t)n);
+  for (int64_t ip = 0; ip < n; ++ip) iperm[perm[ip]] = (int32_t)ip;
+  std::vector<int64_t> rp((size_t)n + 1, 0);
+  for (int64_t ip = 0; ip < n; ++ip) {
+    const int32_t i = perm[ip];
+    rp[ip + 1] = rp[ip] + (p->h_row_ptr[i + 1] - p->h_row_ptr[i]);
+  }
+  std::vector<int32_t> ccol((size_t)nnz);
+  std::vector<double2> cval((size_t)nf * (size_t)nnz);
+  for (int64_t ip = 0; ip < n; ++ip) {
+    const int32_t i = perm[ip];
+    const int64_t r0 = p->h_row_ptr[i], len = p->h_row_ptr[i + 1] - r0;
+    for (int64_t j = 0; j < len; ++j) {
+      ccol[(size_t)(rp[ip] + j)] = iperm[p->h_col[r0 + j]];
+      for (int32_t s = 0; s < nf; ++s)
+        cval[(size_t)s * nnz + (size_t)(rp[ip] + j)] = B[(size_t)s * nnz + (size_t)(r0 + j)];
+    }
+  }
+  int rc = RM_OK;
+  if ((rc = upload(&p->sub_rp, rp.data(), n + 1)) || (rc = upload(&p->sub_ccol, ccol.data(), nnz)) ||
+      (rc = upload(&p->sub_cval, cval.data(), (int64_t)nf * nnz)))
+    return rc;
+  p->h_sub_rp.swap(rp);
+  return RM_OK;
+}
+
 // Connected components of the sparsity graph (host union-find), the row
 // permutation that makes them contiguous, and the per-slot operator B_s in
 // permuted ELL order; uploaded once per plan.
@@ -475,48 +483,45 @@ int sub_build(rm_plan* p) {
   }
   p->nsub = nc;
   p->max_sub = mx;
-  // B_s per CSR entry: sum over pairs p of K_p * sum_{q in term_p, field_q = s} M_q
   const int64_t nnz = p->nnz;
   const int32_t nf = std::max(p->nfield, 1);
-  std::vector<double2> B((size_t)nf * (size_t)nnz, make_double2(0.0, 0.0));
-  for (int64_t e = 0; e < nnz; ++e)
-    for (int64_t q = p->h_pair_ptr[e]; q < p->h_pair_ptr[e + 1]; ++q) {
-      const int32_t t = p->h_pair_term[q];
-      const double2 kv = p->h_pair_kval[q];
-      for (int64_t m = p->h_term_ptr[t]; m < p->h_term_ptr[t + 1]; ++m) {
-        double2& d = B[(size_t)p->h_term_field[m] * nnz + e];
-        const double2 pr = h_cmul(kv, p->h_term_mval[m]);
-        d.x += pr.x;
-        d.y += pr.y;
-      }
-    }
-  const int32_t W = p->ell_w;
-  const size_t ne = (size_t)W * (size_t)n;
-  std::vector<int32_t> hcol(ne);
-  std::vector<double2> hval((size_t)nf * ne, make_double2(0.0, 0.0));
-  for (int64_t ip = 0; ip < n; ++ip) {
-    const int32_t i = perm[ip];
-    const int64_t r0 = p->h_row_ptr[i], len = p->h_row_ptr[i + 1] - r0;
-    for (int32_t j = 0; j < W; ++j) {
-      const size_t q = (size_t)j * n + ip;
-      if (j < len) {
-        hcol[q] = iperm[p->h_col[r0 + j]];
-        for (int32_t s = 0; s < nf; ++s) hval[(size_t)s * ne + q] = B[(size_t)s * nnz + r0 + j];
-      } else {
-        hcol[q] = (int32_t)ip;
-      }
-    }
-  }
+  const std::vector<double2> B = entry_values(p);
+  int32_t mn = INT32_MAX;
+  for (int32_t c = 0; c < nc; ++c) mn = std::min(mn, hi[c] - lo[c]);
   p->h_sub_start.assign(lo.begin(), lo.end() - 1);
   p->h_sub_end = hi;
   p->h_sub_label_perm.resize((size_t)n);
   for (int64_t ip = 0; ip < n; ++ip) p->h_sub_label_perm[ip] = label[perm[ip]];
   int rc = RM_OK;
   if ((rc = upload(&p->sub_perm, perm.data(), n)) || (rc = upload(&p->sub_label, label.data(), n)) ||
-      (rc = upload(&p->sub_lo, lo.data(), nc)) || (rc = upload(&p->sub_hi, hi.data(), nc)) ||
-      (rc = upload(&p->sub_col, hcol.data(), (int64_t)ne)) ||
-      (rc = upload(&p->sub_val, hval.data(), (int64_t)((size_t)nf * ne))))
+      (rc = upload(&p->sub_lo, lo.data(), nc)) || (rc = upload(&p->sub_hi, hi.data(), nc)))
     return rc;
+  // ELL (column-major) for the one-wave kernel: only if some subspace fits a wave
+  const int32_t W = p->ell_w;
+  if (mn <= RM_SUBSPACE_MAX_ROWS) {
+    const size_t ne = (size_t)W * (size_t)n;
+    std::vector<int32_t> hcol(ne);
+    std::vector<double2> hval((size_t)nf * ne, make_double2(0.0, 0.0));
+    for (int64_t ip = 0; ip < n; ++ip) {
+      const int32_t i = perm[ip];
+      const int64_t r0 = p->h_row_ptr[i], len = p->h_row_ptr[i + 1] - r0;
+      for (int32_t j = 0; j < W; ++j) {
+        const size_t q = (size_t)j * n + ip;
+        if (j < len) {
+          hcol[q] = iperm[p->h_col[r0 + j]];
+          for (int32_t s = 0; s < nf; ++s) hval[(size_t)s * ne + q] = B[(size_t)s * nnz + r0 + j];
+        } else {
+          hcol[q] = (int32_t)ip;
+        }
+      }
+    }
+    if ((rc = upload(&p->sub_col, hcol.data(), (int64_t)ne)) ||
+        (rc = upload(&p->sub_val, hval.data(), (int64_t)((size_t)nf * ne))))
+      return rc;
+  }
+  p->h_sub_perm.swap(perm);
+  // CSR (row-major, permuted) for the cooperative kernel: if some subspace does not fit
+  if (mx > RM_SUBSPACE_MAX_ROWS && (rc = csr_build(p, B))) return rc;
   p->sub_w = W;
   p->sub_ready = 1;
   return RM_OK;
@@ -732,6 +737,10 @@ SubOp plan_op(const rm_plan* p) {
   memset(&op, 0, sizeof(op));
   op.col = p->sub_col;
   op.val = p->sub_val;
+  op.rp = p->sub_rp;
+  op.ccol = p->sub_ccol;
+  op.cval = p->sub_cval;
+  op.nnz = p->nnz;
   op.perm = p->sub_perm;
   op.n = (int32_t)p->n;
   op.w = p->sub_w;
@@ -767,19 +776,25 @@ extern "C" int rm_plan_set_precision(rm_plan* p, int32_t precision) {
 int sub_propagate(rm_plan* p, void* psi, int64_t ld, int64_t ncols, int64_t nsteps,
                   const double* fprod_dev, double f_re, double f_im, const void* phase,
                   int32_t krylov_fixed, int32_t maxorder, double tol, int32_t* kused,
-                  int32_t* status, hipStream_t s, bool* done) {
+                  int32_t* status, bool force_coop, hipStream_t s, bool* done) {
   *done = false;
   if (nsteps > INT32_MAX) return RM_OK;
   int rc = sub_build(p);
   if (rc) return rc;
   if ((rc = sub_domains(p, psi, ld, ncols, s))) return rc;
-  std::vector<int4> cls[NCLASS];
+  std::vector<int4> cls[NCLASS], big;
   for (int64_t b = 0; b < ncols; ++b) {
     const int32_t lo = p->h_dom[2 * b], hi = p->h_dom[2 * b + 1];
     if (hi <= lo) continue;   // zero column: stays zero (kused 0)
-    if (hi - lo > RM_SUBSPACE_MAX_ROWS) return RM_OK;
-    cls[sub_class(p->sub_w, hi - lo)].push_back(make_int4((int)b, lo, hi - lo, 0));
+    if (force_coop || hi - lo > RM_SUBSPACE_MAX_ROWS || !p->sub_col)
+      big.push_back(make_int4((int)b, lo, hi - lo, 0));
+    else
+      cls[sub_class(p->sub_w, hi - lo)].push_back(make_int4((int)b, lo, hi - lo, 0));
   }
+  // the cooperative kernel takes long domains one wavepacket at a time: for
+  // many of them the full-vector kernels (all columns at once) are faster
+  if (!force_coop && (int64_t)big.size() > COOP_MAX_COLS) return RM_OK;
+  if (!big.empty() && !p->sub_rp && (rc = csr_build(p, entry_values(p)))) return rc;
   for (auto& v : cls)
     std::stable_sort(v.begin(), v.end(), [](const int4& x, const int4& y) { return x.z > y.z; });
   if ((rc = grow(&p->sflag, &p->sflag_cap, nsteps))) return rc;
@@ -806,8 +821,17 @@ int sub_propagate(rm_plan* p, void* psi, int64_t ld, int64_t ncols, int64_t nste
   op.nsteps = (int32_t)nsteps;
   if (!p->sub_scratch) p->sub_scratch = new (std::nothrow) SubScratch();
   if (!p->sub_scratch) return fail(RM_ERR_ALLOC, "rm_propagate: host allocation");
-  return sub_run((SubScratch*)p->sub_scratch, ops, cls, krylov_fixed > 0 ? krylov_fixed : maxorder,
-                 krylov_fixed, maxorder, tol, p->precision == RM_PREC_FP32, p->n_cu, s, p);
+  const int32_t kcap = krylov_fixed > 0 ? krylov_fixed : maxorder;
+  const bool f32 = p->precision == RM_PREC_FP32;
+  if ((rc = sub_run((SubScratch*)p->sub_scratch, ops, cls, kcap, krylov_fixed, maxorder, tol, f32, p->n_cu,
+                    s, p)))
+    return rc;
+  if (big.empty()) return RM_OK;
+  if (p->timing) HIPCHK(timing_begin(p, s));
+  if ((rc = coop_run(p, &p->coop_scratch, op, big, kcap, krylov_fixed, maxorder, tol, f32, p->n_cu, s)))
+    return rc;
+  if (p->timing) HIPCHK(timing_end(p, s));
+  return RM_OK;
 }
 
 // ---------------------------------------------------------------------------

@@ -50,6 +50,7 @@ class Ensemble:
         self.plan = None
         self._active = None
         self.kernel = "auto"
+        self.precision = "fp64"
         dev = torch.device("cuda", torch.cuda.current_device())
         self.psi = torch.as_tensor(np.ascontiguousarray(psi0, dtype=np.complex128)).to(dev)
         self.out = torch.empty_like(self.psi)
@@ -76,8 +77,17 @@ class Ensemble:
         self.plan.set_field(self.H, self.stream)
 
     def set_kernel(self, kernel):
-        """'auto', 'tuned', 'global', 'resident', 'ell' or 'ell:<i>'."""
+        """'auto', 'tuned', 'global', 'resident', 'ell', 'ell:<i>' or 'subspace'."""
         self.kernel = kernel
+        if self.plan is not None:
+            self._apply_kernel()
+
+    def set_precision(self, precision):
+        """'fp64' or 'fp32' (FP32 operator and matvec inputs, FP64 elsewhere)."""
+        from . import _lib
+        if precision not in _lib.PRECISIONS:
+            raise ValueError(f"unknown precision '{precision}'")
+        self.precision = precision
         if self.plan is not None:
             self._apply_kernel()
 
@@ -106,13 +116,18 @@ class Ensemble:
         return torch.from_numpy(tab).to(self.psi.device)
 
     def _apply_kernel(self):
-        """Set the plan's kernel; 'tuned' takes the autotuner's cached choice
-        for this shape, or 'auto' when the shape was never tuned."""
+        """Set the plan's kernel and precision; 'tuned' takes the autotuner's
+        cached choice for this shape, or 'auto' when the shape was never tuned.
+        Plans are shared between ensembles over the same operator, so this
+        runs before every launch."""
         k = self.kernel
         if k == "tuned":
             from . import autotune
             k = autotune.lookup(self.plan, self.psi.shape[0], self.krylov_fixed) or "auto"
-        self.plan.set_kernel(k)
+        if self.plan.kernel != k:
+            self.plan.set_kernel(k)
+        if self.plan.precision != self.precision:
+            self.plan.set_precision(self.precision)
 
     def propagate(self, fields, thresh=None, table=None, kused=None):
         """Advance every row through len(fields) steps with one device call
@@ -120,12 +135,14 @@ class Ensemble:
         if table is None:
             table = self.prepare(fields, thresh)
         th = None if thresh is None else [thresh]
+        self._apply_kernel()
         self.plan.propagate(self.psi, self.out, table, self.f, phase=self.phase, thresh=th,
                             krylov_fixed=self.krylov_fixed, maxorder=self.maxorder, tol=self.tol,
                             kused=kused, status=self.status, stream=self.stream)
 
     def launch(self):
         """The Krylov step of all rows with the field already on the device."""
+        self._apply_kernel()
         self.plan.step(self.psi, self.out, self.f, phase=self.phase, krylov_fixed=self.krylov_fixed,
                        maxorder=self.maxorder, tol=self.tol, kused=self.kused, status=self.status,
                        stream=self.stream)
@@ -140,5 +157,10 @@ class Ensemble:
         return out
 
     def check(self):
-        if self.krylov_fixed == 0 and bool((self.status != 0).any()):
+        from . import _lib
+        st = self.status.cpu().numpy()
+        if np.any(st & _lib.RM_DEVICE_TIMEOUT):
+            raise RuntimeError("a cooperative propagation launch timed out at a grid barrier; "
+                               "its wavepackets are invalid")
+        if self.krylov_fixed == 0 and bool(np.any(st & _lib.RM_NOT_CONVERGED)):
             raise ValueError(f"Lanczos reached maximum order of '{self.maxorder}' without convergence")

@@ -206,14 +206,16 @@ class DevicePlan:
                 pass
             self.handle = None
 
-    KERNELS = {"auto": 0, "global": 1, "resident": 2, "ell": 3, "subspace": 4}
+    KERNELS = {"auto": 0, "global": 1, "resident": 2, "ell": 3, "subspace": 4, "coop": 5}
     ELL_CFG0 = 16
 
     def set_kernel(self, variant):
         """Select the step kernel: 'auto', 'global', 'resident', 'ell',
-        'ell:<i>' for ELL launch configuration i (see ell_configs()), or
+        'ell:<i>' for ELL launch configuration i (see ell_configs()),
         'subspace' (propagate only: one wave per wavepacket on its invariant
-        subspace; an error when it does not apply)."""
+        subspaces, a cooperative grid-wide launch per wavepacket for long
+        domains; an error when it does not apply) or 'coop' (propagate only:
+        every wavepacket through the cooperative kernel)."""
         if isinstance(variant, str) and variant.startswith("ell:"):
             code = self.ELL_CFG0 + int(variant[4:])
         else:

@@ -206,6 +206,9 @@ class TDSE:
             plan = None
         else:
             plan = _plan.plan_for(H)
+            # plans are shared with ensembles that may have picked another kernel
+            if plan.kernel != "auto":
+                plan.set_kernel("auto")
 
         phase = None
         if H0 is not None:

@@ -47,7 +47,7 @@ def _device_run(h0, H, fields, X, *, kernel="subspace", fixed=0, tol=1e-15, dt=1
     e = ensemble.Ensemble(H, h0, f, X, krylov_fixed=fixed, tol=tol)
     e.set_kernel(kernel)
     table = e.prepare(fields)
-    e.plan.set_precision(precision)
+    e.set_precision(precision)
     kused = torch.zeros((len(fields), len(X)), dtype=torch.int32, device=e.psi.device)
     e.propagate(None, table=table, kused=kused)
     e.check()
@@ -191,18 +191,51 @@ def test_auto_equals_subspace_and_ell_agrees(device):
     assert np.abs(a - c).max() <= 1e-12
 
 
-def test_subspace_refuses_thresholds_and_long_domains(device):
-    """Forced 'subspace' raises when the path does not apply; 'auto' falls
-    back to the full-vector kernels with the same result as the oracle."""
+def test_subspace_refuses_thresholds_long_domains_go_cooperative(device):
+    """Forced 'subspace' raises with MF thresholds; domains longer than a
+    wave holds are advanced by the cooperative kernel (same result as the
+    oracle)."""
+    import torch
+    from richmol_amd import ensemble
     from richmol_amd._lib import RichmolAmdError
+    from richmol_amd.tdse import TDSE
     h0, H, cos2, bas = linear_rotor(40)
     n = 1681
     rng = np.random.default_rng(9)
     X = rng.normal(size=(2, n)) + 1j * rng.normal(size=(2, n))
     X /= np.linalg.norm(X, axis=1)[:, None]
     fields = [[0.3e9, 0.0, 1e9]] * 2     # X+Z: one parity block holds ~840 states, domain = all 1681
+    f = TDSE(t_end=20.0, dt=10.0, t_units="fs").exp_fac
+    e = ensemble.Ensemble(H, h0, f, X)
+    e.set_kernel("subspace")
+    table = e.prepare(fields, thresh=1e-3)
     with pytest.raises(RichmolAmdError):
-        _device_run(h0, H, fields, X, kernel="subspace")
-    psi_d, k_d, _ = _device_run(h0, H, fields, X, kernel="auto")
+        e.propagate(None, thresh=1e-3, table=table)
+    psi_d, k_d, _ = _device_run(h0, H, fields, X, kernel="subspace")
     psi_o, k_o = _oracle_run(h0, H, fields, X)
     assert np.abs(psi_d - psi_o).max() <= TOL_PSI
+    assert np.array_equal(k_d, k_o)
+
+
+def test_fp32_thermal_ensemble_cos2(device):
+    """FP32-matvec mode on the headline workload's shape (J<=40, Boltzmann
+    columns at 1 K, fixed m=12) through the pulse peak: every column's
+    <cos^2 theta> within 1e-6 of the FP64 oracle after each block of steps,
+    and the ensemble average too."""
+    from oracle.cartens import OracleCarTens
+    from richmol_amd import ensemble as _ens
+    h0, H, cos2, bas = linear_rotor(40)
+    n = 1681
+    idx = _ens.thermal_sample(bas, 1.0, 48, seed=11)
+    X = _basis_columns(n, idx)
+    C = OracleCarTens.from_attrs(cos2).tomat(form="full", cart="0")
+    c2 = lambda P: np.real(np.einsum("bi,bi->b", P.conj(), (C @ P.T).T)) + 1 / 3
+    psi_d, psi_o = X.copy(), X.copy()
+    for blk in range(3):
+        t = (np.arange(8) + 26 + 8 * blk) * 10.0 + 5.0
+        fields = [[0.0, 0.0, gaussian_pulse_field(tt)] for tt in t]
+        psi_d, _, _ = _device_run(h0, H, fields, psi_d, kernel="auto", fixed=12, precision="fp32")
+        psi_o, _ = _oracle_run(h0, H, fields, psi_o, fixed=12)
+        err = np.abs(c2(psi_d) - c2(psi_o))
+        assert err.max() <= 1e-6, (blk, err.max())
+        assert abs(c2(psi_d).mean() - c2(psi_o).mean()) <= 1e-6
