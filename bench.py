@@ -150,9 +150,12 @@ def asymtop_leg(nsteps, cpu_steps=3):
     kms, nl = ens.plan.kernel_time()
     ens.plan.timing(False)
     ens.check()
+    rows, ents = ens.plan.domain_sizes(X)
     out = {"n_states": n, "nnz": ens.plan.nnz, "steps": nsteps, "dt_fs": dt,
            "wavepacket_steps_per_s": nsteps / wall, "ms_per_step": wall / nsteps * 1e3,
-           "krylov_kernel_ms_per_step": kms / max(nl, 1), "kernel": ens.plan.kernel,
+           "kernel_ms_per_step": kms / nsteps, "launches": nl,
+           "domain_rows": int(rows[0]), "domain_entries": int(ents[0]),
+           "kernel": "k_long (cooperative form, one launch for the run)" if rows[0] > 1024 else ens.plan.kernel,
            "mean_krylov_dim": float(kused.float().mean()),
            "norm_err": float(abs(np.linalg.norm(ens.psi.cpu().numpy()) - 1)),
            "ground_population_end": float(abs(ens.psi[0, 0].item()) ** 2)}

@@ -157,19 +157,22 @@ int rm_plan_kernel_time(rm_plan* plan, double* total_ms, int64_t* launches);
  * equal the full-vector kernels' to rounding (entries outside the domain stay
  * exactly zero).  Domains of more than RM_SUBSPACE_MAX_ROWS rows are advanced
  * by a cooperative launch (every workgroup of the device on one wavepacket,
- * grid barriers between the Lanczos reductions), one wavepacket at a time;
- * with more than 32 such wavepackets RM_KERNEL_AUTO takes the full-vector
- * kernels instead.
+ * grid barriers between the Lanczos reductions) when there are at most 8 of
+ * them, otherwise by the block kernel (one workgroup per wavepacket, all in
+ * one launch).
  *
- * RM_KERNEL_COOP (rm_propagate only): every wavepacket through the
- * cooperative kernel, whatever its domain (no MF thresholds). */
+ * RM_KERNEL_COOP, RM_KERNEL_BLOCK (rm_propagate only): every wavepacket
+ * through the cooperative, or the block, kernel whatever its domain (no MF
+ * thresholds). */
 #define RM_KERNEL_AUTO      0
 #define RM_KERNEL_GLOBAL    1
 #define RM_KERNEL_RESIDENT  2
 #define RM_KERNEL_ELL       3
 #define RM_KERNEL_SUBSPACE  4
 #define RM_KERNEL_COOP      5
+#define RM_KERNEL_BLOCK     6
 #define RM_SUBSPACE_MAX_ROWS 1024
+#define RM_BLOCK_MAX_ROWS   65536   /* longest domain of an rm_batch_propagate job */
 /* RM_KERNEL_ELL_CFG0 + i: ELL kernel with explicit launch configuration i
  * (rm_ell_config() describes it); the autotuner's search space. */
 #define RM_KERNEL_ELL_CFG0  16
@@ -213,12 +216,14 @@ int rm_plan_step_flags(rm_plan* plan, int64_t nsteps, const double* fprod_dev,
 /*
  * Batches of independent propagation jobs (no reference counterpart: the
  * serving loop's device call).  One rm_batch_propagate call advances every
- * job by its own number of steps with the subspace kernel: one wave per job,
- * all jobs in as many launches as there are launch classes (domain size and
- * operator width; a job's result does not depend on the other jobs of the
- * call, bit for bit).  Jobs may share plans; each has its own wavepacket,
- * phases and field table.  Calls on one batch object must use one stream
- * (descriptor staging is double-buffered and reused in call order).
+ * job by its own number of steps with the subspace kernels: one wave per job
+ * on a domain of up to RM_SUBSPACE_MAX_ROWS rows, one workgroup per job on a
+ * longer one, all jobs in as many launches as there are launch classes
+ * (domain size and operator width; a job's result does not depend on the
+ * other jobs of the call, bit for bit).  Jobs may share plans; each has its
+ * own wavepacket, phases and field table.  Calls on one batch object must
+ * use one stream (descriptor staging is double-buffered and reused in call
+ * order).
  */
 typedef struct rm_batch rm_batch;
 typedef struct {
@@ -226,7 +231,7 @@ typedef struct {
   void* psi;                 /* device: the wavepacket is column `col` at psi + col*ld */
   int64_t ld;
   int32_t col;
-  int32_t lo, len;           /* its domain, from rm_plan_domains (len <= RM_SUBSPACE_MAX_ROWS) */
+  int32_t lo, len;           /* its domain, from rm_plan_domains (len <= RM_BLOCK_MAX_ROWS) */
   const double* fprod_dev;   /* device: field products of this call's steps, nsteps x nfield complex */
   const int32_t* sflag_dev;  /* device: rm_plan_step_flags of those rows */
   int32_t nsteps;

@@ -142,7 +142,6 @@ struct rm_plan {
   int64_t sflag_cap = 0;
   std::vector<int32_t> h_dom;
   void* sub_scratch = nullptr;   // launch staging and workspace (rm_subspace.hip)
-  void* coop_scratch = nullptr;  // workspace of the cooperative kernel (rm_coop.hip)
   // optional HIP-event timing of the Krylov step launches
   int timing = 0;
   int nrec = 0;
@@ -200,12 +199,21 @@ static int grow(T** buf, int64_t* cap, int64_t count) {
 int ensure_ws(rm_plan* p, size_t bytes);
 // Free a plan's subspace-launch scratch (rm_subspace.hip).
 void sub_scratch_release(void* sc);
-// Free a plan's cooperative-kernel workspace (rm_coop.hip).
+// Free the long-domain kernels' workspace (rm_coop.hip).
 void coop_scratch_release(void* sc);
-// Propagate wavepackets (column, lo, len, -) of one job with the cooperative
-// kernel, one launch each (rm_coop.hip).
-int coop_run(rm_plan* p, void** scratch, const SubOp& op, const std::vector<int4>& items, int32_t kcap,
-             int32_t fixed_m, int32_t maxorder, double tol, bool f32, int n_cu, hipStream_t s);
+// A long-domain wavepacket: (column, lo, len, op), its domain's stored
+// entries, longest row and field slots.
+struct LongItem {
+  int4 item;
+  int64_t nent;
+  int32_t maxrow, nslot;
+};
+// Long-domain wavepackets (rm_coop.hip): blk one workgroup each in one launch,
+// coop the whole device each (one cooperative launch each).  d_ops, d_blk,
+// d_coop: device copies of the jobs and of the two item lists.
+int long_run(void** scratch, const SubOp* d_ops, const int4* d_blk, const std::vector<LongItem>& blk,
+             const int4* d_coop, const std::vector<LongItem>& coop, int32_t kcap, int32_t fixed_m,
+             int32_t maxorder, double tol, bool f32, int n_cu, hipStream_t s);
 // Event-bracketed launch helpers for the plan's optional timing.
 hipError_t timing_begin(rm_plan* p, hipStream_t s);
 hipError_t timing_end(rm_plan* p, hipStream_t s);

@@ -29,7 +29,7 @@ namespace {
 
 constexpr int NSR = 2;                         // field slots held in registers
 constexpr size_t SUB_LDS_BASIS = 32 * 1024;    // LDS budget of a wave's Krylov basis
-constexpr int64_t COOP_MAX_COLS = 32;          // long-domain wavepackets per call for the cooperative kernel
+constexpr int64_t COOP_MAX_COLS = 8;           // long-domain wavepackets per call for the cooperative kernel
 
 struct SubArgs {
   const SubOp* ops;
@@ -614,6 +614,7 @@ struct SubScratch {
   int cur = 0;
   void* ws = nullptr;
   size_t ws_bytes = 0;
+  void* long_ws = nullptr;   // the long-domain kernels' workspace (rm_coop.hip)
 };
 
 void scratch_release(SubScratch* sc) {
@@ -625,15 +626,19 @@ void scratch_release(SubScratch* sc) {
     if (sc->done[i]) (void)hipEventDestroy(sc->done[i]);
   }
   if (sc->ws) (void)hipFree(sc->ws);
+  coop_scratch_release(sc->long_ws);
   delete sc;
 }
 
-// Stage ops and items (items grouped by class) and launch one kernel per
-// non-empty class on s; with timing on, the launches are bracketed as one.
+// Stage ops and items (the wave classes, then the block and cooperative
+// lists) and launch one kernel per non-empty wave class, the block kernel
+// and the cooperative launches on s; with timing on, the launches are
+// bracketed as one.
 int sub_run(SubScratch* sc, const std::vector<SubOp>& ops, std::vector<int4> (&cls)[NCLASS],
-            int32_t kcap, int32_t fixed_m, int32_t maxorder, double tol, bool f32, int n_cu,
-            hipStream_t s, rm_plan* timed) {
-  size_t nitems = 0;
+            const std::vector<LongItem>& blk, const std::vector<LongItem>& coop, int32_t kcap,
+            int32_t fixed_m, int32_t maxorder, double tol, bool f32, int n_cu, hipStream_t s,
+            rm_plan* timed) {
+  size_t nitems = blk.size() + coop.size();
   for (int c = 0; c < NCLASS; ++c) nitems += cls[c].size();
   if (nitems == 0) return RM_OK;
   const size_t ops_bytes = (ops.size() * sizeof(SubOp) + 255) & ~(size_t)255;
@@ -659,6 +664,16 @@ int sub_run(SubScratch* sc, const std::vector<SubOp>& ops, std::vector<int4> (&c
     off[c] = pos;
     if (!cls[c].empty()) memcpy(sc->host[bi] + pos, cls[c].data(), cls[c].size() * sizeof(int4));
     pos += cls[c].size() * sizeof(int4);
+  }
+  const size_t off_blk = pos;
+  for (const LongItem& it : blk) {
+    memcpy(sc->host[bi] + pos, &it.item, sizeof(int4));
+    pos += sizeof(int4);
+  }
+  const size_t off_coop = pos;
+  for (const LongItem& it : coop) {
+    memcpy(sc->host[bi] + pos, &it.item, sizeof(int4));
+    pos += sizeof(int4);
   }
   HIPCHK(hipMemcpyAsync(sc->dev[bi], sc->host[bi], bytes, hipMemcpyHostToDevice, s));
 
@@ -702,6 +717,9 @@ int sub_run(SubScratch* sc, const std::vector<SubOp>& ops, std::vector<int4> (&c
     a.cscr = a.gram + (fixed_m > 0 ? 0 : (size_t)waves * kcap * kcap);
     HIPCHK(launch_class(c, a, f32, n_cu, s, &waves, false));
   }
+  int rc = long_run(&sc->long_ws, (const SubOp*)sc->dev[bi], (const int4*)(sc->dev[bi] + off_blk), blk,
+                    (const int4*)(sc->dev[bi] + off_coop), coop, kcap, fixed_m, maxorder, tol, f32, n_cu, s);
+  if (rc) return rc;
   if (timed) HIPCHK(timing_end(timed, s));
   if (!sc->done[bi]) HIPCHK(hipEventCreateWithFlags(&sc->done[bi], hipEventDisableTiming));
   HIPCHK(hipEventRecord(sc->done[bi], s));
@@ -769,32 +787,53 @@ extern "C" int rm_plan_set_precision(rm_plan* p, int32_t precision) {
   return RM_OK;
 }
 
+// A long-domain item of plan p with its domain's entry count and longest row.
+LongItem long_item(const rm_plan* p, int col, int lo, int len, int op) {
+  LongItem it;
+  it.item = make_int4(col, lo, len, op);
+  it.nent = p->h_sub_rp[lo + len] - p->h_sub_rp[lo];
+  int64_t mr = 0;
+  for (int64_t r = lo; r < (int64_t)lo + len; ++r) mr = std::max(mr, p->h_sub_rp[r + 1] - p->h_sub_rp[r]);
+  it.maxrow = (int32_t)std::min<int64_t>(mr, INT32_MAX);
+  it.nslot = p->nfield;
+  return it;
+}
+
+void sort_long(std::vector<LongItem>& v) {
+  std::stable_sort(v.begin(), v.end(), [](const LongItem& x, const LongItem& y) { return x.nent > y.nent; });
+}
+
 // The subspace path of rm_propagate.  Arguments are validated by the caller
-// (ld, ncols, Krylov parameters, pointers).  *done = false when the path does
-// not apply to these wavepackets (a domain longer than RM_SUBSPACE_MAX_ROWS);
-// nothing has then been written to psi.
+// (ld, ncols, Krylov parameters, pointers).  Domains of up to
+// RM_SUBSPACE_MAX_ROWS rows go to the one-wave kernel; longer ones to the
+// cooperative kernel when there are at most COOP_MAX_COLS of them, else to
+// the block kernel.  force: RM_KERNEL_COOP / RM_KERNEL_BLOCK send every
+// wavepacket to that kernel (0: the rule above).  *done = false when the
+// path does not apply; nothing has then been written to psi.
 int sub_propagate(rm_plan* p, void* psi, int64_t ld, int64_t ncols, int64_t nsteps,
                   const double* fprod_dev, double f_re, double f_im, const void* phase,
                   int32_t krylov_fixed, int32_t maxorder, double tol, int32_t* kused,
-                  int32_t* status, bool force_coop, hipStream_t s, bool* done) {
+                  int32_t* status, int force, hipStream_t s, bool* done) {
   *done = false;
   if (nsteps > INT32_MAX) return RM_OK;
   int rc = sub_build(p);
   if (rc) return rc;
   if ((rc = sub_domains(p, psi, ld, ncols, s))) return rc;
-  std::vector<int4> cls[NCLASS], big;
+  std::vector<int4> cls[NCLASS];
+  std::vector<int4> big;
   for (int64_t b = 0; b < ncols; ++b) {
     const int32_t lo = p->h_dom[2 * b], hi = p->h_dom[2 * b + 1];
     if (hi <= lo) continue;   // zero column: stays zero (kused 0)
-    if (force_coop || hi - lo > RM_SUBSPACE_MAX_ROWS || !p->sub_col)
+    if (force || hi - lo > RM_SUBSPACE_MAX_ROWS || !p->sub_col)
       big.push_back(make_int4((int)b, lo, hi - lo, 0));
     else
       cls[sub_class(p->sub_w, hi - lo)].push_back(make_int4((int)b, lo, hi - lo, 0));
   }
-  // the cooperative kernel takes long domains one wavepacket at a time: for
-  // many of them the full-vector kernels (all columns at once) are faster
-  if (!force_coop && (int64_t)big.size() > COOP_MAX_COLS) return RM_OK;
   if (!big.empty() && !p->sub_rp && (rc = csr_build(p, entry_values(p)))) return rc;
+  std::vector<LongItem> blk, coop;
+  const bool to_coop = force == RM_KERNEL_COOP || (force == 0 && (int64_t)big.size() <= COOP_MAX_COLS);
+  for (const int4& it : big) (to_coop ? coop : blk).push_back(long_item(p, it.x, it.y, it.z, 0));
+  sort_long(blk);
   for (auto& v : cls)
     std::stable_sort(v.begin(), v.end(), [](const int4& x, const int4& y) { return x.z > y.z; });
   if ((rc = grow(&p->sflag, &p->sflag_cap, nsteps))) return rc;
@@ -822,16 +861,8 @@ int sub_propagate(rm_plan* p, void* psi, int64_t ld, int64_t ncols, int64_t nste
   if (!p->sub_scratch) p->sub_scratch = new (std::nothrow) SubScratch();
   if (!p->sub_scratch) return fail(RM_ERR_ALLOC, "rm_propagate: host allocation");
   const int32_t kcap = krylov_fixed > 0 ? krylov_fixed : maxorder;
-  const bool f32 = p->precision == RM_PREC_FP32;
-  if ((rc = sub_run((SubScratch*)p->sub_scratch, ops, cls, kcap, krylov_fixed, maxorder, tol, f32, p->n_cu,
-                    s, p)))
-    return rc;
-  if (big.empty()) return RM_OK;
-  if (p->timing) HIPCHK(timing_begin(p, s));
-  if ((rc = coop_run(p, &p->coop_scratch, op, big, kcap, krylov_fixed, maxorder, tol, f32, p->n_cu, s)))
-    return rc;
-  if (p->timing) HIPCHK(timing_end(p, s));
-  return RM_OK;
+  return sub_run((SubScratch*)p->sub_scratch, ops, cls, blk, coop, kcap, krylov_fixed, maxorder, tol,
+                 p->precision == RM_PREC_FP32, p->n_cu, s, p);
 }
 
 // ---------------------------------------------------------------------------
@@ -911,6 +942,8 @@ extern "C" int rm_batch_propagate(rm_batch* bt, const rm_job* jobs, int32_t njob
   ops.reserve((size_t)njobs);
   std::vector<int4> cls[NCLASS];
   std::vector<int64_t> work[NCLASS];
+  std::vector<LongItem> blk;
+  std::vector<int64_t> blk_work;
   for (int32_t j = 0; j < njobs; ++j) {
     const rm_job& jb = jobs[j];
     rm_plan* p = jb.plan;
@@ -923,11 +956,13 @@ extern "C" int rm_batch_propagate(rm_batch* bt, const rm_job* jobs, int32_t njob
       return fail(RM_ERR_ARG, "rm_batch_propagate: job %d: bad wavepacket, field table or flags", j);
     // the domain must be a union of whole subspaces, or gathers would leave it
     const int32_t lo = jb.lo, len = jb.len;
-    if (len < 1 || len > RM_SUBSPACE_MAX_ROWS || lo < 0 || (int64_t)lo + len > p->n ||
+    if (len < 1 || len > RM_BLOCK_MAX_ROWS || lo < 0 || (int64_t)lo + len > p->n ||
         p->h_sub_start[p->h_sub_label_perm[lo]] != lo ||
         p->h_sub_end[p->h_sub_label_perm[lo + len - 1]] != lo + len)
       return fail(RM_ERR_ARG, "rm_batch_propagate: job %d: [%d, %d) is not a domain of %d rows at most "
-                  "(use rm_plan_domains)", j, lo, lo + len, RM_SUBSPACE_MAX_ROWS);
+                  "(use rm_plan_domains)", j, lo, lo + len, RM_BLOCK_MAX_ROWS);
+    const bool longdom = len > RM_SUBSPACE_MAX_ROWS || !p->sub_col;
+    if (longdom && !p->sub_rp && (rc = csr_build(p, entry_values(p)))) return rc;
     SubOp op = plan_op(p);
     op.phase = (const double2*)jb.phase;
     op.table = (const double2*)jb.fprod_dev;
@@ -940,9 +975,14 @@ extern "C" int rm_batch_propagate(rm_batch* bt, const rm_job* jobs, int32_t njob
     op.status = jb.status;
     op.f = make_double2(jb.f_re, jb.f_im);
     op.nsteps = jb.nsteps;
-    const int c = sub_class(p->sub_w, len);
-    cls[c].push_back(make_int4(jb.col, lo, len, (int)ops.size()));
-    work[c].push_back((int64_t)jb.nsteps * len);
+    if (longdom) {
+      blk.push_back(long_item(p, jb.col, lo, len, (int)ops.size()));
+      blk_work.push_back((int64_t)jb.nsteps * blk.back().nent);
+    } else {
+      const int c = sub_class(p->sub_w, len);
+      cls[c].push_back(make_int4(jb.col, lo, len, (int)ops.size()));
+      work[c].push_back((int64_t)jb.nsteps * len);
+    }
     ops.push_back(op);
   }
   // longest jobs first within a class (the waves take items in order)
@@ -954,6 +994,16 @@ extern "C" int rm_batch_propagate(rm_batch* bt, const rm_job* jobs, int32_t njob
     for (size_t t = 0; t < ix.size(); ++t) sorted[t] = cls[c][ix[t]];
     cls[c].swap(sorted);
   }
-  return sub_run(bt->sc, ops, cls, krylov_fixed > 0 ? krylov_fixed : maxorder, krylov_fixed, maxorder,
-                 tol, precision == RM_PREC_FP32, bt->n_cu, (hipStream_t)stream, nullptr);
+  // long domains: one workgroup each (block kernel), most work first
+  {
+    std::vector<size_t> ix(blk.size());
+    std::iota(ix.begin(), ix.end(), 0);
+    std::stable_sort(ix.begin(), ix.end(), [&](size_t x, size_t y) { return blk_work[x] > blk_work[y]; });
+    std::vector<LongItem> sorted(ix.size());
+    for (size_t t = 0; t < ix.size(); ++t) sorted[t] = blk[ix[t]];
+    blk.swap(sorted);
+  }
+  const std::vector<LongItem> none;
+  return sub_run(bt->sc, ops, cls, blk, none, krylov_fixed > 0 ? krylov_fixed : maxorder, krylov_fixed,
+                 maxorder, tol, precision == RM_PREC_FP32, bt->n_cu, (hipStream_t)stream, nullptr);
 }

@@ -41,7 +41,6 @@ static void plan_free(rm_plan* p) {
                   p->sub_hi, p->sub_col, p->sub_val, p->sub_rp, p->sub_ccol, p->sub_cval, p->dom,
                   p->sflag};
   sub_scratch_release(p->sub_scratch);
-  coop_scratch_release(p->coop_scratch);
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (hipEvent_t e : p->ev0) (void)hipEventDestroy(e);
@@ -1350,7 +1349,7 @@ extern "C" int rm_ell_config(int32_t i, int32_t* threads, int32_t* rows, int32_t
 extern "C" int rm_plan_set_kernel(rm_plan* p, int32_t variant) {
   if (!p) return fail(RM_ERR_ARG, "rm_plan_set_kernel: NULL plan");
   const bool cfg = variant >= RM_KERNEL_ELL_CFG0 && variant < RM_KERNEL_ELL_CFG0 + N_ELL_CFGS;
-  if (!cfg && (variant < RM_KERNEL_AUTO || variant > RM_KERNEL_COOP))
+  if (!cfg && (variant < RM_KERNEL_AUTO || variant > RM_KERNEL_BLOCK))
     return fail(RM_ERR_ARG, "rm_plan_set_kernel: unknown variant %d", variant);
   if (cfg && p->n > (int64_t)ELL_CFGS[variant - RM_KERNEL_ELL_CFG0].ntr * ELL_CFGS[variant - RM_KERNEL_ELL_CFG0].r)
     return fail(RM_ERR_ARG, "rm_plan_set_kernel: n=%lld exceeds ELL configuration %d",
@@ -1407,7 +1406,8 @@ hipError_t timing_end(rm_plan* p, hipStream_t s) {
 static hipError_t timed_launch_step(rm_plan* p, const StepArgs& a, unsigned ncols, hipStream_t s) {
   hipError_t e = timing_begin(p, s);
   if (e != hipSuccess) return e;
-  const bool prop_only = p->variant == RM_KERNEL_SUBSPACE || p->variant == RM_KERNEL_COOP;
+  const bool prop_only = p->variant == RM_KERNEL_SUBSPACE || p->variant == RM_KERNEL_COOP ||
+                         p->variant == RM_KERNEL_BLOCK;
   e = launch_step(a, ncols, s, prop_only ? RM_KERNEL_AUTO : p->variant, p->n_cu);
   if (e != hipSuccess) return e;
   return timing_end(p, s);
@@ -1512,7 +1512,7 @@ __global__ void k_copy_cols(int64_t n, int64_t ld, const double2* __restrict__ s
 int sub_propagate(rm_plan* p, void* psi, int64_t ld, int64_t ncols, int64_t nsteps,
                   const double* fprod_dev, double f_re, double f_im, const void* phase,
                   int32_t krylov_fixed, int32_t maxorder, double tol, int32_t* kused,
-                  int32_t* status, bool force_coop, hipStream_t s, bool* done);
+                  int32_t* status, int force, hipStream_t s, bool* done);
 
 extern "C" int rm_propagate(rm_plan* p, void* psi, void* work, int64_t ld, int64_t ncols,
                             int64_t nsteps, const double* fprod_dev, const double* thresh,
@@ -1538,18 +1538,19 @@ extern "C" int rm_propagate(rm_plan* p, void* psi, void* work, int64_t ld, int64
       fa.thresh[i] = thresh[i];
       any_thresh |= thresh[i] > 0.0;
     }
-  const bool sub_kernel = p->variant == RM_KERNEL_SUBSPACE || p->variant == RM_KERNEL_COOP;
+  const bool sub_kernel = p->variant == RM_KERNEL_SUBSPACE || p->variant == RM_KERNEL_COOP ||
+                          p->variant == RM_KERNEL_BLOCK;
   if ((p->variant == RM_KERNEL_AUTO || sub_kernel) && !any_thresh) {
     bool done = false;
+    const int force = p->variant == RM_KERNEL_COOP || p->variant == RM_KERNEL_BLOCK ? p->variant : 0;
     rc = sub_propagate(p, psi, ld, ncols, nsteps, fprod_dev, f_re, f_im, phase, krylov_fixed,
-                       maxorder, tol, kused, status, p->variant == RM_KERNEL_COOP, s, &done);
+                       maxorder, tol, kused, status, force, s, &done);
     if (rc) return rc;
     if (done) return RM_OK;
   }
   if (sub_kernel)
-    return fail(RM_ERR_ARG, "rm_propagate: the %s kernel does not apply (%s)",
-                p->variant == RM_KERNEL_COOP ? "cooperative" : "subspace",
-                any_thresh ? "MF thresholds given" : "more long-domain wavepackets than the cooperative kernel takes");
+    return fail(RM_ERR_ARG, "rm_propagate: the subspace kernels do not apply (%s)",
+                any_thresh ? "MF thresholds given" : "more than 2^31 steps");
   if (status) hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(ncols, 256)), dim3(256), 0, s, status, ncols, 0);
   a.status = status;
   a.status_sticky = 1;

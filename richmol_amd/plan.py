@@ -206,7 +206,7 @@ class DevicePlan:
                 pass
             self.handle = None
 
-    KERNELS = {"auto": 0, "global": 1, "resident": 2, "ell": 3, "subspace": 4, "coop": 5}
+    KERNELS = {"auto": 0, "global": 1, "resident": 2, "ell": 3, "subspace": 4, "coop": 5, "block": 6}
     ELL_CFG0 = 16
 
     def set_kernel(self, variant):
@@ -214,8 +214,9 @@ class DevicePlan:
         'ell:<i>' for ELL launch configuration i (see ell_configs()),
         'subspace' (propagate only: one wave per wavepacket on its invariant
         subspaces, a cooperative grid-wide launch per wavepacket for long
-        domains; an error when it does not apply) or 'coop' (propagate only:
-        every wavepacket through the cooperative kernel)."""
+        domains; an error when it does not apply), 'coop' or 'block'
+        (propagate only: every wavepacket through the cooperative kernel, or
+        through the one-workgroup-per-wavepacket block kernel)."""
         if isinstance(variant, str) and variant.startswith("ell:"):
             code = self.ELL_CFG0 + int(variant[4:])
         else:
