@@ -242,6 +242,11 @@ typedef struct {
 } rm_job;
 int rm_batch_create(rm_batch** out);
 int rm_batch_destroy(rm_batch* batch);
+/* Jobs whose domain has more than `rows` rows run on the block kernel (one
+ * workgroup per job) instead of one wave; 0 <= rows <= RM_SUBSPACE_MAX_ROWS,
+ * default RM_SUBSPACE_MAX_ROWS.  The class of a job still depends only on the
+ * job, so results stay independent of the mix. */
+int rm_batch_set_block_rows(rm_batch* batch, int32_t rows);
 /* job_bytes: sizeof(rm_job) as the caller laid the array out; any other
  * value is rejected (RM_ERR_ARG) before a job is read. */
 int rm_batch_propagate(rm_batch* batch, const rm_job* jobs, int32_t njobs, int32_t job_bytes,

@@ -51,6 +51,7 @@ SIGNATURES = {
     "rm_plan_step_flags": (ctypes.c_int, [_vp, ctypes.c_int64, _vp, _vp, _vp]),
     "rm_batch_create": (ctypes.c_int, [ctypes.POINTER(_vp)]),
     "rm_batch_destroy": (ctypes.c_int, [_vp]),
+    "rm_batch_set_block_rows": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "rm_batch_propagate": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                           ctypes.c_int32, ctypes.c_double, ctypes.c_int32, _vp]),
 }

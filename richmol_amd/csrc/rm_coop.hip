@@ -35,16 +35,16 @@ namespace {
 
 enum { GRID_REG = 0, GRID_MEM = 1, BLOCK = 2 };
 
-// threads per workgroup: 512 for a cooperative launch (256 VGPRs per lane),
-// 256 for a workgroup per wavepacket (two workgroups per CU)
+// threads per workgroup (256 VGPRs per lane)
 template <int MODE>
-constexpr int ct_of() { return MODE == BLOCK ? 256 : 512; }
+constexpr int ct_of() { return 512; }
 
 constexpr int LPR = 16;          // lanes per row in the matvec (one DPP row)
 constexpr int RPW = 64 / LPR;    // rows per wave in the matvec
 constexpr int UR = 12;           // register-resident entries per lane
 constexpr int UM = 8;            // entries per lane loaded together (memory mode)
 constexpr int GL = 32;           // Gram matrix order held in LDS (adaptive)
+constexpr int XL_ROWS = 2048;    // block form: matvec input staged in LDS up to this many rows
 constexpr unsigned SPIN_LIMIT = 1u << 22;   // polls of a grid barrier before the launch gives up
 
 struct LongArgs {
@@ -175,9 +175,15 @@ __device__ __forceinline__ double inv_beta(double beta) {
   return ib;
 }
 
-template <int MODE, bool FIXED, bool F32>
+// XL (block form, domains of at most XL_ROWS rows): the matvec input is
+// staged in LDS by the threads that write it, and gathered from there.
+template <int MODE, bool FIXED, bool F32, bool XL = false>
 __global__ __launch_bounds__(ct_of<MODE>()) void k_long(LongArgs a) {
   using XT = typename XType<F32>::T;
+  static_assert(!XL || MODE == BLOCK, "LDS-staged input is for the block form");
+  extern __shared__ char s_dyn[];
+  XT* s_x = (XT*)s_dyn;                                   // [XL_ROWS] matvec input
+  int* s_rp = (int*)(s_dyn + sizeof(XT) * XL_ROWS);       // [XL_ROWS + 1] row offsets from e_lo
   constexpr int CT = ct_of<MODE>();
   constexpr int CWV = CT / 64;
   constexpr bool REG = MODE == GRID_REG;
@@ -208,7 +214,9 @@ __global__ __launch_bounds__(ct_of<MODE>()) void k_long(LongArgs a) {
   double2* P = (double2*)(base + a.offP);
   XT* Ast = (XT*)(base + a.offA);
   const XT* Xin;
-  if constexpr (F32)
+  if constexpr (XL)
+    Xin = s_x;
+  else if constexpr (F32)
     Xin = Wf;
   else
     Xin = Wb;
@@ -229,6 +237,15 @@ __global__ __launch_bounds__(ct_of<MODE>()) void k_long(LongArgs a) {
       S[i] = op.psi[(int64_t)b * op.ld + orig];
       P[i] = op.phase ? op.phase[orig] : make_double2(1.0, 0.0);
     }
+    if constexpr (XL)
+      for (int64_t i = t0; i <= len; i += nthr) s_rp[i] = (int)(op.rp[lo + i] - e_lo);
+    // first entry of domain row r (XL: from LDS, published by the first barrier of a step)
+    auto row_start = [&](int r) __attribute__((always_inline)) -> int64_t {
+      if constexpr (XL)
+        return e_lo + s_rp[r];
+      else
+        return op.rp[lo + r];
+    };
     // this lane's row in the matvec (REG: the wave's only row group) and its entries
     const int rreg = gw * RPW + rw;
     int64_t ebeg = 0, eend = 0;
@@ -298,6 +315,7 @@ __global__ __launch_bounds__(ct_of<MODE>()) void k_long(LongArgs a) {
         V[i] = v;
         Wb[i] = v;
         if constexpr (F32) Wf[i] = to_x<true>(v);
+        if constexpr (XL) s_x[i] = to_x<F32>(v);
       }
       sync_all<MODE>(a, ss, &s_ab);
 
@@ -318,7 +336,12 @@ __global__ __launch_bounds__(ct_of<MODE>()) void k_long(LongArgs a) {
           w.y = row_sum16(w.y);
           if (sub == 0 && r < len) {
             w = cscale(ib, w);
-            const double2 v = cscale(ib, Wb[r]);
+            double2 wr;
+            if constexpr (XL && !F32)
+              wr = s_x[r];   // W itself
+            else
+              wr = Wb[r];
+            const double2 v = cscale(ib, wr);
             Vit[r] = v;
             Y[r] = w;
             red[0] += w.x * v.x + w.y * v.y;
@@ -334,32 +357,45 @@ __global__ __launch_bounds__(ct_of<MODE>()) void k_long(LongArgs a) {
             finish_row(rreg, acc);
           }
         } else {
-          for (int rg = gw; rg < ngroups; rg += nwaves) {
-            const int r = rg * RPW + rw;
-            XT acc = to_x<F32>(zero);
-            if (r < len) {
-              const int64_t e1 = op.rp[lo + r + 1];
-              for (int64_t e = op.rp[lo + r] + sub; e < e1; e += (int64_t)LPR * UM) {
-                int c[UM];
-                XT av[UM];
+          // two row groups per pass (rows ra, rb), their loads in flight together
+          auto ecol = [&](int64_t q, int64_t e1) __attribute__((always_inline)) -> int {
+            return q < e1 ? op.ccol[q] - lo : 0;
+          };
+          auto eval = [&](int64_t q, int64_t e1) __attribute__((always_inline)) -> XT {
+            if (q >= e1) return to_x<F32>(make_double2(0.0, 0.0));
+            return direct ? to_x<F32>(op.cval[q]) : Ast[q - e_lo];
+          };
+          for (int rg = gw; rg < ngroups; rg += 2 * nwaves) {
+            const int ra = rg * RPW + rw, rb = (rg + nwaves) * RPW + rw;
+            const int64_t a0 = ra < len ? row_start(ra) : 0, a1 = ra < len ? row_start(ra + 1) : 0;
+            const int64_t b0 = rb < len ? row_start(rb) : 0, b1 = rb < len ? row_start(rb + 1) : 0;
+            XT acca = to_x<F32>(zero), accb = to_x<F32>(zero);
+            const int64_t lm = std::max(a1 - a0, b1 - b0);
+            for (int64_t t = sub; t < lm; t += (int64_t)LPR * UM) {
+              int ca[UM], cb[UM];
+              XT va[UM], vb[UM];
 #pragma unroll
-                for (int u = 0; u < UM; ++u) {
-                  const int64_t q = e + (int64_t)u * LPR;
-                  const bool ok = q < e1;
-                  c[u] = ok ? op.ccol[q] - lo : 0;
-                  if (direct)
-                    av[u] = ok ? to_x<F32>(op.cval[q]) : to_x<F32>(zero);
-                  else
-                    av[u] = ok ? Ast[q - e_lo] : to_x<F32>(zero);
-                }
-                XT part = to_x<F32>(zero);
-#pragma unroll
-                for (int u = 0; u < UM; ++u) part = xfma(av[u], Xin[c[u]], part);
-                acc = xadd(acc, part);
+              for (int u = 0; u < UM; ++u) {
+                ca[u] = ecol(a0 + t + (int64_t)u * LPR, a1);
+                va[u] = eval(a0 + t + (int64_t)u * LPR, a1);
+                cb[u] = ecol(b0 + t + (int64_t)u * LPR, b1);
+                vb[u] = eval(b0 + t + (int64_t)u * LPR, b1);
               }
-              if (direct) acc = xfma(f0, acc, to_x<F32>(zero));
+              XT pa = to_x<F32>(zero), pb = to_x<F32>(zero);
+#pragma unroll
+              for (int u = 0; u < UM; ++u) {
+                pa = xfma(va[u], Xin[ca[u]], pa);
+                pb = xfma(vb[u], Xin[cb[u]], pb);
+              }
+              acca = xadd(acca, pa);
+              accb = xadd(accb, pb);
             }
-            finish_row(r, acc);
+            if (direct) {
+              acca = xfma(f0, acca, to_x<F32>(zero));
+              accb = xfma(f0, accb, to_x<F32>(zero));
+            }
+            finish_row(ra, acca);
+            finish_row(rb, accb);
           }
         }
         if (it == 0 && !FIXED) {
@@ -380,6 +416,7 @@ __global__ __launch_bounds__(ct_of<MODE>()) void k_long(LongArgs a) {
           if (it > 0) wn = csub(wn, cscale(beta, Vprev[i]));
           Wb[i] = wn;
           if constexpr (F32) Wf[i] = to_x<true>(wn);
+          if constexpr (XL) s_x[i] = to_x<F32>(wn);
           bs += wn.x * wn.x + wn.y * wn.y;
         }
         k = it + 1;
@@ -452,6 +489,7 @@ __global__ __launch_bounds__(ct_of<MODE>()) void k_long(LongArgs a) {
           Wb[i] = v;
           Vk[i] = v;
           if constexpr (F32) Wf[i] = to_x<true>(v);
+          if constexpr (XL) s_x[i] = to_x<F32>(v);
         }
         if (!FIXED) {
           // Gram column of the restart vector, explicit dots
@@ -519,33 +557,35 @@ struct LongScratch {
   size_t bytes = 0;
 };
 
-template <int MODE, bool FIXED, bool F32>
-hipError_t launch_r(LongArgs& a, int grid, hipStream_t s) {
+template <int MODE, bool FIXED, bool F32, bool XL>
+hipError_t launch_r(LongArgs& a, int grid, size_t lds, hipStream_t s) {
   if (MODE == BLOCK) {
-    hipLaunchKernelGGL((k_long<MODE, FIXED, F32>), dim3(grid), dim3(ct_of<MODE>()), 0, s, a);
+    hipLaunchKernelGGL((k_long<MODE, FIXED, F32, XL>), dim3(grid), dim3(ct_of<MODE>()), lds, s, a);
     return hipGetLastError();
   }
   void* args[] = {&a};
-  return hipLaunchCooperativeKernel((const void*)k_long<MODE, FIXED, F32>, dim3(grid), dim3(ct_of<MODE>()),
-                                    args, 0, s);
+  return hipLaunchCooperativeKernel((const void*)k_long<MODE, FIXED, F32, XL>, dim3(grid), dim3(ct_of<MODE>()),
+                                    args, lds, s);
 }
 
-template <int MODE>
-hipError_t launch_m(LongArgs& a, int grid, bool f32, hipStream_t s) {
+template <int MODE, bool XL = false>
+hipError_t launch_m(LongArgs& a, int grid, bool f32, size_t lds, hipStream_t s) {
   const bool fixed = a.fixed_m > 0;
-  if (fixed) return f32 ? launch_r<MODE, true, true>(a, grid, s) : launch_r<MODE, true, false>(a, grid, s);
-  return f32 ? launch_r<MODE, false, true>(a, grid, s) : launch_r<MODE, false, false>(a, grid, s);
+  if (fixed)
+    return f32 ? launch_r<MODE, true, true, XL>(a, grid, lds, s) : launch_r<MODE, true, false, XL>(a, grid, lds, s);
+  return f32 ? launch_r<MODE, false, true, XL>(a, grid, lds, s) : launch_r<MODE, false, false, XL>(a, grid, lds, s);
 }
 
-// Workgroups of one form the device holds at once (the smallest over its instantiations).
-template <int MODE>
-hipError_t max_resident(int n_cu, int* grid) {
-  const void* ks[] = {(const void*)k_long<MODE, true, true>, (const void*)k_long<MODE, true, false>,
-                      (const void*)k_long<MODE, false, true>, (const void*)k_long<MODE, false, false>};
+// Workgroups of one form the device holds at once (the smallest over its
+// instantiations) with lds bytes of dynamic LDS each.
+template <int MODE, bool XL = false>
+hipError_t max_resident(int n_cu, size_t lds, int* grid) {
+  const void* ks[] = {(const void*)k_long<MODE, true, true, XL>, (const void*)k_long<MODE, true, false, XL>,
+                      (const void*)k_long<MODE, false, true, XL>, (const void*)k_long<MODE, false, false, XL>};
   int occ = INT32_MAX;
   for (const void* k : ks) {
     int o = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k, ct_of<MODE>(), 0);
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k, ct_of<MODE>(), lds);
     if (e != hipSuccess) return e;
     occ = std::min(occ, o);
   }
@@ -587,10 +627,15 @@ int long_run(void** scratch, const SubOp* d_ops, const int4* d_blk, const std::v
              const int4* d_coop, const std::vector<LongItem>& coop, int32_t kcap, int32_t fixed_m,
              int32_t maxorder, double tol, bool f32, int n_cu, hipStream_t s) {
   if (blk.empty() && coop.empty()) return RM_OK;
-  int gres_r = 1, gres_m = 1, gres_b = 1;
-  HIPCHK(max_resident<GRID_REG>(n_cu, &gres_r));
-  HIPCHK(max_resident<GRID_MEM>(n_cu, &gres_m));
-  HIPCHK(max_resident<BLOCK>(n_cu, &gres_b));
+  int gres_r = 1, gres_m = 1, gres_b = 1, gres_x = 1;
+  HIPCHK((max_resident<GRID_REG>(n_cu, 0, &gres_r)));
+  HIPCHK((max_resident<GRID_MEM>(n_cu, 0, &gres_m)));
+  HIPCHK((max_resident<BLOCK>(n_cu, 0, &gres_b)));
+  // block items on domains of at most XL_ROWS rows come first (staged in LDS)
+  size_t nx = 0;
+  while (nx < blk.size() && blk[nx].item.z <= XL_ROWS) ++nx;
+  const size_t lds_x = (f32 ? sizeof(float2) : sizeof(double2)) * XL_ROWS + sizeof(int) * (XL_ROWS + 1);
+  if (nx) HIPCHK((max_resident<BLOCK, true>(n_cu, lds_x, &gres_x)));
   gres_r = std::min(gres_r, n_cu);
   gres_m = std::min(gres_m, n_cu);
   const bool fixed = fixed_m > 0;
@@ -608,7 +653,7 @@ int long_run(void** scratch, const SubOp* d_ops, const int4* d_blk, const std::v
       need_A |= it.nslot > 1;
     }
     const size_t region = region_layout(ab, lmax, emax, kcap, f32, need_A);
-    nwg = (int)std::min<int64_t>((int64_t)blk.size(), gres_b);
+    nwg = (int)std::max(std::min<int64_t>((int64_t)nx, gres_x), std::min<int64_t>((int64_t)(blk.size() - nx), gres_b));
     ab.wg_bytes = (int64_t)region;
     ab.lmax = lmax;
     b_bytes = region * (size_t)nwg;
@@ -667,11 +712,17 @@ int long_run(void** scratch, const SubOp* d_ops, const int4* d_blk, const std::v
     a->maxorder = maxorder;
     a->tol = tol;
   }
-  if (!blk.empty()) {
+  if (nx) {
     ab.ws = wb;
     ab.items = d_blk;
-    ab.nitems = (int32_t)blk.size();
-    HIPCHK(launch_m<BLOCK>(ab, nwg, f32, s));
+    ab.nitems = (int32_t)nx;
+    HIPCHK((launch_m<BLOCK, true>(ab, (int)std::min<int64_t>((int64_t)nx, gres_x), f32, lds_x, s)));
+  }
+  if (blk.size() > nx) {
+    ab.ws = wb;
+    ab.items = d_blk + nx;
+    ab.nitems = (int32_t)(blk.size() - nx);
+    HIPCHK((launch_m<BLOCK>(ab, (int)std::min<int64_t>((int64_t)ab.nitems, gres_b), f32, 0, s)));
   }
   ac.ws = wc;
   ac.wg_bytes = 0;
@@ -689,9 +740,9 @@ int long_run(void** scratch, const SubOp* d_ops, const int4* d_blk, const std::v
     hipLaunchKernelGGL(k_long_reset, dim3(1), dim3(64), 0, s, bar, abort_flag);
     HIPCHK(hipGetLastError());
     if (reg)
-      HIPCHK(launch_m<GRID_REG>(ac, grid, f32, s));
+      HIPCHK((launch_m<GRID_REG>(ac, grid, f32, 0, s)));
     else
-      HIPCHK(launch_m<GRID_MEM>(ac, grid, f32, s));
+      HIPCHK((launch_m<GRID_MEM>(ac, grid, f32, 0, s)));
   }
   return RM_OK;
 }

@@ -615,6 +615,10 @@ struct SubScratch {
   void* ws = nullptr;
   size_t ws_bytes = 0;
   void* long_ws = nullptr;   // the long-domain kernels' workspace (rm_coop.hip)
+  // launch classes run concurrently on auxiliary streams (default priority)
+  hipStream_t aux[NCLASS + 1] = {};
+  hipEvent_t fork = nullptr;
+  hipEvent_t join[NCLASS + 1] = {};
 };
 
 void scratch_release(SubScratch* sc) {
@@ -627,6 +631,14 @@ void scratch_release(SubScratch* sc) {
   }
   if (sc->ws) (void)hipFree(sc->ws);
   coop_scratch_release(sc->long_ws);
+  for (int i = 0; i <= NCLASS; ++i) {
+    if (sc->aux[i]) {
+      (void)hipStreamSynchronize(sc->aux[i]);
+      (void)hipStreamDestroy(sc->aux[i]);
+    }
+    if (sc->join[i]) (void)hipEventDestroy(sc->join[i]);
+  }
+  if (sc->fork) (void)hipEventDestroy(sc->fork);
   delete sc;
 }
 
@@ -684,15 +696,19 @@ int sub_run(SubScratch* sc, const std::vector<SubOp>& ops, std::vector<int4> (&c
   a.fixed_m = fixed_m;
   a.maxorder = maxorder;
   a.tol = tol;
-  size_t need = 0;
+  // one workspace region per class (the classes may run at the same time)
+  size_t need = 0, wsoff[NCLASS] = {};
+  int ngroups = blk.empty() ? 0 : 1;
   for (int c = 0; c < NCLASS; ++c) {
     if (cls[c].empty()) continue;
+    ++ngroups;
     a.nitems = (int32_t)cls[c].size();
     int waves = 0;
     HIPCHK(launch_class(c, a, f32, n_cu, s, &waves, true));
     const size_t per_wave = (class_basis_lds(c, kcap) ? 0 : (size_t)kcap * 64 * CLASS_R[c]) +
                             (fixed_m > 0 ? 0 : (size_t)kcap * kcap + 2 * (size_t)kcap);
-    need = std::max(need, sizeof(double2) * std::max<size_t>(per_wave, 1) * (size_t)waves);
+    wsoff[c] = need;
+    need += (sizeof(double2) * std::max<size_t>(per_wave, 1) * (size_t)waves + 255) & ~(size_t)255;
   }
   if (sc->ws_bytes < need) {
     if (sc->ws) {
@@ -705,6 +721,27 @@ int sub_run(SubScratch* sc, const std::vector<SubOp>& ops, std::vector<int4> (&c
     sc->ws_bytes = need;
   }
   if (timed) HIPCHK(timing_begin(timed, s));
+  // several launches and no cooperative one: each on an auxiliary stream
+  // forked from s and joined back into it, so they run side by side
+  const bool fork = ngroups > 1 && coop.empty();
+  int nused = 0;
+  auto next_stream = [&](hipStream_t* out) -> int {
+    if (!fork) {
+      *out = s;
+      return RM_OK;
+    }
+    const int i = nused++;
+    if (!sc->aux[i]) HIPCHK(hipStreamCreateWithFlags(&sc->aux[i], hipStreamNonBlocking));
+    if (!sc->join[i]) HIPCHK(hipEventCreateWithFlags(&sc->join[i], hipEventDisableTiming));
+    HIPCHK(hipStreamWaitEvent(sc->aux[i], sc->fork, 0));
+    *out = sc->aux[i];
+    return RM_OK;
+  };
+  if (fork) {
+    if (!sc->fork) HIPCHK(hipEventCreateWithFlags(&sc->fork, hipEventDisableTiming));
+    HIPCHK(hipEventRecord(sc->fork, s));
+  }
+  int rc = RM_OK;
   for (int c = 0; c < NCLASS; ++c) {
     if (cls[c].empty()) continue;
     a.items = (const int4*)(sc->dev[bi] + off[c]);
@@ -712,14 +749,21 @@ int sub_run(SubScratch* sc, const std::vector<SubOp>& ops, std::vector<int4> (&c
     int waves = 0;
     HIPCHK(launch_class(c, a, f32, n_cu, s, &waves, true));
     const bool bl = class_basis_lds(c, kcap);
-    a.basis = (double2*)sc->ws;
+    a.basis = (double2*)((char*)sc->ws + wsoff[c]);
     a.gram = a.basis + (bl ? 0 : (size_t)waves * kcap * 64 * CLASS_R[c]);
     a.cscr = a.gram + (fixed_m > 0 ? 0 : (size_t)waves * kcap * kcap);
-    HIPCHK(launch_class(c, a, f32, n_cu, s, &waves, false));
+    hipStream_t cs;
+    if ((rc = next_stream(&cs))) return rc;
+    HIPCHK(launch_class(c, a, f32, n_cu, cs, &waves, false));
+    if (fork) HIPCHK(hipEventRecord(sc->join[nused - 1], cs));
   }
-  int rc = long_run(&sc->long_ws, (const SubOp*)sc->dev[bi], (const int4*)(sc->dev[bi] + off_blk), blk,
-                    (const int4*)(sc->dev[bi] + off_coop), coop, kcap, fixed_m, maxorder, tol, f32, n_cu, s);
+  hipStream_t bs = s;
+  if (!blk.empty() && (rc = next_stream(&bs))) return rc;
+  rc = long_run(&sc->long_ws, (const SubOp*)sc->dev[bi], (const int4*)(sc->dev[bi] + off_blk), blk,
+                (const int4*)(sc->dev[bi] + off_coop), coop, kcap, fixed_m, maxorder, tol, f32, n_cu, bs);
   if (rc) return rc;
+  if (fork && !blk.empty()) HIPCHK(hipEventRecord(sc->join[nused - 1], bs));
+  for (int i = 0; i < (fork ? nused : 0); ++i) HIPCHK(hipStreamWaitEvent(s, sc->join[i], 0));
   if (timed) HIPCHK(timing_end(timed, s));
   if (!sc->done[bi]) HIPCHK(hipEventCreateWithFlags(&sc->done[bi], hipEventDisableTiming));
   HIPCHK(hipEventRecord(sc->done[bi], s));
@@ -799,8 +843,15 @@ LongItem long_item(const rm_plan* p, int col, int lo, int len, int op) {
   return it;
 }
 
+// Most work first, with the domains the block kernel stages in LDS
+// (RM_XL_ROWS rows at most) ahead of the longer ones.
+constexpr int RM_XL_ROWS = 2048;
 void sort_long(std::vector<LongItem>& v) {
-  std::stable_sort(v.begin(), v.end(), [](const LongItem& x, const LongItem& y) { return x.nent > y.nent; });
+  std::stable_sort(v.begin(), v.end(), [](const LongItem& x, const LongItem& y) {
+    const bool lx = x.item.z <= RM_XL_ROWS, ly = y.item.z <= RM_XL_ROWS;
+    if (lx != ly) return lx;
+    return x.nent > y.nent;
+  });
 }
 
 // The subspace path of rm_propagate.  Arguments are validated by the caller
@@ -872,6 +923,7 @@ struct rm_batch {
   SubScratch* sc = nullptr;
   int device = 0;
   int n_cu = 256;
+  int32_t block_rows = RM_SUBSPACE_MAX_ROWS;   // longer domains: one workgroup per job
 };
 
 extern "C" int rm_batch_create(rm_batch** out) {
@@ -889,6 +941,14 @@ extern "C" int rm_batch_create(rm_batch** out) {
       b->n_cu <= 0)
     b->n_cu = 256;
   *out = b;
+  return RM_OK;
+}
+
+extern "C" int rm_batch_set_block_rows(rm_batch* b, int32_t rows) {
+  if (!b) return fail(RM_ERR_ARG, "rm_batch_set_block_rows: NULL batch");
+  if (rows < 0 || rows > RM_SUBSPACE_MAX_ROWS)
+    return fail(RM_ERR_ARG, "rm_batch_set_block_rows: rows=%d outside [0, %d]", rows, RM_SUBSPACE_MAX_ROWS);
+  b->block_rows = rows;
   return RM_OK;
 }
 
@@ -961,7 +1021,7 @@ extern "C" int rm_batch_propagate(rm_batch* bt, const rm_job* jobs, int32_t njob
         p->h_sub_end[p->h_sub_label_perm[lo + len - 1]] != lo + len)
       return fail(RM_ERR_ARG, "rm_batch_propagate: job %d: [%d, %d) is not a domain of %d rows at most "
                   "(use rm_plan_domains)", j, lo, lo + len, RM_BLOCK_MAX_ROWS);
-    const bool longdom = len > RM_SUBSPACE_MAX_ROWS || !p->sub_col;
+    const bool longdom = len > bt->block_rows || !p->sub_col;
     if (longdom && !p->sub_rp && (rc = csr_build(p, entry_values(p)))) return rc;
     SubOp op = plan_op(p);
     op.phase = (const double2*)jb.phase;
@@ -994,11 +1054,16 @@ extern "C" int rm_batch_propagate(rm_batch* bt, const rm_job* jobs, int32_t njob
     for (size_t t = 0; t < ix.size(); ++t) sorted[t] = cls[c][ix[t]];
     cls[c].swap(sorted);
   }
-  // long domains: one workgroup each (block kernel), most work first
+  // long domains: one workgroup each (block kernel), LDS-staged domains
+  // first, most work first
   {
     std::vector<size_t> ix(blk.size());
     std::iota(ix.begin(), ix.end(), 0);
-    std::stable_sort(ix.begin(), ix.end(), [&](size_t x, size_t y) { return blk_work[x] > blk_work[y]; });
+    std::stable_sort(ix.begin(), ix.end(), [&](size_t x, size_t y) {
+      const bool lx = blk[x].item.z <= RM_XL_ROWS, ly = blk[y].item.z <= RM_XL_ROWS;
+      if (lx != ly) return lx;
+      return blk_work[x] > blk_work[y];
+    });
     std::vector<LongItem> sorted(ix.size());
     for (size_t t = 0; t < ix.size(); ++t) sorted[t] = blk[ix[t]];
     blk.swap(sorted);

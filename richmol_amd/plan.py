@@ -159,6 +159,23 @@ def flatten_operator(T, active=None):
                 slots=slots, nfield=slot_base, nleaf=len(terms))
 
 
+def matrix_arrays(A, cart):
+    """Plan arrays (the layout of flatten_operator) of H(F) = p(F) A: one
+    term whose M value is 1 in the single field slot ``cart``."""
+    from scipy.sparse import csr_matrix
+    A = csr_matrix(A, dtype=np.complex128)
+    A.sort_indices()
+    n = A.shape[0]
+    if A.shape[1] != n:
+        raise ValueError("the operator matrix must be square")
+    nnz = A.nnz
+    return dict(n=n, n2=n, nnz=nnz, row_ptr=A.indptr.astype(np.int64), col_idx=A.indices.astype(np.int32),
+                pair_ptr=np.arange(nnz + 1, dtype=np.int64), pair_term=np.zeros(nnz, np.int32),
+                pair_kval=np.ascontiguousarray(A.data), nterm=1, term_ptr=np.array([0, 1], np.int64),
+                term_field=np.zeros(1, np.int32), term_mval=np.ones(1, np.complex128),
+                term_leaf=np.zeros(1, np.int32), slots=[[cart]], nfield=1, nleaf=1)
+
+
 class DevicePlan:
     """rm_plan handle plus the slot layout of the field products.
 
@@ -166,13 +183,16 @@ class DevicePlan:
     refuses a field that activates any other component (plan_for() then
     returns the plan for the new set)."""
 
-    def __init__(self, T, active=None):
+    def __init__(self, T, active=None, arrays=None):
         import ctypes
         self.lib = _lib.load()
         self.kernel = "auto"
         self.precision = "fp64"
-        self.active = active_carts(T) if active is None else active
-        arr = flatten_operator(T, self.active)
+        if arrays is None:
+            self.active = active_carts(T) if active is None else active
+            arr = flatten_operator(T, self.active)
+        else:
+            self.active, arr = active, arrays
         if arr["n"] != arr["n2"]:
             raise ValueError("bra and ket bases differ: the propagator needs a square operator")
         if arr["nfield"] > _lib.RM_MAX_FIELD:
@@ -196,6 +216,13 @@ class DevicePlan:
             _lib.ptr(arr["term_mval"].view(np.float64), c_dbl), _lib.ptr(arr["term_leaf"], c_i32),
             self.nfield, self.nleaf, ctypes.byref(h)), "rm_plan_create")
         self.handle = h
+
+    @classmethod
+    def from_matrix(cls, A, cart):
+        """Plan of H(F) = p(F) A for one field product p (component label
+        ``cart``, e.g. 'zz' for F_z^2): a single summed tensor with a single
+        term, every entry of the sparse matrix A its K value."""
+        return cls(None, active=[[cart]], arrays=matrix_arrays(A, cart))
 
     def __del__(self):
         h = getattr(self, "handle", None)

@@ -162,13 +162,31 @@ def rigid_rotor_matrix(J, Bx, By, Bz):
     k = -J..J (molecule-fixed angular momentum, z the quantization axis):
     diagonal (Bx+By)/2 (J(J+1) - k^2) + Bz k^2, and (Bx-By)/4 times the
     Delta k = +-2 ladder elements."""
-    ks = np.arange(-J, J + 1)
+    ks = np.arange(-J, J + 1, dtype=np.float64)
     jj = J * (J + 1.0)
-    H = np.diag((Bx + By) / 2.0 * (jj - ks ** 2) + Bz * ks ** 2).astype(np.float64)
-    for i, k in enumerate(ks[:-2]):
-        v = (Bx - By) / 4.0 * math.sqrt((jj - k * (k + 1)) * (jj - (k + 1) * (k + 2)))
-        H[i + 2, i] = H[i, i + 2] = v
+    H = np.diag((Bx + By) / 2.0 * (jj - ks ** 2) + Bz * ks ** 2)
+    if J >= 1:
+        k = ks[:-2]
+        v = (Bx - By) / 4.0 * np.sqrt((jj - k * (k + 1)) * (jj - (k + 1) * (k + 2)))
+        i = np.arange(k.size)
+        H[i + 2, i] = v
+        H[i, i + 2] = v
     return H
+
+
+def _wang(J):
+    """Wang functions of J as columns over k = -J..J (K = 0, then K = 1..J
+    with the + and - combinations) and their D2 species labels."""
+    Kv = np.concatenate([[0], np.repeat(np.arange(1, J + 1), 2)]).astype(np.int64)
+    Sv = np.concatenate([[1], np.tile([1, -1], J)]).astype(np.int64)
+    W = np.zeros((2 * J + 1, Kv.size))
+    c = np.arange(Kv.size)
+    W[J, 0] = 1.0
+    W[J + Kv[1:], c[1:]] = 1.0 / math.sqrt(2.0)
+    W[J - Kv[1:], c[1:]] = Sv[1:] / math.sqrt(2.0)
+    c2x = Sv * (-1) ** J
+    labels = np.array([_D2[(int(K) % 2, int(x))] for K, x in zip(Kv, c2x)])
+    return W, labels
 
 
 def asymtop_basis(Bx, By, Bz, Jmax, Jmin=0, m_filter=None, J_filter=None):
@@ -189,24 +207,15 @@ def asymtop_basis(Bx, By, Bz, Jmax, Jmin=0, m_filter=None, J_filter=None):
         ks = list(range(-J, J + 1))
         H = rigid_rotor_matrix(J, Bx, By, Bz)
         # Wang functions (columns over primitive k) and their D2 species
-        cols = {}
-        for K in range(0, J + 1):
-            for sgn in ((1,) if K == 0 else (1, -1)):
-                w = np.zeros(2 * J + 1)
-                if K == 0:
-                    w[J] = 1.0
-                else:
-                    w[J + K] = 1.0 / math.sqrt(2.0)
-                    w[J - K] = sgn / math.sqrt(2.0)
-                c2x = sgn * (-1) ** J
-                cols.setdefault(_D2[(K % 2, c2x)], []).append(w)
+        W, labels = _wang(J)
         jl.append(J)
         ml[J] = ms
         kf[J], en[J] = {}, {}
         for sym in ("A", "B1", "B2", "B3"):
-            if sym not in cols:
+            sel = labels == sym
+            if not np.any(sel):
                 continue
-            Wm = np.array(cols[sym]).T
+            Wm = W[:, sel]
             e, v = np.linalg.eigh(Wm.T @ H @ Wm)
             kf[J][sym] = (ks, (Wm @ v).astype(np.complex128))
             en[J][sym] = e
@@ -337,6 +346,139 @@ def tensor(tens, basis):
     """Lab-frame operator of a rank-1 or rank-2 molecular-frame tensor (in au)."""
     rank, comps, ucols = spherical_components(tens)
     return _build(basis, rank, CART[rank], comps, ucols)
+
+
+_KPRIM = {}
+_MZ = {}
+
+
+def zfield_m0_operator(basis, tens, start=None):
+    """Operator of a rank-1 or rank-2 molecular-frame tensor on an m = 0
+    basis (every J holds m = 0 only) for a field along Z: H(F) = F_z^rank A.
+    A is one sparse matrix in the block order of the built operators (J,
+    then species, then state) -- the matrix that tensor(tens, basis)
+    flattens to under a Z field, up to rounding -- made from the same
+    primitives with one dense product per (J1, J2, species of J2):
+    K_w = C1^T P_w c2 over every species of J1 at once, scaled by
+    M_w,z..z(m=0, m=0).  Entries below 1e-12 of their product's largest
+    (rounding left by symmetry-forbidden couplings) are dropped.  The
+    primitive-k matrices P_w and the M elements are kept across calls.
+
+    start=None: returns (A, energies) over the whole basis.  start = (J,
+    species): only the blocks reachable from that block are built -- the
+    invariant subspace its states live in under every Z field -- and
+    (A, energies, index) is returned, index[i] being the whole-basis
+    position of domain state i (domain states in basis order)."""
+    for J in basis.Jlist:
+        if list(basis.mlist[J]) != [0]:
+            raise ValueError("zfield_m0_operator needs a basis with m = 0 only")
+    rank, comps, ucols = spherical_components(tens)
+    cart = "z" * rank
+    blocks, pos = [], 0
+    boff, rows_of = {}, {}
+    for J in basis.Jlist:
+        r = 0
+        for sp in basis.symlist[J]:
+            ns = basis.kfun[J][sp][1].shape[1]
+            boff[(J, sp)] = pos
+            rows_of[(J, sp)] = (r, r + ns)
+            blocks.append((J, sp))
+            pos += ns
+            r += ns
+    n = pos
+    Cs = {J: np.concatenate([basis.kfun[J][sp][1] for sp in basis.symlist[J]], axis=1) for J in basis.Jlist}
+    real = all(np.all(C.imag == 0) for C in Cs.values())
+    Cr = {J: (C.real if real else C) for J, C in Cs.items()}
+    dJ = max(comps.keys()) if comps else 0
+    ckey = tuple((w, tuple(sorted(comps[w].items()))) for w in sorted(comps))
+
+    def mz(w, J1, J2):
+        key = (rank, w, J1, J2)
+        v = _MZ.get(key)
+        if v is None:
+            v = _MZ[key] = _m_block(ucols[w], w, J1, [0], J2, [0], CART[rank])[cart][0, 0]
+        return v
+
+    def prim(w, J1, J2):
+        key = (ckey, w, J1, J2)
+        P = _KPRIM.get(key)
+        if P is None:
+            P = _k_prim(comps[w], w, J1, list(range(-J1, J1 + 1)), J2, list(range(-J2, J2 + 1)))
+            if real and np.all(P.imag == 0):
+                P = P.real
+            P = csr_matrix(P)   # Delta k = sigma: at most five diagonals
+            _KPRIM[key] = P
+        return P
+
+    def column(b2):
+        """Non-zero blocks (b1, K) of the column block b2 = (J2, species)."""
+        J2, s2 = b2
+        r0, r1 = rows_of[b2]
+        c2 = Cr[J2][:, r0:r1]
+        out = []
+        for J1 in range(J2 - dJ, J2 + dJ + 1):
+            if J1 not in Cr:
+                continue
+            acc = None
+            for w in sorted(comps):
+                m = mz(w, J1, J2)
+                if abs(m) < _EPS:
+                    continue
+                K = Cr[J1].conj().T @ (prim(w, J1, J2) @ c2)
+                K[np.abs(K) < _EPS] = 0
+                acc = K * m if acc is None else acc + K * m
+            if acc is None:
+                continue
+            top = np.abs(acc).max()
+            if top == 0:
+                continue
+            acc[np.abs(acc) < 1e-12 * top] = 0
+            for s1 in basis.symlist[J1]:
+                q0, q1 = rows_of[(J1, s1)]
+                sub = acc[q0:q1]
+                if np.any(sub):
+                    out.append(((J1, s1), sub))
+        return out
+
+    ent = {}
+    if start is None:
+        keep = blocks
+        for b2 in blocks:
+            for b1, K in column(b2):
+                ent[(b1, b2)] = K
+    else:
+        start = tuple(start)
+        if start not in boff:
+            raise ValueError(f"start block {start} is not in the basis")
+        seen, queue = {start}, [start]
+        while queue:
+            b2 = queue.pop()
+            for b1, K in column(b2):
+                ent[(b1, b2)] = K
+                if b1 not in seen:
+                    seen.add(b1)
+                    queue.append(b1)
+        keep = [b for b in blocks if b in seen]
+    doff, index, dpos = {}, [], 0
+    for b in keep:
+        ns = rows_of[b][1] - rows_of[b][0]
+        doff[b] = dpos
+        index.append(boff[b] + np.arange(ns))
+        dpos += ns
+    index = np.concatenate(index) if index else np.zeros(0, dtype=np.int64)
+    R, Ccol, V = [], [], []
+    for (b1, b2), K in ent.items():
+        r, c = np.nonzero(K)
+        R.append(r + doff[b1]); Ccol.append(c + doff[b2]); V.append(K[r, c].astype(np.complex128))
+    if R:
+        A = csr_matrix((np.concatenate(V), (np.concatenate(R), np.concatenate(Ccol))), shape=(dpos, dpos))
+    else:
+        A = csr_matrix((dpos, dpos), dtype=np.complex128)
+    A.sort_indices()
+    energies = np.concatenate([basis.energy[J][sp] for (J, sp) in keep]) if keep else np.zeros(0)
+    if start is None:
+        return A, energies
+    return A, energies, index
 
 
 def named(name, basis):

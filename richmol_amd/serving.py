@@ -4,18 +4,25 @@ A request is one field-driven propagation: a rotor Hamiltonian, a pulse, a
 number of steps, an initial state.  The server keeps every admitted request's
 wavepacket, phases and field table resident on the device and advances all
 active requests together, ``chunk`` steps per tick, with one
-``rm_batch_propagate`` call (one wave per request on its invariant subspace,
-jobs of different operators in the same launch).  Requests join at tick
-boundaries as they arrive -- admitted while slots and the HBM budget allow --
-and leave when their last step is done; host-side admission of new requests
-overlaps the device work of the running tick.
+``rm_batch_propagate`` call (one wave -- or, for domains longer than a wave
+holds, one workgroup -- per request on its invariant subspace, jobs of
+different operators in the same launch).  Requests join at tick boundaries
+as they arrive -- admitted while slots and the HBM budget allow -- and leave
+when their last step is done; host-side admission of new requests (operator
+build included) overlaps the device work of the running tick.
+
+Requests are linear rotors (B, polarizability LINEAR_ALPHA, all m; operators
+shared per Jmax) or asymmetric tops (rotational constants A >= B >= C along
+z, x, y, polarizability ASYM_ALPHA; the m = 0 block, which a Z pulse never
+leaves from |J=0>; operator built per request).  psi0 is the J = 0 state.
 
 Latency is measured from arrival to the completion of the tick that finished
 the request.  ``benchmark`` drives a seeded Poisson stream of requests drawn
-as the serving configuration describes (linear rotors: B U(0.05, 30) cm^-1,
-Jmax U{10..60}, pulse FWHM U(100, 1000) fs, peak intensity log-U(1e10, 1e12)
-W/cm^2, 200-2000 steps of 10 fs, psi0 = |J=0, m=0>) at 0.8 of the capacity
-measured on the same stream in a closed-loop run.
+as the serving configuration describes (linear or asymmetric top 50/50,
+rotational constants U(0.05, 30) cm^-1, Jmax U{10..60}, pulse FWHM
+U(100, 1000) fs, peak intensity log-U(1e10, 1e12) W/cm^2, 200-2000 steps of
+10 fs, complex64 matvec with FP64 Krylov) at 0.8 of the capacity measured on
+the same stream in a closed-loop run.
 """
 import ctypes
 import time
@@ -26,6 +33,7 @@ import numpy as np
 from . import _lib, rot, units
 
 LINEAR_ALPHA = (26.0, 26.0, 53.6)   # alpha_perp, alpha_perp, alpha_par [au]
+ASYM_ALPHA = (12.0, 8.0, 16.0)      # alpha_xx, alpha_yy, alpha_zz [au]; x = b, y = c, z = a axis
 DT_FS = 10.0
 
 
@@ -63,18 +71,36 @@ def batch_propagate(lib, handle, jobs, krylov_fixed, maxorder, tol, precision, s
 class Request:
     """One propagation job.  ``pulse``: (fwhm_fs, intensity W/cm^2), a Z
     Gaussian envelope centred at 4 fwhm and cut beyond 4 fwhm (as in the
-    benchmark workload); ``B``: rotational constant [cm^-1]."""
+    benchmark workload); ``B``: rotational constant of a linear rotor
+    [cm^-1]; ``abc``: (A, B, C) of an asymmetric top (kind 'asym'), whose
+    polarizability is ``alpha`` (3 x 3, molecular frame; default
+    diag(ASYM_ALPHA))."""
 
-    __slots__ = ("rid", "arrival", "kind", "B", "jmax", "fwhm", "intensity", "nsteps", "dt",
-                 "t_admit", "t_done", "psi", "steps_done", "kused")
+    __slots__ = ("rid", "arrival", "kind", "B", "abc", "alpha", "jmax", "fwhm", "intensity", "nsteps", "dt",
+                 "t_admit", "t_done", "psi", "steps_done", "kused", "index", "cumfield")
 
-    def __init__(self, rid, arrival, B, jmax, fwhm, intensity, nsteps, kind="linear", dt=DT_FS):
+    def __init__(self, rid, arrival, B, jmax, fwhm, intensity, nsteps, kind="linear", dt=DT_FS, abc=None,
+                 alpha=None):
+        if kind not in ("linear", "asym"):
+            raise ValueError(f"unknown request kind '{kind}'")
+        if kind == "asym" and (abc is None or len(abc) != 3):
+            raise ValueError("an asymmetric-top request needs abc = (A, B, C)")
         self.rid, self.arrival, self.kind = rid, float(arrival), kind
         self.B, self.jmax, self.fwhm, self.intensity = float(B), int(jmax), float(fwhm), float(intensity)
+        self.abc = None if abc is None else tuple(float(x) for x in abc)
+        # molecular-frame polarizability of an asymmetric top (default diag(ASYM_ALPHA))
+        self.alpha = None if alpha is None else np.asarray(alpha, dtype=np.float64)
         self.nsteps, self.dt = int(nsteps), float(dt)
         self.t_admit = self.t_done = None
         self.psi = None
+        self.index = None   # basis positions of the propagated states (asymmetric top: its subspace)
         self.steps_done = 0
+
+    @property
+    def n(self):
+        """States of the request's basis: (Jmax+1)^2 for both kinds (all m of
+        a linear rotor; the m = 0 block of an asymmetric top)."""
+        return (self.jmax + 1) ** 2
 
     def fields(self):
         """(nsteps, 3) Z field at the step midpoints [V/m]."""
@@ -89,18 +115,55 @@ class Request:
 
 
 def make_requests(n, seed=0, rate=None, jmax=(10, 60), B=(0.05, 30.0), fwhm=(100.0, 1000.0),
-                  intensity=(1e10, 1e12), steps=(200, 2000)):
+                  intensity=(1e10, 1e12), steps=(200, 2000), asym_fraction=0.0):
     """Seeded request stream; Poisson arrivals at ``rate`` per second (all at
-    t = 0 when rate is None)."""
+    t = 0 when rate is None); a fraction ``asym_fraction`` of asymmetric
+    tops, their three constants drawn from ``B`` and sorted A >= B >= C."""
     rng = np.random.default_rng(seed)
     arr = np.zeros(n) if rate is None else np.cumsum(rng.exponential(1.0 / rate, size=n))
     out = []
     for i in range(n):
-        out.append(Request(i, arr[i], B=rng.uniform(*B), jmax=rng.integers(jmax[0], jmax[1] + 1),
-                           fwhm=rng.uniform(*fwhm),
-                           intensity=10 ** rng.uniform(np.log10(intensity[0]), np.log10(intensity[1])),
-                           nsteps=rng.integers(steps[0], steps[1] + 1)))
+        r = Request(i, arr[i], B=rng.uniform(*B), jmax=rng.integers(jmax[0], jmax[1] + 1),
+                    fwhm=rng.uniform(*fwhm),
+                    intensity=10 ** rng.uniform(np.log10(intensity[0]), np.log10(intensity[1])),
+                    nsteps=rng.integers(steps[0], steps[1] + 1))
+        if asym_fraction > 0:
+            asym = rng.random() < asym_fraction
+            abc = np.sort(rng.uniform(*B, size=3))[::-1]
+            if asym:
+                r.kind, r.abc = "asym", tuple(float(x) for x in abc)
+        out.append(r)
     return out
+
+
+def _domain(A, n):
+    """Domain [lo, lo + len) of the J = 0 state (row 0) in the plan's
+    permuted order: the subspaces are numbered by their first row, so row 0's
+    subspace comes first (lo = 0) and len is its size (host, from the
+    sparsity pattern; no device call)."""
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.csgraph import breadth_first_order
+    P = csr_matrix(A)
+    P = csr_matrix((np.ones(P.nnz), P.indices, P.indptr), shape=(n, n))
+    reach = breadth_first_order(P + P.T, 0, directed=False, return_predecessors=False)
+    return 0, int(reach.size)
+
+
+# fixed latency of a field step (its ~12 Krylov iterations' reductions and
+# barriers), in the units of _cost: stored entries (about 20 ns each)
+STEP_FIXED_COST = 2000.0
+
+
+def _cost(A, n):
+    """Time of one field step of the J = 0 state's job in stored-entry units:
+    the entries of its domain rows, those rows, and STEP_FIXED_COST (the
+    per-step cost the tick balances)."""
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.csgraph import breadth_first_order
+    P = csr_matrix(A)
+    P = csr_matrix((np.ones(P.nnz), P.indices, P.indptr), shape=(n, n))
+    reach = breadth_first_order(P + P.T, 0, directed=False, return_predecessors=False)
+    return float(np.diff(P.indptr)[reach].sum() + reach.size + STEP_FIXED_COST)
 
 
 class _Model:
@@ -109,21 +172,23 @@ class _Model:
 
     def __init__(self, master, jmax):
         from .plan import DevicePlan, field_table
-        import torch
         self.jmax = jmax
         self.H = rot.restrict(master, jmax) * (-0.5 * units.AUpol_x_Vm_to_invcm())
         self.n = (jmax + 1) ** 2
         self.J = np.repeat(np.arange(jmax + 1), 2 * np.arange(jmax + 1) + 1).astype(np.float64)
         active, _, _ = field_table(self.H, np.array([[0.0, 0.0, 1.0]]))
         self.plan = DevicePlan(self.H, active)
+        self.plan.subspaces()
         self.nslot = self.plan.nfield
-        psi0 = torch.zeros((1, self.n), dtype=torch.complex128, device="cuda")
-        psi0[0, 0] = 1.0
-        lohi = np.zeros(2, dtype=np.int32)
-        _lib.check(self.plan.lib.rm_plan_domains(self.plan.handle, ctypes.c_void_p(psi0.data_ptr()),
-                                                 self.n, 1, _lib.ptr(lohi, ctypes.c_int32), None),
-                   "rm_plan_domains")
-        self.lo, self.len = int(lohi[0]), int(lohi[1] - lohi[0])
+        arr = self.plan._arrays
+        pattern = (np.ones(arr["nnz"]), arr["col_idx"], arr["row_ptr"])
+        from scipy.sparse import csr_matrix
+        P = csr_matrix(pattern, shape=(self.n, self.n))
+        self.lo, self.len = _domain(P, self.n)
+        self.cost = _cost(P, self.n)
+
+    def energies(self, req):
+        return req.B * self.J * (self.J + 1.0)
 
     def table(self, F):
         from .plan import field_table
@@ -131,28 +196,73 @@ class _Model:
         return self.plan.table_array(cols, nst)
 
 
+class _AsymModel:
+    """Operator and device plan of one asymmetric-top request: H(F) =
+    F_z^2 A on the invariant subspace of the J = 0 state inside the m = 0
+    block (rot.zfield_m0_operator from block (0, 'A'); the polarizability
+    ASYM_ALPHA unless the request gives its own, -1/2 alpha F F), energies
+    of the rigid rotor with A, B, C along z, x, y.  index: the m = 0 basis
+    position of every subspace state (the J = 0 state is the first)."""
+
+    def __init__(self, req):
+        from .plan import DevicePlan
+        a, b, c = req.abc
+        bas = rot.asymtop_basis(b, c, a, req.jmax, m_filter=lambda J, m: m == 0)
+        alpha = np.diag(ASYM_ALPHA) if req.alpha is None else req.alpha
+        A0, self.E, self.index = rot.zfield_m0_operator(bas, alpha, start=(0, "A"))
+        self.nfull = req.n
+        self.H = A0 * (-0.5 * units.AUpol_x_Vm_to_invcm())
+        self.n = A0.shape[0]
+        self.plan = DevicePlan.from_matrix(self.H, "zz")
+        self.plan.subspaces()   # the subspace arrays now, during the running tick
+        self.nslot = 1
+        self.lo, self.len = _domain(self.H, self.n)
+        self.cost = _cost(self.H, self.n)
+
+    def energies(self, req):
+        return self.E
+
+    def table(self, F):
+        tab = np.zeros((len(F), 1), dtype=np.complex128)
+        tab[:, 0] = F[:, 2] * F[:, 2]
+        return tab
+
+
 class Server:
     """Continuous-batching engine on one device stream.
 
     hbm_budget: bytes of device memory the resident request state may use
     (admission stops at the first queued request that does not fit);
-    max_jobs: concurrent requests; chunk: steps per tick per request;
-    krylov_fixed: fixed Krylov dimension (0: adaptive with tol, maxorder).
+    max_jobs: concurrent requests; chunk: field steps per tick of the
+    costliest active request -- a request whose field steps cost c runs
+    chunk * c_max / c of them per tick, plus the field-free steps (phases
+    only) between them, at most chunk_max steps in all, so every request's
+    part of a tick takes about as long and cheap requests are not held back
+    by expensive ones (a result does not depend on how its steps are split);
+    krylov_fixed: fixed Krylov dimension (0: adaptive with tol, maxorder);
+    block_rows: jobs on domains longer than this run one workgroup each
+    (default: only those longer than a wave holds).
     """
 
     def __init__(self, hbm_budget=8 << 30, max_jobs=8192, chunk=64, krylov_fixed=12, maxorder=100,
-                 tol=1e-15, precision="fp64", jmax_master=60):
+                 tol=1e-15, precision="fp64", jmax_master=60, block_rows=64, chunk_max=4096):
         import torch
         self.torch = torch
         self.lib = _lib.load()
         self.budget, self.max_jobs, self.chunk = int(hbm_budget), int(max_jobs), int(chunk)
+        self.chunk_max = int(chunk_max)
         self.kfix, self.maxorder, self.tol = int(krylov_fixed), int(maxorder), float(tol)
         self.precision = _lib.PRECISIONS[precision]
         self.f = -1j * DT_FS * 1e-15 * units.enr_units()["invcm"] / units.HBAR
-        self.stream = torch.cuda.current_stream()
+        # ticks run on a stream of their own, so admission's uploads (plan
+        # creation, tensors) do not wait for the running tick
+        self.stream = torch.cuda.Stream()
+        self.t_host = 0.0     # seconds spent admitting (operators, uploads)
         h = ctypes.c_void_p()
         _lib.check(self.lib.rm_batch_create(ctypes.byref(h)), "rm_batch_create")
         self.handle = h
+        if block_rows is not None:
+            _lib.check(self.lib.rm_batch_set_block_rows(h, int(block_rows)), "rm_batch_set_block_rows")
         bas = rot.linear_basis(1.0, jmax_master)
         self.master = rot.tensor(np.diag(LINEAR_ALPHA), bas)
         self.models = {}
@@ -174,22 +284,35 @@ class Server:
 
     def footprint(self, req):
         """Resident device bytes of a request: wavepacket, phases, field table,
-        step flags, Krylov dimensions."""
-        n = (req.jmax + 1) ** 2
-        return 16 * n * 2 + req.nsteps * (16 * 1 + 4 + 4)
+        step flags, Krylov dimensions; an asymmetric top also holds its own
+        operator plan (about 50 stored entries per state at Jmax 60, ~100
+        bytes each over the plan's arrays).  An asymmetric top is counted on
+        its whole m = 0 block, an upper bound of the subspace it runs on."""
+        n = req.n
+        own = 0 if req.kind == "linear" else 100 * 50 * n
+        return 16 * n * 2 + req.nsteps * (16 * 1 + 4 + 4) + own
 
     # -- admission: device state of one request, job record in numpy form
     def _admit(self, req, now):
+        t0 = time.perf_counter()
+        out = self._admit_one(req, now)
+        self.t_host += time.perf_counter() - t0
+        return out
+
+    def _admit_one(self, req, now):
         torch = self.torch
-        m = self.model(req.jmax)
+        m = self.model(req.jmax) if req.kind == "linear" else _AsymModel(req)
         dev = "cuda"
-        E = req.B * m.J * (m.J + 1.0)
+        E = m.energies(req)
         d = torch.from_numpy(E.astype(np.complex128)).to(dev, non_blocking=False)
         phase = torch.empty(m.n, dtype=torch.complex128, device=dev)
         _lib.check(self.lib.rm_phase_init(m.n, ctypes.c_void_p(d.data_ptr()), float(self.f.real),
                                           float(self.f.imag), ctypes.c_void_p(phase.data_ptr()),
                                           ctypes.c_void_p(self.stream.cuda_stream)), "rm_phase_init")
-        tab = torch.from_numpy(m.table(req.fields())).to(dev)
+        F = req.fields()
+        # field steps up to each step (the ones that run a Lanczos expv)
+        req.cumfield = np.concatenate([[0], np.cumsum(np.any(F != 0, axis=1))]).astype(np.int64)
+        tab = torch.from_numpy(m.table(F)).to(dev)
         sflag = torch.empty(req.nsteps, dtype=torch.int32, device=dev)
         _lib.check(self.lib.rm_plan_step_flags(m.plan.handle, req.nsteps, ctypes.c_void_p(tab.data_ptr()),
                                                ctypes.c_void_p(sflag.data_ptr()),
@@ -212,7 +335,7 @@ class Server:
         rec["status"] = status.data_ptr()
         req.t_admit = now
         req.steps_done = 0
-        keep = (d, phase, tab, sflag, psi, kused, status)
+        keep = (d, phase, tab, sflag, psi, kused, status, m)   # m: an asymmetric top's own plan
         self.used += self.footprint(req)
         return rec, keep, m.nslot
 
@@ -220,6 +343,9 @@ class Server:
         """Serve ``requests`` (arrival times in seconds from the start);
         returns them with t_admit / t_done set and psi on the host."""
         torch = self.torch
+        self.t_host = 0.0
+        self.ticks, self.t_device = 0, 0.0
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         pending = deque(sorted(requests, key=lambda r: r.arrival))
         queue = deque()
         active, recs, keeps, nslots = [], [], [], []
@@ -253,7 +379,16 @@ class Server:
             # this tick: each active request advances min(chunk, remaining) steps
             steps_done = np.array([r.steps_done for r in active], dtype=np.int64)
             total = np.array([r.nsteps for r in active], dtype=np.int64)
-            nst = np.minimum(self.chunk, total - steps_done)
+            # every request's part of the tick costs about chunk field steps of
+            # the costliest one; field-free steps (phases only) are not counted
+            cost = np.array([k[7].cost for k in keeps])
+            fsteps = np.maximum(1, (self.chunk * cost.max() / cost).astype(np.int64))
+            nst = np.empty(len(active), dtype=np.int64)
+            for i, r in enumerate(active):
+                cf = r.cumfield
+                end = np.searchsorted(cf, cf[r.steps_done] + fsteps[i], side="right") - 1
+                nst[i] = max(1, min(end - r.steps_done, self.chunk_max))
+            nst = np.minimum(nst, total - steps_done)
             base_tab = np.array([k[2].data_ptr() for k in keeps], dtype=np.uint64)
             base_flag = np.array([k[3].data_ptr() for k in keeps], dtype=np.uint64)
             base_k = np.array([k[5].data_ptr() for k in keeps], dtype=np.uint64)
@@ -262,14 +397,20 @@ class Server:
             jobs["sflag_dev"] = base_flag + steps_done.astype(np.uint64) * np.uint64(4)
             jobs["kused"] = base_k + steps_done.astype(np.uint64) * np.uint64(4)
             jobs["nsteps"] = nst.astype(np.int32)
+            # admission initialised the new requests' tensors on the default stream
+            self.stream.wait_stream(torch.cuda.current_stream())
+            ev0.record(self.stream)
             batch_propagate(self.lib, self.handle, jobs, self.kfix, self.maxorder, self.tol, self.precision,
                             self.stream.cuda_stream)
+            ev1.record(self.stream)
             ev.record(self.stream)
             # overlap: admit the requests that arrived meanwhile (they join the next tick)
             n_before = len(active)
             admit(clock() - t0)
             ev.synchronize()
             t_tick = clock() - t0
+            self.ticks += 1
+            self.t_device += ev0.elapsed_time(ev1) * 1e-3
             keep_idx = []
             for i in range(n_before):
                 r = active[i]
@@ -278,6 +419,7 @@ class Server:
                     r.t_done = t_tick
                     r.psi = keeps[i][4]
                     r.kused = keeps[i][5]
+                    r.index = getattr(keeps[i][7], "index", None)
                     self.used -= self.footprint(r)
                     done.append(r)
                 else:
@@ -292,8 +434,14 @@ class Server:
             elif len(jobs) != len(active):
                 jobs = job_array(recs)
         torch.cuda.synchronize()
+        self.t_host_run = self.t_host
         for r in done:
-            r.psi = r.psi.cpu().numpy()[0]
+            v = r.psi.cpu().numpy()[0]
+            if r.index is not None:   # back to the whole m = 0 basis
+                full = np.zeros(r.n, dtype=np.complex128)
+                full[r.index] = v
+                v = full
+            r.psi = v
             r.kused = r.kused.cpu().numpy()
         return done
 
@@ -307,25 +455,30 @@ def summarize(done, wall):
             "latency_max_ms": float(lat.max() * 1e3), "mean_steps": steps / len(done)}
 
 
-def benchmark(n_requests=200, seed=0, load=0.8, chunk=64, **kw):
+def benchmark(n_requests=200, seed=0, load=0.8, chunk=64, asym_fraction=0.5, precision="fp32", **kw):
     """Closed-loop capacity, then a Poisson stream at ``load`` x capacity."""
-    srv = Server(chunk=chunk, **kw)
-    reqs = make_requests(n_requests, seed=seed)
-    for j in sorted({r.jmax for r in reqs}):   # operator cache warm (shared per Jmax)
+    srv = Server(chunk=chunk, precision=precision, **kw)
+    reqs = make_requests(n_requests, seed=seed, asym_fraction=asym_fraction)
+    for j in sorted({r.jmax for r in reqs if r.kind == "linear"}):   # linear operators (shared per Jmax)
         srv.model(j)
     t0 = time.perf_counter()
     done = srv.run(reqs)
     closed = time.perf_counter() - t0
     cap = len(done) / closed
     rate = load * cap
-    reqs2 = make_requests(n_requests, seed=seed, rate=rate)
+    reqs2 = make_requests(n_requests, seed=seed, rate=rate, asym_fraction=asym_fraction)
     t0 = time.perf_counter()
     done2 = srv.run(reqs2)
     wall = time.perf_counter() - t0
     rep = summarize(done2, wall)
-    rep.update({"offered_rate_per_s": rate, "closed_loop_capacity_per_s": cap, "load": load,
+    rep.update({"admission_host_s": srv.t_host_run, "ticks": srv.ticks, "device_busy_s": srv.t_device,
+                "queue_wait_p50_ms": float(np.percentile([r.t_admit - r.arrival for r in done2], 50) * 1e3),
+                "queue_wait_p99_ms": float(np.percentile([r.t_admit - r.arrival for r in done2], 99) * 1e3),"offered_rate_per_s": rate, "closed_loop_capacity_per_s": cap, "load": load,
                 "chunk_steps": chunk, "krylov": srv.kfix, "models": len(srv.models),
-                "stream": "linear rotors, B U(0.05,30) cm^-1, Jmax U{10..60}, FWHM U(100,1000) fs, "
-                          "I log-U(1e10,1e12) W/cm^2, 200-2000 steps of 10 fs, psi0=|0,0>, "
+                "precision": precision, "asym_fraction": asym_fraction,
+                "requests_asym": sum(r.kind == "asym" for r in done2),
+                "stream": f"linear rotors / asymmetric tops ({asym_fraction:.0%} asym), rotational constants "
+                          "U(0.05,30) cm^-1, Jmax U{10..60}, FWHM U(100,1000) fs, I log-U(1e10,1e12) W/cm^2, "
+                          "200-2000 steps of 10 fs, Z pulse, psi0 = J=0 state, "
                           f"Poisson arrivals at {load} x closed-loop capacity (seeded)"})
     return rep

@@ -183,3 +183,35 @@ def test_asymtop_tensors_match_camphor():
                 assert rel > 1e-3          # unrotated README dipole: clearly resolved
             else:
                 assert rel < 2e-7, (key, rel)
+
+
+def _flat_dense(T):
+    """Dense matrix of a plan's operator at unit field products."""
+    from richmol_amd import plan as P
+    arr = P.flatten_operator(T, P.active_carts(T))
+    n = arr["n"]
+    mf = np.array([arr["term_mval"][arr["term_ptr"][t]:arr["term_ptr"][t + 1]].sum() for t in range(arr["nterm"])])
+    D = np.zeros((n, n), dtype=np.complex128)
+    for i in range(n):
+        for e in range(arr["row_ptr"][i], arr["row_ptr"][i + 1]):
+            p0, p1 = arr["pair_ptr"][e], arr["pair_ptr"][e + 1]
+            D[i, arr["col_idx"][e]] = np.sum(arr["pair_kval"][p0:p1] * mf[arr["pair_term"][p0:p1]])
+    return D
+
+
+@pytest.mark.parametrize("tens", [np.diag([12.0, 8.0, 16.0]),
+                                  np.array([[10.0, 1.5, -0.7], [1.5, 7.0, 0.4], [-0.7, 0.4, 15.0]]),
+                                  np.array([0.3, -0.2, 0.9])])
+def test_zfield_m0_operator_equals_cartens(tens):
+    """The serving path's direct m = 0 operator equals the general builder's
+    operator (flattened under a unit Z field: F_z^rank = 1) entry by entry,
+    and its energies equal the field-free Hamiltonian's diagonal."""
+    bas = rot.asymtop_basis(14.0, 3.0, 27.0, 12, m_filter=lambda J, m: m == 0)
+    T = rot.tensor(tens, bas)
+    T.field([0.0, 0.0, 1.0])
+    D = _flat_dense(T)
+    A, E = rot.zfield_m0_operator(bas, tens)
+    assert A.shape == D.shape
+    assert np.abs(A.toarray() - D).max() <= 1e-13 * np.abs(D).max()
+    h0 = rot.hamiltonian(bas)
+    assert np.array_equal(E, h0.tomat(form="full", cart="0").diagonal().real)

@@ -72,17 +72,20 @@ def test_water_j6_adaptive(device, kernel):
     assert np.array_equal(k_d, k_o)
 
 
-def test_water_j20_configuration(device):
+@pytest.mark.parametrize("kernel", ["auto", "block"])
+def test_water_j20_configuration(device, kernel):
     """The configuration's basis: J <= 20 with all m (N = 12 341, 1.5 M stored
     entries, four invariant subspaces of ~3 100 states), ground state, a few
-    steps at the pulse peak."""
+    steps at the pulse peak: 'auto' takes the cooperative kernel (one
+    wavepacket on a long domain), 'block' one workgroup reading its matvec
+    input from memory (a domain longer than the block kernel stages in LDS)."""
     h0, leaves, bas = water(20)
     n = 12341
     X = np.zeros((1, n), dtype=np.complex128)
     X[0, 0] = 1.0
     t = (np.arange(4) + 79.5) * DT
     fields = water_fields(t)
-    psi_d, k_d, e = _device(h0, leaves, fields, X)
+    psi_d, k_d, e = _device(h0, leaves, fields, X, kernel=kernel)
     assert e.plan.n == n
     psi_o, k_o = _oracle(h0, leaves, fields, X)
     assert np.abs(psi_d - psi_o).max() <= TOL_PSI
