@@ -201,6 +201,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--asym-steps", type=int, default=2000,
                     help="steps of the asymmetric-top leg (0: skip)")
+    ap.add_argument("--asym-only", action="store_true",
+                    help="only the asymmetric-top leg (for profiling its kernel) and exit")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-sized device buffer 3 times (reference for the HBM "
                          "byte counters) and exit")
@@ -215,6 +217,9 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"calibrate_copy_bytes_read": nb, "calibrate_copy_bytes_written": nb,
                           "copies": 3}))
+        return
+    if args.asym_only:
+        print(json.dumps({"asymtop_water_j20": asymtop_leg(args.asym_steps, cpu_steps=0)}), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
