@@ -179,6 +179,65 @@ def asymtop_leg(nsteps, cpu_steps=3):
     return out
 
 
+def kron_leg(nsteps=3, ncols=64, jmax=80, krylov=12, t0_fs=50000.0):
+    """The large asymmetric-top configuration: H2S-like rotor (A, B, C =
+    10.36, 9.02, 4.73 cm^-1), J <= jmax with all m (N = 708 561 at J <= 80),
+    dipole + full polarizability, circularly polarised XY field chirped
+    0 -> 10 THz over 100 ps (E0 = 1e8 V/m), dt = 10 fs, fixed Krylov
+    dimension 12, ncols wavepackets (the ncols lowest field-free states) on
+    the Kronecker-block operator (rm_kplan_propagate: per output J one dense
+    product over all wavepackets).  One warm-up step, then nsteps timed
+    steps from t0_fs, in FP64 and with FP32 dense products from the same
+    start; <cos^2 theta> (molecular z = C2 axis against lab Z) of both."""
+    import torch
+    from richmol_amd.kron import DeviceKronPlan, KronOperator, slot_products
+    from tests._fixtures import centrifuge_fields, large_top
+    t_b = time.perf_counter()
+    kop, bas = large_top(jmax)
+    dp = DeviceKronPlan(kop)
+    kc = KronOperator(bas, [np.diag([0.0, 0.0, 1.0])], [1.0])
+    dc = DeviceKronPlan(kc)
+    build_s = time.perf_counter() - t_b
+    n = kop.n
+    f = exp_fac(10.0)
+    t = t0_fs + (np.arange(nsteps + 1) + 0.5) * 10.0
+    fp = slot_products(kop, centrifuge_fields(t))
+    cfp = torch.from_numpy(slot_products(kc, [np.array([[0.0, 0.0, 1.0]])])[0]).cuda()
+    order = np.argsort(kop.energies, kind="stable")[:ncols]
+    psi0 = torch.zeros((ncols, n), dtype=torch.complex128, device="cuda:0")
+    psi0[torch.arange(ncols), torch.from_numpy(order).cuda()] = 1.0
+    ph = dp.phases(f)
+    y = torch.empty_like(psi0)
+
+    def cos2(psi):
+        dc.matvec(psi, y, cfp)
+        return torch.real(torch.sum(psi.conj() * y, dim=1)).cpu().numpy()
+
+    out = {"n_states": n, "ncols": ncols, "jmax": jmax, "krylov_fixed": krylov, "dt_fs": 10.0,
+           "blocks": len(kop.blocks), "host_build_s": build_s, "steps_timed": nsteps,
+           "t_start_fs": float(t[1] - 5.0), "config_steps": 10000,
+           "gemm_flops_per_step": dp.gemm_flops_per_col * ncols * krylov}
+    res = {}
+    for prec, peak in (("fp64", 78.6), ("fp32", 157.3)):
+        psi = psi0.clone()
+        dp.propagate(psi, fp[:1], f, phase=ph, krylov_fixed=krylov, precision=prec)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        dp.propagate(psi, fp[1:], f, phase=ph, krylov_fixed=krylov, precision=prec)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t1
+        tf = out["gemm_flops_per_step"] * nsteps / wall / 1e12
+        res[prec] = (psi, cos2(psi))
+        out[prec] = {"wavepacket_steps_per_s": ncols * nsteps / wall, "ms_per_step": wall / nsteps * 1e3,
+                     "dense_tflops_effective": tf, "peak_tflops": peak, "frac_of_peak": tf / peak,
+                     "norm_err": float((torch.linalg.vector_norm(psi, dim=1) - 1).abs().max())}
+        del psi
+    out["fp32_vs_fp64"] = {"max_abs_amplitude": float((res["fp32"][0] - res["fp64"][0]).abs().max()),
+                           "max_abs_cos2": float(np.abs(res["fp32"][1] - res["fp64"][1]).max()),
+                           "cos2_fp64_mean": float(res["fp64"][1].mean())}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -203,6 +262,10 @@ def main():
                     help="steps of the asymmetric-top leg (0: skip)")
     ap.add_argument("--asym-only", action="store_true",
                     help="only the asymmetric-top leg (for profiling its kernel) and exit")
+    ap.add_argument("--kron-steps", type=int, default=3,
+                    help="timed steps of the large asymmetric-top (Kronecker-block) leg (0: skip)")
+    ap.add_argument("--kron-only", action="store_true",
+                    help="only the large asymmetric-top leg (for profiling its kernels) and exit")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-sized device buffer 3 times (reference for the HBM "
                          "byte counters) and exit")
@@ -217,6 +280,9 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"calibrate_copy_bytes_read": nb, "calibrate_copy_bytes_written": nb,
                           "copies": 3}))
+        return
+    if args.kron_only:
+        print(json.dumps({"large_top_j80": kron_leg(args.kron_steps)}), flush=True)
         return
     if args.asym_only:
         print(json.dumps({"asymtop_water_j20": asymtop_leg(args.asym_steps, cpu_steps=0)}), flush=True)
@@ -359,6 +425,11 @@ def main():
         if args.asym_steps > 0:
             result["asymtop_water_j20"] = asymtop_leg(args.asym_steps)
             print(json.dumps({"progress": "asymtop"}), file=sys.stderr, flush=True)
+
+        # ---- the large asymmetric-top configuration (64 wavepackets, N = 708 561)
+        if args.kron_steps > 0:
+            result["large_top_j80"] = kron_leg(args.kron_steps)
+            print(json.dumps({"progress": "large_top"}), file=sys.stderr, flush=True)
 
         # ---- serving: continuous batching of independent propagation requests
         if args.serve_requests > 0:

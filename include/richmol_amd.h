@@ -263,6 +263,54 @@ int rm_phase_apply(int64_t n, const void* psi_in, void* psi_out, int64_t ld, int
 int rm_expect(rm_plan* plan, const void* psi, int64_t ld, int64_t ncols,
               void* out, void* stream);
 
+/*
+ * Kronecker-block operators for large rotor bases (the factored form of
+ * CarTens.vec, field.py:1145-1245: Y[J1] += MF_w X[J2] K_w^T per block and
+ * irrep w).  The basis has every J of `Js` (ascending) with all m; a
+ * wavepacket is stored per J as the (2J+1) x (2J+1) matrix X[J][m][k]
+ * (m-major, k over all species), J blocks in the order of `Js` -- the
+ * "internal order" below.  Block q couples J1[q] <- J2[q]:
+ *   K:     complex (2J1+1) x (2J2+1) row-major at K + 2*k_off[q] (doubles);
+ *   sigma: its m-diagonals sig[sig_off[q] .. sig_off[q+1]) (m2 = m1 - sigma,
+ *          |sigma| <= 2);
+ *   mdiag: complex [nsig][ncart[q]][2J1+1] at mdiag + 2*d_off[q]: the values
+ *          of M_w,cart on those diagonals (indexed by m1);
+ *   slot0: the field products of its Cartesian components are slots
+ *          slot0[q] .. slot0[q]+ncart[q]-1 of a step's nslot products.
+ * Per step MF_q = sum_c fprod[slot0+c] M_q,c (CarTens.field,
+ * field.py:1073-1142); per output J1 the stacked MF_q X[J2] of every
+ * wavepacket go through one dense product with the stacked K_q (rocBLAS).
+ */
+typedef struct rm_kplan rm_kplan;
+int rm_kplan_create(int32_t nJ, const int32_t* Js, int32_t nslot, int32_t nblk,
+                    const int32_t* J1, const int32_t* J2, const int32_t* slot0,
+                    const int32_t* ncart, const int64_t* k_off, const double* K,
+                    const int64_t* sig_off, const int32_t* sig, const int64_t* d_off,
+                    const double* mdiag, rm_kplan** out);
+int rm_kplan_destroy(rm_kplan* plan);
+/* n: states per wavepacket; gemm_flops_per_col: real flops of one matvec's
+ * dense products per wavepacket. */
+int rm_kplan_info(const rm_kplan* plan, int64_t* n, double* gemm_flops_per_col);
+/* y = H(F) x for ncols wavepackets (device, complex, row b at b*ld, internal
+ * order); fprod_dev: device, nslot complex.  precision RM_PREC_FP32: the
+ * products in FP32 (complex64 operands, FP32 accumulation). */
+int rm_kplan_matvec(rm_kplan* plan, const void* x, void* y, int64_t ld, int32_t ncols,
+                    const double* fprod_dev, int32_t precision, void* stream);
+/*
+ * nsteps steps of TDSE.update (tdse.py:365-414) for ncols wavepackets (rows
+ * of psi, in place) with a fixed Krylov dimension krylov_fixed
+ * (tdse.py:417-486; an exact breakdown continues with a zero vector, whose
+ * coefficient in exp(fT)e0 is zero as the reference's restart vector's is).
+ * fprod: HOST, nsteps x nslot complex; a step whose products are all zero
+ * applies the phases only (the reference skips the Lanczos when the field
+ * leaves no MF block).  phase: device, n complex, internal order (from
+ * rm_phase_init) or NULL.  RM_PREC_FP32: the dense products in FP32; the
+ * recurrence, norms, exp(fT)e0, phases and the wavepackets stay FP64.
+ */
+int rm_kplan_propagate(rm_kplan* plan, void* psi, int64_t ld, int32_t ncols, int64_t nsteps,
+                       const double* fprod, double f_re, double f_im, const void* phase,
+                       int32_t krylov_fixed, int32_t precision, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -8,7 +8,8 @@ LIB_NAME = "librichmol_amd.so"
 LIB_PATH = os.path.join(PKG_DIR, LIB_NAME)
 SOURCES = [os.path.join(PKG_DIR, "csrc", "rm_tdse.hip"),
            os.path.join(PKG_DIR, "csrc", "rm_subspace.hip"),
-           os.path.join(PKG_DIR, "csrc", "rm_coop.hip")]
+           os.path.join(PKG_DIR, "csrc", "rm_coop.hip"),
+           os.path.join(PKG_DIR, "csrc", "rm_kron.hip")]
 
 HIPCC_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared",
@@ -17,6 +18,8 @@ HIPCC_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I", os.path.join(REPO_DIR, "include"),
 ]
+# dense products of the Kronecker-block operator
+LINK_FLAGS = ["-L/opt/rocm/lib", "-lrocblas"]
 
 
 def hipcc():
@@ -27,7 +30,7 @@ def hipcc():
 def build(verbose=False):
     """Compile SOURCES into LIB_PATH (always recompiles)."""
     tmp = LIB_PATH + ".tmp"
-    cmd = [hipcc(), *HIPCC_FLAGS, "-o", tmp, *SOURCES]
+    cmd = [hipcc(), *HIPCC_FLAGS, "-o", tmp, *SOURCES, *LINK_FLAGS]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)

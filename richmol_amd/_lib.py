@@ -54,6 +54,16 @@ SIGNATURES = {
     "rm_batch_set_block_rows": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "rm_batch_propagate": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                           ctypes.c_int32, ctypes.c_double, ctypes.c_int32, _vp]),
+    "rm_kplan_create": (ctypes.c_int, [ctypes.c_int32, _c_i32p, ctypes.c_int32, ctypes.c_int32, _c_i32p,
+                                       _c_i32p, _c_i32p, _c_i32p, _c_i64p, _c_dblp, _c_i64p, _c_i32p,
+                                       _c_i64p, _c_dblp, ctypes.POINTER(_vp)]),
+    "rm_kplan_destroy": (ctypes.c_int, [_vp]),
+    "rm_kplan_info": (ctypes.c_int, [_vp, _c_i64p, _c_dblp]),
+    "rm_kplan_matvec": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int64, ctypes.c_int32, _vp, ctypes.c_int32,
+                                       _vp]),
+    "rm_kplan_propagate": (ctypes.c_int, [_vp, _vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int64, _c_dblp,
+                                          ctypes.c_double, ctypes.c_double, _vp, ctypes.c_int32,
+                                          ctypes.c_int32, _vp]),
 }
 
 
@@ -83,6 +93,8 @@ def load():
         raise RichmolAmdError(
             f"{LIB_PATH} is missing: build it with `python -m richmol_amd._build` "
             "(the HIP path has no CPU fallback)")
+    # torch first: its HIP runtime and rocBLAS are then the ones the library binds to
+    import torch  # noqa: F401
     lib = ctypes.CDLL(LIB_PATH)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

@@ -1,0 +1,743 @@
+// richmol_amd: Kronecker-block operators (factored H = sum K_w (x) MF_w) for
+// large rotor bases, and a batched fixed-order Lanczos propagator on them.
+//
+// richmol applies a tensor block by block as Y[J1] += MF_w X[J2] K_w^T
+// (field.py:1235-1237).  Here, per output J1, every source block
+// q = (J2, w, tensor) is stacked: Z_J1 = [MF_q X[J2]]_q (the banded m-mixing
+// applied to the matvec input of all wavepackets at once), and
+// Y[J1] = Z_J1 K_J1^T with K_J1 = [K_q]_q -- one dense product per J1 whose
+// rows are (wavepacket, m1), the GEMM the matrix cores run (rocBLAS).  The
+// complex product is done in real arithmetic on split layouts:
+//   [Yr | Yi] = [Zr | Zi] [[Kr^T, Ki^T], [-Ki^T, Kr^T]].
+// Vectors are kept per J block as rows (wavepacket, m) of 2 (2J+1) reals
+// [re(k) | im(k)]; a wavepacket's (J) chunk is contiguous, so its dot
+// products are two-pass sums over (J, wavepacket) chunks in a fixed order.
+//
+// The Lanczos recurrence of tdse.py:417-486 with a fixed Krylov dimension m
+// (unnormalised first vector, V_{k+1} = W/beta formed where W is read, an
+// exact breakdown giving a zero vector, whose coefficient in exp(fT)e0 is
+// exactly zero as the reference's restart vector's is), the split-operator
+// phases around it, and exp(fT)e0 per wavepacket (Taylor with scaling, one
+// wave each).  FP64, or FP32 (complex64 Z, K and matvec output; the
+// recurrence, norms, exp(fT), phases and the wavepacket in FP64).
+
+#include <rocblas/rocblas.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "rm_common.h"
+
+namespace {
+
+// One (output J1, source block) pair: Z_J1 columns [koff, koff + dk2) of
+// each half (re / im), computed from the J2 block of the input.
+struct KTask {
+  int32_t J1, J2, dm1, dm2, dk2, koff, ktot, nsig;
+  int32_t sig0;        // first entry of the block's sigma list
+  int32_t blk;
+  int64_t mf_off;      // MF values of the block: [nsig][dm1] complex
+  int64_t zoff;        // Z_J1 (reals) for this call's wavepackets
+  int64_t xoff;        // X[J2] (reals) in the split vector layout
+};
+
+struct KBlk {          // per block: MF assembly from the field products
+  int32_t dm1, nsig, ncart, slot0;
+  int64_t mf_off, d_off;
+};
+
+}  // namespace
+
+struct rm_kplan {
+  int32_t nJ = 0, nslot = 0, nblk = 0;
+  std::vector<int32_t> Js;              // ascending
+  std::vector<int64_t> h_voff;          // per J: complex offset of the J block per wavepacket
+  int64_t n = 0;                        // states per wavepacket
+  std::vector<KBlk> h_blk;
+  std::vector<int32_t> h_J1, h_J2, h_sig;
+  std::vector<int64_t> h_sig_off;
+  std::vector<std::vector<int32_t>> src;   // per J index: source blocks
+  std::vector<int32_t> ktot;               // per J index
+  std::vector<int64_t> kq_off;             // per block: column offset within its J1's K stack
+  double2* d_mdiag = nullptr;
+  int32_t* d_sig = nullptr;
+  KBlk* d_blk = nullptr;
+  double2* d_mf = nullptr;
+  int64_t mf_count = 0;
+  std::vector<double*> kbig64;            // per J index: (2 Ktot) x (2 dk) row-major
+  std::vector<float*> kbig32;
+  rocblas_handle rb = nullptr;
+  int64_t* d_voff = nullptr;            // per J: complex offset (per wavepacket)
+  int32_t* d_Js = nullptr;
+  // per-call workspace
+  void* ws = nullptr;
+  size_t ws_bytes = 0;
+  KTask* d_tasks = nullptr;
+  int64_t tasks_cap = 0;
+  double2* d_fp = nullptr;              // field products of a call's steps
+  int64_t fp_cap = 0;
+};
+
+namespace {
+
+__host__ __device__ inline int64_t jdim(int32_t J) { return 2 * (int64_t)J + 1; }
+
+// MF[blk][t][m1] = sum_c fprod[slot0 + c] mdiag[blk][t][c][m1]
+__global__ void k_kp_mf(const KBlk* __restrict__ blk, const double2* __restrict__ mdiag,
+                        const double2* __restrict__ fp, double2* __restrict__ mf) {
+  const KBlk bk = blk[blockIdx.y];
+  const int64_t cnt = (int64_t)bk.nsig * bk.dm1;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < cnt; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t t = e / bk.dm1, m1 = e % bk.dm1;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int c = 0; c < bk.ncart; ++c)
+      acc = cfma(fp[bk.slot0 + c], mdiag[bk.d_off + (t * bk.ncart + c) * bk.dm1 + m1], acc);
+    mf[bk.mf_off + e] = acc;
+  }
+}
+
+// Z rows (b, m1) of one task: sum over the block's sigmas of MF[t][m1] X[J2](b, m2 = m1 - J1 - s + J2) * ib[b]
+template <typename ZT>
+__global__ __launch_bounds__(256) void k_kp_z(const KTask* __restrict__ tasks, const int32_t* __restrict__ sig,
+                                              const double2* __restrict__ mf, const double* __restrict__ X,
+                                              const double* __restrict__ ib, ZT* __restrict__ Z, int32_t ncols) {
+  const KTask tk = tasks[blockIdx.y];
+  const int lane = threadIdx.x & 63, rsub = threadIdx.x >> 6;   // 4 rows per block pass
+  const int64_t rows = (int64_t)ncols * tk.dm1;
+  const int64_t zld = 2 * (int64_t)tk.ktot, xld = 2 * (int64_t)tk.dk2;
+  for (int64_t r = (int64_t)blockIdx.x * 4 + rsub; r < rows; r += (int64_t)gridDim.x * 4) {
+    const int64_t b = r / tk.dm1;
+    const int m1 = (int)(r % tk.dm1);
+    const double s = ib[b];
+    for (int k2 = lane; k2 < tk.dk2; k2 += 64) {
+      double zr = 0.0, zi = 0.0;
+      for (int t = 0; t < tk.nsig; ++t) {
+        const int m2 = m1 - tk.J1 - sig[tk.sig0 + t] + tk.J2;
+        if (m2 < 0 || m2 >= tk.dm2) continue;
+        const double2 f = mf[tk.mf_off + (int64_t)t * tk.dm1 + m1];
+        const double* xrow = X + tk.xoff + (b * tk.dm2 + m2) * xld;
+        const double xr = xrow[k2], xi = xrow[tk.dk2 + k2];
+        zr += f.x * xr - f.y * xi;
+        zi += f.x * xi + f.y * xr;
+      }
+      ZT* zrow = Z + tk.zoff + r * zld;
+      zrow[tk.koff + k2] = (ZT)(zr * s);
+      zrow[tk.ktot + tk.koff + k2] = (ZT)(zi * s);
+    }
+  }
+}
+
+// Per (J, wavepacket) chunk: V_it = ib W (stored), alpha partial = vdot(Y, V_it)
+template <typename YT>
+__global__ __launch_bounds__(256) void k_kp_dot(const int64_t* __restrict__ voff, const int32_t* __restrict__ Js,
+                                                const double* __restrict__ W, const double* __restrict__ ib,
+                                                const YT* __restrict__ Y, double* __restrict__ V,
+                                                double* __restrict__ part, int32_t nJ) {
+  const int j = blockIdx.x, b = blockIdx.y;
+  const int64_t d = jdim(Js[j]), len = 2 * d * d;
+  const int64_t base = voff[j] * 2 * gridDim.y + (int64_t)b * len;   // reals
+  const double s = ib[b];
+  double a0 = 0.0, a1 = 0.0;
+  for (int64_t i = threadIdx.x; i < d * d; i += blockDim.x) {
+    const int64_t row = i / d, k = i % d;
+    const int64_t pr = base + row * 2 * d + k, pi = pr + d;
+    const double vr = s * W[pr], vi = s * W[pi];
+    V[pr] = vr;
+    V[pi] = vi;
+    const double yr = (double)Y[pr], yi = (double)Y[pi];
+    a0 += yr * vr + yi * vi;
+    a1 += yr * vi - yi * vr;
+  }
+  __shared__ double red[2][4];
+  a0 = wave_allsum(a0);
+  a1 = wave_allsum(a1);
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = a0;
+    red[1][threadIdx.x >> 6] = a1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    part[((int64_t)b * nJ + j) * 2] = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+    part[((int64_t)b * nJ + j) * 2 + 1] = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
+  }
+}
+
+// W = Y - alpha V_it - beta V_{it-1} per chunk, partial |W|^2
+template <typename YT>
+__global__ __launch_bounds__(256) void k_kp_upd(const int64_t* __restrict__ voff, const int32_t* __restrict__ Js,
+                                                const YT* __restrict__ Y, const double* __restrict__ Vit,
+                                                const double* __restrict__ Vprev, const double2* __restrict__ alpha,
+                                                const double* __restrict__ beta, int32_t it, int32_t kcap,
+                                                double* __restrict__ W, double* __restrict__ part, int32_t nJ) {
+  const int j = blockIdx.x, b = blockIdx.y;
+  const int64_t d = jdim(Js[j]), len = 2 * d * d;
+  const int64_t base = voff[j] * 2 * gridDim.y + (int64_t)b * len;
+  const double2 al = alpha[(int64_t)b * kcap + it];
+  const double be = it > 0 ? beta[(int64_t)b * (kcap + 1) + it] : 0.0;
+  double acc = 0.0;
+  for (int64_t i = threadIdx.x; i < d * d; i += blockDim.x) {
+    const int64_t row = i / d, k = i % d;
+    const int64_t pr = base + row * 2 * d + k, pi = pr + d;
+    const double vr = Vit[pr], vi = Vit[pi];
+    double wr = (double)Y[pr] - (al.x * vr - al.y * vi);
+    double wi = (double)Y[pi] - (al.x * vi + al.y * vr);
+    if (it > 0) {
+      wr -= be * Vprev[pr];
+      wi -= be * Vprev[pi];
+    }
+    W[pr] = wr;
+    W[pi] = wi;
+    acc += wr * wr + wi * wi;
+  }
+  __shared__ double red[4];
+  acc = wave_allsum(acc);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) part[(int64_t)b * nJ + j] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// Second pass: per wavepacket the chunks' partials in J order.  mode 0:
+// alpha (complex) -> alpha[b][it]; mode 1: |W|^2 -> beta[b][it+1] and ib.
+__global__ void k_kp_fin(const double* __restrict__ part, int32_t nJ, int32_t ncols, int32_t mode, int32_t it,
+                         int32_t kcap, double2* __restrict__ alpha, double* __restrict__ beta,
+                         double* __restrict__ ib) {
+  const int b = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (b >= ncols) return;
+  double s0 = 0.0, s1 = 0.0;
+  for (int j = lane; j < nJ; j += 64) {
+    if (mode == 0) {
+      s0 += part[((int64_t)b * nJ + j) * 2];
+      s1 += part[((int64_t)b * nJ + j) * 2 + 1];
+    } else {
+      s0 += part[(int64_t)b * nJ + j];
+    }
+  }
+  s0 = wave_allsum(s0);
+  s1 = wave_allsum(s1);
+  if (lane != 0) return;
+  if (mode == 0) {
+    alpha[(int64_t)b * kcap + it] = make_double2(s0, s1);
+  } else {
+    const double be = sqrt(s0);
+    beta[(int64_t)b * (kcap + 1) + it + 1] = be;
+    // exact breakdown: a zero vector stands for the reference's restart (its coefficient is 0)
+    ib[b] = be != 0.0 ? 1.0 / be : 0.0;
+  }
+}
+
+// Start of a step: V_0 = P S (the phases of the split operator), W = V_0, ib = 1.
+__global__ void k_kp_start(const int64_t* __restrict__ voff, const int32_t* __restrict__ Js,
+                           const double2* __restrict__ ph, const double* __restrict__ S, double* __restrict__ W,
+                           double* __restrict__ ib, double* __restrict__ beta, int32_t kcap) {
+  const int j = blockIdx.x, b = blockIdx.y;
+  const int64_t d = jdim(Js[j]), len = 2 * d * d;
+  const int64_t base = voff[j] * 2 * gridDim.y + (int64_t)b * len;
+  for (int64_t i = threadIdx.x; i < d * d; i += blockDim.x) {
+    const int64_t row = i / d, k = i % d;
+    const int64_t pr = base + row * 2 * d + k, pi = pr + d;
+    const double2 p = ph ? ph[voff[j] + i] : make_double2(1.0, 0.0);
+    const double sr = S[pr], si = S[pi];
+    W[pr] = p.x * sr - p.y * si;
+    W[pi] = p.x * si + p.y * sr;
+  }
+  if (j == 0 && threadIdx.x == 0) {
+    ib[b] = 1.0;
+    beta[(int64_t)b * (kcap + 1)] = 0.0;
+  }
+}
+
+// A step without field: S = P P S (tdse.py:368-380 with the expv skipped)
+__global__ void k_kp_phase2(const int64_t* __restrict__ voff, const int32_t* __restrict__ Js,
+                            const double2* __restrict__ ph, double* __restrict__ S) {
+  const int j = blockIdx.x, b = blockIdx.y;
+  const int64_t d = jdim(Js[j]), len = 2 * d * d;
+  const int64_t base = voff[j] * 2 * gridDim.y + (int64_t)b * len;
+  for (int64_t i = threadIdx.x; i < d * d; i += blockDim.x) {
+    const int64_t row = i / d, k = i % d;
+    const int64_t pr = base + row * 2 * d + k, pi = pr + d;
+    const double2 p = ph[voff[j] + i];
+    double sr = S[pr], si = S[pi];
+    double tr = p.x * sr - p.y * si, ti = p.x * si + p.y * sr;
+    S[pr] = p.x * tr - p.y * ti;
+    S[pi] = p.x * ti + p.y * tr;
+  }
+}
+
+// exp(f T) e0 per wavepacket (one wave each)
+__global__ __launch_bounds__(64) void k_kp_expm(const double2* __restrict__ alpha, const double* __restrict__ beta,
+                                                int32_t m, int32_t kcap, double2 f, double2* __restrict__ coef) {
+  __shared__ double2 s_a[KMAX];
+  __shared__ double s_b[KMAX + 1];
+  __shared__ double2 s_c[KMAX];
+  const int b = blockIdx.x, l = threadIdx.x;
+  for (int i = l; i < m; i += 64) {
+    s_a[i] = alpha[(int64_t)b * kcap + i];
+    s_b[i] = beta[(int64_t)b * (kcap + 1) + i];
+  }
+  wave_sync();
+  if (m == 1) {
+    if (l == 0) s_c[0] = make_double2(1.0, 0.0);
+  } else {
+    expm_e0_any(m, s_a, s_b, f, s_c);
+  }
+  wave_sync();
+  for (int i = l; i < m; i += 64) coef[(int64_t)b * kcap + i] = s_c[i];
+}
+
+// S = P sum_j c_j V_j
+__global__ void k_kp_comb(const int64_t* __restrict__ voff, const int32_t* __restrict__ Js,
+                          const double2* __restrict__ ph, const double* __restrict__ V, int64_t vstride,
+                          const double2* __restrict__ coef, int32_t m, int32_t kcap, double* __restrict__ S) {
+  const int j = blockIdx.x, b = blockIdx.y;
+  const int64_t d = jdim(Js[j]), len = 2 * d * d;
+  const int64_t base = voff[j] * 2 * gridDim.y + (int64_t)b * len;
+  for (int64_t i = threadIdx.x; i < d * d; i += blockDim.x) {
+    const int64_t row = i / d, k = i % d;
+    const int64_t pr = base + row * 2 * d + k, pi = pr + d;
+    double ur = 0.0, ui = 0.0;
+    for (int q = 0; q < m; ++q) {
+      const double2 c = coef[(int64_t)b * kcap + q];
+      const double vr = V[q * vstride + pr], vi = V[q * vstride + pi];
+      ur += c.x * vr - c.y * vi;
+      ui += c.x * vi + c.y * vr;
+    }
+    const double2 p = ph ? ph[voff[j] + i] : make_double2(1.0, 0.0);
+    S[pr] = p.x * ur - p.y * ui;
+    S[pi] = p.x * ui + p.y * ur;
+  }
+}
+
+// (B, n) complex rows in the internal order <-> split layout
+__global__ void k_kp_pack(const int64_t* __restrict__ voff, const int32_t* __restrict__ Js,
+                          const double2* __restrict__ psi, int64_t ld, double* __restrict__ S, int32_t to_split) {
+  const int j = blockIdx.x, b = blockIdx.y;
+  const int64_t d = jdim(Js[j]), len = 2 * d * d;
+  const int64_t base = voff[j] * 2 * gridDim.y + (int64_t)b * len;
+  for (int64_t i = threadIdx.x; i < d * d; i += blockDim.x) {
+    const int64_t row = i / d, k = i % d;
+    const int64_t pr = base + row * 2 * d + k;
+    double2* p = (double2*)psi + (int64_t)b * ld + voff[j] + i;
+    if (to_split) {
+      const double2 v = *p;
+      S[pr] = v.x;
+      S[pr + d] = v.y;
+    } else {
+      *p = make_double2(S[pr], S[pr + d]);
+    }
+  }
+}
+
+// matvec output (split layout, FP64 or FP32) -> complex rows
+template <typename YT>
+__global__ void k_kp_out(const int64_t* __restrict__ voff, const int32_t* __restrict__ Js, const YT* __restrict__ Y,
+                         double2* __restrict__ psi, int64_t ld) {
+  const int j = blockIdx.x, b = blockIdx.y;
+  const int64_t d = jdim(Js[j]), len = 2 * d * d;
+  const int64_t base = voff[j] * 2 * gridDim.y + (int64_t)b * len;
+  for (int64_t i = threadIdx.x; i < d * d; i += blockDim.x) {
+    const int64_t row = i / d, k = i % d;
+    const int64_t pr = base + row * 2 * d + k;
+    psi[(int64_t)b * ld + voff[j] + i] = make_double2((double)Y[pr], (double)Y[pr + d]);
+  }
+}
+
+#define RBCHK(expr)                                                                    \
+  do {                                                                                 \
+    rocblas_status st_ = (expr);                                                       \
+    if (st_ != rocblas_status_success)                                                 \
+      return fail(RM_ERR_HIP, "%s failed: %s", #expr, rocblas_status_to_string(st_)); \
+  } while (0)
+
+void kplan_free(rm_kplan* p) {
+  if (!p) return;
+  void* bufs[] = {p->d_mdiag, p->d_sig, p->d_blk, p->d_mf, p->ws, p->d_tasks, p->d_voff, p->d_Js, p->d_fp};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  for (double* k : p->kbig64)
+    if (k) (void)hipFree(k);
+  for (float* k : p->kbig32)
+    if (k) (void)hipFree(k);
+  if (p->rb) (void)rocblas_destroy_handle(p->rb);
+  delete p;
+}
+
+}  // namespace
+
+extern "C" int rm_kplan_create(int32_t nJ, const int32_t* Js, int32_t nslot, int32_t nblk, const int32_t* J1,
+                               const int32_t* J2, const int32_t* slot0, const int32_t* ncart,
+                               const int64_t* k_off, const double* K, const int64_t* sig_off,
+                               const int32_t* sig, const int64_t* d_off, const double* mdiag, rm_kplan** out) {
+  if (!out) return fail(RM_ERR_ARG, "rm_kplan_create: out is NULL");
+  *out = nullptr;
+  if (nJ < 1 || nblk < 0 || nslot < 0 || !Js || (nblk > 0 && (!J1 || !J2 || !slot0 || !ncart || !k_off || !K ||
+                                                              !sig_off || !sig || !d_off || !mdiag)))
+    return fail(RM_ERR_ARG, "rm_kplan_create: bad arguments");
+  rm_kplan* p = new (std::nothrow) rm_kplan();
+  if (!p) return fail(RM_ERR_ALLOC, "rm_kplan_create: host allocation");
+  p->nJ = nJ;
+  p->nslot = nslot;
+  p->nblk = nblk;
+  p->Js.assign(Js, Js + nJ);
+  std::vector<int> jidx(1024, -1);
+  int64_t pos = 0;
+  for (int j = 0; j < nJ; ++j) {
+    if (Js[j] < 0 || Js[j] >= 1024 || (j > 0 && Js[j] <= Js[j - 1])) {
+      kplan_free(p);
+      return fail(RM_ERR_ARG, "rm_kplan_create: J list must be ascending in [0, 1024)");
+    }
+    jidx[Js[j]] = j;
+    p->h_voff.push_back(pos);
+    pos += jdim(Js[j]) * jdim(Js[j]);
+  }
+  p->n = pos;
+  p->src.assign(nJ, {});
+  p->ktot.assign(nJ, 0);
+  p->kq_off.assign(nblk, 0);
+  int64_t mfpos = 0;
+  for (int q = 0; q < nblk; ++q) {
+    const int a = J1[q] >= 0 && J1[q] < 1024 ? jidx[J1[q]] : -1;
+    const int c = J2[q] >= 0 && J2[q] < 1024 ? jidx[J2[q]] : -1;
+    const int64_t ns = sig_off[q + 1] - sig_off[q];
+    if (a < 0 || c < 0 || ns < 1 || ncart[q] < 1 || slot0[q] < 0 || slot0[q] + ncart[q] > nslot ||
+        k_off[q + 1] - k_off[q] != jdim(J1[q]) * jdim(J2[q]) ||
+        d_off[q + 1] - d_off[q] != ns * ncart[q] * jdim(J1[q])) {
+      kplan_free(p);
+      return fail(RM_ERR_ARG, "rm_kplan_create: block %d is inconsistent", q);
+    }
+    for (int64_t t = sig_off[q]; t < sig_off[q + 1]; ++t)
+      if (std::abs(sig[t]) > 2) {
+        kplan_free(p);
+        return fail(RM_ERR_ARG, "rm_kplan_create: block %d has |sigma| > 2", q);
+      }
+    p->kq_off[q] = p->ktot[a];
+    p->ktot[a] += (int32_t)jdim(J2[q]);
+    p->src[a].push_back(q);
+    KBlk bk;
+    bk.dm1 = (int32_t)jdim(J1[q]);
+    bk.nsig = (int32_t)ns;
+    bk.ncart = ncart[q];
+    bk.slot0 = slot0[q];
+    bk.mf_off = mfpos;
+    bk.d_off = d_off[q];
+    mfpos += ns * jdim(J1[q]);
+    p->h_blk.push_back(bk);
+  }
+  p->h_J1.assign(J1, J1 + nblk);
+  p->h_J2.assign(J2, J2 + nblk);
+  p->h_sig_off.assign(sig_off, sig_off + nblk + 1);
+  p->h_sig.assign(sig, sig + (nblk ? sig_off[nblk] : 0));
+  p->mf_count = mfpos;
+  int rc = RM_OK;
+  const int64_t ndiag = nblk ? d_off[nblk] : 0;
+  if ((rc = upload(&p->d_mdiag, (const double2*)mdiag, ndiag)) ||
+      (rc = upload(&p->d_sig, p->h_sig.data(), (int64_t)p->h_sig.size())) ||
+      (rc = upload(&p->d_blk, p->h_blk.data(), (int64_t)p->h_blk.size())) ||
+      (rc = upload(&p->d_mf, (const double2*)nullptr, std::max<int64_t>(mfpos, 1))) ||
+      (rc = upload(&p->d_voff, p->h_voff.data(), nJ)) || (rc = upload(&p->d_Js, p->Js.data(), nJ))) {
+    kplan_free(p);
+    return rc;
+  }
+  // per J1: the real form of the stacked K, [[Kr^T, Ki^T], [-Ki^T, Kr^T]] (2 Ktot x 2 dk1)
+  p->kbig64.assign(nJ, nullptr);
+  p->kbig32.assign(nJ, nullptr);
+  for (int a = 0; a < nJ; ++a) {
+    if (p->src[a].empty()) continue;
+    const int64_t dk1 = jdim(Js[a]), kt = p->ktot[a], ncol = 2 * dk1;
+    std::vector<double> kb((size_t)(2 * kt * ncol), 0.0);
+    for (int q : p->src[a]) {
+      const int64_t dk2 = jdim(J2[q]), ko = p->kq_off[q];
+      const double* Kq = K + 2 * k_off[q];   // complex (dk1 x dk2) row-major
+      for (int64_t k1 = 0; k1 < dk1; ++k1)
+        for (int64_t k2 = 0; k2 < dk2; ++k2) {
+          const double kr = Kq[2 * (k1 * dk2 + k2)], ki = Kq[2 * (k1 * dk2 + k2) + 1];
+          const int64_t r0 = ko + k2, r1 = kt + ko + k2;
+          kb[(size_t)(r0 * ncol + k1)] = kr;
+          kb[(size_t)(r0 * ncol + dk1 + k1)] = ki;
+          kb[(size_t)(r1 * ncol + k1)] = -ki;
+          kb[(size_t)(r1 * ncol + dk1 + k1)] = kr;
+        }
+    }
+    std::vector<float> kf(kb.begin(), kb.end());
+    if ((rc = upload(&p->kbig64[a], kb.data(), (int64_t)kb.size())) ||
+        (rc = upload(&p->kbig32[a], kf.data(), (int64_t)kf.size()))) {
+      kplan_free(p);
+      return rc;
+    }
+  }
+  if (rocblas_create_handle(&p->rb) != rocblas_status_success) {
+    p->rb = nullptr;
+    kplan_free(p);
+    return fail(RM_ERR_HIP, "rm_kplan_create: rocblas_create_handle failed");
+  }
+  *out = p;
+  return RM_OK;
+}
+
+extern "C" int rm_kplan_destroy(rm_kplan* p) {
+  kplan_free(p);
+  return RM_OK;
+}
+
+extern "C" int rm_kplan_info(const rm_kplan* p, int64_t* n, double* gemm_flops_per_col) {
+  if (!p) return fail(RM_ERR_ARG, "rm_kplan_info: NULL plan");
+  if (n) *n = p->n;
+  if (gemm_flops_per_col) {
+    double fl = 0.0;
+    for (int a = 0; a < p->nJ; ++a) {
+      const double d = (double)jdim(p->Js[a]);
+      fl += 2.0 * d * (2.0 * d) * (2.0 * p->ktot[a]);
+    }
+    *gemm_flops_per_col = fl;
+  }
+  return RM_OK;
+}
+
+namespace {
+
+struct KWork {
+  double *S, *W, *V, *ib, *part, *beta;
+  double2 *alpha, *coef;
+  void *Y, *Z;
+  int64_t vreal;   // reals per vector (all wavepackets)
+};
+
+size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Workspace for ncols wavepackets and Krylov dimension m (m = 0: a matvec only).
+int kwork(rm_kplan* p, int32_t ncols, int32_t m, bool f32, KWork* w) {
+  const int64_t vreal = 2 * p->n * ncols;
+  int64_t zreal = 0;
+  for (int a = 0; a < p->nJ; ++a) zreal += (int64_t)ncols * jdim(p->Js[a]) * 2 * p->ktot[a];
+  const size_t es = f32 ? sizeof(float) : sizeof(double);
+  const size_t mm = (size_t)std::max(m, 1);
+  const size_t b_vec = al256(sizeof(double) * (size_t)vreal);
+  const size_t b_y = al256(es * (size_t)vreal), b_z = al256(es * (size_t)std::max<int64_t>(zreal, 1));
+  const size_t b_col = al256(sizeof(double) * (size_t)ncols);
+  const size_t b_part = al256(sizeof(double) * 2 * (size_t)ncols * p->nJ);
+  const size_t b_beta = al256(sizeof(double) * (size_t)ncols * (mm + 1));
+  const size_t b_al = al256(sizeof(double2) * (size_t)ncols * mm);
+  const size_t need = b_vec * (2 + mm) + b_y + b_z + b_col + b_part + b_beta + 2 * b_al;
+  if (p->ws_bytes < need) {
+    if (p->ws) {
+      HIPCHK(hipDeviceSynchronize());
+      HIPCHK(hipFree(p->ws));
+      p->ws = nullptr;
+      p->ws_bytes = 0;
+    }
+    HIPCHK(hipMalloc(&p->ws, need));
+    p->ws_bytes = need;
+  }
+  char* c = (char*)p->ws;
+  w->S = (double*)c; c += b_vec;
+  w->W = (double*)c; c += b_vec;
+  w->V = (double*)c; c += b_vec * mm;
+  w->Y = c; c += b_y;
+  w->Z = c; c += b_z;
+  w->ib = (double*)c; c += b_col;
+  w->part = (double*)c; c += b_part;
+  w->beta = (double*)c; c += b_beta;
+  w->alpha = (double2*)c; c += b_al;
+  w->coef = (double2*)c; c += b_al;
+  w->vreal = vreal;
+  return RM_OK;
+}
+
+// The (output J1, source block) tasks for ncols wavepackets, uploaded once per call.
+int ktasks(rm_kplan* p, int32_t ncols, int64_t* ntask, std::vector<int64_t>& zoff_J, hipStream_t s) {
+  std::vector<KTask> h;
+  zoff_J.assign(p->nJ, 0);
+  int64_t zpos = 0;
+  for (int a = 0; a < p->nJ; ++a) {
+    zoff_J[a] = zpos;
+    if (p->src[a].empty()) continue;
+    const int64_t dm1 = jdim(p->Js[a]);
+    for (int q : p->src[a]) {
+      KTask t;
+      t.J1 = p->h_J1[q];
+      t.J2 = p->h_J2[q];
+      t.dm1 = (int32_t)dm1;
+      t.dm2 = (int32_t)jdim(t.J2);
+      t.dk2 = t.dm2;
+      t.koff = (int32_t)p->kq_off[q];
+      t.ktot = p->ktot[a];
+      t.nsig = (int32_t)(p->h_sig_off[q + 1] - p->h_sig_off[q]);
+      t.sig0 = (int32_t)p->h_sig_off[q];
+      t.blk = q;
+      t.mf_off = p->h_blk[q].mf_off;
+      t.zoff = zpos;
+      const int jb = (int)(std::lower_bound(p->Js.begin(), p->Js.end(), t.J2) - p->Js.begin());
+      t.xoff = p->h_voff[jb] * 2 * ncols;
+      h.push_back(t);
+    }
+    zpos += (int64_t)ncols * dm1 * 2 * p->ktot[a];
+  }
+  *ntask = (int64_t)h.size();
+  int rc = grow(&p->d_tasks, &p->tasks_cap, std::max<int64_t>(*ntask, 1));
+  if (rc) return rc;
+  if (*ntask) HIPCHK(hipMemcpyAsync(p->d_tasks, h.data(), sizeof(KTask) * h.size(), hipMemcpyHostToDevice, s));
+  return RM_OK;
+}
+
+__global__ void k_kp_zero(double* __restrict__ x, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    x[i] = 0.0;
+}
+
+// Y = H(F) (ib X) for the split-layout input X; the MF values of the step are in p->d_mf.
+int kmatvec(rm_kplan* p, const double* X, const double* ib, void* Y, void* Z, int32_t ncols, int64_t ntask,
+            const std::vector<int64_t>& zoff_J, bool f32, hipStream_t s) {
+  if (ntask) {
+    const int64_t maxrows = (int64_t)ncols * jdim(p->Js[p->nJ - 1]);
+    const dim3 grid((unsigned)std::min<int64_t>((maxrows + 3) / 4, 4096), (unsigned)ntask);
+    if (f32)
+      hipLaunchKernelGGL(k_kp_z<float>, grid, dim3(256), 0, s, p->d_tasks, p->d_sig, p->d_mf, X, ib, (float*)Z,
+                         ncols);
+    else
+      hipLaunchKernelGGL(k_kp_z<double>, grid, dim3(256), 0, s, p->d_tasks, p->d_sig, p->d_mf, X, ib, (double*)Z,
+                         ncols);
+    HIPCHK(hipGetLastError());
+  }
+  RBCHK(rocblas_set_stream(p->rb, s));
+  for (int a = 0; a < p->nJ; ++a) {
+    const int64_t d = jdim(p->Js[a]);
+    const int64_t yoff = p->h_voff[a] * 2 * ncols;
+    if (p->src[a].empty()) {
+      // no source block reaches this J: its output rows are zero
+      if (f32) {
+        const int64_t cnt = (int64_t)ncols * d * d;   // floats as pairs: 2 * cnt floats = cnt doubles
+        hipLaunchKernelGGL(k_kp_zero, dim3(grid_for(cnt, 256)), dim3(256), 0, s, (double*)((float*)Y + yoff), cnt);
+      } else {
+        const int64_t cnt = 2 * (int64_t)ncols * d * d;
+        hipLaunchKernelGGL(k_kp_zero, dim3(grid_for(cnt, 256)), dim3(256), 0, s, (double*)Y + yoff, cnt);
+      }
+      continue;
+    }
+    const rocblas_int M = (rocblas_int)(ncols * d), N = (rocblas_int)(2 * d), K = (rocblas_int)(2 * p->ktot[a]);
+    if (f32) {
+      const float one = 1.0f, zero = 0.0f;
+      RBCHK(rocblas_sgemm(p->rb, rocblas_operation_none, rocblas_operation_none, N, M, K, &one, p->kbig32[a], N,
+                          (const float*)Z + zoff_J[a], K, &zero, (float*)Y + yoff, N));
+    } else {
+      const double one = 1.0, zero = 0.0;
+      RBCHK(rocblas_dgemm(p->rb, rocblas_operation_none, rocblas_operation_none, N, M, K, &one, p->kbig64[a], N,
+                          (const double*)Z + zoff_J[a], K, &zero, (double*)Y + yoff, N));
+    }
+  }
+  HIPCHK(hipGetLastError());
+  return RM_OK;
+}
+
+// Launch helpers of the chunk kernels: grid (nJ, ncols), 256 threads.
+dim3 chunk_grid(const rm_kplan* p, int32_t ncols) { return dim3((unsigned)p->nJ, (unsigned)ncols); }
+
+}  // namespace
+
+// y = H(F) x for ncols wavepackets given as rows (stride ld) of complex
+// values in the plan's internal order; fprod: device, nslot complex.
+extern "C" int rm_kplan_matvec(rm_kplan* p, const void* x, void* y, int64_t ld, int32_t ncols,
+                               const double* fprod_dev, int32_t precision, void* stream) {
+  if (!p) return fail(RM_ERR_ARG, "rm_kplan_matvec: NULL plan");
+  if (ncols < 1 || ncols > 65535 || ld < p->n || !x || !y || (p->nslot > 0 && !fprod_dev))
+    return fail(RM_ERR_ARG, "rm_kplan_matvec: bad arguments");
+  if (precision != RM_PREC_FP64 && precision != RM_PREC_FP32)
+    return fail(RM_ERR_ARG, "rm_kplan_matvec: unknown precision %d", precision);
+  hipStream_t s = (hipStream_t)stream;
+  const bool f32 = precision == RM_PREC_FP32;
+  KWork w;
+  int rc = kwork(p, ncols, 0, f32, &w);
+  if (rc) return rc;
+  int64_t ntask = 0;
+  std::vector<int64_t> zoff_J;
+  if ((rc = ktasks(p, ncols, &ntask, zoff_J, s))) return rc;
+  if (p->nblk)
+    hipLaunchKernelGGL(k_kp_mf, dim3(8, (unsigned)p->nblk), dim3(256), 0, s, p->d_blk, p->d_mdiag,
+                       (const double2*)fprod_dev, p->d_mf);
+  hipLaunchKernelGGL(k_kp_pack, chunk_grid(p, ncols), dim3(256), 0, s, p->d_voff, p->d_Js, (const double2*)x, ld,
+                     w.S, 1);
+  hipLaunchKernelGGL(k_kp_start, chunk_grid(p, ncols), dim3(256), 0, s, p->d_voff, p->d_Js,
+                     (const double2*)nullptr, w.S, w.W, w.ib, w.beta, 0);
+  if ((rc = kmatvec(p, w.W, w.ib, w.Y, w.Z, ncols, ntask, zoff_J, f32, s))) return rc;
+  if (f32)
+    hipLaunchKernelGGL(k_kp_out<float>, chunk_grid(p, ncols), dim3(256), 0, s, p->d_voff, p->d_Js,
+                       (const float*)w.Y, (double2*)y, ld);
+  else
+    hipLaunchKernelGGL(k_kp_out<double>, chunk_grid(p, ncols), dim3(256), 0, s, p->d_voff, p->d_Js,
+                       (const double*)w.Y, (double2*)y, ld);
+  HIPCHK(hipGetLastError());
+  return RM_OK;
+}
+
+// nsteps split-operator Lanczos steps (tdse.py:365-414, 417-486) with a fixed
+// Krylov dimension for ncols wavepackets; see the header for the arguments.
+extern "C" int rm_kplan_propagate(rm_kplan* p, void* psi, int64_t ld, int32_t ncols, int64_t nsteps,
+                                  const double* fprod, double f_re, double f_im, const void* phase,
+                                  int32_t krylov_fixed, int32_t precision, void* stream) {
+  if (!p) return fail(RM_ERR_ARG, "rm_kplan_propagate: NULL plan");
+  if (ncols < 1 || ncols > 65535 || ld < p->n || !psi || nsteps < 0 || (nsteps > 0 && p->nslot > 0 && !fprod))
+    return fail(RM_ERR_ARG, "rm_kplan_propagate: bad arguments");
+  if (krylov_fixed < 1 || krylov_fixed > KMAX)
+    return fail(RM_ERR_ARG, "rm_kplan_propagate: krylov_fixed must be in [1, %d]", KMAX);
+  if (precision != RM_PREC_FP64 && precision != RM_PREC_FP32)
+    return fail(RM_ERR_ARG, "rm_kplan_propagate: unknown precision %d", precision);
+  if (nsteps == 0) return RM_OK;
+  hipStream_t s = (hipStream_t)stream;
+  const bool f32 = precision == RM_PREC_FP32;
+  const int32_t m = krylov_fixed;
+  KWork w;
+  int rc = kwork(p, ncols, m, f32, &w);
+  if (rc) return rc;
+  int64_t ntask = 0;
+  std::vector<int64_t> zoff_J;
+  if ((rc = ktasks(p, ncols, &ntask, zoff_J, s))) return rc;
+  // the field products of every step, uploaded once
+  const int64_t nf = std::max<int32_t>(p->nslot, 1);
+  if ((rc = grow(&p->d_fp, &p->fp_cap, nsteps * nf))) return rc;
+  if (p->nslot)
+    HIPCHK(hipMemcpyAsync(p->d_fp, fprod, sizeof(double2) * nsteps * p->nslot, hipMemcpyHostToDevice, s));
+  const double2* ph = (const double2*)phase;
+  const double2 f = make_double2(f_re, f_im);
+  const dim3 cg = chunk_grid(p, ncols), cb(256);
+  const dim3 fin_grid((unsigned)((ncols + 3) / 4)), fin_block(256);
+  const int64_t vs = w.vreal;
+  hipLaunchKernelGGL(k_kp_pack, cg, cb, 0, s, p->d_voff, p->d_Js, (const double2*)psi, ld, w.S, 1);
+  for (int64_t st = 0; st < nsteps; ++st) {
+    bool field = false;
+    for (int32_t c = 0; c < p->nslot && !field; ++c)
+      field = fprod[2 * (st * p->nslot + c)] != 0.0 || fprod[2 * (st * p->nslot + c) + 1] != 0.0;
+    if (!field) {
+      if (ph) hipLaunchKernelGGL(k_kp_phase2, cg, cb, 0, s, p->d_voff, p->d_Js, ph, w.S);
+      continue;
+    }
+    if (p->nblk)
+      hipLaunchKernelGGL(k_kp_mf, dim3(8, (unsigned)p->nblk), dim3(256), 0, s, p->d_blk, p->d_mdiag,
+                         p->d_fp + st * p->nslot, p->d_mf);
+    hipLaunchKernelGGL(k_kp_start, cg, cb, 0, s, p->d_voff, p->d_Js, ph, w.S, w.W, w.ib, w.beta, m);
+    for (int32_t it = 0; it < m; ++it) {
+      double* Vit = w.V + it * vs;
+      const double* Vprev = it > 0 ? w.V + (it - 1) * vs : Vit;
+      if ((rc = kmatvec(p, w.W, w.ib, w.Y, w.Z, ncols, ntask, zoff_J, f32, s))) return rc;
+      if (f32)
+        hipLaunchKernelGGL(k_kp_dot<float>, cg, cb, 0, s, p->d_voff, p->d_Js, w.W, w.ib, (const float*)w.Y, Vit,
+                           w.part, p->nJ);
+      else
+        hipLaunchKernelGGL(k_kp_dot<double>, cg, cb, 0, s, p->d_voff, p->d_Js, w.W, w.ib, (const double*)w.Y, Vit,
+                           w.part, p->nJ);
+      hipLaunchKernelGGL(k_kp_fin, fin_grid, fin_block, 0, s, w.part, p->nJ, ncols, 0, it, m, w.alpha, w.beta, w.ib);
+      if (it + 1 == m) break;
+      if (f32)
+        hipLaunchKernelGGL(k_kp_upd<float>, cg, cb, 0, s, p->d_voff, p->d_Js, (const float*)w.Y, Vit, Vprev,
+                           w.alpha, w.beta, it, m, w.W, w.part, p->nJ);
+      else
+        hipLaunchKernelGGL(k_kp_upd<double>, cg, cb, 0, s, p->d_voff, p->d_Js, (const double*)w.Y, Vit, Vprev,
+                           w.alpha, w.beta, it, m, w.W, w.part, p->nJ);
+      hipLaunchKernelGGL(k_kp_fin, fin_grid, fin_block, 0, s, w.part, p->nJ, ncols, 1, it, m, w.alpha, w.beta, w.ib);
+    }
+    hipLaunchKernelGGL(k_kp_expm, dim3((unsigned)ncols), dim3(64), 0, s, w.alpha, w.beta, m, m, f, w.coef);
+    hipLaunchKernelGGL(k_kp_comb, cg, cb, 0, s, p->d_voff, p->d_Js, ph, w.V, vs, w.coef, m, m, w.S);
+    HIPCHK(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_kp_pack, cg, cb, 0, s, p->d_voff, p->d_Js, (const double2*)psi, ld, w.S, 0);
+  HIPCHK(hipGetLastError());
+  return RM_OK;
+}

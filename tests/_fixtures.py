@@ -98,3 +98,42 @@ def gaussian_pulse_field(t_fs, fwhm_fs=300.0, intensity=5e11):
     e0 = units.intensity_to_field(intensity)
     t0 = 2.5 * fwhm_fs / 2
     return e0 * np.exp(-2 * np.log(2) * (t_fs - t0) ** 2 / fwhm_fs ** 2) / np.sqrt(2.0)
+
+
+# Large asymmetric top (the Kronecker-block configuration): H2S-like rotor,
+# A, B, C = 10.36, 9.02, 4.73 cm^-1, frame x = c, y = a, z = b (the C2 axis),
+# dipole 0.3816 au (0.97 D) along z, and a full polarizability tensor (au;
+# the off-diagonal elements make every Cartesian component act).  The field
+# is circularly polarised in the XY plane with a linearly chirped frequency,
+# 0 -> 10 THz over 100 ps: E(t) = E0 (cos phi, sin phi, 0), phi = pi r t^2,
+# r = 1e23 Hz/s, E0 = 1e8 V/m, acting on both the dipole and the
+# polarizability (H = -mu.E - 1/2 E.alpha.E).
+LARGE_B = (4.73, 10.36, 9.02)           # (Bx, By, Bz)
+LARGE_MU = (0.0, 0.0, 0.3816)
+LARGE_ALPHA = ((24.0, 0.6, -0.4), (0.6, 25.5, 0.3), (-0.4, 0.3, 24.7))
+LARGE_E0 = 1e8
+LARGE_CHIRP = 1e23
+
+
+def large_top_tensors():
+    """The two interaction tensors and their energy scales (cm^-1 per field
+    product): -mu in au x V/m, -1/2 alpha in au x (V/m)^2."""
+    return ([np.array(LARGE_MU), np.array(LARGE_ALPHA)],
+            [-units.AUdip_x_Vm_to_invcm(), -0.5 * units.AUpol_x_Vm_to_invcm()])
+
+
+def large_top(jmax):
+    """(KronOperator, basis) of the large asymmetric top with J <= jmax, all m."""
+    from richmol_amd.kron import KronOperator
+    bas = rot.asymtop_basis(*LARGE_B, jmax)
+    tens, scales = large_top_tensors()
+    return KronOperator(bas, tens, scales), bas
+
+
+def centrifuge_fields(t_fs, e0=LARGE_E0, rate=LARGE_CHIRP):
+    """Per-tensor (nsteps, 3) field arrays of the chirped circular field at
+    times t_fs (the same field on the dipole and the polarizability)."""
+    t = np.asarray(t_fs, dtype=np.float64) * 1e-15
+    phi = np.pi * rate * t ** 2
+    E = np.stack([e0 * np.cos(phi), e0 * np.sin(phi), np.zeros_like(t)], axis=1)
+    return [E, E]

@@ -1,0 +1,188 @@
+"""Kronecker-block operator on the device (rm_kplan: per output J one dense
+product of the stacked K blocks with the m-mixed input of every wavepacket,
+rocBLAS) and its fixed-order Lanczos propagator, against the host
+factorisation and the FP64 oracle (tdse.py:365-486 restated) on the large
+asymmetric top (dipole + full polarizability, chirped circular XY field) at
+small J.  Tolerances: FP64 1e-10 absolute in every amplitude; FP32 products
+2e-6 in amplitudes and 1e-6 in <cos^2 theta>."""
+import numpy as np
+import pytest
+
+from _fixtures import centrifuge_fields, large_top, large_top_tensors
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev_plan(kop):
+    from richmol_amd.kron import DeviceKronPlan
+    return DeviceKronPlan(kop)
+
+
+def _to_int(kop, Xf):
+    Xi = np.zeros_like(Xf)
+    Xi[:, kop.perm] = Xf
+    return Xi
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("fp32", 2e-6)])
+def test_matvec_vs_host_factorisation(device, precision, tol):
+    """J <= 7 (680 states), 5 wavepackets, a general field on both tensors:
+    the device matvec equals KronOperator.matvec (relative to the largest
+    output)."""
+    import torch
+    from richmol_amd.kron import slot_products
+    kop, bas = large_top(7)
+    dp = _dev_plan(kop)
+    rng = np.random.default_rng(11)
+    X = rng.normal(size=(5, kop.n)) + 1j * rng.normal(size=(5, kop.n))
+    F = [np.array([[0.3e8, -0.8e8, 0.5e8]]), np.array([[0.6e8, 0.2e8, -0.9e8]])]
+    fp = slot_products(kop, F)
+    ref = kop.matvec(X, kop.field_products([F[0][0], F[1][0]]))
+    x = torch.from_numpy(X).to(device)
+    y = torch.zeros_like(x)
+    dp.matvec(x, y, torch.from_numpy(fp[0]).to(device), precision=precision)
+    got = y.cpu().numpy()
+    assert np.abs(got - ref).max() <= tol * np.abs(ref).max()
+
+
+def test_matvec_strided_rows(device):
+    """Rows with a leading dimension larger than n; the padding is untouched."""
+    import torch
+    from richmol_amd.kron import slot_products
+    kop, bas = large_top(3)
+    dp = _dev_plan(kop)
+    rng = np.random.default_rng(12)
+    n = kop.n
+    buf = torch.from_numpy(rng.normal(size=(2, n + 7)) + 1j * rng.normal(size=(2, n + 7))).to(device)
+    out = torch.full((2, n + 7), 3.0 + 4.0j, dtype=torch.complex128, device=device)
+    F = [np.array([[1e8, 2e7, 0.0]])] * 2
+    fp = slot_products(kop, F)
+    dp.matvec(buf[:, :n], out[:, :n], torch.from_numpy(fp[0]).to(device))
+    ref = kop.matvec(buf[:, :n].cpu().numpy(), kop.field_products([F[0][0], F[1][0]]))
+    assert np.abs(out[:, :n].cpu().numpy() - ref).max() <= 1e-13 * np.abs(ref).max()
+    assert torch.all(out[:, n:] == 3.0 + 4.0j)
+
+
+def _oracle_run(bas, fields, Xf, fixed, dt):
+    from oracle.cartens import OracleCarTens
+    from oracle.tdse import OracleTDSE
+    from richmol_amd import rot
+    tens, scales = large_top_tensors()
+    OL = [OracleCarTens.from_attrs(rot.tensor(t, bas) * s) for t, s in zip(tens, scales)]
+    Oh0 = OracleCarTens.from_attrs(rot.hamiltonian(bas))
+    nst = len(fields[0])
+    to = OracleTDSE(t_start=0.0, t_end=dt * nst, dt=dt, t_units="fs")
+    to.time_grid()
+    psi = Xf.copy()
+    for i in range(nst):
+        for O, F in zip(OL, fields):
+            O.field(list(F[i]))
+        psi, _ = to.update(OL[0] + OL[1], psi, H0=Oh0, fixed_m=fixed)
+    return psi, to.exp_fac
+
+
+def _device_run(kop, fields, Xf, fixed, f, precision="fp64"):
+    import torch
+    from richmol_amd.kron import slot_products
+    dp = _dev_plan(kop)
+    psi = torch.from_numpy(_to_int(kop, Xf)).cuda()
+    ph = dp.phases(f)
+    dp.propagate(psi, slot_products(kop, fields), f, phase=ph, krylov_fixed=fixed, precision=precision)
+    return psi.cpu().numpy()[:, kop.perm]
+
+
+def _columns(n, seed):
+    rng = np.random.default_rng(seed)
+    X = np.zeros((4, n), dtype=np.complex128)
+    X[0, 0] = 1.0                                  # ground state
+    X[1, 5] = 1.0                                  # an excited basis state
+    X[2] = rng.normal(size=n) + 1j * rng.normal(size=n)
+    X[2] /= np.linalg.norm(X[2])
+    X[3] = 0.4 * (rng.normal(size=n) + 1j * rng.normal(size=n)) / np.sqrt(n)   # norm != 1
+    return X
+
+
+@pytest.mark.parametrize("fixed", [12, 5])
+def test_propagate_matches_oracle(device, fixed):
+    """J <= 5 (286 states), four wavepackets (basis states, dense, unnormalised),
+    eight 10 fs steps of the chirped circular field starting late in the
+    sweep (several THz), plus two zero-field steps (phases only): FP64 within
+    1e-10 of the oracle."""
+    kop, bas = large_top(5)
+    dt = 10.0
+    t = 40000.0 + (np.arange(8) + 0.5) * dt
+    fields = centrifuge_fields(t, e0=3e9)
+    for F in fields:
+        F[3] = 0.0
+        F[6] = 0.0
+    X = _columns(kop.n, 3)
+    ref, f = _oracle_run(bas, fields, X, fixed, dt)
+    got = _device_run(kop, fields, X, fixed, f)
+    assert np.abs(got - ref).max() <= 1e-10
+    if fixed == 12:   # converged expv: the norms are kept (unitary step)
+        assert np.abs(np.linalg.norm(got, axis=1) - np.linalg.norm(X, axis=1)).max() <= 1e-12
+
+
+def test_propagate_fp32_products(device):
+    """FP32 dense products (complex64 operands and output; recurrence, norms,
+    exp(fT), phases and wavepackets FP64): amplitudes within 2e-6 and
+    <cos^2 theta> within 1e-6 of the FP64 oracle over 12 steps at J <= 6."""
+    from oracle.cartens import OracleCarTens
+    from richmol_amd import rot
+    kop, bas = large_top(6)
+    dt = 10.0
+    t = 20000.0 + (np.arange(12) + 0.5) * dt
+    fields = centrifuge_fields(t, e0=3e9)
+    X = _columns(kop.n, 5)[:3]
+    ref, f = _oracle_run(bas, fields, X, 12, dt)
+    got = _device_run(kop, fields, X, 12, f, precision="fp32")
+    C = OracleCarTens.from_attrs(rot.named("cos2theta", bas)).tomat(form="full", cart="0")
+    c2 = lambda P: np.real(np.einsum("bi,bi->b", P.conj(), (C @ P.T).T))
+    assert np.abs(got - ref).max() <= 2e-6
+    assert np.abs(c2(got) - c2(ref)).max() <= 1e-6
+
+
+def test_propagate_exact_breakdown(device):
+    """Isotropic polarizability only: the operator is a multiple of the
+    identity, so a basis-state column breaks down exactly after the first
+    Krylov vector (the reference restarts, tdse.py:459-465; here a zero vector
+    with zero coefficient); a dense column beside it is unaffected."""
+    import torch
+    from oracle.cartens import OracleCarTens
+    from oracle.tdse import OracleTDSE
+    from richmol_amd import rot, units
+    from richmol_amd.kron import DeviceKronPlan, KronOperator, slot_products
+    from _fixtures import LARGE_B
+    bas = rot.asymtop_basis(*LARGE_B, 3)
+    a = np.diag([20.0, 20.0, 20.0])
+    s = -0.5 * units.AUpol_x_Vm_to_invcm()
+    kop = KronOperator(bas, [a], [s])
+    dt = 10.0
+    E = np.tile([[2e9, 1e9, 0.5e9]], (3, 1))
+    X = _columns(kop.n, 7)[:3]
+    O = OracleCarTens.from_attrs(rot.tensor(a, bas) * s)
+    Oh0 = OracleCarTens.from_attrs(rot.hamiltonian(bas))
+    to = OracleTDSE(t_start=0.0, t_end=dt * 3, dt=dt, t_units="fs")
+    to.time_grid()
+    ref = X.copy()
+    for i in range(3):
+        O.field(list(E[i]))
+        ref, _ = to.update(O, ref, H0=Oh0, fixed_m=6)
+    dp = DeviceKronPlan(kop)
+    psi = torch.from_numpy(_to_int(kop, X)).cuda()
+    dp.propagate(psi, slot_products(kop, [E]), to.exp_fac, phase=dp.phases(to.exp_fac), krylov_fixed=6)
+    got = psi.cpu().numpy()[:, kop.perm]
+    assert np.abs(got - ref).max() <= 1e-10
+
+
+def test_propagate_rejects_bad_arguments(device):
+    import torch
+    from richmol_amd import _lib
+    kop, bas = large_top(2)
+    dp = _dev_plan(kop)
+    psi = torch.zeros((1, kop.n), dtype=torch.complex128, device=device)
+    fp = np.zeros((1, dp.nslot), dtype=np.complex128)
+    with pytest.raises(_lib.RichmolAmdError):
+        dp.propagate(psi, fp, 1j, krylov_fixed=0)
+    with pytest.raises(_lib.RichmolAmdError):
+        dp.propagate(psi, fp, 1j, krylov_fixed=_lib.RM_MAX_KRYLOV + 1)
