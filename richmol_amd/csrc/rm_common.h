@@ -260,6 +260,19 @@ __device__ __forceinline__ double wave_allsum32(double v) {
   return readlane_d(v, 0) + readlane_d(v, 16);
 }
 
+// Sums over each half of the wave (lanes 0..31 and 32..63) for two problems
+// held side by side: the operations of wave_allsum32 on each half, the lane's
+// own half's sum returned.
+__device__ __forceinline__ double half_allsum32(double v) {
+  v += dpp_mov<0x128>(v);
+  v += dpp_mov<0x124>(v);
+  v += dpp_mov<0x122>(v);
+  v += dpp_mov<0x121>(v);
+  const double s0 = readlane_d(v, 0) + readlane_d(v, 16);
+  const double s1 = readlane_d(v, 32) + readlane_d(v, 48);
+  return (threadIdx.x & 32) ? s1 : s0;
+}
+
 __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
@@ -371,6 +384,11 @@ __device__ __forceinline__ double row_max16(double v) {
 // row i and its neighbours arrive by DPP row shifts (no LDS traffic); the
 // Taylor sum of each factor stops at the first term below 2^-60 of the
 // bound (h |f T|_1)^j / j!, at most TAYLOR_TERMS terms.  Lanes 0..s-1 write c.
+// PAIR: the two halves of the wave (lanes 0..31, 32..63) hold two independent
+// problems; each lane passes its own half's alpha, beta and c, and the result
+// is written by lanes (l & 31) < s of each half.  Every row computes the same
+// values as in the single form (the row operations never cross rows).
+template <bool PAIR = false>
 static __device__ void expm_e0_16(int s, const double2* alpha, const double* beta, double2 f, double2* c) {
   const int l = threadIdx.x & 63;
   const int i = l & 15;
@@ -414,7 +432,11 @@ static __device__ void expm_e0_16(int s, const double2* alpha, const double* bet
     }
     y = acc;
   }
-  if (l < s) c[l] = y;
+  if (PAIR) {
+    if ((l & 31) < s) c[l & 31] = y;
+  } else if (l < s) {
+    c[l] = y;
+  }
 }
 
 // Adaptive stopping metric after iteration k (one wave; tdse.py:473-476).
@@ -430,7 +452,7 @@ static __device__ void expm_e0_16(int s, const double2* alpha, const double* bet
 __device__ __forceinline__ void expm_e0_any(int s, const double2* alpha, const double* beta, double2 f,
                                             double2* c) {
   if (s <= 16)
-    expm_e0_16(s, alpha, beta, f, c);
+    expm_e0_16<false>(s, alpha, beta, f, c);
   else
     expm_e0(s, alpha, beta, f, c);
 }
@@ -460,7 +482,7 @@ static __device__ double adaptive_metric(int k, bool breakdown, double2* G, int 
   if (tid == 0) G[k * kc + k] = make_double2(1.0, 0.0);
   __threadfence_block();
   if (FAST && k + 1 <= 16)
-    expm_e0_16(k + 1, s_alpha, s_beta, f, s_c);
+    expm_e0_16<false>(k + 1, s_alpha, s_beta, f, s_c);
   else
     expm_e0(k + 1, s_alpha, s_beta, f, s_c);
   __threadfence_block();

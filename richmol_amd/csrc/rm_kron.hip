@@ -74,6 +74,17 @@ struct rm_kplan {
   size_t ws_bytes = 0;
   KTask* d_tasks = nullptr;
   int64_t tasks_cap = 0;
+  int64_t* d_meta = nullptr;            // per call: row offsets and task ranges per J
+  int64_t meta_cap = 0;
+  int64_t ztiles = 0;
+  int32_t zr = 16;                      // Z rows per tile of the Z-build kernel
+  // the dense products run on their own stream, chunk c after the Z build of chunk c
+  hipStream_t gs = nullptr;
+  std::vector<hipEvent_t> ev;           // KCHUNK Z-done events + one products-done event
+  std::vector<int32_t> chunk_j;         // per call: J index bounds of the chunks (KCHUNK + 1)
+  size_t z_lds = 0;                     // its LDS bytes
+  std::vector<KTask> h_tasks;
+  std::vector<int64_t> h_meta;          // host copy (its source must outlive the async upload)
   double2* d_fp = nullptr;              // field products of a call's steps
   int64_t fp_cap = 0;
 };
@@ -81,6 +92,10 @@ struct rm_kplan {
 namespace {
 
 __host__ __device__ inline int64_t jdim(int32_t J) { return 2 * (int64_t)J + 1; }
+
+// Chunks of output J per matvec: the Z build of chunk c + 1 overlaps the dense
+// products of chunk c (memory-bound next to MFMA-bound work).
+constexpr int KCHUNK = 5;
 
 // MF[blk][t][m1] = sum_c fprod[slot0 + c] mdiag[blk][t][c][m1]
 __global__ void k_kp_mf(const KBlk* __restrict__ blk, const double2* __restrict__ mdiag,
@@ -96,34 +111,76 @@ __global__ void k_kp_mf(const KBlk* __restrict__ blk, const double2* __restrict_
   }
 }
 
-// Z rows (b, m1) of one task: sum over the block's sigmas of MF[t][m1] X[J2](b, m2 = m1 - J1 - s + J2) * ib[b]
+// Z rows of one tile (output J1, wavepacket b, rows m1 in [m0, m0 + nr)) for
+// every source block q of J1:
+//   Z[(b, m1)][koff_q + k2] = ib[b] sum_t MF_q[t][m1] X[J2_q](b, m2_t)[k2]
+// (re and im halves), m2_t = m1 - J1 - sigma_t + J2_q.  Per block, the X
+// rows the tile needs (nr + 4 at most: |sigma| <= 2) are one contiguous range
+// of the input, staged in LDS with coalesced loads; the tile's MF values
+// beside them.  Tiles are found from the per-J tile offsets meta[0..nJ] and
+// task ranges meta[nJ+1..2nJ+1]; zr = rows per tile.
 template <typename ZT>
-__global__ __launch_bounds__(256) void k_kp_z(const KTask* __restrict__ tasks, const int32_t* __restrict__ sig,
+__global__ __launch_bounds__(256) void k_kp_z(const KTask* __restrict__ tasks, const int64_t* __restrict__ meta,
+                                              int32_t nJ, const int32_t* __restrict__ sig,
                                               const double2* __restrict__ mf, const double* __restrict__ X,
-                                              const double* __restrict__ ib, ZT* __restrict__ Z, int32_t ncols) {
-  const KTask tk = tasks[blockIdx.y];
-  const int lane = threadIdx.x & 63, rsub = threadIdx.x >> 6;   // 4 rows per block pass
-  const int64_t rows = (int64_t)ncols * tk.dm1;
-  const int64_t zld = 2 * (int64_t)tk.ktot, xld = 2 * (int64_t)tk.dk2;
-  for (int64_t r = (int64_t)blockIdx.x * 4 + rsub; r < rows; r += (int64_t)gridDim.x * 4) {
-    const int64_t b = r / tk.dm1;
-    const int m1 = (int)(r % tk.dm1);
-    const double s = ib[b];
-    for (int k2 = lane; k2 < tk.dk2; k2 += 64) {
-      double zr = 0.0, zi = 0.0;
-      for (int t = 0; t < tk.nsig; ++t) {
-        const int m2 = m1 - tk.J1 - sig[tk.sig0 + t] + tk.J2;
-        if (m2 < 0 || m2 >= tk.dm2) continue;
-        const double2 f = mf[tk.mf_off + (int64_t)t * tk.dm1 + m1];
-        const double* xrow = X + tk.xoff + (b * tk.dm2 + m2) * xld;
-        const double xr = xrow[k2], xi = xrow[tk.dk2 + k2];
-        zr += f.x * xr - f.y * xi;
-        zi += f.x * xi + f.y * xr;
-      }
-      ZT* zrow = Z + tk.zoff + r * zld;
-      zrow[tk.koff + k2] = (ZT)(zr * s);
-      zrow[tk.ktot + tk.koff + k2] = (ZT)(zi * s);
+                                              const double* __restrict__ ib, ZT* __restrict__ Z, int32_t zr,
+                                              int64_t tile0) {
+  extern __shared__ double smem[];
+  const int64_t tile = tile0 + blockIdx.x;
+  const int64_t* tile_off = meta;
+  const int64_t* task_off = meta + nJ + 1;
+  int lo = 0, hi = nJ;   // tile_off[lo] <= tile < tile_off[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (tile_off[mid] <= tile) lo = mid; else hi = mid;
+  }
+  const int64_t q0 = task_off[lo], q1 = task_off[lo + 1];
+  const int dm1 = tasks[q0].dm1;
+  const int nmt = (dm1 + zr - 1) / zr;
+  const int64_t tt = tile - tile_off[lo];
+  const int64_t b = tt / nmt;
+  const int m0 = (int)(tt % nmt) * zr;
+  const int nr = min(zr, dm1 - m0);
+  const double s = ib[b];
+  double2* s_mf = (double2*)smem;      // [5][zr]
+  double* s_x = smem + 10 * zr;        // [nr + 4][2 dk2]
+  for (int64_t q = q0; q < q1; ++q) {
+    const KTask tk = tasks[q];
+    int sg[5];
+#pragma unroll
+    for (int t = 0; t < 5; ++t) sg[t] = t < tk.nsig ? sig[tk.sig0 + t] : 0;
+    const int64_t xld = 2 * (int64_t)tk.dk2;
+    const int lo2 = max(0, m0 - tk.J1 + tk.J2 - 2);
+    const int hi2 = min(tk.dm2 - 1, m0 + nr - 1 - tk.J1 + tk.J2 + 2);
+    const int64_t nx = hi2 >= lo2 ? (int64_t)(hi2 - lo2 + 1) * xld : 0;
+    const double* src = X + tk.xoff + (b * tk.dm2 + lo2) * xld;
+    for (int64_t i = threadIdx.x; i < nx; i += blockDim.x) s_x[i] = src[i];
+    for (int i = threadIdx.x; i < tk.nsig * nr; i += blockDim.x) {
+      const int t = i / nr, r = i % nr;
+      s_mf[t * zr + r] = mf[tk.mf_off + (int64_t)t * tk.dm1 + m0 + r];
     }
+    __syncthreads();
+    ZT* zt = Z + tk.zoff + (b * tk.dm1 + m0) * 2 * (int64_t)tk.ktot;
+    for (int e = threadIdx.x; e < nr * tk.dk2; e += blockDim.x) {
+      const int r = e / tk.dk2, k2 = e % tk.dk2;
+      const int m1 = m0 + r;
+      double zre = 0.0, zim = 0.0;
+#pragma unroll
+      for (int t = 0; t < 5; ++t) {
+        if (t >= tk.nsig) break;
+        const int m2 = m1 - tk.J1 - sg[t] + tk.J2;
+        if (m2 < 0 || m2 >= tk.dm2) continue;
+        const double2 f = s_mf[t * zr + r];
+        const double* xr = s_x + (m2 - lo2) * xld;
+        const double a = xr[k2], c = xr[tk.dk2 + k2];
+        zre += f.x * a - f.y * c;
+        zim += f.x * c + f.y * a;
+      }
+      ZT* zrow = zt + (int64_t)r * 2 * tk.ktot;
+      zrow[tk.koff + k2] = (ZT)(zre * s);
+      zrow[tk.ktot + tk.koff + k2] = (ZT)(zim * s);
+    }
+    __syncthreads();
   }
 }
 
@@ -351,7 +408,7 @@ __global__ void k_kp_out(const int64_t* __restrict__ voff, const int32_t* __rest
 
 void kplan_free(rm_kplan* p) {
   if (!p) return;
-  void* bufs[] = {p->d_mdiag, p->d_sig, p->d_blk, p->d_mf, p->ws, p->d_tasks, p->d_voff, p->d_Js, p->d_fp};
+  void* bufs[] = {p->d_mdiag, p->d_sig, p->d_blk, p->d_mf, p->ws, p->d_tasks, p->d_voff, p->d_Js, p->d_fp, p->d_meta};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (double* k : p->kbig64)
@@ -359,6 +416,9 @@ void kplan_free(rm_kplan* p) {
   for (float* k : p->kbig32)
     if (k) (void)hipFree(k);
   if (p->rb) (void)rocblas_destroy_handle(p->rb);
+  for (hipEvent_t e : p->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (p->gs) (void)hipStreamDestroy(p->gs);
   delete p;
 }
 
@@ -391,6 +451,22 @@ extern "C" int rm_kplan_create(int32_t nJ, const int32_t* Js, int32_t nslot, int
     pos += jdim(Js[j]) * jdim(Js[j]);
   }
   p->n = pos;
+  // Z-build tiles: as many rows as keep the staged X rows and MF values in 64 KiB of LDS
+  {
+    const int64_t dkmax = jdim(Js[nJ - 1]);
+    p->zr = 0;
+    for (int zr = 16; zr >= 1 && !p->zr; zr >>= 1) {
+      const size_t lds = sizeof(double) * (size_t)(10 * zr + (zr + 4) * 2 * dkmax);
+      if (lds <= 65536) {
+        p->zr = zr;
+        p->z_lds = lds;
+      }
+    }
+    if (!p->zr) {
+      kplan_free(p);
+      return fail(RM_ERR_ARG, "rm_kplan_create: J = %d is too large for the Z-build tiles", Js[nJ - 1]);
+    }
+  }
   p->src.assign(nJ, {});
   p->ktot.assign(nJ, 0);
   p->kq_off.assign(nblk, 0);
@@ -470,6 +546,22 @@ extern "C" int rm_kplan_create(int32_t nJ, const int32_t* Js, int32_t nslot, int
     kplan_free(p);
     return fail(RM_ERR_HIP, "rm_kplan_create: rocblas_create_handle failed");
   }
+  p->ev.assign(KCHUNK + 1, nullptr);
+  if (hipStreamCreateWithFlags(&p->gs, hipStreamNonBlocking) != hipSuccess) {
+    p->gs = nullptr;
+    kplan_free(p);
+    return fail(RM_ERR_HIP, "rm_kplan_create: stream creation failed");
+  }
+  for (auto& e : p->ev)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+      e = nullptr;
+      kplan_free(p);
+      return fail(RM_ERR_HIP, "rm_kplan_create: event creation failed");
+    }
+  if (rocblas_set_stream(p->rb, p->gs) != rocblas_status_success) {
+    kplan_free(p);
+    return fail(RM_ERR_HIP, "rm_kplan_create: rocblas_set_stream failed");
+  }
   *out = p;
   return RM_OK;
 }
@@ -545,7 +637,8 @@ int kwork(rm_kplan* p, int32_t ncols, int32_t m, bool f32, KWork* w) {
 
 // The (output J1, source block) tasks for ncols wavepackets, uploaded once per call.
 int ktasks(rm_kplan* p, int32_t ncols, int64_t* ntask, std::vector<int64_t>& zoff_J, hipStream_t s) {
-  std::vector<KTask> h;
+  std::vector<KTask>& h = p->h_tasks;
+  h.clear();
   zoff_J.assign(p->nJ, 0);
   int64_t zpos = 0;
   for (int a = 0; a < p->nJ; ++a) {
@@ -573,9 +666,36 @@ int ktasks(rm_kplan* p, int32_t ncols, int64_t* ntask, std::vector<int64_t>& zof
     zpos += (int64_t)ncols * dm1 * 2 * p->ktot[a];
   }
   *ntask = (int64_t)h.size();
+  if (*ntask > INT32_MAX) return fail(RM_ERR_ARG, "too many operator blocks");
+  // per J: first Z row and first task (tasks are grouped by output J in J order)
+  p->h_meta.assign(2 * (size_t)(p->nJ + 1), 0);
+  int64_t tiles = 0, tq = 0;
+  for (int a = 0; a < p->nJ; ++a) {
+    p->h_meta[a] = tiles;
+    p->h_meta[p->nJ + 1 + a] = tq;
+    if (!p->src[a].empty()) tiles += (int64_t)ncols * ((jdim(p->Js[a]) + p->zr - 1) / p->zr);
+    tq += (int64_t)p->src[a].size();
+  }
+  p->h_meta[p->nJ] = tiles;
+  p->h_meta[2 * p->nJ + 1] = tq;
+  p->ztiles = tiles;
+  // chunk bounds by Z tiles (ascending J): a short first chunk (its build is
+  // not overlapped), then equal shares
+  p->chunk_j.assign(KCHUNK + 1, p->nJ);
+  p->chunk_j[0] = 0;
+  for (int c = 1; c < KCHUNK; ++c) {
+    const double share = 1.0 / 16.0 + (15.0 / 16.0) * (double)(c - 1) / (double)(KCHUNK - 1);
+    const int64_t target = (int64_t)(share * (double)tiles);
+    int a = p->chunk_j[c - 1];
+    while (a < p->nJ && p->h_meta[a + 1] <= target) ++a;
+    p->chunk_j[c] = std::max(a, p->chunk_j[c - 1]);
+  }
   int rc = grow(&p->d_tasks, &p->tasks_cap, std::max<int64_t>(*ntask, 1));
   if (rc) return rc;
+  if ((rc = grow(&p->d_meta, &p->meta_cap, (int64_t)p->h_meta.size()))) return rc;
   if (*ntask) HIPCHK(hipMemcpyAsync(p->d_tasks, h.data(), sizeof(KTask) * h.size(), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(p->d_meta, p->h_meta.data(), sizeof(int64_t) * p->h_meta.size(), hipMemcpyHostToDevice,
+                        s));
   return RM_OK;
 }
 
@@ -584,47 +704,52 @@ __global__ void k_kp_zero(double* __restrict__ x, int64_t n) {
     x[i] = 0.0;
 }
 
-// Y = H(F) (ib X) for the split-layout input X; the MF values of the step are in p->d_mf.
+// Y = H(F) (ib X) for the split-layout input X; the MF values of the step are
+// in p->d_mf.  The Z build runs on s, chunk by chunk; the dense products of a
+// chunk run on the plan's product stream once that chunk's Z is done, and s
+// waits for the last of them.
 int kmatvec(rm_kplan* p, const double* X, const double* ib, void* Y, void* Z, int32_t ncols, int64_t ntask,
             const std::vector<int64_t>& zoff_J, bool f32, hipStream_t s) {
-  if (ntask) {
-    const int64_t maxrows = (int64_t)ncols * jdim(p->Js[p->nJ - 1]);
-    const dim3 grid((unsigned)std::min<int64_t>((maxrows + 3) / 4, 4096), (unsigned)ntask);
-    if (f32)
-      hipLaunchKernelGGL(k_kp_z<float>, grid, dim3(256), 0, s, p->d_tasks, p->d_sig, p->d_mf, X, ib, (float*)Z,
-                         ncols);
-    else
-      hipLaunchKernelGGL(k_kp_z<double>, grid, dim3(256), 0, s, p->d_tasks, p->d_sig, p->d_mf, X, ib, (double*)Z,
-                         ncols);
-    HIPCHK(hipGetLastError());
-  }
-  RBCHK(rocblas_set_stream(p->rb, s));
-  for (int a = 0; a < p->nJ; ++a) {
-    const int64_t d = jdim(p->Js[a]);
-    const int64_t yoff = p->h_voff[a] * 2 * ncols;
-    if (p->src[a].empty()) {
-      // no source block reaches this J: its output rows are zero
-      if (f32) {
-        const int64_t cnt = (int64_t)ncols * d * d;   // floats as pairs: 2 * cnt floats = cnt doubles
-        hipLaunchKernelGGL(k_kp_zero, dim3(grid_for(cnt, 256)), dim3(256), 0, s, (double*)((float*)Y + yoff), cnt);
-      } else {
-        const int64_t cnt = 2 * (int64_t)ncols * d * d;
-        hipLaunchKernelGGL(k_kp_zero, dim3(grid_for(cnt, 256)), dim3(256), 0, s, (double*)Y + yoff, cnt);
-      }
-      continue;
+  if (p->ztiles > INT32_MAX) return fail(RM_ERR_ARG, "too many Z tiles (%lld)", (long long)p->ztiles);
+  for (int c = 0; c < KCHUNK; ++c) {
+    const int j0 = p->chunk_j[c], j1 = p->chunk_j[c + 1];
+    const int64_t t0 = p->h_meta[j0], nt = p->h_meta[j1] - t0;
+    if (ntask && nt > 0) {
+      if (f32)
+        hipLaunchKernelGGL(k_kp_z<float>, dim3((unsigned)nt), dim3(256), p->z_lds, s, p->d_tasks, p->d_meta, p->nJ,
+                           p->d_sig, p->d_mf, X, ib, (float*)Z, p->zr, t0);
+      else
+        hipLaunchKernelGGL(k_kp_z<double>, dim3((unsigned)nt), dim3(256), p->z_lds, s, p->d_tasks, p->d_meta,
+                           p->nJ, p->d_sig, p->d_mf, X, ib, (double*)Z, p->zr, t0);
+      HIPCHK(hipGetLastError());
     }
-    const rocblas_int M = (rocblas_int)(ncols * d), N = (rocblas_int)(2 * d), K = (rocblas_int)(2 * p->ktot[a]);
-    if (f32) {
-      const float one = 1.0f, zero = 0.0f;
-      RBCHK(rocblas_sgemm(p->rb, rocblas_operation_none, rocblas_operation_none, N, M, K, &one, p->kbig32[a], N,
-                          (const float*)Z + zoff_J[a], K, &zero, (float*)Y + yoff, N));
-    } else {
-      const double one = 1.0, zero = 0.0;
-      RBCHK(rocblas_dgemm(p->rb, rocblas_operation_none, rocblas_operation_none, N, M, K, &one, p->kbig64[a], N,
-                          (const double*)Z + zoff_J[a], K, &zero, (double*)Y + yoff, N));
+    HIPCHK(hipEventRecord(p->ev[c], s));
+    HIPCHK(hipStreamWaitEvent(p->gs, p->ev[c], 0));
+    for (int a = j0; a < j1; ++a) {
+      const int64_t d = jdim(p->Js[a]);
+      const int64_t yoff = p->h_voff[a] * 2 * ncols;
+      if (p->src[a].empty()) {
+        // no source block reaches this J: its output rows are zero
+        const int64_t cnt = f32 ? (int64_t)ncols * d * d : 2 * (int64_t)ncols * d * d;   // in doubles
+        double* y0 = f32 ? (double*)((float*)Y + yoff) : (double*)Y + yoff;
+        hipLaunchKernelGGL(k_kp_zero, dim3(grid_for(cnt, 256)), dim3(256), 0, p->gs, y0, cnt);
+        continue;
+      }
+      const rocblas_int M = (rocblas_int)(ncols * d), N = (rocblas_int)(2 * d), K = (rocblas_int)(2 * p->ktot[a]);
+      if (f32) {
+        const float one = 1.0f, zero = 0.0f;
+        RBCHK(rocblas_sgemm(p->rb, rocblas_operation_none, rocblas_operation_none, N, M, K, &one, p->kbig32[a], N,
+                            (const float*)Z + zoff_J[a], K, &zero, (float*)Y + yoff, N));
+      } else {
+        const double one = 1.0, zero = 0.0;
+        RBCHK(rocblas_dgemm(p->rb, rocblas_operation_none, rocblas_operation_none, N, M, K, &one, p->kbig64[a], N,
+                            (const double*)Z + zoff_J[a], K, &zero, (double*)Y + yoff, N));
+      }
     }
   }
   HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(p->ev[KCHUNK], p->gs));
+  HIPCHK(hipStreamWaitEvent(s, p->ev[KCHUNK], 0));
   return RM_OK;
 }
 

@@ -345,6 +345,153 @@ __global__ __launch_bounds__(64) void k_sub_prop(SubArgs a) {
   }
 }
 
+// Two wavepackets per wave (fixed Krylov dimension <= 16, domains <= 32
+// rows, the same operator): lanes 32 h + i hold row i of the pair's
+// wavepacket h.  Per half, every operation is the one k_sub_prop<1, WM, true,
+// F32, true> performs for a domain of <= 32 rows (its lane-local arithmetic,
+// the half-wave sums of wave_allsum32 and the row-local exp(fT)e0), so a
+// wavepacket's result does not depend on its partner, bit for bit, and
+// equals the single-wave kernel's.  Item 2p + h of the launch is half h of
+// pair p; an item with column -1 is an empty half.
+template <int WM, bool F32>
+__global__ __launch_bounds__(64) void k_sub_pair(SubArgs a) {
+  using XT = typename XType<F32>::T;
+  constexpr int WMR = WM > 0 ? WM : 1;
+  __shared__ XT s_x[64];
+  __shared__ double2 s_V[16 * 64];
+  __shared__ double2 s_alpha[2][16];
+  __shared__ double s_beta[2][17];
+  __shared__ double2 s_cf[2][16];
+  const int l = threadIdx.x;
+  const int h = l >> 5, i = l & 31;
+  const double2 zero = make_double2(0.0, 0.0);
+  const int npair = (a.nitems + 1) >> 1;
+  for (int pr = blockIdx.x; pr < npair; pr += gridDim.x) {
+    const int itm = 2 * pr + h;
+    const int4 item = itm < a.nitems ? a.items[itm] : make_int4(-1, 0, 0, a.items[2 * pr].w);
+    const int b = item.x, lo = item.y, len = b >= 0 ? item.z : 0;
+    const SubOp op = a.ops[item.w];   // the same for both halves
+    const int n = op.n, W = op.w;
+    const bool breg = WM > 0 && op.nslot <= NSR;
+    const bool ok = i < len;
+    const int gq = len > 0 ? lo + min(i, len - 1) : 0;
+    const int orig = op.perm[gq];
+    double2 st = ok ? op.psi[(int64_t)b * op.ld + orig] : make_double2(0.0, 0.0);
+    const double2 ph = op.phase ? op.phase[orig] : make_double2(1.0, 0.0);
+    int cidx[WMR];
+    double2 Bv[NSR][WMR];
+#pragma unroll
+    for (int j = 0; j < WMR; ++j) {
+      cidx[j] = ((WM > 0 && j < W) ? op.col[(int64_t)j * n + gq] - lo : gq - lo) + 32 * h;
+      if (len == 0) cidx[j] = l;
+#pragma unroll
+      for (int sl = 0; sl < NSR; ++sl)
+        Bv[sl][j] = (breg && j < W && sl < op.nslot) ? op.val[((int64_t)sl * W + j) * n + gq]
+                                                    : make_double2(0.0, 0.0);
+    }
+    double2* sa = s_alpha[h];
+    double* sb = s_beta[h];
+    double2* sc = s_cf[h];
+    for (int step = 0; step < op.nsteps; ++step) {
+      double2 vk = op.phase ? cmul(st, ph) : st;
+      if (op.sflag[step] == 0) {
+        st = op.phase ? cmul(vk, ph) : vk;
+        if (i == 0 && b >= 0 && op.kused) op.kused[(int64_t)step * op.kused_ld + b - op.kcol0] = 0;
+        continue;
+      }
+      const double2* fp = op.table + (int64_t)step * op.nslot;
+      XT A[WMR];
+      if (WM > 0) {
+        double2 f0 = zero, f1 = zero;
+        if (breg) {
+          f0 = fp[0];
+          if (op.nslot > 1) f1 = fp[1];
+        }
+#pragma unroll
+        for (int j = 0; j < WMR; ++j) {
+          double2 v;
+          if (breg) {
+            v = cmul(f0, Bv[0][j]);
+            if (NSR > 1) v = cfma(f1, Bv[1][j], v);
+          } else {
+            v = make_double2(0.0, 0.0);
+            if (j < W)
+              for (int sl = 0; sl < op.nslot; ++sl) v = cfma(fp[sl], op.val[((int64_t)sl * W + j) * n + gq], v);
+          }
+          A[j] = to_x<F32>(v);
+        }
+      }
+      auto matvec = [&]() __attribute__((always_inline)) -> double2 {
+        const double2 z0 = make_double2(0.0, 0.0);
+        XT acc = to_x<F32>(z0);
+#pragma unroll
+        for (int j = 0; j < WMR; ++j) acc = xfma(A[j], s_x[cidx[j]], acc);
+        return ok ? to_d2(acc) : z0;
+      };
+      s_V[l] = vk;
+      s_x[l] = to_x<F32>(vk);
+      wave_sync();
+      double2 Wr = matvec(), vp = zero;
+      double acc3[2];
+      acc3[0] = Wr.x * vk.x + Wr.y * vk.y;
+      acc3[1] = Wr.x * vk.y - Wr.y * vk.x;
+      double2 al = make_double2(half_allsum32(acc3[0]), half_allsum32(acc3[1]));
+      Wr = csub(Wr, cmul(al, vk));
+      sa[0] = al;
+      int k = 1;
+      for (int it = 1; it < a.fixed_m; ++it) {
+        const double beta = sqrt(half_allsum32(Wr.x * Wr.x + Wr.y * Wr.y));
+        const bool brk = !(beta != 0.0);
+        if (!brk) {
+          double ib = __builtin_amdgcn_rcp(beta);
+          ib = fma(fma(-beta, ib, 1.0), ib, ib);
+          ib = fma(fma(-beta, ib, 1.0), ib, ib);
+          vp = vk;
+          vk = cscale(ib, Wr);
+        } else {
+          // exact breakdown: Gram-Schmidt of ones against the basis (tdse.py:459-465)
+          vp = vk;
+          Wr = ok ? make_double2(1.0, 0.0) : zero;
+          for (int j = 0; j < it; ++j) {
+            double d0 = 0.0, d1 = 0.0;
+            if (ok) {
+              const double2 x = s_V[j * 64 + l];
+              d0 = x.x * Wr.x + x.y * Wr.y;
+              d1 = x.x * Wr.y - x.y * Wr.x;
+            }
+            const double2 proj = make_double2(half_allsum32(d0), half_allsum32(d1));
+            if (ok) Wr = csub(Wr, cmul(proj, s_V[j * 64 + l]));
+          }
+          const double nv = sqrt(half_allsum32(Wr.x * Wr.x + Wr.y * Wr.y));
+          vk = nv > 0.0 ? make_double2(Wr.x / nv, Wr.y / nv) : zero;
+        }
+        s_V[it * 64 + l] = vk;
+        s_x[l] = to_x<F32>(vk);
+        sb[it] = beta;
+        wave_sync();
+        Wr = matvec();
+        const double a0 = Wr.x * vk.x + Wr.y * vk.y, a1 = Wr.x * vk.y - Wr.y * vk.x;
+        al = make_double2(half_allsum32(a0), half_allsum32(a1));
+        Wr = csub(csub(Wr, cmul(al, vk)), cscale(beta, vp));
+        sa[it] = al;
+        k = it + 1;
+      }
+      wave_sync();
+      if (k > 1) expm_e0_16<true>(k, sa, sb, op.f, sc);
+      else if (i == 0) sc[0] = make_double2(1.0, 0.0);
+      __threadfence_block();
+      wave_sync();
+      double2 u = zero;
+      for (int j = 0; j < k - 1; ++j) u = cfma(sc[j], s_V[j * 64 + (len > 0 ? 32 * h + min(i, len - 1) : l)], u);
+      const double2 uu = cfma(sc[k - 1], vk, u);
+      st = ok ? (op.phase ? cmul(uu, ph) : uu) : zero;
+      if (i == 0 && b >= 0 && op.kused) op.kused[(int64_t)step * op.kused_ld + b - op.kcol0] = k;
+      wave_sync();
+    }
+    if (ok) op.psi[(int64_t)b * op.ld + orig] = st;
+  }
+}
+
 __global__ void k_sub_fill(int32_t* x, int64_t n, int32_t v0, int32_t v1) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) x[i] = (i & 1) ? v1 : v0;
@@ -533,8 +680,13 @@ int sub_build(rm_plan* p) {
 // -- does not depend on the other wavepackets of the launch.
 //   0: R=1, 1: R=2, 2: R=4 with the operator in registers (ELL width <= 4)
 //   3: R=4, 4: R=8, 5: R=16 reading the operator at every matvec
-constexpr int NCLASS = 6;
-constexpr int CLASS_R[NCLASS] = {1, 2, 4, 4, 8, 16};
+//   6: two wavepackets per wave (k_sub_pair): fixed Krylov dimension <= 16,
+//      domains <= 32 rows, ELL width <= 4 -- the class-0 items that qualify,
+//      paired with an item of the same operator or with an empty half
+constexpr int NCLASS = 7;
+constexpr int PAIR_CLASS = 6;
+constexpr int PAIR_MAX_M = 16;
+constexpr int CLASS_R[NCLASS] = {1, 2, 4, 4, 8, 16, 1};
 
 int sub_class(int w, int len) {
   if (w <= 4) {
@@ -549,7 +701,24 @@ int sub_class(int w, int len) {
 
 // Krylov basis in LDS: register classes whose basis fits.
 bool class_basis_lds(int c, int kcap) {
+  if (c == PAIR_CLASS) return true;   // static LDS of k_sub_pair
   return c <= 2 && (size_t)kcap * 64 * CLASS_R[c] * sizeof(double2) <= SUB_LDS_BASIS;
+}
+
+template <bool F32>
+hipError_t launch_pair(const SubArgs& a0, int n_cu, hipStream_t s, int* waves_out, bool dry) {
+  auto kern = k_sub_pair<4, F32>;
+  int occ = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)kern, 64, 0);
+  if (e != hipSuccess) return e;
+  if (occ < 1) occ = 1;
+  const int64_t npair = ((int64_t)a0.nitems + 1) / 2;
+  int waves = (int)std::min<int64_t>(npair, (int64_t)occ * n_cu);
+  if (waves < 1) waves = 1;
+  *waves_out = waves;
+  if (dry) return hipSuccess;
+  hipLaunchKernelGGL(kern, dim3(waves), dim3(64), 0, s, a0);
+  return hipGetLastError();
 }
 
 template <int R, int WM, bool FIXED, bool F32, bool BL>
@@ -595,6 +764,7 @@ hipError_t launch_class_p(int c, const SubArgs& a, int n_cu, hipStream_t s, int*
 }
 
 hipError_t launch_class(int c, const SubArgs& a, bool f32, int n_cu, hipStream_t s, int* waves, bool dry) {
+  if (c == PAIR_CLASS) return f32 ? launch_pair<true>(a, n_cu, s, waves, dry) : launch_pair<false>(a, n_cu, s, waves, dry);
   const bool fixed = a.fixed_m > 0;
   if (fixed && f32) return launch_class_p<true, true>(c, a, n_cu, s, waves, dry);
   if (fixed) return launch_class_p<true, false>(c, a, n_cu, s, waves, dry);
@@ -650,6 +820,23 @@ int sub_run(SubScratch* sc, const std::vector<SubOp>& ops, std::vector<int4> (&c
             const std::vector<LongItem>& blk, const std::vector<LongItem>& coop, int32_t kcap,
             int32_t fixed_m, int32_t maxorder, double tol, bool f32, int n_cu, hipStream_t s,
             rm_plan* timed) {
+  // the pair class: class-0 items of <= 32 rows under a fixed Krylov
+  // dimension <= 16, grouped by operator, each group padded to an even count
+  if (fixed_m >= 1 && fixed_m <= PAIR_MAX_M && cls[PAIR_CLASS].empty()) {
+    std::vector<int4> keep, pairable;
+    for (const int4& it : cls[0]) (it.z <= 32 ? pairable : keep).push_back(it);
+    if (!pairable.empty()) {
+      std::stable_sort(pairable.begin(), pairable.end(), [](const int4& x, const int4& y) { return x.w < y.w; });
+      std::vector<int4>& pc = cls[PAIR_CLASS];
+      for (size_t i = 0; i < pairable.size();) {
+        size_t j = i;
+        while (j < pairable.size() && pairable[j].w == pairable[i].w) pc.push_back(pairable[j++]);
+        if ((j - i) & 1) pc.push_back(make_int4(-1, 0, 0, pairable[i].w));
+        i = j;
+      }
+      cls[0].swap(keep);
+    }
+  }
   size_t nitems = blk.size() + coop.size();
   for (int c = 0; c < NCLASS; ++c) nitems += cls[c].size();
   if (nitems == 0) return RM_OK;

@@ -26,12 +26,12 @@ def _to_int(kop, Xf):
 
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("fp32", 2e-6)])
 def test_matvec_vs_host_factorisation(device, precision, tol):
-    """J <= 7 (680 states), 5 wavepackets, a general field on both tensors:
-    the device matvec equals KronOperator.matvec (relative to the largest
-    output)."""
+    """J <= 12 (2625 states; m blocks of up to 25 rows span two Z tiles), 5
+    wavepackets, a general field on both tensors: the device matvec equals
+    KronOperator.matvec (relative to the largest output)."""
     import torch
     from richmol_amd.kron import slot_products
-    kop, bas = large_top(7)
+    kop, bas = large_top(12)
     dp = _dev_plan(kop)
     rng = np.random.default_rng(11)
     X = rng.normal(size=(5, kop.n)) + 1j * rng.normal(size=(5, kop.n))
