@@ -43,6 +43,7 @@ PULSE_STEPS = int(round(8 * FWHM_FS / DT_FS))   # steps with a non-zero field
 MEASURED_TRAFFIC = {
     ("ell", 40, 512, 12, 1): (351.1e6, "profiles/k_lanczos_ell_linear_j40_b512_m12_pmc.md"),
     ("subspace", 40, 512, 12, 200): (2.12e6, "profiles/k_sub_prop_linear_j40_b512_m12.md"),
+    ("pair", 40, 512, 12, 200): (2.05e6, "profiles/k_sub_pair_linear_j40_b512_m12.md"),
 }
 
 
@@ -328,17 +329,20 @@ def main():
     # algorithmic FP64 work per wavepacket-step on the rows it spans (d rows,
     # e stored entries): m matvecs of 8 e flops + 7 complex vector operations
     # of 8 d flops per iteration
+    pair = False
     if used == "subspace":
         d_rows, d_nnz = p.domain_sizes(psi0)
         flops_launch = float(np.sum(8.0 * m * d_nnz + 8.0 * m * d_rows * 7)) * args.steps
-        kernel_name, launch_steps = "k_sub_prop", args.steps
+        # fixed dimension <= 16 and domains <= 32 rows: two wavepackets per wave
+        pair = m <= 16 and int(np.max(d_rows)) <= 32
+        kernel_name, launch_steps = ("k_sub_pair" if pair else "k_sub_prop"), args.steps
     else:
         flops_launch = args.ncols * (8.0 * m * p.nnz + 8.0 * m * n * 7)
         kernel_name, launch_steps = "k_lanczos_ell", 1
     kern_ms = kms / nl
     tflops = flops_launch / (kern_ms * 1e-3) / 1e12
-    meas = MEASURED_TRAFFIC.get(("subspace" if used == "subspace" else "ell", args.jmax, args.ncols, m,
-                                 launch_steps))
+    meas = MEASURED_TRAFFIC.get(("pair" if pair else ("subspace" if used == "subspace" else "ell"), args.jmax,
+                                 args.ncols, m, launch_steps))
 
     result = {
         "metric": METRIC,
@@ -359,7 +363,8 @@ def main():
                    "jmax": args.jmax, "n_states": n, "wavepackets": args.ncols, "krylov": m,
                    "dt_fs": DT_FS, "kernel": args.kernel, "kernel_used": used,
                    "launch": "one rm_propagate call for the timed steps",
-                   "parallelism": "one GPU; one wave per wavepacket on its invariant subspace"
+                   "parallelism": ("one GPU; two wavepackets per wave on their invariant subspaces" if pair
+                                   else "one GPU; one wave per wavepacket on its invariant subspace")
                    if used == "subspace" else "one GPU; wavepackets split over persistent workgroups"},
         "roofline": {"bound": "mfma", "achieved": tflops, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": tflops / FP64_PEAK_TFLOPS,

@@ -291,6 +291,14 @@ int rm_kplan_destroy(rm_kplan* plan);
 /* n: states per wavepacket; gemm_flops_per_col: real flops of one matvec's
  * dense products per wavepacket. */
 int rm_kplan_info(const rm_kplan* plan, int64_t* n, double* gemm_flops_per_col);
+/* Matvec form (no reference counterpart).  RM_KKERNEL_GEMM (default): the
+ * m-mixed input (MF_w X) stored, then one rocBLAS product per output J, the
+ * build of the next J chunk overlapping the products of the last.
+ * RM_KKERNEL_FUSED: one launch per matvec on the matrix cores, the m-mixed
+ * input built in LDS chunk by chunk inside the product (no stored copy). */
+#define RM_KKERNEL_FUSED 0
+#define RM_KKERNEL_GEMM  1
+int rm_kplan_set_kernel(rm_kplan* plan, int32_t kernel);
 /* y = H(F) x for ncols wavepackets (device, complex, row b at b*ld, internal
  * order); fprod_dev: device, nslot complex.  precision RM_PREC_FP32: the
  * products in FP32 (complex64 operands, FP32 accumulation). */

@@ -58,6 +58,7 @@ SIGNATURES = {
                                        _c_i32p, _c_i32p, _c_i32p, _c_i64p, _c_dblp, _c_i64p, _c_i32p,
                                        _c_i64p, _c_dblp, ctypes.POINTER(_vp)]),
     "rm_kplan_destroy": (ctypes.c_int, [_vp]),
+    "rm_kplan_set_kernel": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "rm_kplan_info": (ctypes.c_int, [_vp, _c_i64p, _c_dblp]),
     "rm_kplan_matvec": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int64, ctypes.c_int32, _vp, ctypes.c_int32,
                                        _vp]),

@@ -67,6 +67,21 @@ struct rm_kplan {
   std::vector<double*> kbig64;            // per J index: (2 Ktot) x (2 dk) row-major
   std::vector<float*> kbig32;
   rocblas_handle rb = nullptr;
+  // fused form: per J1 the stacked K with columns padded to groups of four
+  // (FGK) and the rows of a group interleaved [re x 4 | im x 4], output
+  // columns padded to FBN; colmap: per padded stacked column, (block << 16 | k2)
+  // of the J1's source blocks, -1 for padding
+  int32_t kernel = RM_KKERNEL_GEMM;
+  std::vector<int32_t> kpad, npad;      // per J index
+  std::vector<int64_t> kf_off, cm_off;  // per J index
+  double* d_kf64 = nullptr;
+  float* d_kf32 = nullptr;
+  int32_t* d_colmap = nullptr;
+  char* d_fz = nullptr;                 // per call: J descriptors and tiles
+  int64_t fz_cap = 0;
+  const int4* fz_tiles = nullptr;
+  int64_t ntiles = 0;
+  std::vector<char> h_fz;
   int64_t* d_voff = nullptr;            // per J: complex offset (per wavepacket)
   int32_t* d_Js = nullptr;
   // per-call workspace
@@ -96,6 +111,26 @@ __host__ __device__ inline int64_t jdim(int32_t J) { return 2 * (int64_t)J + 1; 
 // Chunks of output J per matvec: the Z build of chunk c + 1 overlaps the dense
 // products of chunk c (memory-bound next to MFMA-bound work).
 constexpr int KCHUNK = 5;
+
+// Fused matvec tiles: FBM output rows (wavepacket, m1) x FBN output columns
+// per workgroup of four waves (2 x 2, each 4 x 4 MFMA blocks of 16 x 16), the
+// stacked K dimension in chunks of FBK reals (two groups of four complex
+// columns); at most FMAXT source blocks per output J.
+constexpr int FBM = 128, FBN = 128, FBK = 16, FGK = 4, FMAXT = 32;
+
+// Per-call descriptor of one output J of the fused matvec.
+struct FJ {
+  int32_t dm1, N, Npad, kpad;    // m rows per wavepacket, output columns (2 dk1), padded; padded stacked columns
+  int32_t task0, ntask;          // its source blocks in the call's task list
+  int64_t M;                     // ncols * dm1
+  int64_t cm_off, kf_off, yoff;  // column map, K stack, output block
+};
+
+// Source block of the fused kernel, staged in LDS.
+struct FT {
+  int32_t J1, J2, dm2, dk2, nsig, sg[5];
+  int64_t mf_off, xoff;
+};
 
 // MF[blk][t][m1] = sum_c fprod[slot0 + c] mdiag[blk][t][c][m1]
 __global__ void k_kp_mf(const KBlk* __restrict__ blk, const double2* __restrict__ mdiag,
@@ -182,6 +217,138 @@ __global__ __launch_bounds__(256) void k_kp_z(const KTask* __restrict__ tasks, c
     }
     __syncthreads();
   }
+}
+
+template <typename T>
+struct FAcc;
+template <>
+struct FAcc<double> {
+  typedef double V __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ V mfma(double a, double b, V c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  // row of D held in item v of lane l (column l % 16)
+  static __device__ __forceinline__ int row(int l, int v) { return (l >> 4) + 4 * v; }
+};
+template <>
+struct FAcc<float> {
+  typedef float V __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ V mfma(float a, float b, V c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int l, int v) { return 4 * (l >> 4) + v; }
+};
+
+// The matvec Y[J1] = Z_J1 K_J1^T (real split form) with Z never stored:
+// every chunk of FBK stacked reals is built in LDS from the input and the MF
+// diagonals (as k_kp_z does, scaled by ib[b]) next to the matching rows of
+// the K stack, and multiplied on the matrix cores (16 x 16 x 4 MFMA; lane l
+// supplies A[l % 16][l / 16] and B[l / 16][l % 16]; item v of lane l holds
+// D[l / 16 + 4 v][l % 16] in FP64 and D[4 (l / 16) + v][l % 16] in FP32).
+// One workgroup per (output J, FBM rows, FBN columns) tile of the tile list.
+template <typename T>
+__global__ __launch_bounds__(256) void k_kp_fused(const int4* __restrict__ tiles, const FJ* __restrict__ jd,
+                                                  const KTask* __restrict__ tasks, const int32_t* __restrict__ sig,
+                                                  const int32_t* __restrict__ colmap,
+                                                  const double2* __restrict__ mf, const double* __restrict__ X,
+                                                  const double* __restrict__ ib, const T* __restrict__ Kf,
+                                                  T* __restrict__ Y) {
+  using V = typename FAcc<T>::V;
+  __shared__ T sZ[FBK][FBM];
+  __shared__ T sK[FBK][FBN];
+  __shared__ FT sT[FMAXT];
+  const int4 tl = tiles[blockIdx.x];
+  const FJ j = jd[tl.x];
+  const int64_t r0 = (int64_t)tl.y * FBM;
+  const int n0 = tl.z * FBN;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int wr = w & 1, wc = w >> 1;
+  for (int q = t; q < j.ntask; q += blockDim.x) {
+    const KTask tk = tasks[j.task0 + q];
+    FT f;
+    f.J1 = tk.J1;
+    f.J2 = tk.J2;
+    f.dm2 = tk.dm2;
+    f.dk2 = tk.dk2;
+    f.nsig = tk.nsig;
+    for (int u = 0; u < 5; ++u) f.sg[u] = u < tk.nsig ? sig[tk.sig0 + u] : 0;
+    f.mf_off = tk.mf_off;
+    f.xoff = tk.xoff;
+    sT[q] = f;
+  }
+  V acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[a][c] = V{0, 0, 0, 0};
+  __syncthreads();
+  const int nchunk = 2 * j.kpad / FBK;
+  for (int ch = 0; ch < nchunk; ++ch) {
+    // Z chunk: complex stacked columns 8 ch .. 8 ch + 7, rows r0 .. r0 + FBM - 1
+#pragma unroll
+    for (int u = 0; u < (FBM * 8) / 256; ++u) {
+      const int e = t + 256 * u;
+      const int jc = e & 7, row = e >> 3;
+      const int64_t r = r0 + row;
+      const int cm = colmap[j.cm_off + 8 * ch + jc];
+      double zr = 0.0, zi = 0.0;
+      if (cm >= 0 && r < j.M) {
+        const FT& f = sT[cm >> 16];
+        const int k2 = cm & 0xffff;
+        const int64_t b = r / j.dm1;
+        const int m1 = (int)(r - b * j.dm1);
+        const int64_t xld = 2 * (int64_t)f.dk2;
+        const double* xb = X + f.xoff + b * f.dm2 * xld + k2;
+        for (int u2 = 0; u2 < f.nsig; ++u2) {
+          const int m2 = m1 - f.J1 - f.sg[u2] + f.J2;
+          if (m2 < 0 || m2 >= f.dm2) continue;
+          const double2 fv = mf[f.mf_off + (int64_t)u2 * j.dm1 + m1];
+          const double xr = xb[m2 * xld], xi = xb[m2 * xld + f.dk2];
+          zr += fv.x * xr - fv.y * xi;
+          zi += fv.x * xi + fv.y * xr;
+        }
+        const double s = ib[b];
+        zr *= s;
+        zi *= s;
+      }
+      const int kk = (jc >> 2) * 8 + (jc & 3);
+      sZ[kk][row] = (T)zr;
+      sZ[kk + 4][row] = (T)zi;
+    }
+    // K chunk: stacked rows 16 ch .. 16 ch + 15, columns n0 .. n0 + FBN - 1
+    const T* kr = Kf + j.kf_off + (int64_t)(FBK * ch) * j.Npad + n0;
+#pragma unroll
+    for (int u = 0; u < (FBK * FBN) / 256; ++u) {
+      const int e = t + 256 * u;
+      sK[e / FBN][e % FBN] = kr[(int64_t)(e / FBN) * j.Npad + (e % FBN)];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < FBK / 4; ++ks) {
+      const int kq = ks * 4 + (lane >> 4);
+      T av[4], bv[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) av[a] = sZ[kq][wr * 64 + a * 16 + (lane & 15)];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) bv[c] = sK[kq][wc * 64 + c * 16 + (lane & 15)];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[a][c] = FAcc<T>::mfma(av[a], bv[c], acc[a][c]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int col = n0 + wc * 64 + c * 16 + (lane & 15);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t row = r0 + wr * 64 + a * 16 + FAcc<T>::row(lane, v);
+        if (row < j.M && col < j.N) Y[j.yoff + row * j.N + col] = acc[a][c][v];
+      }
+    }
 }
 
 // Per (J, wavepacket) chunk: V_it = ib W (stored), alpha partial = vdot(Y, V_it)
@@ -408,7 +575,7 @@ __global__ void k_kp_out(const int64_t* __restrict__ voff, const int32_t* __rest
 
 void kplan_free(rm_kplan* p) {
   if (!p) return;
-  void* bufs[] = {p->d_mdiag, p->d_sig, p->d_blk, p->d_mf, p->ws, p->d_tasks, p->d_voff, p->d_Js, p->d_fp, p->d_meta};
+  void* bufs[] = {p->d_mdiag, p->d_sig, p->d_blk, p->d_mf, p->ws, p->d_tasks, p->d_voff, p->d_Js, p->d_fp, p->d_meta, p->d_kf64, p->d_kf32, p->d_colmap, p->d_fz};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (double* k : p->kbig64)
@@ -541,6 +708,64 @@ extern "C" int rm_kplan_create(int32_t nJ, const int32_t* Js, int32_t nslot, int
       return rc;
     }
   }
+  // fused form: padded stacked columns (groups of FGK, the total a multiple
+  // of FBK / 2), interleaved group rows, output columns padded to FBN
+  p->kpad.assign(nJ, 0);
+  p->npad.assign(nJ, 0);
+  p->kf_off.assign(nJ, 0);
+  p->cm_off.assign(nJ, 0);
+  int64_t kfpos = 0, cmpos = 0;
+  for (int a = 0; a < nJ; ++a) {
+    const int64_t dk1 = jdim(Js[a]);
+    int64_t kp = 0;
+    for (int q : p->src[a]) kp += (jdim(J2[q]) + FGK - 1) / FGK * FGK;
+    kp = (kp + FBK / 2 - 1) / (FBK / 2) * (FBK / 2);
+    if ((int64_t)p->src[a].size() > FMAXT) {
+      kplan_free(p);
+      return fail(RM_ERR_ARG, "rm_kplan_create: more than %d source blocks for J = %d", FMAXT, Js[a]);
+    }
+    p->kpad[a] = (int32_t)kp;
+    p->npad[a] = (int32_t)((2 * dk1 + FBN - 1) / FBN * FBN);
+    p->kf_off[a] = kfpos;
+    p->cm_off[a] = cmpos;
+    kfpos += 2 * kp * p->npad[a];
+    cmpos += kp;
+  }
+  {
+    std::vector<double> kf((size_t)std::max<int64_t>(kfpos, 1), 0.0);
+    std::vector<int32_t> cm((size_t)std::max<int64_t>(cmpos, 1), -1);
+    for (int a = 0; a < nJ; ++a) {
+      const int64_t dk1 = jdim(Js[a]), np = p->npad[a];
+      double* kb = kf.data() + p->kf_off[a];
+      int64_t c0 = 0;
+      for (size_t li = 0; li < p->src[a].size(); ++li) {
+        const int q = p->src[a][li];
+        const int64_t dk2 = jdim(J2[q]);
+        const double* Kq = K + 2 * k_off[q];
+        for (int64_t k2 = 0; k2 < dk2; ++k2) {
+          const int64_t col = c0 + k2, g = col / FGK, i = col % FGK;
+          cm[(size_t)(p->cm_off[a] + col)] = (int32_t)((li << 16) | k2);
+          double* rre = kb + (2 * FGK * g + i) * np;
+          double* rim = kb + (2 * FGK * g + FGK + i) * np;
+          for (int64_t k1 = 0; k1 < dk1; ++k1) {
+            const double kr = Kq[2 * (k1 * dk2 + k2)], ki = Kq[2 * (k1 * dk2 + k2) + 1];
+            rre[k1] = kr;
+            rre[dk1 + k1] = ki;
+            rim[k1] = -ki;
+            rim[dk1 + k1] = kr;
+          }
+        }
+        c0 += (dk2 + FGK - 1) / FGK * FGK;
+      }
+    }
+    std::vector<float> kf32(kf.begin(), kf.end());
+    if ((rc = upload(&p->d_kf64, kf.data(), (int64_t)kf.size())) ||
+        (rc = upload(&p->d_kf32, kf32.data(), (int64_t)kf32.size())) ||
+        (rc = upload(&p->d_colmap, cm.data(), (int64_t)cm.size()))) {
+      kplan_free(p);
+      return rc;
+    }
+  }
   if (rocblas_create_handle(&p->rb) != rocblas_status_success) {
     p->rb = nullptr;
     kplan_free(p);
@@ -600,7 +825,8 @@ size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
 int kwork(rm_kplan* p, int32_t ncols, int32_t m, bool f32, KWork* w) {
   const int64_t vreal = 2 * p->n * ncols;
   int64_t zreal = 0;
-  for (int a = 0; a < p->nJ; ++a) zreal += (int64_t)ncols * jdim(p->Js[a]) * 2 * p->ktot[a];
+  if (p->kernel != RM_KKERNEL_FUSED)
+    for (int a = 0; a < p->nJ; ++a) zreal += (int64_t)ncols * jdim(p->Js[a]) * 2 * p->ktot[a];
   const size_t es = f32 ? sizeof(float) : sizeof(double);
   const size_t mm = (size_t)std::max(m, 1);
   const size_t b_vec = al256(sizeof(double) * (size_t)vreal);
@@ -690,6 +916,39 @@ int ktasks(rm_kplan* p, int32_t ncols, int64_t* ntask, std::vector<int64_t>& zof
     while (a < p->nJ && p->h_meta[a + 1] <= target) ++a;
     p->chunk_j[c] = std::max(a, p->chunk_j[c - 1]);
   }
+  // fused matvec: J descriptors and the tile list (longest stacks first)
+  {
+    std::vector<FJ> fj((size_t)p->nJ);
+    std::vector<int4> tl;
+    for (int a = 0; a < p->nJ; ++a) {
+      FJ& d = fj[(size_t)a];
+      d.dm1 = (int32_t)jdim(p->Js[a]);
+      d.N = 2 * d.dm1;
+      d.Npad = p->npad[a];
+      d.kpad = p->kpad[a];
+      d.task0 = (int32_t)p->h_meta[p->nJ + 1 + a];
+      d.ntask = (int32_t)p->src[a].size();
+      d.M = (int64_t)ncols * d.dm1;
+      d.cm_off = p->cm_off[a];
+      d.kf_off = p->kf_off[a];
+      d.yoff = p->h_voff[a] * 2 * ncols;
+      const int64_t nrt = (d.M + FBM - 1) / FBM, nct = (d.N + FBN - 1) / FBN;
+      if (nrt > INT32_MAX) return fail(RM_ERR_ARG, "too many fused tiles");
+      for (int64_t rt = 0; rt < nrt; ++rt)
+        for (int64_t ct = 0; ct < nct; ++ct) tl.push_back(make_int4(a, (int)rt, (int)ct, 0));
+    }
+    std::stable_sort(tl.begin(), tl.end(), [&](const int4& x, const int4& y) { return p->kpad[x.x] > p->kpad[y.x]; });
+    p->ntiles = (int64_t)tl.size();
+    if (p->ntiles > INT32_MAX) return fail(RM_ERR_ARG, "too many fused tiles");
+    const size_t fjb = (sizeof(FJ) * fj.size() + 255) & ~(size_t)255;
+    p->h_fz.assign(fjb + sizeof(int4) * tl.size(), 0);
+    memcpy(p->h_fz.data(), fj.data(), sizeof(FJ) * fj.size());
+    if (!tl.empty()) memcpy(p->h_fz.data() + fjb, tl.data(), sizeof(int4) * tl.size());
+    int rc0 = grow(&p->d_fz, &p->fz_cap, (int64_t)p->h_fz.size());
+    if (rc0) return rc0;
+    HIPCHK(hipMemcpyAsync(p->d_fz, p->h_fz.data(), p->h_fz.size(), hipMemcpyHostToDevice, s));
+    p->fz_tiles = (const int4*)((const char*)p->d_fz + fjb);
+  }
   int rc = grow(&p->d_tasks, &p->tasks_cap, std::max<int64_t>(*ntask, 1));
   if (rc) return rc;
   if ((rc = grow(&p->d_meta, &p->meta_cap, (int64_t)p->h_meta.size()))) return rc;
@@ -710,6 +969,19 @@ __global__ void k_kp_zero(double* __restrict__ x, int64_t n) {
 // waits for the last of them.
 int kmatvec(rm_kplan* p, const double* X, const double* ib, void* Y, void* Z, int32_t ncols, int64_t ntask,
             const std::vector<int64_t>& zoff_J, bool f32, hipStream_t s) {
+  if (p->kernel == RM_KKERNEL_FUSED) {
+    if (p->ntiles > 0) {
+      const FJ* jd = (const FJ*)p->d_fz;
+      if (f32)
+        hipLaunchKernelGGL(k_kp_fused<float>, dim3((unsigned)p->ntiles), dim3(256), 0, s, p->fz_tiles, jd,
+                           p->d_tasks, p->d_sig, p->d_colmap, p->d_mf, X, ib, p->d_kf32, (float*)Y);
+      else
+        hipLaunchKernelGGL(k_kp_fused<double>, dim3((unsigned)p->ntiles), dim3(256), 0, s, p->fz_tiles, jd,
+                           p->d_tasks, p->d_sig, p->d_colmap, p->d_mf, X, ib, p->d_kf64, (double*)Y);
+      HIPCHK(hipGetLastError());
+    }
+    return RM_OK;
+  }
   if (p->ztiles > INT32_MAX) return fail(RM_ERR_ARG, "too many Z tiles (%lld)", (long long)p->ztiles);
   for (int c = 0; c < KCHUNK; ++c) {
     const int j0 = p->chunk_j[c], j1 = p->chunk_j[c + 1];
@@ -864,5 +1136,13 @@ extern "C" int rm_kplan_propagate(rm_kplan* p, void* psi, int64_t ld, int32_t nc
   }
   hipLaunchKernelGGL(k_kp_pack, cg, cb, 0, s, p->d_voff, p->d_Js, (const double2*)psi, ld, w.S, 0);
   HIPCHK(hipGetLastError());
+  return RM_OK;
+}
+
+extern "C" int rm_kplan_set_kernel(rm_kplan* p, int32_t kernel) {
+  if (!p) return fail(RM_ERR_ARG, "rm_kplan_set_kernel: NULL plan");
+  if (kernel != RM_KKERNEL_FUSED && kernel != RM_KKERNEL_GEMM)
+    return fail(RM_ERR_ARG, "rm_kplan_set_kernel: unknown kernel %d", kernel);
+  p->kernel = kernel;
   return RM_OK;
 }

@@ -261,12 +261,22 @@ class DeviceKronPlan:
         _lib.check(lib.rm_kplan_info(h, ctypes.byref(n), ctypes.byref(fl)), "rm_kplan_info")
         self.n = int(n.value)
         self.gemm_flops_per_col = float(fl.value)
+        self.kernel = "gemm"
 
     def __del__(self):
         h = getattr(self, "handle", None)
         if h is not None and h.value:
             self._lib.load().rm_kplan_destroy(h)
             self.handle = None
+
+    KERNELS = {"fused": 0, "gemm": 1}
+
+    def set_kernel(self, kernel):
+        """'gemm' (default): the m-mixed input stored, rocBLAS per output J;
+        'fused': one MFMA launch per matvec, the m-mixed input built in LDS."""
+        self._lib.check(self._lib.load().rm_kplan_set_kernel(self.handle, self.KERNELS[kernel]),
+                        "rm_kplan_set_kernel")
+        self.kernel = kernel
 
     def matvec(self, x, y, fprod_row, precision="fp64", stream=None):
         """y = H(F) x (device complex128 (ncols, n) tensors); fprod_row:

@@ -24,8 +24,9 @@ def _to_int(kop, Xf):
     return Xi
 
 
+@pytest.mark.parametrize("kernel", ["fused", "gemm"])
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("fp32", 2e-6)])
-def test_matvec_vs_host_factorisation(device, precision, tol):
+def test_matvec_vs_host_factorisation(device, precision, tol, kernel):
     """J <= 12 (2625 states; m blocks of up to 25 rows span two Z tiles), 5
     wavepackets, a general field on both tensors: the device matvec equals
     KronOperator.matvec (relative to the largest output)."""
@@ -33,6 +34,7 @@ def test_matvec_vs_host_factorisation(device, precision, tol):
     from richmol_amd.kron import slot_products
     kop, bas = large_top(12)
     dp = _dev_plan(kop)
+    dp.set_kernel(kernel)
     rng = np.random.default_rng(11)
     X = rng.normal(size=(5, kop.n)) + 1j * rng.normal(size=(5, kop.n))
     F = [np.array([[0.3e8, -0.8e8, 0.5e8]]), np.array([[0.6e8, 0.2e8, -0.9e8]])]
@@ -63,6 +65,32 @@ def test_matvec_strided_rows(device):
     assert torch.all(out[:, n:] == 3.0 + 4.0j)
 
 
+def test_fused_matvec_long_stacks(device):
+    """J <= 24 (2 401 states; output blocks of up to 49 x 98 reals, stacks of
+    up to nine source blocks, several row and column tiles per J): the fused
+    kernel equals the stored-Z rocBLAS form to rounding, and the host
+    factorisation, in FP64; 3 wavepackets."""
+    import torch
+    from richmol_amd.kron import slot_products
+    kop, bas = large_top(24)
+    dp = _dev_plan(kop)
+    rng = np.random.default_rng(13)
+    X = rng.normal(size=(3, kop.n)) + 1j * rng.normal(size=(3, kop.n))
+    F = [np.array([[0.4e8, 0.9e8, -0.3e8]]), np.array([[-0.2e8, 0.5e8, 0.8e8]])]
+    fp = torch.from_numpy(slot_products(kop, F)[0]).to(device)
+    x = torch.from_numpy(X).to(device)
+    out = {}
+    for k in ("fused", "gemm"):
+        dp.set_kernel(k)
+        y = torch.zeros_like(x)
+        dp.matvec(x, y, fp)
+        out[k] = y.cpu().numpy()
+    ref = kop.matvec(X, kop.field_products([F[0][0], F[1][0]]))
+    scale = np.abs(ref).max()
+    assert np.abs(out["fused"] - out["gemm"]).max() <= 1e-13 * scale
+    assert np.abs(out["fused"] - ref).max() <= 1e-13 * scale
+
+
 def _oracle_run(bas, fields, Xf, fixed, dt):
     from oracle.cartens import OracleCarTens
     from oracle.tdse import OracleTDSE
@@ -81,10 +109,11 @@ def _oracle_run(bas, fields, Xf, fixed, dt):
     return psi, to.exp_fac
 
 
-def _device_run(kop, fields, Xf, fixed, f, precision="fp64"):
+def _device_run(kop, fields, Xf, fixed, f, precision="fp64", kernel="gemm"):
     import torch
     from richmol_amd.kron import slot_products
     dp = _dev_plan(kop)
+    dp.set_kernel(kernel)
     psi = torch.from_numpy(_to_int(kop, Xf)).cuda()
     ph = dp.phases(f)
     dp.propagate(psi, slot_products(kop, fields), f, phase=ph, krylov_fixed=fixed, precision=precision)
@@ -102,8 +131,9 @@ def _columns(n, seed):
     return X
 
 
+@pytest.mark.parametrize("kernel", ["gemm", "fused"])
 @pytest.mark.parametrize("fixed", [12, 5])
-def test_propagate_matches_oracle(device, fixed):
+def test_propagate_matches_oracle(device, fixed, kernel):
     """J <= 5 (286 states), four wavepackets (basis states, dense, unnormalised),
     eight 10 fs steps of the chirped circular field starting late in the
     sweep (several THz), plus two zero-field steps (phases only): FP64 within
@@ -117,13 +147,14 @@ def test_propagate_matches_oracle(device, fixed):
         F[6] = 0.0
     X = _columns(kop.n, 3)
     ref, f = _oracle_run(bas, fields, X, fixed, dt)
-    got = _device_run(kop, fields, X, fixed, f)
+    got = _device_run(kop, fields, X, fixed, f, kernel=kernel)
     assert np.abs(got - ref).max() <= 1e-10
     if fixed == 12:   # converged expv: the norms are kept (unitary step)
         assert np.abs(np.linalg.norm(got, axis=1) - np.linalg.norm(X, axis=1)).max() <= 1e-12
 
 
-def test_propagate_fp32_products(device):
+@pytest.mark.parametrize("kernel", ["gemm", "fused"])
+def test_propagate_fp32_products(device, kernel):
     """FP32 dense products (complex64 operands and output; recurrence, norms,
     exp(fT), phases and wavepackets FP64): amplitudes within 2e-6 and
     <cos^2 theta> within 1e-6 of the FP64 oracle over 12 steps at J <= 6."""
@@ -135,7 +166,7 @@ def test_propagate_fp32_products(device):
     fields = centrifuge_fields(t, e0=3e9)
     X = _columns(kop.n, 5)[:3]
     ref, f = _oracle_run(bas, fields, X, 12, dt)
-    got = _device_run(kop, fields, X, 12, f, precision="fp32")
+    got = _device_run(kop, fields, X, 12, f, precision="fp32", kernel=kernel)
     C = OracleCarTens.from_attrs(rot.named("cos2theta", bas)).tomat(form="full", cart="0")
     c2 = lambda P: np.real(np.einsum("bi,bi->b", P.conj(), (C @ P.T).T))
     assert np.abs(got - ref).max() <= 2e-6
