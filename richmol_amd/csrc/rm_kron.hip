@@ -113,10 +113,10 @@ __host__ __device__ inline int64_t jdim(int32_t J) { return 2 * (int64_t)J + 1; 
 constexpr int KCHUNK = 5;
 
 // Fused matvec tiles: FBM output rows (wavepacket, m1) x FBN output columns
-// per workgroup of four waves (2 x 2, each 4 x 4 MFMA blocks of 16 x 16), the
-// stacked K dimension in chunks of FBK reals (two groups of four complex
-// columns); at most FMAXT source blocks per output J.
-constexpr int FBM = 128, FBN = 128, FBK = 16, FGK = 4, FMAXT = 32;
+// per workgroup (four multiplying waves, 2 x 2, each 2 x 4 MFMA blocks of
+// 16 x 16), the stacked K dimension in chunks of FBK reals (two groups of four
+// complex columns); at most FMAXT source blocks per output J.
+constexpr int FBM = 64, FBN = 128, FBK = 16, FGK = 4, FMAXT = 32;
 
 // Per-call descriptor of one output J of the fused matvec.
 struct FJ {
@@ -245,24 +245,33 @@ struct FAcc<float> {
 // the K stack, and multiplied on the matrix cores (16 x 16 x 4 MFMA; lane l
 // supplies A[l % 16][l / 16] and B[l / 16][l % 16]; item v of lane l holds
 // D[l / 16 + 4 v][l % 16] in FP64 and D[4 (l / 16) + v][l % 16] in FP32).
-// One workgroup per (output J, FBM rows, FBN columns) tile of the tile list.
+// One workgroup of eight waves per (output J, FBM rows, FBN columns) tile of
+// the tile list: waves 0-3 multiply, waves 4-7 gather the next chunk into the
+// other half of the double-buffered LDS tiles meanwhile (each SIMD holds one
+// wave of each kind); every wave passes the same barriers, one per chunk.  A
+// gathering thread builds one stacked column (t % 8) of two fixed rows; its
+// column map entry is read one chunk ahead.  Offsets into the input, output
+// and K stack are 32-bit (the host checks the sizes).
 template <typename T>
-__global__ __launch_bounds__(256) void k_kp_fused(const int4* __restrict__ tiles, const FJ* __restrict__ jd,
+__global__ __launch_bounds__(512) void k_kp_fused(const int4* __restrict__ tiles, const FJ* __restrict__ jd,
                                                   const KTask* __restrict__ tasks, const int32_t* __restrict__ sig,
                                                   const int32_t* __restrict__ colmap,
                                                   const double2* __restrict__ mf, const double* __restrict__ X,
                                                   const double* __restrict__ ib, const T* __restrict__ Kf,
                                                   T* __restrict__ Y) {
   using V = typename FAcc<T>::V;
-  __shared__ T sZ[FBK][FBM];
-  __shared__ T sK[FBK][FBN];
+  constexpr int RU = (FBM * 8) / 256;            // rows per gathering thread
+  constexpr int KU = (FBK * FBN) / 256;          // K values per gathering thread
+  __shared__ T sZ[2][FBK][FBM];
+  __shared__ T sK[2][FBK][FBN];
   __shared__ FT sT[FMAXT];
   const int4 tl = tiles[blockIdx.x];
   const FJ j = jd[tl.x];
-  const int64_t r0 = (int64_t)tl.y * FBM;
+  const int r0 = tl.y * FBM;
   const int n0 = tl.z * FBN;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const int wr = w & 1, wc = w >> 1;
+  const bool prod = w >= 4;
+  const int pt = t & 255;                        // gathering thread index
   for (int q = t; q < j.ntask; q += blockDim.x) {
     const KTask tk = tasks[j.task0 + q];
     FT f;
@@ -276,77 +285,100 @@ __global__ __launch_bounds__(256) void k_kp_fused(const int4* __restrict__ tiles
     f.xoff = tk.xoff;
     sT[q] = f;
   }
-  V acc[4][4];
+  const int jc = pt & 7;
+  const int kk = (jc >> 2) * 8 + (jc & 3);
+  int rb[RU], rm[RU];
+  bool rv[RU];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int c = 0; c < 4; ++c) acc[a][c] = V{0, 0, 0, 0};
+  for (int u = 0; u < RU; ++u) {
+    const int r = r0 + (pt >> 3) + 32 * u;
+    rv[u] = r < j.M;
+    rb[u] = rv[u] ? r / j.dm1 : 0;
+    rm[u] = rv[u] ? r - rb[u] * j.dm1 : 0;
+  }
   __syncthreads();
   const int nchunk = 2 * j.kpad / FBK;
-  for (int ch = 0; ch < nchunk; ++ch) {
-    // Z chunk: complex stacked columns 8 ch .. 8 ch + 7, rows r0 .. r0 + FBM - 1
-#pragma unroll
-    for (int u = 0; u < (FBM * 8) / 256; ++u) {
-      const int e = t + 256 * u;
-      const int jc = e & 7, row = e >> 3;
-      const int64_t r = r0 + row;
-      const int cm = colmap[j.cm_off + 8 * ch + jc];
-      double zr = 0.0, zi = 0.0;
-      if (cm >= 0 && r < j.M) {
-        const FT& f = sT[cm >> 16];
-        const int k2 = cm & 0xffff;
-        const int64_t b = r / j.dm1;
-        const int m1 = (int)(r - b * j.dm1);
-        const int64_t xld = 2 * (int64_t)f.dk2;
-        const double* xb = X + f.xoff + b * f.dm2 * xld + k2;
-        for (int u2 = 0; u2 < f.nsig; ++u2) {
-          const int m2 = m1 - f.J1 - f.sg[u2] + f.J2;
-          if (m2 < 0 || m2 >= f.dm2) continue;
-          const double2 fv = mf[f.mf_off + (int64_t)u2 * j.dm1 + m1];
-          const double xr = xb[m2 * xld], xi = xb[m2 * xld + f.dk2];
-          zr += fv.x * xr - fv.y * xi;
-          zi += fv.x * xi + fv.y * xr;
-        }
-        const double s = ib[b];
-        zr *= s;
-        zi *= s;
-      }
-      const int kk = (jc >> 2) * 8 + (jc & 3);
-      sZ[kk][row] = (T)zr;
-      sZ[kk + 4][row] = (T)zi;
-    }
-    // K chunk: stacked rows 16 ch .. 16 ch + 15, columns n0 .. n0 + FBN - 1
+  int cm_next = (prod && nchunk > 0) ? colmap[j.cm_off + jc] : -1;
+  // gather chunk ch into LDS buffer buf (gathering waves only)
+  auto produce = [&](int ch, int buf) __attribute__((always_inline)) {
+    const int cm = cm_next;
+    if (ch + 1 < nchunk) cm_next = colmap[j.cm_off + 8 * (ch + 1) + jc];
     const T* kr = Kf + j.kf_off + (int64_t)(FBK * ch) * j.Npad + n0;
 #pragma unroll
-    for (int u = 0; u < (FBK * FBN) / 256; ++u) {
-      const int e = t + 256 * u;
-      sK[e / FBN][e % FBN] = kr[(int64_t)(e / FBN) * j.Npad + (e % FBN)];
+    for (int u = 0; u < KU; ++u) {
+      const int e = pt + 256 * u;
+      sK[buf][e / FBN][e % FBN] = kr[(e / FBN) * j.Npad + (e % FBN)];
     }
-    __syncthreads();
+    const bool live = cm >= 0;
+    const FT& f = sT[live ? (cm >> 16) : 0];
+    const int k2 = live ? (cm & 0xffff) : 0;
+    const int xld = 2 * f.dk2;
+    double zre[RU], zim[RU];
 #pragma unroll
-    for (int ks = 0; ks < FBK / 4; ++ks) {
-      const int kq = ks * 4 + (lane >> 4);
-      T av[4], bv[4];
+    for (int u = 0; u < RU; ++u) {
+      zre[u] = 0.0;
+      zim[u] = 0.0;
+    }
 #pragma unroll
-      for (int a = 0; a < 4; ++a) av[a] = sZ[kq][wr * 64 + a * 16 + (lane & 15)];
+    for (int s5 = 0; s5 < 5; ++s5) {
 #pragma unroll
-      for (int c = 0; c < 4; ++c) bv[c] = sK[kq][wc * 64 + c * 16 + (lane & 15)];
+      for (int u = 0; u < RU; ++u) {
+        const int m2 = rm[u] - f.J1 - f.sg[s5] + f.J2;
+        const bool ok = live && rv[u] && s5 < f.nsig && m2 >= 0 && m2 < f.dm2;
+        const int xo = (int)f.xoff + (rb[u] * f.dm2 + (ok ? m2 : 0)) * xld + k2;
+        const double2 fv = ok ? mf[(int)f.mf_off + s5 * j.dm1 + rm[u]] : make_double2(0.0, 0.0);
+        const double xr = ok ? X[xo] : 0.0, xi = ok ? X[xo + f.dk2] : 0.0;
+        zre[u] += fv.x * xr - fv.y * xi;
+        zim[u] += fv.x * xi + fv.y * xr;
+      }
+    }
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
+    for (int u = 0; u < RU; ++u) {
+      const double sc = ib[rb[u]];
+      const int row = (pt >> 3) + 32 * u;
+      sZ[buf][kk][row] = (T)(zre[u] * sc);
+      sZ[buf][kk + 4][row] = (T)(zim[u] * sc);
+    }
+  };
+  V acc[2][4];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) acc[a][c] = FAcc<T>::mfma(av[a], bv[c], acc[a][c]);
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[a][c] = V{0, 0, 0, 0};
+  const int wr = w & 1, wc = (w >> 1) & 1;
+  if (prod && nchunk > 0) produce(0, 0);
+  __syncthreads();
+  for (int ch = 0; ch < nchunk; ++ch) {
+    const int cur = ch & 1;
+    if (prod) {
+      if (ch + 1 < nchunk) produce(ch + 1, cur ^ 1);
+    } else {
+#pragma unroll
+      for (int ks = 0; ks < FBK / 4; ++ks) {
+        const int kq = ks * 4 + (lane >> 4);
+        T av[2], bv[4];
+#pragma unroll
+        for (int a = 0; a < 2; ++a) av[a] = sZ[cur][kq][wr * 32 + a * 16 + (lane & 15)];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) bv[c] = sK[cur][kq][wc * 64 + c * 16 + (lane & 15)];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int c = 0; c < 4; ++c) acc[a][c] = FAcc<T>::mfma(av[a], bv[c], acc[a][c]);
+      }
     }
     __syncthreads();
   }
+  if (prod) return;
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
       const int col = n0 + wc * 64 + c * 16 + (lane & 15);
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int64_t row = r0 + wr * 64 + a * 16 + FAcc<T>::row(lane, v);
-        if (row < j.M && col < j.N) Y[j.yoff + row * j.N + col] = acc[a][c][v];
+        const int row = r0 + wr * 32 + a * 16 + FAcc<T>::row(lane, v);
+        if (row < j.M && col < j.N) Y[j.yoff + (int64_t)row * j.N + col] = acc[a][c][v];
       }
     }
 }
@@ -970,13 +1002,16 @@ __global__ void k_kp_zero(double* __restrict__ x, int64_t n) {
 int kmatvec(rm_kplan* p, const double* X, const double* ib, void* Y, void* Z, int32_t ncols, int64_t ntask,
             const std::vector<int64_t>& zoff_J, bool f32, hipStream_t s) {
   if (p->kernel == RM_KKERNEL_FUSED) {
+    if (2 * p->n * (int64_t)ncols >= INT32_MAX || p->mf_count >= INT32_MAX)
+      return fail(RM_ERR_ARG, "fused matvec: %lld wavepackets of %lld states exceed its 32-bit offsets",
+                  (long long)ncols, (long long)p->n);
     if (p->ntiles > 0) {
       const FJ* jd = (const FJ*)p->d_fz;
       if (f32)
-        hipLaunchKernelGGL(k_kp_fused<float>, dim3((unsigned)p->ntiles), dim3(256), 0, s, p->fz_tiles, jd,
+        hipLaunchKernelGGL(k_kp_fused<float>, dim3((unsigned)p->ntiles), dim3(512), 0, s, p->fz_tiles, jd,
                            p->d_tasks, p->d_sig, p->d_colmap, p->d_mf, X, ib, p->d_kf32, (float*)Y);
       else
-        hipLaunchKernelGGL(k_kp_fused<double>, dim3((unsigned)p->ntiles), dim3(256), 0, s, p->fz_tiles, jd,
+        hipLaunchKernelGGL(k_kp_fused<double>, dim3((unsigned)p->ntiles), dim3(512), 0, s, p->fz_tiles, jd,
                            p->d_tasks, p->d_sig, p->d_colmap, p->d_mf, X, ib, p->d_kf64, (double*)Y);
       HIPCHK(hipGetLastError());
     }
