@@ -179,6 +179,7 @@ __global__ __launch_bounds__(256) void k_kp_z(const KTask* __restrict__ tasks, c
   const double s = ib[b];
   double2* s_mf = (double2*)smem;      // [5][zr]
   double* s_x = smem + 10 * zr;        // [nr + 4][2 dk2]
+  int staged = -1;   // J2 whose rows s_x holds
   for (int64_t q = q0; q < q1; ++q) {
     const KTask tk = tasks[q];
     int sg[5];
@@ -187,9 +188,13 @@ __global__ __launch_bounds__(256) void k_kp_z(const KTask* __restrict__ tasks, c
     const int64_t xld = 2 * (int64_t)tk.dk2;
     const int lo2 = max(0, m0 - tk.J1 + tk.J2 - 2);
     const int hi2 = min(tk.dm2 - 1, m0 + nr - 1 - tk.J1 + tk.J2 + 2);
-    const int64_t nx = hi2 >= lo2 ? (int64_t)(hi2 - lo2 + 1) * xld : 0;
-    const double* src = X + tk.xoff + (b * tk.dm2 + lo2) * xld;
-    for (int64_t i = threadIdx.x; i < nx; i += blockDim.x) s_x[i] = src[i];
+    if (tk.J2 != staged) {
+      // blocks of one J2 are consecutive: their rows are staged once
+      const int64_t nx = hi2 >= lo2 ? (int64_t)(hi2 - lo2 + 1) * xld : 0;
+      const double* src = X + tk.xoff + (b * tk.dm2 + lo2) * xld;
+      for (int64_t i = threadIdx.x; i < nx; i += blockDim.x) s_x[i] = src[i];
+      staged = tk.J2;
+    }
     for (int i = threadIdx.x; i < tk.nsig * nr; i += blockDim.x) {
       const int t = i / nr, r = i % nr;
       s_mf[t * zr + r] = mf[tk.mf_off + (int64_t)t * tk.dm1 + m0 + r];
@@ -685,8 +690,6 @@ extern "C" int rm_kplan_create(int32_t nJ, const int32_t* Js, int32_t nslot, int
         kplan_free(p);
         return fail(RM_ERR_ARG, "rm_kplan_create: block %d has |sigma| > 2", q);
       }
-    p->kq_off[q] = p->ktot[a];
-    p->ktot[a] += (int32_t)jdim(J2[q]);
     p->src[a].push_back(q);
     KBlk bk;
     bk.dm1 = (int32_t)jdim(J1[q]);
@@ -697,6 +700,15 @@ extern "C" int rm_kplan_create(int32_t nJ, const int32_t* Js, int32_t nslot, int
     bk.d_off = d_off[q];
     mfpos += ns * jdim(J1[q]);
     p->h_blk.push_back(bk);
+  }
+  // per output J the source blocks ordered by J2 (those of one J2 share their
+  // input rows in the Z build), stacked in that order
+  for (int a = 0; a < nJ; ++a) {
+    std::stable_sort(p->src[a].begin(), p->src[a].end(), [&](int x, int y) { return J2[x] < J2[y]; });
+    for (int q : p->src[a]) {
+      p->kq_off[q] = p->ktot[a];
+      p->ktot[a] += (int32_t)jdim(J2[q]);
+    }
   }
   p->h_J1.assign(J1, J1 + nblk);
   p->h_J2.assign(J2, J2 + nblk);
