@@ -180,7 +180,7 @@ def asymtop_leg(nsteps, cpu_steps=3):
     return out
 
 
-def kron_leg(nsteps=3, ncols=64, jmax=80, krylov=12, t0_fs=50000.0, kernel="gemm"):
+def kron_leg(nsteps=3, ncols=64, jmax=80, krylov=12, t0_fs=50000.0, kernel="gemm", cpu_matvecs=0):
     """The large asymmetric-top configuration: H2S-like rotor (A, B, C =
     10.36, 9.02, 4.73 cm^-1), J <= jmax with all m (N = 708 561 at J <= 80),
     dipole + full polarizability, circularly polarised XY field chirped
@@ -235,6 +235,20 @@ def kron_leg(nsteps=3, ncols=64, jmax=80, krylov=12, t0_fs=50000.0, kernel="gemm
                      "dense_tflops_effective": tf, "peak_tflops": peak, "frac_of_peak": tf / peak,
                      "norm_err": float((torch.linalg.vector_norm(psi, dim=1) - 1).abs().max())}
         del psi
+    if cpu_matvecs:
+        # CPU sample: the reference's block loop Y[J1] += MF_w X[J2] K_w^T (field.py:1235-1237)
+        # on the same factored operator in numpy, one wavepacket, one thread
+        from threadpoolctl import threadpool_limits
+        fpr = kop.field_products([F[0] for F in centrifuge_fields(t[:1])])
+        x1 = psi0[:1].cpu().numpy()
+        with threadpool_limits(limits=1):
+            t1 = time.perf_counter()
+            for _ in range(cpu_matvecs):
+                kop.matvec(x1, fpr)
+            tmv = (time.perf_counter() - t1) / cpu_matvecs
+        out["cpu_port"] = {"wavepacket_steps_per_s": 1.0 / (krylov * tmv), "matvec_s": tmv, "cores": 1,
+                           "sample": f"{cpu_matvecs} host matvec(s) of one wavepacket (numpy block loop on "
+                                     f"the factored operator, 1 thread); a step is {krylov} matvecs"}
     out["fp32_vs_fp64"] = {"max_abs_amplitude": float((res["fp32"][0] - res["fp64"][0]).abs().max()),
                            "max_abs_cos2": float(np.abs(res["fp32"][1] - res["fp64"][1]).max()),
                            "cos2_fp64_mean": float(res["fp64"][1].mean())}
@@ -437,7 +451,8 @@ def main():
 
         # ---- the large asymmetric-top configuration (64 wavepackets, N = 708 561)
         if args.kron_steps > 0:
-            result["large_top_j80"] = kron_leg(args.kron_steps, kernel=args.kron_kernel)
+            result["large_top_j80"] = kron_leg(args.kron_steps, kernel=args.kron_kernel,
+                                               cpu_matvecs=0 if args.no_cpu else 1)
             print(json.dumps({"progress": "large_top"}), file=sys.stderr, flush=True)
 
         # ---- serving: continuous batching of independent propagation requests

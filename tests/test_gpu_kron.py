@@ -217,3 +217,45 @@ def test_propagate_rejects_bad_arguments(device):
         dp.propagate(psi, fp, 1j, krylov_fixed=0)
     with pytest.raises(_lib.RichmolAmdError):
         dp.propagate(psi, fp, 1j, krylov_fixed=_lib.RM_MAX_KRYLOV + 1)
+
+
+def test_full_size_large_top_properties(device):
+    """The large configuration itself (J <= 80 with all m, N = 708 561), two
+    wavepackets (ground state and a dense random row), where the oracle
+    cannot run: the device operator is Hermitian (<x|Hy> = conj<y|Hx>), two
+    steps keep the norms (1e-12), the fused and stored matvec forms give the
+    same wavepackets (1e-11), and FP32 products stay within 1e-6 of FP64."""
+    import torch
+    from richmol_amd.kron import slot_products
+    kop, bas = large_top(80)
+    assert kop.n == 708561
+    dp = _dev_plan(kop)
+    n = kop.n
+    rng = np.random.default_rng(17)
+    F = centrifuge_fields(np.array([50005.0, 50015.0]))
+    fp = slot_products(kop, F)
+    fpd = torch.from_numpy(fp[0]).to(device)
+    xy = torch.from_numpy(rng.normal(size=(2, n)) + 1j * rng.normal(size=(2, n))).to(device)
+    h = torch.zeros_like(xy)
+    dp.matvec(xy, h, fpd)
+    a = torch.vdot(xy[0], h[1]).item()
+    b = torch.vdot(xy[1], h[0]).item()
+    scale = float(torch.linalg.vector_norm(xy[0]) * torch.linalg.vector_norm(h[1]))
+    assert abs(a - np.conj(b)) <= 1e-12 * scale
+    X = np.zeros((2, n), dtype=np.complex128)
+    X[0, int(np.argmin(kop.energies))] = 1.0
+    X[1] = rng.normal(size=n) + 1j * rng.normal(size=n)
+    X[1] /= np.linalg.norm(X[1])
+    from richmol_amd.tdse import TDSE
+    f = TDSE(t_end=20.0, dt=10.0, t_units="fs").exp_fac
+    ph = dp.phases(f)
+    out = {}
+    for kern, prec in (("gemm", "fp64"), ("fused", "fp64"), ("gemm", "fp32")):
+        dp.set_kernel(kern)
+        psi = torch.from_numpy(X).to(device)
+        dp.propagate(psi, fp, f, phase=ph, krylov_fixed=12, precision=prec)
+        out[(kern, prec)] = psi.cpu().numpy()
+    ref = out[("gemm", "fp64")]
+    assert np.abs(np.linalg.norm(ref, axis=1) - 1.0).max() <= 1e-12
+    assert np.abs(out[("fused", "fp64")] - ref).max() <= 1e-11
+    assert np.abs(out[("gemm", "fp32")] - ref).max() <= 1e-6
