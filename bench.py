@@ -180,7 +180,8 @@ def asymtop_leg(nsteps, cpu_steps=3):
     return out
 
 
-def kron_leg(nsteps=3, ncols=64, jmax=80, krylov=12, t0_fs=50000.0, kernel="gemm", cpu_matvecs=0):
+def kron_leg(nsteps=3, ncols=64, jmax=80, krylov=12, t0_fs=50000.0, kernel="gemm", cpu_matvecs=0,
+             accuracy_steps=0):
     """The large asymmetric-top configuration: H2S-like rotor (A, B, C =
     10.36, 9.02, 4.73 cm^-1), J <= jmax with all m (N = 708 561 at J <= 80),
     dipole + full polarizability, circularly polarised XY field chirped
@@ -249,6 +250,25 @@ def kron_leg(nsteps=3, ncols=64, jmax=80, krylov=12, t0_fs=50000.0, kernel="gemm
         out["cpu_port"] = {"wavepacket_steps_per_s": 1.0 / (krylov * tmv), "matvec_s": tmv, "cores": 1,
                            "sample": f"{cpu_matvecs} host matvec(s) of one wavepacket (numpy block loop on "
                                      f"the factored operator, 1 thread); a step is {krylov} matvecs"}
+    if accuracy_steps:
+        # FP32 products against FP64 over a longer stretch from the same start:
+        # <cos^2 theta> of every wavepacket and the norms every 10 steps
+        ta = t0_fs + (np.arange(accuracy_steps) + 0.5) * 10.0
+        fpa = slot_products(kop, centrifuge_fields(ta))
+        pa = {q: psi0.clone() for q in ("fp64", "fp32")}
+        worst_c2, worst_norm, c2_track = 0.0, 0.0, []
+        for k0 in range(0, accuracy_steps, 10):
+            for q in ("fp64", "fp32"):
+                dp.propagate(pa[q], fpa[k0:k0 + 10], f, phase=ph, krylov_fixed=krylov, precision=q)
+            c64, c32 = cos2(pa["fp64"]), cos2(pa["fp32"])
+            worst_c2 = max(worst_c2, float(np.abs(c32 - c64).max()))
+            worst_norm = max(worst_norm, float((torch.linalg.vector_norm(pa["fp32"], dim=1) - 1).abs().max()))
+            c2_track.append(float(c64.mean()))
+        out["fp32_vs_fp64_long"] = {"steps": accuracy_steps, "max_abs_cos2_any_wavepacket": worst_c2,
+                                    "max_norm_err_fp32": worst_norm,
+                                    "max_abs_amplitude_end": float((pa["fp32"] - pa["fp64"]).abs().max()),
+                                    "cos2_fp64_mean_every_10_steps": c2_track}
+        del pa
     out["fp32_vs_fp64"] = {"max_abs_amplitude": float((res["fp32"][0] - res["fp64"][0]).abs().max()),
                            "max_abs_cos2": float(np.abs(res["fp32"][1] - res["fp64"][1]).max()),
                            "cos2_fp64_mean": float(res["fp64"][1].mean())}
@@ -281,6 +301,8 @@ def main():
                     help="only the asymmetric-top leg (for profiling its kernel) and exit")
     ap.add_argument("--kron-steps", type=int, default=3,
                     help="timed steps of the large asymmetric-top (Kronecker-block) leg (0: skip)")
+    ap.add_argument("--kron-accuracy-steps", type=int, default=60,
+                    help="steps of the large-top FP32-vs-FP64 comparison (0: skip)")
     ap.add_argument("--kron-kernel", default="gemm", choices=["gemm", "fused"],
                     help="matvec form of the large asymmetric-top leg")
     ap.add_argument("--kron-only", action="store_true",
@@ -301,7 +323,8 @@ def main():
                           "copies": 3}))
         return
     if args.kron_only:
-        print(json.dumps({"large_top_j80": kron_leg(args.kron_steps, kernel=args.kron_kernel)}), flush=True)
+        print(json.dumps({"large_top_j80": kron_leg(args.kron_steps, kernel=args.kron_kernel,
+                                                    accuracy_steps=args.kron_accuracy_steps)}), flush=True)
         return
     if args.asym_only:
         print(json.dumps({"asymtop_water_j20": asymtop_leg(args.asym_steps, cpu_steps=0)}), flush=True)
@@ -452,7 +475,8 @@ def main():
         # ---- the large asymmetric-top configuration (64 wavepackets, N = 708 561)
         if args.kron_steps > 0:
             result["large_top_j80"] = kron_leg(args.kron_steps, kernel=args.kron_kernel,
-                                               cpu_matvecs=0 if args.no_cpu else 1)
+                                               cpu_matvecs=0 if args.no_cpu else 1,
+                                               accuracy_steps=args.kron_accuracy_steps)
             print(json.dumps({"progress": "large_top"}), file=sys.stderr, flush=True)
 
         # ---- serving: continuous batching of independent propagation requests
