@@ -299,6 +299,33 @@ __device__ __forceinline__ void wave_sync() {
 // ---------------------------------------------------------------------------
 // small dense Krylov pieces, run by the 64 lanes of one wave
 
+// 1/j for the Taylor terms (j = 1..TAYLOR_TERMS; rounded at compile time), so
+// a term costs a multiplication instead of a division on the serial chain.
+__device__ __forceinline__ double taylor_rinv(int j) {
+  constexpr double r[TAYLOR_TERMS + 1] = {0.0,        1.0,        1.0 / 2,  1.0 / 3,  1.0 / 4,
+                                          1.0 / 5,    1.0 / 6,    1.0 / 7,  1.0 / 8,  1.0 / 9,
+                                          1.0 / 10,   1.0 / 11,   1.0 / 12, 1.0 / 13, 1.0 / 14,
+                                          1.0 / 15,   1.0 / 16,   1.0 / 17, 1.0 / 18};
+  return r[j];
+}
+
+// Taylor terms for a scaled norm x <= 1/2: the smallest j with x^j / j! < 2^-60,
+// from the increasing thresholds x_j = (2^-60 j!)^(1/j) (x^j / j! < 2^-60 iff
+// x < x_j), at most TAYLOR_TERMS -- independent comparisons instead of a
+// chain of divisions.
+__device__ __forceinline__ int taylor_terms(double x) {
+  constexpr double thr[TAYLOR_TERMS - 1] = {
+      0x1.0000000000008p-60, 0x1.6a09e667f3bd8p-30, 0x1.d12ed0af1a27ep-20, 0x1.1b4f819c2ff7fp-14,
+      0x1.4d763f00a146ep-11, 0x1.7f34ae136c8e5p-9,  0x1.23257ccb0c5fcp-7,  0x1.54b5ca9e6358bp-6,
+      0x1.4e681294f9348p-5,  0x1.21d6b0d9fac89p-4,  0x1.ca8e511edefb5p-4,  0x1.527c8d13037b4p-3,
+      0x1.d98d19e3bfb28p-3,  0x1.3d6a8eb921352p-2,  0x1.9b19641f27bf7p-2,  0x1.02ca0b1cb9cc0p-1,
+      0x1.3e3dbd2d3b2f7p-1};
+  int n = 1;
+#pragma unroll
+  for (int j = 0; j < TAYLOR_TERMS - 1; ++j) n += x >= thr[j] ? 1 : 0;
+  return n;
+}
+
 // c = exp(f*T) e_0 for the s x s tridiagonal T (diag alpha[0..s-1], off-diagonal
 // beta[1..s-1], T[i-1,i] = T[i,i-1] = beta[i]).  Lane l owns rows l and l+64
 // (s <= 128).  Scaling and stepping: exp(A)e0 = (exp(A/nsub))^nsub e0 with
@@ -323,7 +350,7 @@ static __device__ void expm_e0(int s, const double2* alpha, const double* beta, 
   }
   double norm1 = wave_max(col1);
   int nsub = 1;
-  if (norm1 > 0.5) nsub = (int)ceil(norm1 / 0.5);
+  if (norm1 > 0.5) nsub = (int)ceil(2.0 * norm1);
   if (!(nsub >= 1) || nsub > (1 << 20)) nsub = 1;   // NaN / overflow: result becomes NaN anyway
   const double h = 1.0 / (double)nsub;
   double2 y[2];
@@ -342,7 +369,7 @@ static __device__ void expm_e0(int s, const double2* alpha, const double* beta, 
       double2 prev1 = (l == 0) ? t0_dn /* lane 63 row 63 */ : t1_dn;
       double2 next0 = (l == 63) ? t1_up /* lane 0 row 64 */ : t0_up;
       double2 next1 = (l == 63) ? make_double2(0.0, 0.0) : t1_up;
-      const double sc = h / (double)j;
+      const double sc = h * taylor_rinv(j);
       double2 n0 = cmul(a[0], term[0]);
       n0 = cfma(bl[0], prev0, n0);
       n0 = cfma(bu[0], next0, n0);
@@ -403,21 +430,10 @@ static __device__ void expm_e0_16(int s, const double2* alpha, const double* bet
   }
   const double norm1 = row_max16(col1);
   int nsub = 1;
-  if (norm1 > 0.5) nsub = (int)ceil(norm1 / 0.5);
+  if (norm1 > 0.5) nsub = (int)ceil(2.0 * norm1);
   if (!(nsub >= 1) || nsub > (1 << 20)) nsub = 1;
   const double h = 1.0 / (double)nsub;
-  int nterm = TAYLOR_TERMS;
-  {
-    const double x = h * norm1;
-    double t = 1.0;
-    for (int j = 1; j <= TAYLOR_TERMS; ++j) {
-      t *= x / (double)j;
-      if (t < 0x1p-60) {
-        nterm = j;
-        break;
-      }
-    }
-  }
+  const int nterm = taylor_terms(h * norm1);
   double2 y = make_double2(i == 0 ? 1.0 : 0.0, 0.0);
   for (int sub = 0; sub < nsub; ++sub) {
     double2 term = y, acc = y;
@@ -427,7 +443,7 @@ static __device__ void expm_e0_16(int s, const double2* alpha, const double* bet
       double2 nt = cmul(a, term);
       nt = cfma(bl, prev, nt);
       nt = cfma(bu, next, nt);
-      term = cscale(h / (double)j, nt);
+      term = cscale(h * taylor_rinv(j), nt);
       acc = cadd(acc, term);
     }
     y = acc;
