@@ -163,6 +163,94 @@ class KronOperator:
             e = np.concatenate([basis.energy[J][s] for s in basis.symlist[J]])
             self.energies[off[J]:off[J] + (2 * J + 1) ** 2] = np.tile(e, 2 * J + 1)
 
+    @classmethod
+    def from_cartens(cls, H):
+        """The factored operator of a CarTens leaf or sum as richmol stores it
+        (kmat[(J1, J2)][(s1, s2)][w] K blocks per species pair, mmat[...][w][cart]
+        M matrices, field.py:58-170): per (leaf, J1, J2, w) the species' K
+        blocks assembled over the concatenated k of each J (symlist order) and
+        the M matrix as its diagonals m1 - m2 = sigma.  Needs every J with all
+        m = -J..J and the full k space (2J + 1 states per m); returns the
+        operator and the per-leaf field-product dicts of H's terms."""
+        terms = H.terms()
+        leaf0 = terms[0][0]
+        Jlist = list(leaf0.Jlist1)
+        if list(leaf0.Jlist2) != Jlist:
+            raise ValueError("the Kronecker-block form needs the same basis on both sides")
+        sym = {J: list(leaf0.symlist1[J]) for J in Jlist}
+        dkc = {}
+        for J in Jlist:
+            dms = {leaf0.dim_m1[J][s] for s in sym[J]}
+            dkc[J] = np.concatenate([[0], np.cumsum([leaf0.dim_k1[J][s] for s in sym[J]])]).astype(int)
+            if dms != {2 * J + 1} or dkc[J][-1] != 2 * J + 1:
+                raise ValueError("the Kronecker-block form needs every J with all m and the full k space")
+        self = cls.__new__(cls)
+        self.Jlist = Jlist
+        self.carts, self.ranks, self.blocks = [], [], []
+        fprods = []
+        for li, (leaf, fprod, thr) in enumerate(terms):
+            if fprod is None:
+                raise AttributeError("tensor must be multiplied with a field before it is applied")
+            if thr is not None:
+                raise ValueError("field thresholds (CarTens.field(F, thresh)) are not supported by the "
+                                 "Kronecker-block form")
+            carts = list(leaf.cart)
+            self.carts.append(carts)
+            self.ranks.append(leaf.rank)
+            fprods.append(fprod)
+            for (J1, J2), blk in leaf.kmat.items():
+                ws = sorted({w for d in blk.values() for w in d})
+                for w in ws:
+                    K = np.zeros((2 * J1 + 1, 2 * J2 + 1), dtype=np.complex128)
+                    Mw = None
+                    for (s1, s2), d in blk.items():
+                        if w not in d:
+                            continue
+                        i1, i2 = sym[J1].index(s1), sym[J2].index(s2)
+                        K[dkc[J1][i1]:dkc[J1][i1 + 1], dkc[J2][i2]:dkc[J2][i2 + 1]] = d[w].toarray()
+                        if Mw is None:
+                            Mw = leaf.mmat[(J1, J2)][(s1, s2)][w]
+                    if not np.any(K) or not Mw:
+                        continue
+                    md = {}
+                    for ci, c in enumerate(carts):
+                        if c not in Mw:
+                            continue
+                        M = Mw[c].tocoo()
+                        sig = (M.row - J1) - (M.col - J2)
+                        if np.any(np.abs(sig) > 2):
+                            raise ValueError("M blocks with |m1 - m2| > 2: not a rank <= 2 tensor in m order")
+                        for sv in np.unique(sig):
+                            d = md.setdefault(int(sv), np.zeros((len(carts), 2 * J1 + 1), dtype=np.complex128))
+                            sel = sig == sv
+                            d[ci, M.row[sel]] += M.data[sel]
+                    md = {sv: d for sv, d in md.items() if np.any(d)}
+                    if md:
+                        self.blocks.append((li, J1, J2, w, K, md))
+        off, pos = {}, 0
+        for J in Jlist:
+            off[J] = pos
+            pos += (2 * J + 1) ** 2
+        self.off, self.n = off, pos
+        perm = np.empty(pos, dtype=np.int64)
+        full = 0
+        for J in Jlist:
+            dm = 2 * J + 1
+            for i, s in enumerate(sym[J]):
+                dks = dkc[J][i + 1] - dkc[J][i]
+                im, ik = np.meshgrid(np.arange(dm), np.arange(dks), indexing="ij")
+                perm[full:full + dm * dks] = (off[J] + im * dm + dkc[J][i] + ik).ravel()
+                full += dm * dks
+        self.perm = perm
+        self.energies = None
+        return self, fprods
+
+    def slot_row(self, fprods):
+        """Field products of one step in slot order from per-leaf dicts
+        (CarTens.field products; a missing label is 0)."""
+        return np.array([complex(fp.get(c, 0.0)) for fp, carts in zip(fprods, self.carts) for c in carts],
+                        dtype=np.complex128)
+
     def field_products(self, fields):
         """Per leaf {cart: product of field components} for F (3,) per leaf."""
         out = []

@@ -151,7 +151,13 @@ class TDSE:
         operator), matvec_lib ('hip'), propag ('internal' = the reference's
         adaptive Lanczos, here on the device), tol (default 1e-15).
         Device extensions: maxorder (default 100), krylov_fixed (fixed
-        Krylov dimension, default 0 = adaptive), stream (torch stream).
+        Krylov dimension, default 0 = adaptive), stream (torch stream),
+        operator_form ('auto' (default), 'flat' or 'kron'): 'kron' keeps the
+        operator factored as richmol stores it (Kronecker-block plan; needs
+        every J with all m and the full k space, a fixed Krylov dimension and
+        no field thresholds), 'auto' takes it for such bases of at least
+        KRON_MIN_STATES states; precision ('fp64', or 'fp32' dense products,
+        Kronecker-block form only).
         """
         from . import plan as _plan
         import torch
@@ -185,6 +191,10 @@ class TDSE:
         maxorder = int(kwargs.get("maxorder", 100))
         krylov_fixed = int(kwargs.get("krylov_fixed", 0))
         stream = kwargs.get("stream", None)
+        operator_form = kwargs.get("operator_form", "auto")
+        if operator_form not in ("auto", "flat", "kron"):
+            raise ValueError(f"operator_form has bad value: '{operator_form}' (use 'auto', 'flat', 'kron')")
+        precision = kwargs.get("precision", "fp64")
 
         f = self.exp_fac
         on_device = isinstance(vecs, torch.Tensor)
@@ -200,11 +210,21 @@ class TDSE:
                 raise ValueError("vecs must be a 2-D array of wavepacket rows")
             psi = torch.from_numpy(np.ascontiguousarray(arr, dtype=np.complex128)).cuda()
 
+        kron = None
         if H is None or (not H.is_sum and "_fprod" not in H.__dict__):
             if H0 is None:
                 raise AttributeError("you need to multiply tensor with field before applying it to a vector")
             plan = None
+        elif operator_form == "kron" or (operator_form == "auto" and psi.shape[1] >= KRON_MIN_STATES
+                                         and _kron_eligible(H)):
+            if krylov_fixed <= 0:
+                raise NotImplementedError("the Kronecker-block form propagates with a fixed Krylov dimension: "
+                                          "pass krylov_fixed=m (the adaptive rule needs the flat form)")
+            kron = _kron_plan(H)
+            plan = None
         else:
+            if precision != "fp64":
+                raise ValueError("precision applies to the Kronecker-block form; use Ensemble for FP32 matvecs")
             plan = _plan.plan_for(H)
             # plans are shared with ensembles that may have picked another kernel
             if plan.kernel != "auto":
@@ -226,7 +246,21 @@ class TDSE:
 
         out = torch.empty_like(psi)
         ncols = psi.shape[0]
-        if plan is None:
+        if kron is not None:
+            kop, dp, perm = kron
+            if psi.shape[1] != kop.n:
+                raise ValueError("wavepacket length does not match the operator's basis")
+            row = kop.slot_row([fp for _, fp, _ in H.terms()])[None, :]
+            work = torch.empty_like(psi)
+            work[:, perm] = psi
+            ph = None
+            if phase is not None:
+                ph = torch.empty_like(phase)
+                ph[perm] = phase
+            dp.propagate(work, row, f, phase=ph, krylov_fixed=krylov_fixed, precision=precision, stream=stream)
+            out = work[:, perm]
+            self.last_krylov_dims = torch.full((ncols,), krylov_fixed, dtype=torch.int32, device=psi.device)
+        elif plan is None:
             # no field on the operator: only the two H0 phases (tdse.py:375, 377, 394)
             _phase_apply(psi, out, phase, stream)
         else:
@@ -241,6 +275,45 @@ class TDSE:
         if on_device:
             return out
         return out.cpu().numpy()
+
+
+# 'auto' operator form: bases at least this large with every J carrying all m
+# and the full k space keep the operator factored (a flattened asymmetric top
+# at J <= 80 would hold ~3e9 entries)
+KRON_MIN_STATES = 150_000
+_KRON_CACHE = {}
+
+
+def _kron_eligible(H):
+    leaf = H.terms()[0][0]
+    try:
+        if list(leaf.Jlist1) != list(leaf.Jlist2):
+            return False
+        for J in leaf.Jlist1:
+            syms = leaf.symlist1[J]
+            if any(leaf.dim_m1[J][s] != 2 * J + 1 for s in syms):
+                return False
+            if sum(leaf.dim_k1[J][s] for s in syms) != 2 * J + 1:
+                return False
+    except (AttributeError, KeyError):
+        return False
+    return True
+
+
+def _kron_plan(H):
+    """(KronOperator, DeviceKronPlan, device permutation) for H's leaf
+    structure, built once per set of leaves (the field only changes the
+    per-step products)."""
+    import torch
+    from .kron import DeviceKronPlan, KronOperator
+    key = tuple(leaf._uid for leaf, _, _ in H.terms())
+    hit = _KRON_CACHE.get(key)
+    if hit is None:
+        kop, _ = KronOperator.from_cartens(H)
+        hit = (kop, DeviceKronPlan(kop), torch.from_numpy(kop.perm).cuda())
+        _KRON_CACHE.clear()
+        _KRON_CACHE[key] = hit
+    return hit
 
 
 def _phase_apply(psi, out, phase, stream=None):

@@ -259,3 +259,33 @@ def test_full_size_large_top_properties(device):
     assert np.abs(np.linalg.norm(ref, axis=1) - 1.0).max() <= 1e-12
     assert np.abs(out[("fused", "fp64")] - ref).max() <= 1e-11
     assert np.abs(out[("gemm", "fp32")] - ref).max() <= 1e-6
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32", 2e-6)])
+def test_tdse_update_kronecker_form(device, precision, tol):
+    """The drop-in call: richmol's TDSE.update(H, vecs, H0=...) with H the sum
+    of two field-multiplied CarTens (dipole, full polarizability) routed to
+    the Kronecker-block plan (operator_form='kron'; 'auto' does so from
+    150 000 states): rows in richmol's order in and out, four steps against
+    the oracle's update on the same tensors."""
+    from richmol_amd import rot
+    from richmol_amd.tdse import TDSE
+    kop, bas = large_top(5)
+    tens, scales = large_top_tensors()
+    T = [rot.tensor(t, bas) * s for t, s in zip(tens, scales)]
+    h0 = rot.hamiltonian(bas)
+    dt = 10.0
+    t = 40000.0 + (np.arange(4) + 0.5) * dt
+    fields = centrifuge_fields(t, e0=3e9)
+    X = _columns(kop.n, 23)[:3]
+    ref, f = _oracle_run(bas, fields, X, 12, dt)
+    td = TDSE(t_start=0.0, t_end=dt * 4, dt=dt, t_units="fs")
+    td.time_grid()
+    psi = X.copy()
+    for i in range(4):
+        for Ti, F in zip(T, fields):
+            Ti.field(list(F[i]))
+        psi, _ = td.update(T[0] + T[1], psi, H0=h0, krylov_fixed=12, operator_form="kron", precision=precision)
+    assert np.abs(psi - ref).max() <= tol
+    with pytest.raises(NotImplementedError):
+        td.update(T[0] + T[1], psi, H0=h0, operator_form="kron")

@@ -87,3 +87,40 @@ def test_slot_products_order():
     assert fp.shape == (1, sum(len(c) for c in kop.carts))
     vals = dict(zip([f"{li}:{c}" for li, cs in enumerate(kop.carts) for c in cs], fp[0].real))
     assert vals["0:z"] == 5.0 and vals["1:xy"] == 77.0 and vals["1:zz"] == 169.0
+
+
+def test_from_cartens_equals_builder():
+    """The factored form read from richmol-style CarTens blocks (a sum of a
+    dipole and a full polarizability, each multiplied with its field) equals
+    the builder's factored operator: same matvec on random rows (1e-13
+    relative), same layout and permutation."""
+    from richmol_amd.field import CarTens  # noqa: F401
+    kop, bas = large_top(5)
+    tens, scales = large_top_tensors()
+    T = [rot.tensor(t, bas) * s for t, s in zip(tens, scales)]
+    F = [(0.3e8, -0.8e8, 0.5e8), (1e8, 0.2e8, -0.4e8)]
+    for t, f in zip(T, F):
+        t.field(list(f))
+    H = T[0] + T[1]
+    kc, fprods = kron.KronOperator.from_cartens(H)
+    assert kc.n == kop.n and np.array_equal(kc.perm, kop.perm)
+    rng = np.random.default_rng(9)
+    X = rng.normal(size=(2, kop.n)) + 1j * rng.normal(size=(2, kop.n))
+    a = kop.matvec(X, kop.field_products(F))
+    fp_c = [{c: fp.get(c, 0.0) for c in carts} for fp, carts in zip(fprods, kc.carts)]
+    b = kc.matvec(X, fp_c)
+    assert np.abs(a - b).max() <= 1e-13 * np.abs(a).max()
+    assert np.allclose(kc.slot_row(fprods), kron.slot_products(kop, [np.array([f]) for f in F])[0])
+
+
+def test_from_cartens_refuses_thresholds_and_partial_m():
+    kop, bas = large_top(2)
+    t = rot.tensor(np.array([0.0, 0.0, 1.0]), bas)
+    t.field([0, 0, 1.0], thresh=1e-3)
+    with pytest.raises(ValueError):
+        kron.KronOperator.from_cartens(t)
+    b0 = rot.asymtop_basis(4.73, 10.36, 9.02, 2, m_filter=lambda J, m: m == 0)
+    t0 = rot.tensor(np.array([0.0, 0.0, 1.0]), b0)
+    t0.field([0, 0, 1.0])
+    with pytest.raises(ValueError):
+        kron.KronOperator.from_cartens(t0)
