@@ -306,18 +306,26 @@ int rm_kplan_matvec(rm_kplan* plan, const void* x, void* y, int64_t ld, int32_t 
                     const double* fprod_dev, int32_t precision, void* stream);
 /*
  * nsteps steps of TDSE.update (tdse.py:365-414) for ncols wavepackets (rows
- * of psi, in place) with a fixed Krylov dimension krylov_fixed
- * (tdse.py:417-486; an exact breakdown continues with a zero vector, whose
- * coefficient in exp(fT)e0 is zero as the reference's restart vector's is).
+ * of psi, in place).  krylov_fixed > 0: that fixed Krylov dimension;
+ * krylov_fixed == 0: the reference's adaptive rule (tdse.py:417-486) with
+ * maxorder (<= RM_MAX_KRYLOV) and tol, every wavepacket stopping at its own
+ * dimension (the batch goes on until all have stopped; one device-to-host
+ * read of a counter per Krylov iteration).  An exact breakdown continues with
+ * a zero vector whose coefficient in exp(fT)e0 is zero, as the reference's
+ * restart vector's is (and whose Gram column is the restart vector's).
  * fprod: HOST, nsteps x nslot complex; a step whose products are all zero
  * applies the phases only (the reference skips the Lanczos when the field
  * leaves no MF block).  phase: device, n complex, internal order (from
- * rm_phase_init) or NULL.  RM_PREC_FP32: the dense products in FP32; the
- * recurrence, norms, exp(fT)e0, phases and the wavepackets stay FP64.
+ * rm_phase_init) or NULL.  kused: device int32[nsteps x ncols] or NULL
+ * (dimension per step and wavepacket, 0 on phase-only steps); status: device
+ * int32[ncols] or NULL, zeroed, RM_NOT_CONVERGED where a step reached
+ * maxorder.  RM_PREC_FP32: the dense products in FP32; the recurrence,
+ * norms, exp(fT)e0, phases and the wavepackets stay FP64.
  */
 int rm_kplan_propagate(rm_kplan* plan, void* psi, int64_t ld, int32_t ncols, int64_t nsteps,
                        const double* fprod, double f_re, double f_im, const void* phase,
-                       int32_t krylov_fixed, int32_t precision, void* stream);
+                       int32_t krylov_fixed, int32_t maxorder, double tol, int32_t* kused,
+                       int32_t* status, int32_t precision, void* stream);
 
 #ifdef __cplusplus
 }

@@ -64,7 +64,7 @@ SIGNATURES = {
                                        _vp]),
     "rm_kplan_propagate": (ctypes.c_int, [_vp, _vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int64, _c_dblp,
                                           ctypes.c_double, ctypes.c_double, _vp, ctypes.c_int32,
-                                          ctypes.c_int32, _vp]),
+                                          ctypes.c_int32, ctypes.c_double, _vp, _vp, ctypes.c_int32, _vp]),
 }
 
 

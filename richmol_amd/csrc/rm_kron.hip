@@ -461,7 +461,8 @@ __global__ __launch_bounds__(256) void k_kp_upd(const int64_t* __restrict__ voff
 // alpha (complex) -> alpha[b][it]; mode 1: |W|^2 -> beta[b][it+1] and ib.
 __global__ void k_kp_fin(const double* __restrict__ part, int32_t nJ, int32_t ncols, int32_t mode, int32_t it,
                          int32_t kcap, double2* __restrict__ alpha, double* __restrict__ beta,
-                         double* __restrict__ ib) {
+                         double* __restrict__ ib, double2* __restrict__ G, double2* __restrict__ cb,
+                         int32_t* __restrict__ kdone) {
   const int b = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (b >= ncols) return;
@@ -470,7 +471,7 @@ __global__ void k_kp_fin(const double* __restrict__ part, int32_t nJ, int32_t nc
     if (mode == 0) {
       s0 += part[((int64_t)b * nJ + j) * 2];
       s1 += part[((int64_t)b * nJ + j) * 2 + 1];
-    } else {
+    } else {   // modes 1, 2: one partial per chunk
       s0 += part[(int64_t)b * nJ + j];
     }
   }
@@ -479,6 +480,11 @@ __global__ void k_kp_fin(const double* __restrict__ part, int32_t nJ, int32_t nc
   if (lane != 0) return;
   if (mode == 0) {
     alpha[(int64_t)b * kcap + it] = make_double2(s0, s1);
+  } else if (mode == 2) {
+    // adaptive rule, start of a step: G_00 = |V_0|^2 (V_0 unnormalised), c_prev = [1]
+    G[(int64_t)b * kcap * kcap] = make_double2(s0, 0.0);
+    cb[(int64_t)b * 2 * kcap + kcap] = make_double2(1.0, 0.0);
+    kdone[b] = 0;
   } else {
     const double be = sqrt(s0);
     beta[(int64_t)b * (kcap + 1) + it + 1] = be;
@@ -488,19 +494,31 @@ __global__ void k_kp_fin(const double* __restrict__ part, int32_t nJ, int32_t nc
 }
 
 // Start of a step: V_0 = P S (the phases of the split operator), W = V_0, ib = 1.
+// (adaptive rule: part != NULL receives the chunks' |V_0|^2 partials)
 __global__ void k_kp_start(const int64_t* __restrict__ voff, const int32_t* __restrict__ Js,
                            const double2* __restrict__ ph, const double* __restrict__ S, double* __restrict__ W,
-                           double* __restrict__ ib, double* __restrict__ beta, int32_t kcap) {
+                           double* __restrict__ ib, double* __restrict__ beta, int32_t kcap,
+                           double* __restrict__ part, int32_t nJ) {
   const int j = blockIdx.x, b = blockIdx.y;
   const int64_t d = jdim(Js[j]), len = 2 * d * d;
   const int64_t base = voff[j] * 2 * gridDim.y + (int64_t)b * len;
+  double acc = 0.0;
   for (int64_t i = threadIdx.x; i < d * d; i += blockDim.x) {
     const int64_t row = i / d, k = i % d;
     const int64_t pr = base + row * 2 * d + k, pi = pr + d;
     const double2 p = ph ? ph[voff[j] + i] : make_double2(1.0, 0.0);
     const double sr = S[pr], si = S[pi];
-    W[pr] = p.x * sr - p.y * si;
-    W[pi] = p.x * si + p.y * sr;
+    const double wr = p.x * sr - p.y * si, wi = p.x * si + p.y * sr;
+    W[pr] = wr;
+    W[pi] = wi;
+    acc += wr * wr + wi * wi;
+  }
+  if (part) {
+    __shared__ double red[4];
+    acc = wave_allsum(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) part[(int64_t)b * nJ + j] = (red[0] + red[1]) + (red[2] + red[3]);
   }
   if (j == 0 && threadIdx.x == 0) {
     ib[b] = 1.0;
@@ -546,11 +564,74 @@ __global__ __launch_bounds__(64) void k_kp_expm(const double2* __restrict__ alph
   for (int i = l; i < m; i += 64) coef[(int64_t)b * kcap + i] = s_c[i];
 }
 
-// S = P sum_j c_j V_j
+// Adaptive rule after Krylov vector k (tdse.py:466-476), one wave per
+// wavepacket still running: Gram column k, c = exp(f T_{k+1}) e0 and the
+// change conv = |u_k - u_{k-1}|^2 (adaptive_metric); the wavepacket stops at
+// dimension k + 1 when conv <= tol or k + 1 == maxorder (then status gets
+// RM_NOT_CONVERGED), keeping c as its coefficients.  After an
+// exact breakdown (beta_k == 0) the zero vector that stands for the
+// reference's restart vector gets the restart vector's Gram column e_k.
+// Reaching maxorder sets RM_NOT_CONVERGED (the reference raises whenever its
+// final dimension equals maxorder, tdse.py:480-484).
+// running: incremented by every wavepacket that goes on.
+__global__ __launch_bounds__(64) void k_kp_conv(const double2* __restrict__ alpha, const double* __restrict__ beta,
+                                                int32_t kcap, int32_t k, double2 f, double tol, int32_t maxorder,
+                                                double2* __restrict__ G, double2* __restrict__ cb,
+                                                int32_t* __restrict__ kdone, double2* __restrict__ coef,
+                                                int32_t* __restrict__ status, int32_t* __restrict__ running) {
+  __shared__ double2 s_a[KMAX];
+  __shared__ double s_b[KMAX + 1];
+  const int b = blockIdx.x, l = threadIdx.x;
+  if (kdone[b] != 0) return;
+  for (int i = l; i <= k; i += 64) {
+    s_a[i] = alpha[(int64_t)b * kcap + i];
+    s_b[i] = beta[(int64_t)b * (kcap + 1) + i];
+  }
+  double2* Gb = G + (int64_t)b * kcap * kcap;
+  double2* c = cb + (int64_t)b * 2 * kcap;
+  const bool brk = !(s_b[k] != 0.0);
+  if (brk)
+    for (int i = l; i < k; i += 64) {
+      Gb[i * kcap + k] = make_double2(0.0, 0.0);
+      Gb[k * kcap + i] = make_double2(0.0, 0.0);
+    }
+  wave_sync();
+  const double conv = adaptive_metric<false>(k, brk, Gb, kcap, s_a, s_b, f, c, c + kcap);
+  wave_sync();
+  const bool stop = !(k + 1 < maxorder && conv > tol);
+  if (stop) {
+    for (int i = l; i <= k; i += 64) coef[(int64_t)b * kcap + i] = c[kcap + i];
+    if (l == 0) {
+      kdone[b] = k + 1;
+      if (k + 1 >= maxorder && status) status[b] |= RM_NOT_CONVERGED;
+    }
+  } else if (l == 0) {
+    atomicAdd(running, 1);
+  }
+}
+
+__global__ void k_kp_record(const int32_t* __restrict__ kdone, int32_t ncols, int32_t* __restrict__ kused) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < ncols) kused[b] = kdone[b];
+}
+
+__global__ void k_kp_izero(int32_t* __restrict__ x, int32_t n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = 0;
+}
+
+__global__ void k_kp_fill_i(int32_t* __restrict__ x, int32_t n, int32_t v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = v;
+}
+
+// S = P sum_j c_j V_j (kf != NULL: wavepacket b uses its own dimension kf[b])
 __global__ void k_kp_comb(const int64_t* __restrict__ voff, const int32_t* __restrict__ Js,
                           const double2* __restrict__ ph, const double* __restrict__ V, int64_t vstride,
-                          const double2* __restrict__ coef, int32_t m, int32_t kcap, double* __restrict__ S) {
+                          const double2* __restrict__ coef, int32_t m, int32_t kcap, double* __restrict__ S,
+                          const int32_t* __restrict__ kf) {
   const int j = blockIdx.x, b = blockIdx.y;
+  if (kf) m = kf[b];
   const int64_t d = jdim(Js[j]), len = 2 * d * d;
   const int64_t base = voff[j] * 2 * gridDim.y + (int64_t)b * len;
   for (int64_t i = threadIdx.x; i < d * d; i += blockDim.x) {
@@ -858,6 +939,8 @@ namespace {
 
 struct KWork {
   double *S, *W, *V, *ib, *part, *beta;
+  double2 *G, *cb;          // adaptive rule: Gram matrices, (c, c_prev) per wavepacket
+  int32_t *kdone, *running;
   double2 *alpha, *coef;
   void *Y, *Z;
   int64_t vreal;   // reals per vector (all wavepackets)
@@ -865,8 +948,9 @@ struct KWork {
 
 size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-// Workspace for ncols wavepackets and Krylov dimension m (m = 0: a matvec only).
-int kwork(rm_kplan* p, int32_t ncols, int32_t m, bool f32, KWork* w) {
+// Workspace for ncols wavepackets and Krylov dimension m (m = 0: a matvec only);
+// adaptive: the rule's Gram matrices and coefficient buffers too.
+int kwork(rm_kplan* p, int32_t ncols, int32_t m, bool f32, KWork* w, bool adaptive = false) {
   const int64_t vreal = 2 * p->n * ncols;
   int64_t zreal = 0;
   if (p->kernel != RM_KKERNEL_FUSED)
@@ -879,7 +963,10 @@ int kwork(rm_kplan* p, int32_t ncols, int32_t m, bool f32, KWork* w) {
   const size_t b_part = al256(sizeof(double) * 2 * (size_t)ncols * p->nJ);
   const size_t b_beta = al256(sizeof(double) * (size_t)ncols * (mm + 1));
   const size_t b_al = al256(sizeof(double2) * (size_t)ncols * mm);
-  const size_t need = b_vec * (2 + mm) + b_y + b_z + b_col + b_part + b_beta + 2 * b_al;
+  const size_t b_g = adaptive ? al256(sizeof(double2) * (size_t)ncols * mm * mm) : 0;
+  const size_t b_cb = adaptive ? al256(sizeof(double2) * (size_t)ncols * 2 * mm) : 0;
+  const size_t b_kd = adaptive ? al256(sizeof(int32_t) * ((size_t)ncols + 1)) : 0;
+  const size_t need = b_vec * (2 + mm) + b_y + b_z + b_col + b_part + b_beta + 2 * b_al + b_g + b_cb + b_kd;
   if (p->ws_bytes < need) {
     if (p->ws) {
       HIPCHK(hipDeviceSynchronize());
@@ -901,6 +988,13 @@ int kwork(rm_kplan* p, int32_t ncols, int32_t m, bool f32, KWork* w) {
   w->beta = (double*)c; c += b_beta;
   w->alpha = (double2*)c; c += b_al;
   w->coef = (double2*)c; c += b_al;
+  w->G = adaptive ? (double2*)c : nullptr;
+  c += b_g;
+  w->cb = adaptive ? (double2*)c : nullptr;
+  c += b_cb;
+  w->kdone = adaptive ? (int32_t*)c : nullptr;
+  w->running = adaptive ? (int32_t*)c + ncols : nullptr;
+  c += b_kd;
   w->vreal = vreal;
   return RM_OK;
 }
@@ -1100,7 +1194,7 @@ extern "C" int rm_kplan_matvec(rm_kplan* p, const void* x, void* y, int64_t ld, 
   hipLaunchKernelGGL(k_kp_pack, chunk_grid(p, ncols), dim3(256), 0, s, p->d_voff, p->d_Js, (const double2*)x, ld,
                      w.S, 1);
   hipLaunchKernelGGL(k_kp_start, chunk_grid(p, ncols), dim3(256), 0, s, p->d_voff, p->d_Js,
-                     (const double2*)nullptr, w.S, w.W, w.ib, w.beta, 0);
+                     (const double2*)nullptr, w.S, w.W, w.ib, w.beta, 0, (double*)nullptr, p->nJ);
   if ((rc = kmatvec(p, w.W, w.ib, w.Y, w.Z, ncols, ntask, zoff_J, f32, s))) return rc;
   if (f32)
     hipLaunchKernelGGL(k_kp_out<float>, chunk_grid(p, ncols), dim3(256), 0, s, p->d_voff, p->d_Js,
@@ -1112,24 +1206,31 @@ extern "C" int rm_kplan_matvec(rm_kplan* p, const void* x, void* y, int64_t ld, 
   return RM_OK;
 }
 
-// nsteps split-operator Lanczos steps (tdse.py:365-414, 417-486) with a fixed
-// Krylov dimension for ncols wavepackets; see the header for the arguments.
+// nsteps split-operator Lanczos steps (tdse.py:365-414, 417-486) for ncols
+// wavepackets: a fixed Krylov dimension, or the reference's adaptive rule
+// (krylov_fixed == 0); see the header for the arguments.
 extern "C" int rm_kplan_propagate(rm_kplan* p, void* psi, int64_t ld, int32_t ncols, int64_t nsteps,
                                   const double* fprod, double f_re, double f_im, const void* phase,
-                                  int32_t krylov_fixed, int32_t precision, void* stream) {
+                                  int32_t krylov_fixed, int32_t maxorder, double tol, int32_t* kused,
+                                  int32_t* status, int32_t precision, void* stream) {
   if (!p) return fail(RM_ERR_ARG, "rm_kplan_propagate: NULL plan");
   if (ncols < 1 || ncols > 65535 || ld < p->n || !psi || nsteps < 0 || (nsteps > 0 && p->nslot > 0 && !fprod))
     return fail(RM_ERR_ARG, "rm_kplan_propagate: bad arguments");
-  if (krylov_fixed < 1 || krylov_fixed > KMAX)
-    return fail(RM_ERR_ARG, "rm_kplan_propagate: krylov_fixed must be in [1, %d]", KMAX);
+  const bool adaptive = krylov_fixed == 0;
+  if (krylov_fixed < 0 || krylov_fixed > KMAX)
+    return fail(RM_ERR_ARG, "rm_kplan_propagate: krylov_fixed must be in [0, %d]", KMAX);
+  if (adaptive && (maxorder < 1 || maxorder > KMAX || !(tol > 0.0)))
+    return fail(RM_ERR_ARG, "rm_kplan_propagate: maxorder must be in [1, %d] and tol > 0", KMAX);
   if (precision != RM_PREC_FP64 && precision != RM_PREC_FP32)
     return fail(RM_ERR_ARG, "rm_kplan_propagate: unknown precision %d", precision);
-  if (nsteps == 0) return RM_OK;
   hipStream_t s = (hipStream_t)stream;
   const bool f32 = precision == RM_PREC_FP32;
-  const int32_t m = krylov_fixed;
+  const int32_t m = adaptive ? maxorder : krylov_fixed;   // Krylov vectors held
+  if (status)
+    hipLaunchKernelGGL(k_kp_izero, dim3(grid_for(ncols, 256)), dim3(256), 0, s, status, ncols);
+  if (nsteps == 0) return RM_OK;
   KWork w;
-  int rc = kwork(p, ncols, m, f32, &w);
+  int rc = kwork(p, ncols, m, f32, &w, adaptive);
   if (rc) return rc;
   int64_t ntask = 0;
   std::vector<int64_t> zoff_J;
@@ -1151,12 +1252,20 @@ extern "C" int rm_kplan_propagate(rm_kplan* p, void* psi, int64_t ld, int32_t nc
       field = fprod[2 * (st * p->nslot + c)] != 0.0 || fprod[2 * (st * p->nslot + c) + 1] != 0.0;
     if (!field) {
       if (ph) hipLaunchKernelGGL(k_kp_phase2, cg, cb, 0, s, p->d_voff, p->d_Js, ph, w.S);
+      if (kused)
+        hipLaunchKernelGGL(k_kp_izero, dim3(grid_for(ncols, 256)), dim3(256), 0, s, kused + st * ncols, ncols);
       continue;
     }
     if (p->nblk)
       hipLaunchKernelGGL(k_kp_mf, dim3(8, (unsigned)p->nblk), dim3(256), 0, s, p->d_blk, p->d_mdiag,
                          p->d_fp + st * p->nslot, p->d_mf);
-    hipLaunchKernelGGL(k_kp_start, cg, cb, 0, s, p->d_voff, p->d_Js, ph, w.S, w.W, w.ib, w.beta, m);
+    hipLaunchKernelGGL(k_kp_start, cg, cb, 0, s, p->d_voff, p->d_Js, ph, w.S, w.W, w.ib, w.beta, m,
+                       adaptive ? w.part : (double*)nullptr, p->nJ);
+    const bool rule = adaptive && m > 1;   // maxorder 1: V_0 itself, as a fixed dimension
+    if (rule)
+      hipLaunchKernelGGL(k_kp_fin, fin_grid, fin_block, 0, s, w.part, p->nJ, ncols, 2, 0, m, w.alpha, w.beta, w.ib,
+                         w.G, w.cb, w.kdone);
+    int32_t kdim = m;
     for (int32_t it = 0; it < m; ++it) {
       double* Vit = w.V + it * vs;
       const double* Vprev = it > 0 ? w.V + (it - 1) * vs : Vit;
@@ -1167,7 +1276,21 @@ extern "C" int rm_kplan_propagate(rm_kplan* p, void* psi, int64_t ld, int32_t nc
       else
         hipLaunchKernelGGL(k_kp_dot<double>, cg, cb, 0, s, p->d_voff, p->d_Js, w.W, w.ib, (const double*)w.Y, Vit,
                            w.part, p->nJ);
-      hipLaunchKernelGGL(k_kp_fin, fin_grid, fin_block, 0, s, w.part, p->nJ, ncols, 0, it, m, w.alpha, w.beta, w.ib);
+      hipLaunchKernelGGL(k_kp_fin, fin_grid, fin_block, 0, s, w.part, p->nJ, ncols, 0, it, m, w.alpha, w.beta, w.ib,
+                         w.G, w.cb, w.kdone);
+      if (rule && it >= 1) {
+        // the stopping rule for dimension it + 1; the host waits for the count of wavepackets going on
+        hipLaunchKernelGGL(k_kp_izero, dim3(1), dim3(64), 0, s, w.running, 1);
+        hipLaunchKernelGGL(k_kp_conv, dim3((unsigned)ncols), dim3(64), 0, s, w.alpha, w.beta, m, it, f, tol,
+                           maxorder, w.G, w.cb, w.kdone, w.coef, status, w.running);
+        int32_t h_running = 0;
+        HIPCHK(hipMemcpyAsync(&h_running, w.running, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (h_running == 0) {
+          kdim = it + 1;
+          break;
+        }
+      }
       if (it + 1 == m) break;
       if (f32)
         hipLaunchKernelGGL(k_kp_upd<float>, cg, cb, 0, s, p->d_voff, p->d_Js, (const float*)w.Y, Vit, Vprev,
@@ -1175,10 +1298,24 @@ extern "C" int rm_kplan_propagate(rm_kplan* p, void* psi, int64_t ld, int32_t nc
       else
         hipLaunchKernelGGL(k_kp_upd<double>, cg, cb, 0, s, p->d_voff, p->d_Js, (const double*)w.Y, Vit, Vprev,
                            w.alpha, w.beta, it, m, w.W, w.part, p->nJ);
-      hipLaunchKernelGGL(k_kp_fin, fin_grid, fin_block, 0, s, w.part, p->nJ, ncols, 1, it, m, w.alpha, w.beta, w.ib);
+      hipLaunchKernelGGL(k_kp_fin, fin_grid, fin_block, 0, s, w.part, p->nJ, ncols, 1, it, m, w.alpha, w.beta, w.ib,
+                         w.G, w.cb, w.kdone);
     }
-    hipLaunchKernelGGL(k_kp_expm, dim3((unsigned)ncols), dim3(64), 0, s, w.alpha, w.beta, m, m, f, w.coef);
-    hipLaunchKernelGGL(k_kp_comb, cg, cb, 0, s, p->d_voff, p->d_Js, ph, w.V, vs, w.coef, m, m, w.S);
+    if (rule) {
+      hipLaunchKernelGGL(k_kp_comb, cg, cb, 0, s, p->d_voff, p->d_Js, ph, w.V, vs, w.coef, kdim, m, w.S, w.kdone);
+      if (kused)
+        hipLaunchKernelGGL(k_kp_record, dim3(grid_for(ncols, 256)), dim3(256), 0, s, w.kdone, ncols,
+                           kused + st * ncols);
+    } else {
+      hipLaunchKernelGGL(k_kp_expm, dim3((unsigned)ncols), dim3(64), 0, s, w.alpha, w.beta, m, m, f, w.coef);
+      hipLaunchKernelGGL(k_kp_comb, cg, cb, 0, s, p->d_voff, p->d_Js, ph, w.V, vs, w.coef, m, m, w.S,
+                         (const int32_t*)nullptr);
+      if (kused)
+        hipLaunchKernelGGL(k_kp_fill_i, dim3(grid_for(ncols, 256)), dim3(256), 0, s, kused + st * ncols, ncols, m);
+      if (adaptive && status)   // maxorder 1 is reached at once
+        hipLaunchKernelGGL(k_kp_fill_i, dim3(grid_for(ncols, 256)), dim3(256), 0, s, status, ncols,
+                           RM_NOT_CONVERGED);
+    }
     HIPCHK(hipGetLastError());
   }
   hipLaunchKernelGGL(k_kp_pack, cg, cb, 0, s, p->d_voff, p->d_Js, (const double2*)psi, ld, w.S, 0);

@@ -387,15 +387,19 @@ class DeviceKronPlan:
         _plan_phase(d, complex(f), out, stream)
         return out
 
-    def propagate(self, psi, fprod, f, phase=None, krylov_fixed=12, precision="fp64", stream=None):
+    def propagate(self, psi, fprod, f, phase=None, krylov_fixed=12, precision="fp64", stream=None,
+                  maxorder=100, tol=1e-15, kused=None, status=None):
         """len(fprod) steps of TDSE.update on the rows of psi (in place);
-        fprod: host (nsteps, nslot) complex from slot_products."""
+        fprod: host (nsteps, nslot) complex from slot_products.  krylov_fixed
+        0: the reference's adaptive rule with maxorder and tol; kused (device
+        int32 (nsteps, ncols)) and status (device int32 (ncols,)) optional."""
         import ctypes
         from .plan import _check_cols, _stream_handle
         _check_cols(psi, self.n)
         fp = np.ascontiguousarray(fprod, dtype=np.complex128).reshape(-1, max(self.nslot, 1))
+        vp = lambda t: ctypes.c_void_p(t.data_ptr() if t is not None else 0)
         self._lib.check(self._lib.load().rm_kplan_propagate(
             self.handle, ctypes.c_void_p(psi.data_ptr()), psi.stride(0), psi.shape[0], fp.shape[0],
             self._lib.ptr(fp.view(np.float64), ctypes.c_double), float(complex(f).real), float(complex(f).imag),
-            ctypes.c_void_p(phase.data_ptr() if phase is not None else 0), int(krylov_fixed),
+            vp(phase), int(krylov_fixed), int(maxorder), float(tol), vp(kused), vp(status),
             self._lib.PRECISIONS[precision], ctypes.c_void_p(_stream_handle(stream))), "rm_kplan_propagate")

@@ -154,10 +154,10 @@ class TDSE:
         Krylov dimension, default 0 = adaptive), stream (torch stream),
         operator_form ('auto' (default), 'flat' or 'kron'): 'kron' keeps the
         operator factored as richmol stores it (Kronecker-block plan; needs
-        every J with all m and the full k space, a fixed Krylov dimension and
-        no field thresholds), 'auto' takes it for such bases of at least
-        KRON_MIN_STATES states; precision ('fp64', or 'fp32' dense products,
-        Kronecker-block form only).
+        every J with all m and the full k space and no field thresholds),
+        'auto' takes it for such bases of at least KRON_MIN_STATES states;
+        precision ('fp64', or 'fp32' dense products, Kronecker-block form
+        only).
         """
         from . import plan as _plan
         import torch
@@ -217,9 +217,6 @@ class TDSE:
             plan = None
         elif operator_form == "kron" or (operator_form == "auto" and psi.shape[1] >= KRON_MIN_STATES
                                          and _kron_eligible(H)):
-            if krylov_fixed <= 0:
-                raise NotImplementedError("the Kronecker-block form propagates with a fixed Krylov dimension: "
-                                          "pass krylov_fixed=m (the adaptive rule needs the flat form)")
             kron = _kron_plan(H)
             plan = None
         else:
@@ -257,9 +254,14 @@ class TDSE:
             if phase is not None:
                 ph = torch.empty_like(phase)
                 ph[perm] = phase
-            dp.propagate(work, row, f, phase=ph, krylov_fixed=krylov_fixed, precision=precision, stream=stream)
+            kused = torch.zeros((1, ncols), dtype=torch.int32, device=psi.device)
+            status = torch.zeros(ncols, dtype=torch.int32, device=psi.device)
+            dp.propagate(work, row, f, phase=ph, krylov_fixed=krylov_fixed, precision=precision, stream=stream,
+                         maxorder=maxorder, tol=tol, kused=kused, status=status)
             out = work[:, perm]
-            self.last_krylov_dims = torch.full((ncols,), krylov_fixed, dtype=torch.int32, device=psi.device)
+            self.last_krylov_dims = kused[0]
+            if krylov_fixed == 0 and bool((status != 0).any()):
+                raise ValueError(f"Lanczos reached maximum order of '{maxorder}' without convergence")
         elif plan is None:
             # no field on the operator: only the two H0 phases (tdse.py:375, 377, 394)
             _phase_apply(psi, out, phase, stream)

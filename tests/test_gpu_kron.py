@@ -214,9 +214,13 @@ def test_propagate_rejects_bad_arguments(device):
     psi = torch.zeros((1, kop.n), dtype=torch.complex128, device=device)
     fp = np.zeros((1, dp.nslot), dtype=np.complex128)
     with pytest.raises(_lib.RichmolAmdError):
-        dp.propagate(psi, fp, 1j, krylov_fixed=0)
+        dp.propagate(psi, fp, 1j, krylov_fixed=-1)
     with pytest.raises(_lib.RichmolAmdError):
         dp.propagate(psi, fp, 1j, krylov_fixed=_lib.RM_MAX_KRYLOV + 1)
+    with pytest.raises(_lib.RichmolAmdError):   # adaptive rule: maxorder in [1, RM_MAX_KRYLOV], tol > 0
+        dp.propagate(psi, fp, 1j, krylov_fixed=0, maxorder=0)
+    with pytest.raises(_lib.RichmolAmdError):
+        dp.propagate(psi, fp, 1j, krylov_fixed=0, tol=0.0)
 
 
 def test_full_size_large_top_properties(device):
@@ -287,5 +291,82 @@ def test_tdse_update_kronecker_form(device, precision, tol):
             Ti.field(list(F[i]))
         psi, _ = td.update(T[0] + T[1], psi, H0=h0, krylov_fixed=12, operator_form="kron", precision=precision)
     assert np.abs(psi - ref).max() <= tol
-    with pytest.raises(NotImplementedError):
-        td.update(T[0] + T[1], psi, H0=h0, operator_form="kron")
+
+
+def _oracle_run_adaptive(bas, fields, Xf, dt, maxorder=100, tol=1e-15):
+    from oracle.cartens import OracleCarTens
+    from oracle.tdse import OracleTDSE
+    from richmol_amd import rot
+    tens, scales = large_top_tensors()
+    OL = [OracleCarTens.from_attrs(rot.tensor(t, bas) * s) for t, s in zip(tens, scales)]
+    Oh0 = OracleCarTens.from_attrs(rot.hamiltonian(bas))
+    nst = len(fields[0])
+    to = OracleTDSE(t_start=0.0, t_end=dt * nst, dt=dt, t_units="fs")
+    to.time_grid()
+    psi, ks = Xf.copy(), []
+    for i in range(nst):
+        for O, F in zip(OL, fields):
+            O.field(list(F[i]))
+        stats = []
+        if np.any([np.any(F[i]) for F in fields]):
+            psi, _ = to.update(OL[0] + OL[1], psi, H0=Oh0, tol=tol, maxorder=maxorder, stats=stats)
+        else:
+            psi, _ = to.update(OL[0] + OL[1], psi, H0=Oh0, tol=tol, maxorder=maxorder)
+        ks.append(stats if stats else [0] * len(Xf))
+    return psi, np.array(ks), to.exp_fac
+
+
+@pytest.mark.parametrize("kernel", ["gemm", "fused"])
+def test_propagate_adaptive_matches_oracle(device, kernel):
+    """The reference's adaptive rule (tol 1e-15, maxorder 100) on the
+    Kronecker-block path: four wavepackets at J <= 5 over six steps of the
+    chirped field (one zero-field step): amplitudes within 1e-10 of the oracle
+    and every step's Krylov dimension per wavepacket equal to the oracle's."""
+    import torch
+    from richmol_amd.kron import slot_products
+    kop, bas = large_top(5)
+    dt = 10.0
+    t = 40000.0 + (np.arange(6) + 0.5) * dt
+    fields = centrifuge_fields(t, e0=3e9)
+    for F in fields:
+        F[2] = 0.0
+    X = _columns(kop.n, 29)
+    ref, k_o, f = _oracle_run_adaptive(bas, fields, X, dt)
+    dp = _dev_plan(kop)
+    dp.set_kernel(kernel)
+    psi = torch.from_numpy(_to_int(kop, X)).cuda()
+    kused = torch.zeros((6, 4), dtype=torch.int32, device=device)
+    status = torch.ones(4, dtype=torch.int32, device=device)
+    dp.propagate(psi, slot_products(kop, fields), f, phase=dp.phases(f), krylov_fixed=0, kused=kused, status=status)
+    got = psi.cpu().numpy()[:, kop.perm]
+    assert np.abs(got - ref).max() <= 1e-10
+    assert np.array_equal(kused.cpu().numpy(), k_o)
+    assert int(status.abs().sum()) == 0
+
+
+def test_propagate_adaptive_maxorder_reached(device):
+    """A maxorder too small for tol 1e-15: the steps end at maxorder and
+    status flags every wavepacket (the reference raises ValueError), and
+    TDSE.update on the Kronecker form raises it."""
+    import torch
+    from richmol_amd import rot
+    from richmol_amd.kron import slot_products
+    from richmol_amd.tdse import TDSE
+    kop, bas = large_top(4)
+    F = centrifuge_fields(np.array([40005.0]), e0=3e9)
+    dp = _dev_plan(kop)
+    X = _columns(kop.n, 31)
+    psi = torch.from_numpy(_to_int(kop, X)).cuda()
+    kused = torch.zeros((1, 4), dtype=torch.int32, device=device)
+    status = torch.zeros(4, dtype=torch.int32, device=device)
+    dp.propagate(psi, slot_products(kop, F), -1e-2j, krylov_fixed=0, maxorder=3, kused=kused, status=status)
+    assert np.all(kused.cpu().numpy() == 3)
+    assert np.all(status.cpu().numpy() == 1)
+    tens, scales = large_top_tensors()
+    T = [rot.tensor(t, bas) * s for t, s in zip(tens, scales)]
+    for Ti, Fi in zip(T, F):
+        Ti.field(list(Fi[0]))
+    td = TDSE(t_start=0.0, t_end=10.0, dt=10.0, t_units="fs")
+    td.time_grid()
+    with pytest.raises(ValueError):
+        td.update(T[0] + T[1], X, H0=rot.hamiltonian(bas), operator_form="kron", maxorder=3)
