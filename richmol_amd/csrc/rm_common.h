@@ -129,6 +129,7 @@ struct rm_plan {
   std::vector<int32_t> h_sub_start, h_sub_end;   // per subspace: first / one past last permuted row
   std::vector<int32_t> h_sub_label_perm;         // permuted row -> subspace
   std::vector<int32_t> h_sub_perm;               // permuted row -> original row
+  std::vector<int32_t> h_sub_col;                // host copy of sub_col ([w][n], permuted columns)
   // the B_s as CSR in permuted order, for domains longer than a wave's
   // (built when some subspace has more than RM_SUBSPACE_MAX_ROWS rows)
   int64_t* sub_rp = nullptr;     // [n+1]

@@ -353,7 +353,34 @@ __global__ __launch_bounds__(64) void k_sub_prop(SubArgs a) {
 // wavepacket's result does not depend on its partner, bit for bit, and
 // equals the single-wave kernel's.  Item 2p + h of the launch is half h of
 // pair p; an item with column -1 is an empty half.
-template <int WM, bool F32>
+// BAND: every entry of the domain's rows couples row i to i - 1, i or i + 1
+// (a chain: the linear rotor's (m, J-parity) subspaces), so the matvec takes
+// its neighbours from the adjacent lanes by wavefront DPP shifts -- no LDS
+// gather and no wave barrier on the iteration's critical path; the entries
+// are summed per offset (-1, 0, +1).
+// Lane l <- lane l + 1 (up, wave_shl:1) or l - 1 (wave_shr:1); the lane with
+// no source (63 up, 0 down) receives 0 (bound control with a zero old value:
+// never a leftover register, which a zero coefficient could turn into NaN).
+__device__ __forceinline__ int wave_shift_i(int v, bool up) {
+  return up ? __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, true)
+            : __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, true);
+}
+__device__ __forceinline__ double wave_shift_d(double v, bool up) {
+  return __hiloint2double(wave_shift_i(__double2hiint(v), up), wave_shift_i(__double2loint(v), up));
+}
+template <typename XT>
+__device__ __forceinline__ XT wave_shift(XT v, bool up);
+template <>
+__device__ __forceinline__ double2 wave_shift<double2>(double2 v, bool up) {
+  return make_double2(wave_shift_d(v.x, up), wave_shift_d(v.y, up));
+}
+template <>
+__device__ __forceinline__ float2 wave_shift<float2>(float2 v, bool up) {
+  return make_float2(__int_as_float(wave_shift_i(__float_as_int(v.x), up)),
+                     __int_as_float(wave_shift_i(__float_as_int(v.y), up)));
+}
+
+template <int WM, bool F32, bool BAND = false>
 __global__ __launch_bounds__(64) void k_sub_pair(SubArgs a) {
   using XT = typename XType<F32>::T;
   constexpr int WMR = WM > 0 ? WM : 1;
@@ -389,6 +416,23 @@ __global__ __launch_bounds__(64) void k_sub_pair(SubArgs a) {
         Bv[sl][j] = (breg && j < W && sl < op.nslot) ? op.val[((int64_t)sl * W + j) * n + gq]
                                                     : make_double2(0.0, 0.0);
     }
+    // BAND: the entries per offset -1, 0, +1 (rows past the domain: none)
+    double2 Bo[NSR][3];
+#pragma unroll
+    for (int sl = 0; sl < NSR; ++sl)
+#pragma unroll
+      for (int o = 0; o < 3; ++o) Bo[sl][o] = make_double2(0.0, 0.0);
+    if (BAND && ok) {
+#pragma unroll
+      for (int j = 0; j < WMR; ++j) {
+        const int o = cidx[j] - l + 1;   // 0, 1, 2 for columns i - 1, i, i + 1
+#pragma unroll
+        for (int sl = 0; sl < NSR; ++sl)
+#pragma unroll
+          for (int oo = 0; oo < 3; ++oo)
+            if (o == oo) Bo[sl][oo] = cadd(Bo[sl][oo], Bv[sl][j]);
+      }
+    }
     double2* sa = s_alpha[h];
     double* sb = s_beta[h];
     double2* sc = s_cf[h];
@@ -401,7 +445,12 @@ __global__ __launch_bounds__(64) void k_sub_pair(SubArgs a) {
       }
       const double2* fp = op.table + (int64_t)step * op.nslot;
       XT A[WMR];
-      if (WM > 0) {
+      XT Ab[3];
+      if (BAND) {
+        const double2 f0 = fp[0], f1 = op.nslot > 1 ? fp[1] : zero;
+#pragma unroll
+        for (int o = 0; o < 3; ++o) Ab[o] = to_x<F32>(cfma(f1, Bo[1][o], cmul(f0, Bo[0][o])));
+      } else if (WM > 0) {
         double2 f0 = zero, f1 = zero;
         if (breg) {
           f0 = fp[0];
@@ -424,13 +473,22 @@ __global__ __launch_bounds__(64) void k_sub_pair(SubArgs a) {
       auto matvec = [&]() __attribute__((always_inline)) -> double2 {
         const double2 z0 = make_double2(0.0, 0.0);
         XT acc = to_x<F32>(z0);
+        if (BAND) {
+          const XT x = to_x<F32>(vk);
+          acc = xfma(Ab[0], wave_shift<XT>(x, false), acc);
+          acc = xfma(Ab[1], x, acc);
+          acc = xfma(Ab[2], wave_shift<XT>(x, true), acc);
+        } else {
 #pragma unroll
-        for (int j = 0; j < WMR; ++j) acc = xfma(A[j], s_x[cidx[j]], acc);
+          for (int j = 0; j < WMR; ++j) acc = xfma(A[j], s_x[cidx[j]], acc);
+        }
         return ok ? to_d2(acc) : z0;
       };
       s_V[l] = vk;
-      s_x[l] = to_x<F32>(vk);
-      wave_sync();
+      if (!BAND) {
+        s_x[l] = to_x<F32>(vk);
+        wave_sync();
+      }
       double2 Wr = matvec(), vp = zero;
       double acc3[2];
       acc3[0] = Wr.x * vk.x + Wr.y * vk.y;
@@ -466,9 +524,11 @@ __global__ __launch_bounds__(64) void k_sub_pair(SubArgs a) {
           vk = nv > 0.0 ? make_double2(Wr.x / nv, Wr.y / nv) : zero;
         }
         s_V[it * 64 + l] = vk;
-        s_x[l] = to_x<F32>(vk);
         sb[it] = beta;
-        wave_sync();
+        if (!BAND) {
+          s_x[l] = to_x<F32>(vk);
+          wave_sync();
+        }
         Wr = matvec();
         const double a0 = Wr.x * vk.x + Wr.y * vk.y, a1 = Wr.x * vk.y - Wr.y * vk.x;
         al = make_double2(half_allsum32(a0), half_allsum32(a1));
@@ -662,6 +722,7 @@ int sub_build(rm_plan* p) {
         }
       }
     }
+    p->h_sub_col = hcol;
     if ((rc = upload(&p->sub_col, hcol.data(), (int64_t)ne)) ||
         (rc = upload(&p->sub_val, hval.data(), (int64_t)((size_t)nf * ne))))
       return rc;
@@ -683,10 +744,13 @@ int sub_build(rm_plan* p) {
 //   6: two wavepackets per wave (k_sub_pair): fixed Krylov dimension <= 16,
 //      domains <= 32 rows, ELL width <= 4 -- the class-0 items that qualify,
 //      paired with an item of the same operator or with an empty half
-constexpr int NCLASS = 7;
+//   7: the same with a banded domain (every entry within one row of the
+//      diagonal, <= 2 field slots): neighbours by DPP shifts (k_sub_pair BAND)
+constexpr int NCLASS = 8;
 constexpr int PAIR_CLASS = 6;
+constexpr int PAIR_BAND_CLASS = 7;
 constexpr int PAIR_MAX_M = 16;
-constexpr int CLASS_R[NCLASS] = {1, 2, 4, 4, 8, 16, 1};
+constexpr int CLASS_R[NCLASS] = {1, 2, 4, 4, 8, 16, 1, 1};
 
 int sub_class(int w, int len) {
   if (w <= 4) {
@@ -701,13 +765,13 @@ int sub_class(int w, int len) {
 
 // Krylov basis in LDS: register classes whose basis fits.
 bool class_basis_lds(int c, int kcap) {
-  if (c == PAIR_CLASS) return true;   // static LDS of k_sub_pair
+  if (c == PAIR_CLASS || c == PAIR_BAND_CLASS) return true;   // static LDS of k_sub_pair
   return c <= 2 && (size_t)kcap * 64 * CLASS_R[c] * sizeof(double2) <= SUB_LDS_BASIS;
 }
 
-template <bool F32>
+template <bool F32, bool BAND>
 hipError_t launch_pair(const SubArgs& a0, int n_cu, hipStream_t s, int* waves_out, bool dry) {
-  auto kern = k_sub_pair<4, F32>;
+  auto kern = k_sub_pair<4, F32, BAND>;
   int occ = 0;
   hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)kern, 64, 0);
   if (e != hipSuccess) return e;
@@ -764,7 +828,10 @@ hipError_t launch_class_p(int c, const SubArgs& a, int n_cu, hipStream_t s, int*
 }
 
 hipError_t launch_class(int c, const SubArgs& a, bool f32, int n_cu, hipStream_t s, int* waves, bool dry) {
-  if (c == PAIR_CLASS) return f32 ? launch_pair<true>(a, n_cu, s, waves, dry) : launch_pair<false>(a, n_cu, s, waves, dry);
+  if (c == PAIR_CLASS)
+    return f32 ? launch_pair<true, false>(a, n_cu, s, waves, dry) : launch_pair<false, false>(a, n_cu, s, waves, dry);
+  if (c == PAIR_BAND_CLASS)
+    return f32 ? launch_pair<true, true>(a, n_cu, s, waves, dry) : launch_pair<false, true>(a, n_cu, s, waves, dry);
   const bool fixed = a.fixed_m > 0;
   if (fixed && f32) return launch_class_p<true, true>(c, a, n_cu, s, waves, dry);
   if (fixed) return launch_class_p<true, false>(c, a, n_cu, s, waves, dry);
@@ -820,22 +887,33 @@ int sub_run(SubScratch* sc, const std::vector<SubOp>& ops, std::vector<int4> (&c
             const std::vector<LongItem>& blk, const std::vector<LongItem>& coop, int32_t kcap,
             int32_t fixed_m, int32_t maxorder, double tol, bool f32, int n_cu, hipStream_t s,
             rm_plan* timed) {
-  // the pair class: class-0 items of <= 32 rows under a fixed Krylov
-  // dimension <= 16, grouped by operator, each group padded to an even count
-  if (fixed_m >= 1 && fixed_m <= PAIR_MAX_M && cls[PAIR_CLASS].empty()) {
-    std::vector<int4> keep, pairable;
-    for (const int4& it : cls[0]) (it.z <= 32 ? pairable : keep).push_back(it);
-    if (!pairable.empty()) {
-      std::stable_sort(pairable.begin(), pairable.end(), [](const int4& x, const int4& y) { return x.w < y.w; });
-      std::vector<int4>& pc = cls[PAIR_CLASS];
-      for (size_t i = 0; i < pairable.size();) {
-        size_t j = i;
-        while (j < pairable.size() && pairable[j].w == pairable[i].w) pc.push_back(pairable[j++]);
-        if ((j - i) & 1) pc.push_back(make_int4(-1, 0, 0, pairable[i].w));
-        i = j;
-      }
-      cls[0].swap(keep);
+  // the pair classes: class-0 items of <= 32 rows under a fixed Krylov
+  // dimension <= 16 (the banded ones, placed in PAIR_BAND_CLASS by the
+  // caller, keep their own class), grouped by operator, each group padded to
+  // an even count; other Krylov settings send the banded items back to class 0
+  const bool pairs = fixed_m >= 1 && fixed_m <= PAIR_MAX_M;
+  if (!pairs && !cls[PAIR_BAND_CLASS].empty()) {
+    cls[0].insert(cls[0].end(), cls[PAIR_BAND_CLASS].begin(), cls[PAIR_BAND_CLASS].end());
+    cls[PAIR_BAND_CLASS].clear();
+  }
+  auto pair_up = [](std::vector<int4>& v) {
+    if (v.empty()) return;
+    std::stable_sort(v.begin(), v.end(), [](const int4& x, const int4& y) { return x.w < y.w; });
+    std::vector<int4> out;
+    for (size_t i = 0; i < v.size();) {
+      size_t j = i;
+      while (j < v.size() && v[j].w == v[i].w) out.push_back(v[j++]);
+      if ((j - i) & 1) out.push_back(make_int4(-1, 0, 0, v[i].w));
+      i = j;
     }
+    v.swap(out);
+  };
+  if (pairs && cls[PAIR_CLASS].empty()) {
+    std::vector<int4> keep;
+    for (const int4& it : cls[0]) (it.z <= 32 ? cls[PAIR_CLASS] : keep).push_back(it);
+    cls[0].swap(keep);
+    pair_up(cls[PAIR_CLASS]);
+    pair_up(cls[PAIR_BAND_CLASS]);
   }
   size_t nitems = blk.size() + coop.size();
   for (int c = 0; c < NCLASS; ++c) nitems += cls[c].size();
@@ -957,6 +1035,19 @@ int sub_run(SubScratch* sc, const std::vector<SubOp>& ops, std::vector<int4> (&c
   return RM_OK;
 }
 
+// Every stored entry of rows [lo, hi) couples a row only to itself or to its
+// neighbours in the permuted order.
+bool sub_banded(const rm_plan* p, int32_t lo, int32_t hi) {
+  const int64_t n = p->n;
+  if (p->h_sub_col.size() < (size_t)p->sub_w * (size_t)n) return false;
+  for (int32_t r = lo; r < hi; ++r)
+    for (int32_t j = 0; j < p->sub_w; ++j) {
+      const int32_t c = p->h_sub_col[(size_t)j * n + r];
+      if (c < r - 1 || c > r + 1) return false;
+    }
+  return true;
+}
+
 // Domains [lo, hi) of ncols columns into p->h_dom (synchronises s).
 int sub_domains(rm_plan* p, const void* psi, int64_t ld, int64_t ncols, hipStream_t s) {
   const int64_t n = p->n;
@@ -1062,10 +1153,13 @@ int sub_propagate(rm_plan* p, void* psi, int64_t ld, int64_t ncols, int64_t nste
   for (int64_t b = 0; b < ncols; ++b) {
     const int32_t lo = p->h_dom[2 * b], hi = p->h_dom[2 * b + 1];
     if (hi <= lo) continue;   // zero column: stays zero (kused 0)
-    if (force || hi - lo > RM_SUBSPACE_MAX_ROWS || !p->sub_col)
+    if (force || hi - lo > RM_SUBSPACE_MAX_ROWS || !p->sub_col) {
       big.push_back(make_int4((int)b, lo, hi - lo, 0));
-    else
-      cls[sub_class(p->sub_w, hi - lo)].push_back(make_int4((int)b, lo, hi - lo, 0));
+    } else {
+      const int c = sub_class(p->sub_w, hi - lo);
+      cls[c == 0 && hi - lo <= 32 && p->nfield <= NSR && sub_banded(p, lo, hi) ? PAIR_BAND_CLASS : c]
+          .push_back(make_int4((int)b, lo, hi - lo, 0));
+    }
   }
   if (!big.empty() && !p->sub_rp && (rc = csr_build(p, entry_values(p)))) return rc;
   std::vector<LongItem> blk, coop;

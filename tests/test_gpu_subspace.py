@@ -271,3 +271,50 @@ def test_fp32_thermal_ensemble_cos2(device):
         err = np.abs(c2(psi_d) - c2(psi_o))
         assert err.max() <= 1e-6, (blk, err.max())
         assert abs(c2(psi_d).mean() - c2(psi_o).mean()) <= 1e-6
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_pair_kernel_unbanded_domains(device, precision):
+    """A dipole plus a polarizability on a linear rotor under a Z field: each m
+    block (J = |m|..20, at most 21 rows) couples J to J +- 1 and J +- 2, so its
+    domains fit half a wave but are not banded -- the two-per-wave kernel with
+    the LDS gather; FP64 against the oracle (1e-10), FP32 partner independent."""
+    import torch
+    from richmol_amd import ensemble, rot, units
+    from richmol_amd.tdse import TDSE
+    from oracle.cartens import OracleCarTens
+    from oracle.tdse import OracleTDSE
+    h0, A, cos2, bas = linear_rotor(20)
+    D = rot.tensor(np.array([0.0, 0.0, 0.3]), bas) * (-units.AUdip_x_Vm_to_invcm())
+    n = 441
+    rng = np.random.default_rng(41)
+    X = _basis_columns(n, rng.choice(200, size=7, replace=False))
+    t = np.arange(25, 31) * 10.0 + 5.0
+    pul = np.array([[0.0, 0.0, gaussian_pulse_field(tt)] for tt in t])
+    stat = np.tile([[0.0, 0.0, 2e7]], (len(t), 1))
+    fields = [stat, pul]
+    f = TDSE(t_end=10.0 * len(t), dt=10.0, t_units="fs").exp_fac
+
+    def dev(Xr):
+        D.field(list(stat[0]))
+        A.field(list(pul[0]))
+        e = ensemble.Ensemble(D + A, h0, f, Xr, krylov_fixed=12)
+        e.set_precision(precision)
+        e.propagate(None, table=e.prepare(fields, per_term=True))
+        e.check()
+        return e.psi.cpu().numpy()
+
+    a = dev(X)
+    one = dev(X[2:3])
+    assert np.array_equal(one[0], a[2])
+    if precision == "fp64":
+        OL = [OracleCarTens.from_attrs(D), OracleCarTens.from_attrs(A)]
+        Oh0 = OracleCarTens.from_attrs(h0)
+        to = OracleTDSE(t_start=0.0, t_end=10.0 * len(t), dt=10.0, t_units="fs")
+        to.time_grid()
+        psi = X.copy()
+        for i in range(len(t)):
+            for O, F in zip(OL, fields):
+                O.field(list(F[i]))
+            psi, _ = to.update(OL[0] + OL[1], psi, H0=Oh0, fixed_m=12)
+        assert np.abs(a - psi).max() <= TOL_PSI
