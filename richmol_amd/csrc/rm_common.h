@@ -239,39 +239,68 @@ __device__ __forceinline__ double readlane_d(double v, int lane) {
   return __hiloint2double(hi, lo);
 }
 
+// v_permlane16_swap / v_permlane32_swap with the same value in both operands:
+// every lane receives the (even, odd) row pair of its half, or the (lower,
+// upper) half pair of the wave.
+__device__ __forceinline__ void swap16_d(double v, double* even, double* odd) {
+  const auto lo = __builtin_amdgcn_permlane16_swap(__double2loint(v), __double2loint(v), false, false);
+  const auto hi = __builtin_amdgcn_permlane16_swap(__double2hiint(v), __double2hiint(v), false, false);
+  *even = __hiloint2double(hi[0], lo[0]);
+  *odd = __hiloint2double(hi[1], lo[1]);
+}
+__device__ __forceinline__ void swap32_d(double v, double* lower, double* upper) {
+  const auto lo = __builtin_amdgcn_permlane32_swap(__double2loint(v), __double2loint(v), false, false);
+  const auto hi = __builtin_amdgcn_permlane32_swap(__double2hiint(v), __double2hiint(v), false, false);
+  *lower = __hiloint2double(hi[0], lo[0]);
+  *upper = __hiloint2double(hi[1], lo[1]);
+}
+
 // Sum over the 64 lanes of a wave (all lanes active), bitwise identical on
 // every lane: rotations by 8, 4, 2, 1 inside each 16-lane row pair every lane
 // with partners whose partial sums are the same two operands (addition is
-// commutative), then the four row sums are combined in a fixed order.
+// commutative), then the row sums are combined in a fixed order,
+// (row 0 + row 1) + (row 2 + row 3), through lane swaps between rows and
+// halves (no scalar-register round trip).
 __device__ __forceinline__ double wave_allsum(double v) {
   v += dpp_mov<0x128>(v);   // row_ror:8
   v += dpp_mov<0x124>(v);   // row_ror:4
   v += dpp_mov<0x122>(v);   // row_ror:2
   v += dpp_mov<0x121>(v);   // row_ror:1
-  return (readlane_d(v, 0) + readlane_d(v, 16)) + (readlane_d(v, 32) + readlane_d(v, 48));
+  double e, o, lo, hi;
+  swap16_d(v, &e, &o);
+  swap32_d(e + o, &lo, &hi);
+  return lo + hi;
 }
 
-// wave_allsum for values that are zero on lanes 32..63: the same sum (adding
-// the two zero rows changes nothing), without reading the upper half.
+// wave_allsum for values that are zero on lanes 32..63: row 0 + row 1 (adding
+// the two zero rows changes nothing), on every lane.
 __device__ __forceinline__ double wave_allsum32(double v) {
   v += dpp_mov<0x128>(v);
   v += dpp_mov<0x124>(v);
   v += dpp_mov<0x122>(v);
   v += dpp_mov<0x121>(v);
-  return readlane_d(v, 0) + readlane_d(v, 16);
+  double e, o, lo, hi;
+  swap16_d(v, &e, &o);
+  swap32_d(e + o, &lo, &hi);
+  return lo;
 }
 
 // Sums over each half of the wave (lanes 0..31 and 32..63) for two problems
 // held side by side: the operations of wave_allsum32 on each half, the lane's
 // own half's sum returned.
+// The two row sums of each half meet through v_permlane16_swap (the odd
+// rows of one operand exchanged with the even rows of the other; with the
+// same value in both, every lane gets its half's even-row and odd-row sums),
+// so each half's sum is even + odd -- the readlane form's operands and order,
+// without the round trip through scalar registers.
 __device__ __forceinline__ double half_allsum32(double v) {
   v += dpp_mov<0x128>(v);
   v += dpp_mov<0x124>(v);
   v += dpp_mov<0x122>(v);
   v += dpp_mov<0x121>(v);
-  const double s0 = readlane_d(v, 0) + readlane_d(v, 16);
-  const double s1 = readlane_d(v, 32) + readlane_d(v, 48);
-  return (threadIdx.x & 32) ? s1 : s0;
+  double e, o;
+  swap16_d(v, &e, &o);
+  return e + o;
 }
 
 __device__ __forceinline__ double wave_max(double v) {
