@@ -541,9 +541,16 @@ __global__ __launch_bounds__(64) void k_sub_pair(SubArgs a) {
       else if (i == 0) sc[0] = make_double2(1.0, 0.0);
       __threadfence_block();
       wave_sync();
-      double2 u = zero;
-      for (int j = 0; j < k - 1; ++j) u = cfma(sc[j], s_V[j * 64 + (len > 0 ? 32 * h + min(i, len - 1) : l)], u);
-      const double2 uu = cfma(sc[k - 1], vk, u);
+      // two interleaved partial sums (even and odd j) halve the dependent chain
+      const int vrow = len > 0 ? 32 * h + min(i, len - 1) : l;
+      double2 u0 = zero, u1 = zero;
+      int j = 0;
+      for (; j + 1 < k - 1; j += 2) {
+        u0 = cfma(sc[j], s_V[j * 64 + vrow], u0);
+        u1 = cfma(sc[j + 1], s_V[(j + 1) * 64 + vrow], u1);
+      }
+      if (j < k - 1) u0 = cfma(sc[j], s_V[j * 64 + vrow], u0);
+      const double2 uu = cfma(sc[k - 1], vk, cadd(u0, u1));
       st = ok ? (op.phase ? cmul(uu, ph) : uu) : zero;
       if (i == 0 && b >= 0 && op.kused) op.kused[(int64_t)step * op.kused_ld + b - op.kcol0] = k;
       wave_sync();
