@@ -46,6 +46,8 @@ constexpr int UM = 8;            // entries per lane loaded together (memory mod
 constexpr int GL = 32;           // Gram matrix order held in LDS (adaptive)
 constexpr int XL_ROWS = 2048;    // block form: matvec input staged in LDS up to this many rows
 constexpr unsigned SPIN_LIMIT = 1u << 22;   // polls of a grid barrier before the launch gives up
+constexpr int BAR_LINES = 8;                // arrival counters of a grid barrier
+constexpr int BAR_STRIDE = 32;              // u64 between counters (one 256-byte line each)
 
 struct LongArgs {
   const SubOp* ops;
@@ -57,7 +59,7 @@ struct LongArgs {
   double* part;            // [2][grid][4] workgroup partial sums (cooperative)
   double2* gram;           // [grid][kcap * kcap] (adaptive, orders beyond GL)
   double2* cscr;           // [grid][2 * kcap] (adaptive, orders beyond GL)
-  unsigned long long* bar; // grid-barrier arrivals (zero at launch)
+  unsigned long long* bar; // grid-barrier arrivals, BAR_LINES counters (zero at launch)
   int32_t* abort_flag;     // set when a barrier wait exceeds SPIN_LIMIT polls
   int64_t lmax;            // vector capacity of a region
   int32_t kcap, fixed_m, maxorder;
@@ -72,29 +74,43 @@ struct SyncState {
 __device__ __forceinline__ double2 xadd(double2 a, double2 b) { return cadd(a, b); }
 __device__ __forceinline__ float2 xadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 
-// Grid barrier on an arrival counter that only grows: barrier number t
-// completes when the counter reaches t * grid.  Every wave first waits for
-// its own memory operations; one thread per workgroup then publishes the
-// workgroup's writes (agent-scope release), arrives, waits and acquires
-// (agent-scope acquire; the waves of a workgroup share the CU's caches).  A
-// wait that exceeds SPIN_LIMIT polls (a workgroup that never arrives) sets
-// the abort flag; from then on barriers no longer wait and the launch runs
-// to its end, its results marked invalid.
+// Grid barrier on BAR_LINES arrival counters that only grow, one per 256-byte
+// line: workgroup g arrives on line g % BAR_LINES, so barrier number t
+// completes when line c holds t * (number of workgroups on line c).
+// Spreading the arrivals keeps the atomics of ~100 workgroups from queueing
+// on one address.  Every wave first waits for its own memory operations; one
+// thread per workgroup then publishes the workgroup's writes (agent-scope
+// release) and arrives; lanes 0..BAR_LINES-1 of wave 0 each poll one line
+// and the wave leaves when all lines are complete, then acquires (agent
+// scope; the waves of a workgroup share the CU's caches).  A wait that
+// exceeds SPIN_LIMIT polls (a workgroup that never arrives) sets the abort
+// flag; from then on barriers no longer wait and the launch runs to its end,
+// its results marked invalid.
 __device__ __forceinline__ void grid_barrier(const LongArgs& a, SyncState& ss, int* s_ab) {
-  ss.target += gridDim.x;
+  ss.target += 1;
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
-  if (threadIdx.x == 0 && *s_ab == 0) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __hip_atomic_fetch_add(a.bar, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (threadIdx.x < 64 && *s_ab == 0) {
+    const int lane = threadIdx.x;
+    if (lane == 0) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      __hip_atomic_fetch_add(a.bar + BAR_STRIDE * (blockIdx.x % BAR_LINES), 1ull, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+    }
+    const unsigned c = (unsigned)lane % BAR_LINES;
+    const unsigned long long need =
+        lane < BAR_LINES ? ss.target * (unsigned long long)((gridDim.x + BAR_LINES - 1 - c) / BAR_LINES) : 0ull;
     unsigned spins = 0;
-    while (__hip_atomic_load(a.bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < ss.target) {
+    while (!__all(lane >= BAR_LINES ||
+                  __hip_atomic_load(a.bar + BAR_STRIDE * c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= need)) {
       __builtin_amdgcn_s_sleep(1);
       ++spins;
       if ((spins & 255u) == 0 &&
           (spins >= SPIN_LIMIT || __hip_atomic_load(a.abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
-        __hip_atomic_store(a.abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        *s_ab = 1;
+        if (lane == 0) {
+          __hip_atomic_store(a.abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          *s_ab = 1;
+        }
         break;
       }
     }
@@ -546,10 +562,8 @@ __global__ __launch_bounds__(ct_of<MODE>()) void k_long(LongArgs a) {
 }
 
 __global__ void k_long_reset(unsigned long long* bar, int32_t* abort_flag) {
-  if (threadIdx.x == 0) {
-    *bar = 0ull;
-    *abort_flag = 0;
-  }
+  if (threadIdx.x < BAR_LINES) bar[BAR_STRIDE * threadIdx.x] = 0ull;
+  if (threadIdx.x == 0) *abort_flag = 0;
 }
 
 struct LongScratch {
@@ -675,7 +689,8 @@ int long_run(void** scratch, const SubOp* d_ops, const int4* d_blk, const std::v
   const size_t b_part = al256(sizeof(double) * 2 * 4 * (size_t)gmax);
   const size_t b_gram = fixed ? 0 : al256(sizeof(double2) * (size_t)gmax * kcap * kcap);
   const size_t b_c = fixed ? 0 : al256(sizeof(double2) * (size_t)gmax * 2 * kcap);
-  const size_t need = al256(b_bytes) + al256(c_bytes) + b_part + b_gram + b_c + 256;
+  const size_t b_bar = sizeof(unsigned long long) * BAR_STRIDE * BAR_LINES;
+  const size_t need = al256(b_bytes) + al256(c_bytes) + b_part + b_gram + b_c + b_bar + 256;
   LongScratch* sc = (LongScratch*)*scratch;
   if (!sc) {
     sc = new (std::nothrow) LongScratch();
@@ -699,7 +714,7 @@ int long_run(void** scratch, const SubOp* d_ops, const int4* d_blk, const std::v
   double2* gram = fixed ? nullptr : (double2*)w; w += b_gram;
   double2* cscr = fixed ? nullptr : (double2*)w; w += b_c;
   unsigned long long* bar = (unsigned long long*)w;
-  int32_t* abort_flag = (int32_t*)(w + 64);
+  int32_t* abort_flag = (int32_t*)(w + b_bar);
   for (LongArgs* a : {&ab, &ac}) {
     a->ops = d_ops;
     a->part = part;
