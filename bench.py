@@ -43,7 +43,7 @@ PULSE_STEPS = int(round(8 * FWHM_FS / DT_FS))   # steps with a non-zero field
 MEASURED_TRAFFIC = {
     ("ell", 40, 512, 12, 1): (351.1e6, "profiles/k_lanczos_ell_linear_j40_b512_m12_pmc.md"),
     ("subspace", 40, 512, 12, 200): (2.12e6, "profiles/k_sub_prop_linear_j40_b512_m12.md"),
-    ("pair", 40, 512, 12, 200): (1.99e6, "profiles/k_sub_pair_linear_j40_b512_m12.md"),
+    ("pair", 40, 512, 12, 200): (2.03e6, "profiles/k_sub_pair_linear_j40_b512_m12.md"),
 }
 
 
