@@ -48,6 +48,9 @@ constexpr int XL_ROWS = 2048;    // block form: matvec input staged in LDS up to
 constexpr unsigned SPIN_LIMIT = 1u << 22;   // polls of a grid barrier before the launch gives up
 constexpr int BAR_LINES = 8;                // arrival counters of a grid barrier
 constexpr int BAR_STRIDE = 32;              // u64 between counters (one 256-byte line each)
+// an empty slot of the scalar-only sums' partials: a signalling-NaN bit
+// pattern, which no floating-point operation produces
+constexpr unsigned long long LSENT = 0x7FF4DEAD0000BEEFull;
 
 struct LongArgs {
   const SubOp* ops;
@@ -57,6 +60,7 @@ struct LongArgs {
   int64_t wg_bytes;        // bytes per workgroup region (block form)
   int64_t offV, offWb, offWf, offY, offS, offP, offA;   // byte offsets in a region
   double* part;            // [2][grid][4] workgroup partial sums (cooperative)
+  double* lpart;           // [2][grid][4] partials of the scalar-only sums (LSENT at launch)
   double2* gram;           // [grid][kcap * kcap] (adaptive, orders beyond GL)
   double2* cscr;           // [grid][2 * kcap] (adaptive, orders beyond GL)
   unsigned long long* bar; // grid-barrier arrivals, BAR_LINES counters (zero at launch)
@@ -68,7 +72,10 @@ struct LongArgs {
 
 struct SyncState {
   unsigned long long target = 0;
-  int epoch = 0;
+  int epoch = 0;       // LDS half of the next sum
+  int pepoch = 0;      // half of `part` of the next grid sum
+  int lepoch = 0;      // half of `lpart` of the next scalar-only sum
+  bool lpend = false;  // the last scalar-only sum's slots wait for their reset
 };
 
 __device__ __forceinline__ double2 xadd(double2 a, double2 b) { return cadd(a, b); }
@@ -117,6 +124,16 @@ __device__ __forceinline__ void grid_barrier(const LongArgs& a, SyncState& ss, i
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
   __syncthreads();
+  // every workgroup read the last scalar-only sum before it arrived here: this
+  // workgroup's slots of it go back to empty, which the readers of the next
+  // sum on the same half see after the grid barrier between the two sums
+  if (ss.lpend) {
+    if (threadIdx.x < 4)
+      __hip_atomic_store((unsigned long long*)a.lpart + (size_t)((ss.lepoch - 1) & 1) * gridDim.x * 4 +
+                             blockIdx.x * 4 + threadIdx.x,
+                         LSENT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    ss.lpend = false;
+  }
 }
 
 // Every thread of the cooperating set (the workgroup, or the grid) waits.
@@ -131,7 +148,19 @@ __device__ __forceinline__ void sync_all(const LongArgs& a, SyncState& ss, int* 
 // Sum of NQ values over every thread of the cooperating set; every thread
 // returns the same sums (fixed order at every level).  s_red: [2][CWV][4],
 // the halves used in turn.
-template <int NQ, int CWV, int MODE>
+//
+// SCALAR (cooperative form): the sum orders nothing but itself -- after it a
+// workgroup reads only vector entries it wrote -- so it needs no grid barrier.
+// Each workgroup stores its partials with agent-scope atomic stores into
+// slots that hold LSENT while empty; wave 0 of every workgroup polls all
+// slots (agent-scope atomic loads) until none is empty and adds them in the
+// same order as the barrier form.  Consecutive scalar sums alternate between
+// two halves of `lpart`, and a grid barrier separates any two of them: a
+// workgroup empties its slots of a half after the first grid barrier that
+// follows the sum (all reads of it are done by then), and the next sum on
+// that half comes after a further grid barrier.  The poll is bounded as the
+// grid barrier's wait is.
+template <int NQ, int CWV, int MODE, bool SCALAR = false>
 __device__ void coop_sum(double (&v)[NQ], const LongArgs& a, SyncState& ss, double (*s_red)[4],
                          double* s_tot, int* s_ab) {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -151,8 +180,54 @@ __device__ void coop_sum(double (&v)[NQ], const LongArgs& a, SyncState& ss, doub
       for (int w = 0; w < CWV; ++w) t += red[w][q];
       v[q] = t;
     }
+  } else if constexpr (SCALAR) {
+    unsigned long long* lp = (unsigned long long*)a.lpart + (size_t)(ss.lepoch & 1) * gridDim.x * 4;
+    ss.lepoch += 1;
+    ss.lpend = true;
+    if (tid < NQ) {
+      double t = 0.0;
+      for (int w = 0; w < CWV; ++w) t += red[w][tid];
+      __hip_atomic_store(lp + blockIdx.x * 4 + tid, (unsigned long long)__double_as_longlong(t), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (wv == 0) {
+      double acc[NQ];
+      unsigned spins = 0;
+      for (;;) {
+        bool full = true;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          acc[q] = 0.0;
+          for (int g = lane; g < (int)gridDim.x; g += 64) {
+            const unsigned long long u = __hip_atomic_load(lp + g * 4 + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            full = full && u != LSENT;
+            acc[q] += __longlong_as_double((long long)u);
+          }
+        }
+        if (__all(full) || *s_ab) break;
+        __builtin_amdgcn_s_sleep(1);
+        ++spins;
+        if ((spins & 255u) == 0 &&
+            (spins >= SPIN_LIMIT || __hip_atomic_load(a.abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+          if (lane == 0) {
+            __hip_atomic_store(a.abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            *s_ab = 1;
+          }
+          break;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        const double t = wave_allsum(acc[q]);
+        if (lane == 0) s_tot[q] = t;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) v[q] = s_tot[q];
   } else {
-    double* part = a.part + (size_t)ss.epoch * gridDim.x * 4;
+    double* part = a.part + (size_t)ss.pepoch * gridDim.x * 4;
+    ss.pepoch ^= 1;
     if (tid < NQ) {
       double t = 0.0;
       for (int w = 0; w < CWV; ++w) t += red[w][tid];
@@ -248,7 +323,13 @@ __global__ __launch_bounds__(ct_of<MODE>()) void k_long(LongArgs a) {
     const int b = item.x, lo = item.y, len = item.z;
     const int64_t e_lo = op.rp[lo];
     const int ngroups = (len + RPW - 1) / RPW;
-    for (int64_t i = t0; i < len; i += nthr) {
+    // per-row loops: REG runs each row on its matvec owner (lane sub == 0 of
+    // row gw * RPW + rw), so that after a scalar-only sum a workgroup reads
+    // only vector entries it wrote itself; the other forms stride the rows
+    // over every thread of the cooperating set
+    const int r_beg = !REG ? (int)t0 : (sub == 0 && gw * RPW + rw < len) ? gw * RPW + rw : len;
+    const int r_step = REG ? len : (int)nthr;
+    for (int64_t i = r_beg; i < len; i += r_step) {
       const int orig = op.perm[lo + i];
       S[i] = op.psi[(int64_t)b * op.ld + orig];
       P[i] = op.phase ? op.phase[orig] : make_double2(1.0, 0.0);
@@ -287,7 +368,7 @@ __global__ __launch_bounds__(ct_of<MODE>()) void k_long(LongArgs a) {
       if (op.sflag[step] == 0) {
         // no field: the expv is skipped (tdse.py:377), phases only
         if (op.phase)
-          for (int64_t i = t0; i < len; i += nthr) S[i] = cmul(cmul(S[i], P[i]), P[i]);
+          for (int64_t i = r_beg; i < len; i += r_step) S[i] = cmul(cmul(S[i], P[i]), P[i]);
         if (t0 == 0 && op.kused) op.kused[(int64_t)step * op.kused_ld + b - op.kcol0] = 0;
         continue;
       }
@@ -326,7 +407,7 @@ __global__ __launch_bounds__(ct_of<MODE>()) void k_long(LongArgs a) {
       }
       const XT f0 = to_x<F32>(direct ? fp[0] : zero);
       // V_0 = phi = P psi (unnormalised, tdse.py:443-446), the first matvec input
-      for (int64_t i = t0; i < len; i += nthr) {
+      for (int64_t i = r_beg; i < len; i += r_step) {
         const double2 v = op.phase ? cmul(S[i], P[i]) : S[i];
         V[i] = v;
         Wb[i] = v;
@@ -414,11 +495,12 @@ __global__ __launch_bounds__(ct_of<MODE>()) void k_long(LongArgs a) {
             finish_row(rb, accb);
           }
         }
+        // alpha (and |V_0|^2): REG reads after it only its own rows
         if (it == 0 && !FIXED) {
-          coop_sum<3, CWV, MODE>(red, a, ss, s_red, s_tot, &s_ab);
+          coop_sum<3, CWV, MODE, REG>(red, a, ss, s_red, s_tot, &s_ab);
         } else {
           double r2[2] = {red[0], red[1]};
-          coop_sum<2, CWV, MODE>(r2, a, ss, s_red, s_tot, &s_ab);
+          coop_sum<2, CWV, MODE, REG>(r2, a, ss, s_red, s_tot, &s_ab);
           red[0] = r2[0];
           red[1] = r2[1];
         }
@@ -427,7 +509,7 @@ __global__ __launch_bounds__(ct_of<MODE>()) void k_long(LongArgs a) {
         // W = w - alpha V_it - beta V_{it-1}, and |W|^2
         const double2* Vprev = V + (int64_t)(it > 0 ? it - 1 : 0) * lmax;
         double bs = 0.0;
-        for (int64_t i = t0; i < len; i += nthr) {
+        for (int64_t i = r_beg; i < len; i += r_step) {
           double2 wn = csub(Y[i], cmul(al, Vit[i]));
           if (it > 0) wn = csub(wn, cscale(beta, Vprev[i]));
           Wb[i] = wn;
@@ -539,7 +621,7 @@ __global__ __launch_bounds__(ct_of<MODE>()) void k_long(LongArgs a) {
         }
       }
       __syncthreads();
-      for (int64_t i = t0; i < len; i += nthr) {
+      for (int64_t i = r_beg; i < len; i += r_step) {
         double2 u = zero;
         for (int j = 0; j < k; ++j) u = cfma(s_cf[j], V[(int64_t)j * lmax + i], u);
         S[i] = op.phase ? cmul(u, P[i]) : u;
@@ -551,7 +633,7 @@ __global__ __launch_bounds__(ct_of<MODE>()) void k_long(LongArgs a) {
       // the next step's V_0 is written by the same threads; s_cf and the LDS
       // Gram matrix are rewritten only after the next step's first barrier
     }
-    for (int64_t i = t0; i < len; i += nthr) op.psi[(int64_t)b * op.ld + op.perm[lo + i]] = S[i];
+    for (int64_t i = r_beg; i < len; i += r_step) op.psi[(int64_t)b * op.ld + op.perm[lo + i]] = S[i];
     if (MODE != BLOCK && t0 == 0 && op.status &&
         (s_ab || __hip_atomic_load(a.abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
       op.status[b - op.kcol0] |= RM_DEVICE_TIMEOUT;
@@ -561,9 +643,10 @@ __global__ __launch_bounds__(ct_of<MODE>()) void k_long(LongArgs a) {
   }
 }
 
-__global__ void k_long_reset(unsigned long long* bar, int32_t* abort_flag) {
+__global__ void k_long_reset(unsigned long long* bar, int32_t* abort_flag, unsigned long long* lpart, int n_lpart) {
   if (threadIdx.x < BAR_LINES) bar[BAR_STRIDE * threadIdx.x] = 0ull;
   if (threadIdx.x == 0) *abort_flag = 0;
+  for (int i = threadIdx.x; i < n_lpart; i += blockDim.x) lpart[i] = LSENT;
 }
 
 struct LongScratch {
@@ -687,10 +770,11 @@ int long_run(void** scratch, const SubOp* d_ops, const int4* d_blk, const std::v
   }
   const int gmax = std::max(std::max(gres_r, gres_m), nwg);
   const size_t b_part = al256(sizeof(double) * 2 * 4 * (size_t)gmax);
+  const size_t b_lpart = b_part;
   const size_t b_gram = fixed ? 0 : al256(sizeof(double2) * (size_t)gmax * kcap * kcap);
   const size_t b_c = fixed ? 0 : al256(sizeof(double2) * (size_t)gmax * 2 * kcap);
   const size_t b_bar = sizeof(unsigned long long) * BAR_STRIDE * BAR_LINES;
-  const size_t need = al256(b_bytes) + al256(c_bytes) + b_part + b_gram + b_c + b_bar + 256;
+  const size_t need = al256(b_bytes) + al256(c_bytes) + b_part + b_lpart + b_gram + b_c + b_bar + 256;
   LongScratch* sc = (LongScratch*)*scratch;
   if (!sc) {
     sc = new (std::nothrow) LongScratch();
@@ -711,6 +795,7 @@ int long_run(void** scratch, const SubOp* d_ops, const int4* d_blk, const std::v
   char* wb = w;            w += al256(b_bytes);
   char* wc = w;            w += al256(c_bytes);
   double* part = (double*)w; w += b_part;
+  double* lpart = (double*)w; w += b_lpart;
   double2* gram = fixed ? nullptr : (double2*)w; w += b_gram;
   double2* cscr = fixed ? nullptr : (double2*)w; w += b_c;
   unsigned long long* bar = (unsigned long long*)w;
@@ -718,6 +803,7 @@ int long_run(void** scratch, const SubOp* d_ops, const int4* d_blk, const std::v
   for (LongArgs* a : {&ab, &ac}) {
     a->ops = d_ops;
     a->part = part;
+    a->lpart = lpart;
     a->gram = gram;
     a->cscr = cscr;
     a->bar = bar;
@@ -752,7 +838,8 @@ int long_run(void** scratch, const SubOp* d_ops, const int4* d_blk, const std::v
     const bool reg = groups <= (int64_t)gres_r * wr && it.maxrow <= LPR * UR;
     const int wpg = reg ? wr : wm;
     const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((groups + wpg - 1) / wpg, reg ? gres_r : gres_m));
-    hipLaunchKernelGGL(k_long_reset, dim3(1), dim3(64), 0, s, bar, abort_flag);
+    hipLaunchKernelGGL(k_long_reset, dim3(1), dim3(256), 0, s, bar, abort_flag, (unsigned long long*)lpart,
+                       2 * 4 * gmax);
     HIPCHK(hipGetLastError());
     if (reg)
       HIPCHK((launch_m<GRID_REG>(ac, grid, f32, 0, s)));
