@@ -48,9 +48,12 @@ constexpr int XL_ROWS = 2048;    // block form: matvec input staged in LDS up to
 constexpr unsigned SPIN_LIMIT = 1u << 22;   // polls of a grid barrier before the launch gives up
 constexpr int BAR_LINES = 8;                // arrival counters of a grid barrier
 constexpr int BAR_STRIDE = 32;              // u64 between counters (one 256-byte line each)
-// an empty slot of the scalar-only sums' partials: a signalling-NaN bit
-// pattern, which no floating-point operation produces
+// an empty slot of the slot sums' partials: a signalling-NaN bit pattern,
+// which no floating-point operation produces
 constexpr unsigned long long LSENT = 0x7FF4DEAD0000BEEFull;
+constexpr int LRING = 4;   // slot sums use LRING sets of slots in turn
+// forms of a cooperative sum (coop_sum)
+enum { SUM_BARRIER = 0, SUM_SCALAR = 1, SUM_FENCED = 2 };
 
 struct LongArgs {
   const SubOp* ops;
@@ -60,7 +63,7 @@ struct LongArgs {
   int64_t wg_bytes;        // bytes per workgroup region (block form)
   int64_t offV, offWb, offWf, offY, offS, offP, offA;   // byte offsets in a region
   double* part;            // [2][grid][4] workgroup partial sums (cooperative)
-  double* lpart;           // [2][grid][4] partials of the scalar-only sums (LSENT at launch)
+  double* lpart;           // [LRING][grid][4] partials of the slot sums (LSENT at launch)
   double2* gram;           // [grid][kcap * kcap] (adaptive, orders beyond GL)
   double2* cscr;           // [grid][2 * kcap] (adaptive, orders beyond GL)
   unsigned long long* bar; // grid-barrier arrivals, BAR_LINES counters (zero at launch)
@@ -74,9 +77,22 @@ struct SyncState {
   unsigned long long target = 0;
   int epoch = 0;       // LDS half of the next sum
   int pepoch = 0;      // half of `part` of the next grid sum
-  int lepoch = 0;      // half of `lpart` of the next scalar-only sum
-  bool lpend = false;  // the last scalar-only sum's slots wait for their reset
+  int nsl = 0;         // slot sums so far (the next one uses slot set nsl % LRING)
+  int pend = -1;       // the slot sum whose slots this workgroup has not emptied yet
 };
+
+// Empties this workgroup's slots of slot sum ss.pend.  Called when every
+// workgroup is past its reads of that sum: at the end of the next grid
+// barrier or slot sum.
+__device__ __forceinline__ void slot_release(const LongArgs& a, SyncState& ss) {
+  if (ss.pend >= 0) {
+    if (threadIdx.x < 4)
+      __hip_atomic_store((unsigned long long*)a.lpart + (size_t)(ss.pend % LRING) * gridDim.x * 4 +
+                             blockIdx.x * 4 + threadIdx.x,
+                         LSENT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    ss.pend = -1;
+  }
+}
 
 __device__ __forceinline__ double2 xadd(double2 a, double2 b) { return cadd(a, b); }
 __device__ __forceinline__ float2 xadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
@@ -124,16 +140,7 @@ __device__ __forceinline__ void grid_barrier(const LongArgs& a, SyncState& ss, i
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
   __syncthreads();
-  // every workgroup read the last scalar-only sum before it arrived here: this
-  // workgroup's slots of it go back to empty, which the readers of the next
-  // sum on the same half see after the grid barrier between the two sums
-  if (ss.lpend) {
-    if (threadIdx.x < 4)
-      __hip_atomic_store((unsigned long long*)a.lpart + (size_t)((ss.lepoch - 1) & 1) * gridDim.x * 4 +
-                             blockIdx.x * 4 + threadIdx.x,
-                         LSENT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    ss.lpend = false;
-  }
+  slot_release(a, ss);
 }
 
 // Every thread of the cooperating set (the workgroup, or the grid) waits.
@@ -149,18 +156,24 @@ __device__ __forceinline__ void sync_all(const LongArgs& a, SyncState& ss, int* 
 // returns the same sums (fixed order at every level).  s_red: [2][CWV][4],
 // the halves used in turn.
 //
-// SCALAR (cooperative form): the sum orders nothing but itself -- after it a
-// workgroup reads only vector entries it wrote -- so it needs no grid barrier.
-// Each workgroup stores its partials with agent-scope atomic stores into
-// slots that hold LSENT while empty; wave 0 of every workgroup polls all
-// slots (agent-scope atomic loads) until none is empty and adds them in the
-// same order as the barrier form.  Consecutive scalar sums alternate between
-// two halves of `lpart`, and a grid barrier separates any two of them: a
-// workgroup empties its slots of a half after the first grid barrier that
-// follows the sum (all reads of it are done by then), and the next sum on
-// that half comes after a further grid barrier.  The poll is bounded as the
-// grid barrier's wait is.
-template <int NQ, int CWV, int MODE, bool SCALAR = false>
+// Slot forms (cooperative form only).  Each workgroup stores its partials
+// with agent-scope atomic stores into slots that hold LSENT while empty; wave
+// 0 of every workgroup polls all slots (agent-scope atomic loads) until none
+// is empty and adds them in the barrier form's order (same result).
+// SUM_SCALAR orders nothing but the sum itself: after it a workgroup reads
+// only vector entries it wrote.  SUM_FENCED also publishes the workgroups'
+// writes, as a grid barrier does: every wave waits for its stores, the
+// storing thread releases at agent scope before its slot store, and wave 0
+// acquires at agent scope after the poll (release fence, relaxed store ->
+// relaxed load, acquire fence).  Slot sums use LRING slot sets in turn.  A
+// workgroup empties its slots of a sum at the end of the next grid barrier or
+// slot sum (every workgroup is past its reads by then: it stored its next
+// partial, which depends on them, or it arrived at the barrier), and the
+// callers never run two SUM_SCALAR sums without a grid barrier or a
+// SUM_FENCED sum between them, so between that emptying and the next use of
+// the set (LRING sums on) lies a fenced event that publishes it.  The poll is
+// bounded as the grid barrier's wait is.
+template <int NQ, int CWV, int MODE, int FORM = SUM_BARRIER>
 __device__ void coop_sum(double (&v)[NQ], const LongArgs& a, SyncState& ss, double (*s_red)[4],
                          double* s_tot, int* s_ab) {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -171,6 +184,7 @@ __device__ void coop_sum(double (&v)[NQ], const LongArgs& a, SyncState& ss, doub
   if (lane == 0)
 #pragma unroll
     for (int q = 0; q < NQ; ++q) red[wv][q] = v[q];
+  if constexpr (MODE != BLOCK && FORM == SUM_FENCED) __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   if constexpr (MODE == BLOCK) {
     // the other half is written by the next sum; this one only after the next sum's barrier
@@ -180,13 +194,13 @@ __device__ void coop_sum(double (&v)[NQ], const LongArgs& a, SyncState& ss, doub
       for (int w = 0; w < CWV; ++w) t += red[w][q];
       v[q] = t;
     }
-  } else if constexpr (SCALAR) {
-    unsigned long long* lp = (unsigned long long*)a.lpart + (size_t)(ss.lepoch & 1) * gridDim.x * 4;
-    ss.lepoch += 1;
-    ss.lpend = true;
+  } else if constexpr (FORM != SUM_BARRIER) {
+    const int n = ss.nsl++;
+    unsigned long long* lp = (unsigned long long*)a.lpart + (size_t)(n % LRING) * gridDim.x * 4;
     if (tid < NQ) {
       double t = 0.0;
       for (int w = 0; w < CWV; ++w) t += red[w][tid];
+      if constexpr (FORM == SUM_FENCED) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
       __hip_atomic_store(lp + blockIdx.x * 4 + tid, (unsigned long long)__double_as_longlong(t), __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
     }
@@ -216,6 +230,7 @@ __device__ void coop_sum(double (&v)[NQ], const LongArgs& a, SyncState& ss, doub
           break;
         }
       }
+      if constexpr (FORM == SUM_FENCED) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
         const double t = wave_allsum(acc[q]);
@@ -223,6 +238,8 @@ __device__ void coop_sum(double (&v)[NQ], const LongArgs& a, SyncState& ss, doub
       }
     }
     __syncthreads();
+    slot_release(a, ss);
+    ss.pend = n;
 #pragma unroll
     for (int q = 0; q < NQ; ++q) v[q] = s_tot[q];
   } else {
@@ -496,11 +513,12 @@ __global__ __launch_bounds__(ct_of<MODE>()) void k_long(LongArgs a) {
           }
         }
         // alpha (and |V_0|^2): REG reads after it only its own rows
+        constexpr int FA = REG ? SUM_SCALAR : SUM_BARRIER;
         if (it == 0 && !FIXED) {
-          coop_sum<3, CWV, MODE, REG>(red, a, ss, s_red, s_tot, &s_ab);
+          coop_sum<3, CWV, MODE, FA>(red, a, ss, s_red, s_tot, &s_ab);
         } else {
           double r2[2] = {red[0], red[1]};
-          coop_sum<2, CWV, MODE, REG>(r2, a, ss, s_red, s_tot, &s_ab);
+          coop_sum<2, CWV, MODE, FA>(r2, a, ss, s_red, s_tot, &s_ab);
           red[0] = r2[0];
           red[1] = r2[1];
         }
@@ -549,9 +567,9 @@ __global__ __launch_bounds__(ct_of<MODE>()) void k_long(LongArgs a) {
           __threadfence_block();
           __syncthreads();
         }
-        // beta_k = |W| (tdse.py:453)
+        // beta_k = |W| (tdse.py:453); publishes W, the next matvec input
         double bsum[1] = {bs};
-        coop_sum<1, CWV, MODE>(bsum, a, ss, s_red, s_tot, &s_ab);
+        coop_sum<1, CWV, MODE, REG ? SUM_FENCED : SUM_BARRIER>(bsum, a, ss, s_red, s_tot, &s_ab);
         beta = sqrt(bsum[0]);
         if (tid < 64) s_beta[k] = beta;
         brk = !(beta != 0.0);
@@ -770,7 +788,7 @@ int long_run(void** scratch, const SubOp* d_ops, const int4* d_blk, const std::v
   }
   const int gmax = std::max(std::max(gres_r, gres_m), nwg);
   const size_t b_part = al256(sizeof(double) * 2 * 4 * (size_t)gmax);
-  const size_t b_lpart = b_part;
+  const size_t b_lpart = al256(sizeof(double) * LRING * 4 * (size_t)gmax);
   const size_t b_gram = fixed ? 0 : al256(sizeof(double2) * (size_t)gmax * kcap * kcap);
   const size_t b_c = fixed ? 0 : al256(sizeof(double2) * (size_t)gmax * 2 * kcap);
   const size_t b_bar = sizeof(unsigned long long) * BAR_STRIDE * BAR_LINES;
@@ -839,7 +857,7 @@ int long_run(void** scratch, const SubOp* d_ops, const int4* d_blk, const std::v
     const int wpg = reg ? wr : wm;
     const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((groups + wpg - 1) / wpg, reg ? gres_r : gres_m));
     hipLaunchKernelGGL(k_long_reset, dim3(1), dim3(256), 0, s, bar, abort_flag, (unsigned long long*)lpart,
-                       2 * 4 * gmax);
+                       LRING * 4 * gmax);
     HIPCHK(hipGetLastError());
     if (reg)
       HIPCHK((launch_m<GRID_REG>(ac, grid, f32, 0, s)));
