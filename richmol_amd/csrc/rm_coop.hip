@@ -18,6 +18,8 @@
 // iteration needs two barriers: after the matvec (alpha) and after the
 // three-term update (beta); the basis vector V_k = W/beta is formed inside
 // the matvec from W, so the beta barrier also publishes the matvec input.
+// The cooperative register form replaces both by slot sums (coop_sum): the
+// alpha sum orders nothing but itself, the beta sum is fenced.
 //
 // Cooperative form with every wave holding at most one group of RPW rows and
 // rows of at most LPR*UR entries: a lane keeps the columns and the step's
