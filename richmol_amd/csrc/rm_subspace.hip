@@ -53,6 +53,7 @@ __global__ __launch_bounds__(64) void k_sub_prop(SubArgs a) {
   using XT = typename XType<F32>::T;
   constexpr int NR = 64 * R;
   constexpr int WMR = WM > 0 ? WM : 1;
+  constexpr int UJ = R >= 8 ? 1 : 8 / R;   // entries per row loaded together (WM == 0)
   extern __shared__ char s_raw[];
   XT* s_x = (XT*)s_raw;
   __shared__ double2 s_alpha[KMAX];
@@ -159,16 +160,44 @@ __global__ __launch_bounds__(64) void k_sub_prop(SubArgs a) {
 #pragma unroll
             for (int r = 0; r < R; ++r) acc[r] = xfma(A[j][r], s_x[cidx[j][r]], acc[r]);
         } else {
-          for (int j = 0; j < W; ++j) {
+          bool done = false;
+          if constexpr (R <= 4) {
+            if (op.nslot <= NSR) {
+              // UJ entries per row at a time, all their loads issued before the
+              // first is used (same products, same order as the loop below)
+              for (int j0 = 0; j0 < W; j0 += UJ) {
+                int c[UJ][R];
+                XT v[UJ][R];
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-              const int64_t q = (int64_t)j * n + gq[r];
-              const int c = op.col[q] - lo;
-              double2 v = z0;
-              for (int s = 0; s < op.nslot; ++s) v = cfma(fp[s], op.val[(int64_t)s * W * n + q], v);
-              acc[r] = xfma(to_x<F32>(v), s_x[c], acc[r]);
+                for (int u = 0; u < UJ; ++u)
+#pragma unroll
+                  for (int r = 0; r < R; ++r) {
+                    const int64_t q = (int64_t)(j0 + u < W ? j0 + u : 0) * n + gq[r];
+                    c[u][r] = op.col[q] - lo;
+                    double2 e = cfma(fp[0], op.val[q], z0);
+                    if (NSR > 1 && op.nslot > 1) e = cfma(fp[1], op.val[(int64_t)W * n + q], e);
+                    v[u][r] = to_x<F32>(e);
+                  }
+#pragma unroll
+                for (int u = 0; u < UJ; ++u)
+                  if (j0 + u < W)
+#pragma unroll
+                    for (int r = 0; r < R; ++r) acc[r] = xfma(v[u][r], s_x[c[u][r]], acc[r]);
+              }
+              done = true;
             }
           }
+          if (!done)
+            for (int j = 0; j < W; ++j) {
+#pragma unroll
+              for (int r = 0; r < R; ++r) {
+                const int64_t q = (int64_t)j * n + gq[r];
+                const int c = op.col[q] - lo;
+                double2 v = z0;
+                for (int s = 0; s < op.nslot; ++s) v = cfma(fp[s], op.val[(int64_t)s * W * n + q], v);
+                acc[r] = xfma(to_x<F32>(v), s_x[c], acc[r]);
+              }
+            }
         }
 #pragma unroll
         for (int r = 0; r < R; ++r) Y[r] = ok[r] ? to_d2(acc[r]) : z0;
@@ -753,19 +782,22 @@ int sub_build(rm_plan* p) {
 //      paired with an item of the same operator or with an empty half
 //   7: the same with a banded domain (every entry within one row of the
 //      diagonal, <= 2 field slots): neighbours by DPP shifts (k_sub_pair BAND)
-constexpr int NCLASS = 8;
+//   8: R=1, 9: R=2 reading the operator at every matvec (ELL width > 4)
+constexpr int NCLASS = 10;
 constexpr int PAIR_CLASS = 6;
 constexpr int PAIR_BAND_CLASS = 7;
 constexpr int PAIR_MAX_M = 16;
-constexpr int CLASS_R[NCLASS] = {1, 2, 4, 4, 8, 16, 1, 1};
+constexpr int CLASS_R[NCLASS] = {1, 2, 4, 4, 8, 16, 1, 1, 1, 2};
 
 int sub_class(int w, int len) {
   if (w <= 4) {
     if (len <= 64) return 0;
     if (len <= 128) return 1;
     if (len <= 256) return 2;
-  } else if (len <= 256) {
-    return 3;
+  } else {
+    if (len <= 64) return 8;
+    if (len <= 128) return 9;
+    if (len <= 256) return 3;
   }
   return len <= 512 ? 4 : 5;
 }
@@ -773,7 +805,7 @@ int sub_class(int w, int len) {
 // Krylov basis in LDS: register classes whose basis fits.
 bool class_basis_lds(int c, int kcap) {
   if (c == PAIR_CLASS || c == PAIR_BAND_CLASS) return true;   // static LDS of k_sub_pair
-  return c <= 2 && (size_t)kcap * 64 * CLASS_R[c] * sizeof(double2) <= SUB_LDS_BASIS;
+  return (c <= 2 || c >= 8) && (size_t)kcap * 64 * CLASS_R[c] * sizeof(double2) <= SUB_LDS_BASIS;
 }
 
 template <bool F32, bool BAND>
@@ -830,6 +862,8 @@ hipError_t launch_class_p(int c, const SubArgs& a, int n_cu, hipStream_t s, int*
     case 2: return launch_sub_b<4, 4, FIXED, F32>(c, a, n_cu, s, waves, dry);
     case 3: return launch_sub_k<4, 0, FIXED, F32, false>(a, n_cu, s, waves, dry);
     case 4: return launch_sub_k<8, 0, FIXED, F32, false>(a, n_cu, s, waves, dry);
+    case 8: return launch_sub_b<1, 0, FIXED, F32>(c, a, n_cu, s, waves, dry);
+    case 9: return launch_sub_b<2, 0, FIXED, F32>(c, a, n_cu, s, waves, dry);
     default: return launch_sub_k<16, 0, FIXED, F32, false>(a, n_cu, s, waves, dry);
   }
 }

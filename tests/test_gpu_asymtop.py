@@ -90,3 +90,26 @@ def test_water_j20_configuration(device, kernel):
     psi_o, k_o = _oracle(h0, leaves, fields, X)
     assert np.abs(psi_d - psi_o).max() <= TOL_PSI
     assert np.array_equal(k_d, k_o)
+
+
+@pytest.mark.parametrize("fixed", [0, 12])
+@pytest.mark.parametrize("jmax", [4, 6])
+def test_water_wide_operator_classes(device, jmax, fixed):
+    """Invariant subspaces of 40-45 (J <= 4) and 112-119 (J <= 6) states with
+    operator rows of up to 22 and 41 entries: the one- and two-rows-per-lane
+    subspace classes that read the operator at every matvec.  One basis state
+    per column (each column's domain is one subspace), adaptive and fixed
+    Krylov dimension, against the oracle."""
+    h0, leaves, bas = water(jmax)
+    n = sum(len(bas.mlist[J]) * sum(c.shape[1] for _, c in bas.kfun[J].values()) for J in bas.Jlist)
+    rng = np.random.default_rng(jmax)
+    cols = np.sort(rng.choice(n, size=6, replace=False))
+    X = np.zeros((len(cols), n), dtype=np.complex128)
+    X[np.arange(len(cols)), cols] = 1.0
+    t = (np.arange(10) + 75.5) * DT
+    fields = water_fields(t, fwhm_fs=60.0)
+    psi_d, k_d, _ = _device(h0, leaves, fields, X, fixed=fixed)
+    psi_o, k_o = _oracle(h0, leaves, fields, X, fixed=fixed or None)
+    assert np.abs(psi_d - psi_o).max() <= TOL_PSI
+    if not fixed:
+        assert np.array_equal(k_d, k_o)
