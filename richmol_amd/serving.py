@@ -149,21 +149,26 @@ def _domain(A, n):
     return 0, int(reach.size)
 
 
-# fixed latency of a field step (its ~12 Krylov iterations' reductions and
-# barriers), in the units of _cost: stored entries (about 20 ns each)
-STEP_FIXED_COST = 2000.0
+# Device time of one field step (microseconds, fixed Krylov dimension 12,
+# complex64 matvec) by launch form: (fixed, per stored entry of the domain).
+# Measured on MI355X with one request at a time (2 000 steps, 800 of them with
+# field; the tick overhead of ~10 us per step taken off): linear rotors on the
+# two-per-wave kernel ~6 us at any Jmax; asymmetric tops on one wave (domains
+# of 21 and 36 rows: 24 and 36 us); on one workgroup (66-496 rows, 1.4e3 to
+# 2.1e4 entries: 75, 124, 198, 426 us).
+STEP_US = {"pair": (6.0, 0.0), "wave": (15.0, 0.039), "block": (50.0, 0.0183)}
 
 
-def _cost(A, n):
-    """Time of one field step of the J = 0 state's job in stored-entry units:
-    the entries of its domain rows, those rows, and STEP_FIXED_COST (the
-    per-step cost the tick balances)."""
+def _domain_stats(A, n):
+    """(rows, stored entries, longest row) of the J = 0 state's domain (row
+    0's invariant subspace; host, from the sparsity pattern)."""
     from scipy.sparse import csr_matrix
     from scipy.sparse.csgraph import breadth_first_order
     P = csr_matrix(A)
     P = csr_matrix((np.ones(P.nnz), P.indices, P.indptr), shape=(n, n))
     reach = breadth_first_order(P + P.T, 0, directed=False, return_predecessors=False)
-    return float(np.diff(P.indptr)[reach].sum() + reach.size + STEP_FIXED_COST)
+    lens = np.diff(P.indptr)[reach]
+    return int(reach.size), int(lens.sum()), int(lens.max())
 
 
 class _Model:
@@ -185,7 +190,7 @@ class _Model:
         from scipy.sparse import csr_matrix
         P = csr_matrix(pattern, shape=(self.n, self.n))
         self.lo, self.len = _domain(P, self.n)
-        self.cost = _cost(P, self.n)
+        self.stats = _domain_stats(P, self.n)
 
     def energies(self, req):
         return req.B * self.J * (self.J + 1.0)
@@ -217,7 +222,7 @@ class _AsymModel:
         self.plan.subspaces()   # the subspace arrays now, during the running tick
         self.nslot = 1
         self.lo, self.len = _domain(self.H, self.n)
-        self.cost = _cost(self.H, self.n)
+        self.stats = _domain_stats(self.H, self.n)
 
     def energies(self, req):
         return self.E
@@ -263,6 +268,7 @@ class Server:
         self.handle = h
         if block_rows is not None:
             _lib.check(self.lib.rm_batch_set_block_rows(h, int(block_rows)), "rm_batch_set_block_rows")
+        self.block_rows = 1024 if block_rows is None else int(block_rows)
         bas = rot.linear_basis(1.0, jmax_master)
         self.master = rot.tensor(np.diag(LINEAR_ALPHA), bas)
         self.models = {}
@@ -280,7 +286,21 @@ class Server:
         m = self.models.get(jmax)
         if m is None:
             m = self.models[jmax] = _Model(self.master, jmax)
+            m.cost = self.step_cost(m)
         return m
+
+    def step_cost(self, m):
+        """Modelled device time of one field step of a model's J = 0 job
+        (STEP_US by the launch form rm_batch_propagate picks for its domain)."""
+        rows, nnz, width = m.stats
+        if rows > self.block_rows:
+            form = "block"
+        elif width <= 4 and rows <= 32 and 0 < self.kfix <= 16:
+            form = "pair"
+        else:
+            form = "wave"
+        a, b = STEP_US[form]
+        return a + b * nnz
 
     def footprint(self, req):
         """Resident device bytes of a request: wavepacket, phases, field table,
@@ -301,7 +321,11 @@ class Server:
 
     def _admit_one(self, req, now):
         torch = self.torch
-        m = self.model(req.jmax) if req.kind == "linear" else _AsymModel(req)
+        if req.kind == "linear":
+            m = self.model(req.jmax)
+        else:
+            m = _AsymModel(req)
+            m.cost = self.step_cost(m)
         dev = "cuda"
         E = m.energies(req)
         d = torch.from_numpy(E.astype(np.complex128)).to(dev, non_blocking=False)
