@@ -44,7 +44,12 @@ constexpr int ct_of() { return 512; }
 constexpr int LPR = 16;          // lanes per row in the matvec (one DPP row)
 constexpr int RPW = 64 / LPR;    // rows per wave in the matvec
 constexpr int UR = 12;           // register-resident entries per lane
-constexpr int UM = 8;            // entries per lane loaded together (memory mode)
+// memory-mode matvec: row groups per pass and entries per lane and row
+// loaded together (FP32 operands: 4 x 4; FP64 keeps 2 x 8 within 256 VGPRs)
+template <bool F32>
+constexpr int mg_of() { return F32 ? 4 : 2; }
+template <bool F32>
+constexpr int umg_of() { return F32 ? 4 : 8; }
 constexpr int GL = 32;           // Gram matrix order held in LDS (adaptive)
 constexpr int XL_ROWS = 2048;    // block form: matvec input staged in LDS up to this many rows
 constexpr unsigned SPIN_LIMIT = 1u << 22;   // polls of a grid barrier before the launch gives up
@@ -473,7 +478,6 @@ __global__ __launch_bounds__(ct_of<MODE>()) void k_long(LongArgs a) {
             finish_row(rreg, acc);
           }
         } else {
-          // two row groups per pass (rows ra, rb), their loads in flight together
           auto ecol = [&](int64_t q, int64_t e1) __attribute__((always_inline)) -> int {
             return q < e1 ? op.ccol[q] - lo : 0;
           };
@@ -481,37 +485,56 @@ __global__ __launch_bounds__(ct_of<MODE>()) void k_long(LongArgs a) {
             if (q >= e1) return to_x<F32>(make_double2(0.0, 0.0));
             return direct ? to_x<F32>(op.cval[q]) : Ast[q - e_lo];
           };
-          for (int rg = gw; rg < ngroups; rg += 2 * nwaves) {
-            const int ra = rg * RPW + rw, rb = (rg + nwaves) * RPW + rw;
-            const int64_t a0 = ra < len ? row_start(ra) : 0, a1 = ra < len ? row_start(ra + 1) : 0;
-            const int64_t b0 = rb < len ? row_start(rb) : 0, b1 = rb < len ? row_start(rb + 1) : 0;
-            XT acca = to_x<F32>(zero), accb = to_x<F32>(zero);
-            const int64_t lm = std::max(a1 - a0, b1 - b0);
-            for (int64_t t = sub; t < lm; t += (int64_t)LPR * UM) {
-              int ca[UM], cb[UM];
-              XT va[UM], vb[UM];
+          // MG row groups per pass, their loads in flight together; UMG entries
+          // per lane and row at a time
+          constexpr int MG = mg_of<F32>(), UMG = umg_of<F32>();
+          for (int rg = gw; rg < ngroups; rg += MG * nwaves) {
+            int rr[MG];
+            int64_t r0[MG], r1[MG];
+            XT acc[MG];
+            int64_t lm = 0;
 #pragma unroll
-              for (int u = 0; u < UM; ++u) {
-                ca[u] = ecol(a0 + t + (int64_t)u * LPR, a1);
-                va[u] = eval(a0 + t + (int64_t)u * LPR, a1);
-                cb[u] = ecol(b0 + t + (int64_t)u * LPR, b1);
-                vb[u] = eval(b0 + t + (int64_t)u * LPR, b1);
-              }
-              XT pa = to_x<F32>(zero), pb = to_x<F32>(zero);
+            for (int g = 0; g < MG; ++g) {
+              rr[g] = (rg + g * nwaves) * RPW + rw;
+              r0[g] = rr[g] < len ? row_start(rr[g]) : 0;
+              r1[g] = rr[g] < len ? row_start(rr[g] + 1) : 0;
+              acc[g] = to_x<F32>(zero);
+              lm = std::max(lm, r1[g] - r0[g]);
+            }
+            // the wave's longest row: entries of a pass beyond it are skipped
+            // (a wave-uniform bound, so short rows cost no idle slots)
+            int lmw = (int)lm;
+            lmw = max(lmw, __shfl_xor(lmw, 16));
+            lmw = max(lmw, __shfl_xor(lmw, 32));
+            lmw = __builtin_amdgcn_readfirstlane(lmw);
+            for (int64_t t = sub; t < lm; t += (int64_t)LPR * UMG) {
+              const int tw = (int)(t - sub);   // the same on every lane
+              int cc[MG][UMG];
+              XT vv[MG][UMG];
 #pragma unroll
-              for (int u = 0; u < UM; ++u) {
-                pa = xfma(va[u], Xin[ca[u]], pa);
-                pb = xfma(vb[u], Xin[cb[u]], pb);
-              }
-              acca = xadd(acca, pa);
-              accb = xadd(accb, pb);
+              for (int u = 0; u < UMG; ++u)
+                if (tw + u * LPR < lmw)
+#pragma unroll
+                  for (int g = 0; g < MG; ++g) {
+                    cc[g][u] = ecol(r0[g] + t + (int64_t)u * LPR, r1[g]);
+                    vv[g][u] = eval(r0[g] + t + (int64_t)u * LPR, r1[g]);
+                  }
+              XT pp[MG];
+#pragma unroll
+              for (int g = 0; g < MG; ++g) pp[g] = to_x<F32>(zero);
+#pragma unroll
+              for (int u = 0; u < UMG; ++u)
+                if (tw + u * LPR < lmw)
+#pragma unroll
+                  for (int g = 0; g < MG; ++g) pp[g] = xfma(vv[g][u], Xin[cc[g][u]], pp[g]);
+#pragma unroll
+              for (int g = 0; g < MG; ++g) acc[g] = xadd(acc[g], pp[g]);
             }
-            if (direct) {
-              acca = xfma(f0, acca, to_x<F32>(zero));
-              accb = xfma(f0, accb, to_x<F32>(zero));
+#pragma unroll
+            for (int g = 0; g < MG; ++g) {
+              if (direct) acc[g] = xfma(f0, acc[g], to_x<F32>(zero));
+              finish_row(rr[g], acc[g]);
             }
-            finish_row(ra, acca);
-            finish_row(rb, accb);
           }
         }
         // alpha (and |V_0|^2): REG reads after it only its own rows

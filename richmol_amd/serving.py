@@ -155,8 +155,8 @@ def _domain(A, n):
 # field; the tick overhead of ~10 us per step taken off): linear rotors on the
 # two-per-wave kernel ~6 us at any Jmax; asymmetric tops on one wave (domains
 # of 21 and 36 rows: 24 and 36 us); on one workgroup (66-496 rows, 1.4e3 to
-# 2.1e4 entries: 75, 124, 198, 426 us).
-STEP_US = {"pair": (6.0, 0.0), "wave": (15.0, 0.039), "block": (50.0, 0.0183)}
+# 2.1e4 entries: 48, 93, 147, 320 us).
+STEP_US = {"pair": (6.0, 0.0), "wave": (15.0, 0.039), "block": (30.0, 0.0142)}
 
 
 def _domain_stats(A, n):
