@@ -174,9 +174,11 @@ def rigid_rotor_matrix(J, Bx, By, Bz):
     return H
 
 
+@functools.lru_cache(maxsize=None)
 def _wang(J):
     """Wang functions of J as columns over k = -J..J (K = 0, then K = 1..J
-    with the + and - combinations) and their D2 species labels."""
+    with the + and - combinations) and their D2 species labels (cached:
+    callers read, never modify, the arrays)."""
     Kv = np.concatenate([[0], np.repeat(np.arange(1, J + 1), 2)]).astype(np.int64)
     Sv = np.concatenate([[1], np.tile([1, -1], J)]).astype(np.int64)
     W = np.zeros((2 * J + 1, Kv.size))
@@ -186,17 +188,20 @@ def _wang(J):
     W[J - Kv[1:], c[1:]] = Sv[1:] / math.sqrt(2.0)
     c2x = Sv * (-1) ** J
     labels = np.array([_D2[(int(K) % 2, int(x))] for K, x in zip(Kv, c2x)])
+    W.setflags(write=False)
+    labels.setflags(write=False)
     return W, labels
 
 
-def asymtop_basis(Bx, By, Bz, Jmax, Jmin=0, m_filter=None, J_filter=None):
+def asymtop_basis(Bx, By, Bz, Jmax, Jmin=0, m_filter=None, J_filter=None, species=None):
     """Rigid asymmetric-top basis (rot/solution.py:160-391, rot/basis.py:702-801):
     per J the eigenstates of Bx Jx^2 + By Jy^2 + Bz Jz^2 [cm^-1], with the
     rotational constants along the molecule-frame axes of the tensors that
     will be built on it.  The Wang functions split each J into the four D2
     species (the Hamiltonian couples Delta k = 0, +-2 and commutes with
     k -> -k); a species block is diagonalised on its own and its states are
-    stored as coefficient columns over primitive k = -J..J, ordered by energy."""
+    stored as coefficient columns over primitive k = -J..J, ordered by energy.
+    species: the D2 species to keep (all four by default)."""
     jl, ml, kf, en = [], {}, {}, {}
     for J in range(Jmin, Jmax + 1):
         if J_filter is not None and not J_filter(J):
@@ -208,18 +213,29 @@ def asymtop_basis(Bx, By, Bz, Jmax, Jmin=0, m_filter=None, J_filter=None):
         H = rigid_rotor_matrix(J, Bx, By, Bz)
         # Wang functions (columns over primitive k) and their D2 species
         W, labels = _wang(J)
-        jl.append(J)
-        ml[J] = ms
-        kf[J], en[J] = {}, {}
+        kj, ej = {}, {}
         for sym in ("A", "B1", "B2", "B3"):
             sel = labels == sym
-            if not np.any(sel):
+            if not np.any(sel) or (species is not None and sym not in species):
                 continue
             Wm = W[:, sel]
             e, v = np.linalg.eigh(Wm.T @ H @ Wm)
-            kf[J][sym] = (ks, (Wm @ v).astype(np.complex128))
-            en[J][sym] = e
+            kj[sym] = (ks, (Wm @ v).astype(np.complex128))
+            ej[sym] = e
+        if not kj:   # no kept species at this J
+            continue
+        jl.append(J)
+        ml[J] = ms
+        kf[J], en[J] = kj, ej
     return RotorBasis(jl, ml, kf, en)
+
+
+@functools.lru_cache(maxsize=None)
+def species_sizes(J):
+    """Number of states of each D2 species at J (the Wang-function count;
+    cached, do not modify)."""
+    _, labels = _wang(J)
+    return {sym: int(np.sum(labels == sym)) for sym in ("A", "B1", "B2", "B3")}
 
 
 def hamiltonian(basis):

@@ -201,6 +201,36 @@ class _Model:
         return self.plan.table_array(cols, nst)
 
 
+_SPECIES = {}
+
+
+def _reachable_species(alpha):
+    """D2 species of the m = 0 states reachable from |J=0, A> under a Z field
+    through the polarizability alpha (a symmetry property, the same at every
+    J: found on a J <= 4 basis with generic constants; cached per alpha)."""
+    key = np.asarray(alpha, dtype=np.float64).tobytes()
+    got = _SPECIES.get(key)
+    if got is None:
+        bas = rot.asymtop_basis(1.7, 1.3, 2.9, 4, m_filter=lambda J, m: m == 0)
+        _, _, sub = rot.zfield_m0_operator(bas, alpha, start=(0, "A"))
+        blocks = [(J, sp) for J in bas.Jlist for sp in bas.symlist[J]
+                  for _ in range(bas.kfun[J][sp][1].shape[1])]
+        got = _SPECIES[key] = frozenset(blocks[i][1] for i in sub)
+    return got
+
+
+def _m0_positions(bas, jmax):
+    """Whole-m = 0-basis position (every J <= jmax, all four species, in the
+    order J, species, state) of every state of a species-filtered basis."""
+    full, off = {}, 0
+    for J in range(jmax + 1):
+        for sp, ns in rot.species_sizes(J).items():
+            full[(J, sp)] = off
+            off += ns
+    pos = [full[(J, sp)] + np.arange(bas.kfun[J][sp][1].shape[1]) for J in bas.Jlist for sp in bas.symlist[J]]
+    return np.concatenate(pos).astype(np.int64)
+
+
 class _AsymModel:
     """Operator and device plan of one asymmetric-top request: H(F) =
     F_z^2 A on the invariant subspace of the J = 0 state inside the m = 0
@@ -212,9 +242,12 @@ class _AsymModel:
     def __init__(self, req):
         from .plan import DevicePlan
         a, b, c = req.abc
-        bas = rot.asymtop_basis(b, c, a, req.jmax, m_filter=lambda J, m: m == 0)
-        alpha = np.diag(ASYM_ALPHA) if req.alpha is None else req.alpha
-        A0, self.E, self.index = rot.zfield_m0_operator(bas, alpha, start=(0, "A"))
+        alpha = np.diag(ASYM_ALPHA) if req.alpha is None else np.asarray(req.alpha, dtype=np.float64)
+        # only the species the J = 0 state reaches are diagonalised and coupled
+        keep = _reachable_species(alpha)
+        bas = rot.asymtop_basis(b, c, a, req.jmax, m_filter=lambda J, m: m == 0, species=keep)
+        A0, self.E, sub = rot.zfield_m0_operator(bas, alpha, start=(0, "A"))
+        self.index = _m0_positions(bas, req.jmax)[sub]
         self.nfull = req.n
         self.H = A0 * (-0.5 * units.AUpol_x_Vm_to_invcm())
         self.n = A0.shape[0]
