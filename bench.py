@@ -294,7 +294,8 @@ def main():
                     help="kernel of the full-vector comparison run (tuned = autotuner cache)")
     ap.add_argument("--no-extras", action="store_true",
                     help="headline run only (no full-vector, precision, single-wavepacket, serving legs)")
-    ap.add_argument("--serve-requests", type=int, default=200)
+    ap.add_argument("--serve-requests", type=int, default=2000,
+                    help="requests of the serving stream (the serving configuration's 2000)")
     ap.add_argument("--asym-steps", type=int, default=2000,
                     help="steps of the asymmetric-top leg (0: skip)")
     ap.add_argument("--asym-only", action="store_true",
