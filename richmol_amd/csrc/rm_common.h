@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdint>
@@ -185,14 +186,17 @@ static int upload(T** dst, const T* src, int64_t count) {
 template <typename T>
 static int grow(T** buf, int64_t* cap, int64_t count) {
   if (*cap >= count && *buf) return RM_OK;
+  // headroom (at least 64 KB, at least twice the old size) so that calls of
+  // slightly different sizes do not reallocate
+  const int64_t want = std::max<int64_t>(std::max<int64_t>(count, 2 * *cap), (int64_t)(65536 / sizeof(T)));
   if (*buf) {
     HIPCHK(hipDeviceSynchronize());
     HIPCHK(hipFree(*buf));
     *buf = nullptr;
     *cap = 0;
   }
-  HIPCHK(hipMalloc((void**)buf, sizeof(T) * (size_t)(count > 0 ? count : 1)));
-  *cap = count;
+  HIPCHK(hipMalloc((void**)buf, sizeof(T) * (size_t)want));
+  *cap = want;
   return RM_OK;
 }
 

@@ -965,15 +965,21 @@ int sub_run(SubScratch* sc, const std::vector<SubOp>& ops, std::vector<int4> (&c
   sc->cur ^= 1;
   if (sc->done[bi]) HIPCHK(hipEventSynchronize(sc->done[bi]));
   if (sc->cap[bi] < bytes) {
-    if (sc->dev[bi]) HIPCHK(hipFree(sc->dev[bi]));
-    if (sc->host[bi]) HIPCHK(hipHostFree(sc->host[bi]));
-    sc->dev[bi] = nullptr;
-    sc->host[bi] = nullptr;
-    sc->cap[bi] = 0;
-    const size_t cap = bytes * 2;
-    HIPCHK(hipMalloc((void**)&sc->dev[bi], cap));
-    HIPCHK(hipHostMalloc((void**)&sc->host[bi], cap, hipHostMallocDefault));
-    sc->cap[bi] = cap;
+    // both buffers grow together (the other one after its launches are
+    // done), so that alternating calls of one size allocate only once
+    const size_t cap = std::max<size_t>(bytes * 2, 64 << 10);
+    for (int h : {bi, bi ^ 1}) {
+      if (sc->cap[h] >= cap) continue;
+      if (h != bi && sc->done[h]) HIPCHK(hipEventSynchronize(sc->done[h]));
+      if (sc->dev[h]) HIPCHK(hipFree(sc->dev[h]));
+      if (sc->host[h]) HIPCHK(hipHostFree(sc->host[h]));
+      sc->dev[h] = nullptr;
+      sc->host[h] = nullptr;
+      sc->cap[h] = 0;
+      HIPCHK(hipMalloc((void**)&sc->dev[h], cap));
+      HIPCHK(hipHostMalloc((void**)&sc->host[h], cap, hipHostMallocDefault));
+      sc->cap[h] = cap;
+    }
   }
   memcpy(sc->host[bi], ops.data(), ops.size() * sizeof(SubOp));
   size_t off[NCLASS];
