@@ -411,12 +411,17 @@ class Server:
         t0 = clock()
         done = []
 
-        def admit(now):
+        def admit(now, busy=None):
+            """Admit queued requests that fit; with ``busy`` (the running
+            tick's event) at least one and then only while the tick runs, so
+            host admission never holds the device idle between ticks."""
             nonlocal jobs
             changed = False
             while pending and pending[0].arrival <= now:
                 queue.append(pending.popleft())
             while queue and len(active) < self.max_jobs and self.used + self.footprint(queue[0]) <= self.budget:
+                if busy is not None and changed and busy.query():
+                    break
                 r = queue.popleft()
                 rec, keep, ns = self._admit(r, now)
                 active.append(r); recs.append(rec); keeps.append(keep); nslots.append(ns)
@@ -426,7 +431,8 @@ class Server:
 
         while pending or queue or active:
             now = clock() - t0
-            admit(now)
+            if not active:
+                admit(now)
             if not active:
                 if pending:
                     wait = pending[0].arrival - (clock() - t0)
@@ -463,7 +469,7 @@ class Server:
             ev.record(self.stream)
             # overlap: admit the requests that arrived meanwhile (they join the next tick)
             n_before = len(active)
-            admit(clock() - t0)
+            admit(clock() - t0, busy=ev)
             ev.synchronize()
             t_tick = clock() - t0
             self.ticks += 1
