@@ -215,3 +215,26 @@ def test_zfield_m0_operator_equals_cartens(tens):
     assert np.abs(A.toarray() - D).max() <= 1e-13 * np.abs(D).max()
     h0 = rot.hamiltonian(bas)
     assert np.array_equal(E, h0.tomat(form="full", cart="0").diagonal().real)
+
+
+@pytest.mark.parametrize("alpha", ["diag", "full"])
+def test_species_filtered_m0_operator_equals_full(alpha):
+    """Serving builds an asymmetric top's m = 0 operator only on the D2
+    species the J = 0 state reaches: the operator, energies and whole-basis
+    positions equal those built on the basis with all four species."""
+    from richmol_amd import serving
+    al = (np.diag(serving.ASYM_ALPHA) if alpha == "diag"
+          else np.array([[9.1, 0.3, 0.0], [0.3, 9.8, 0.2], [0.0, 0.2, 9.4]]))
+    keep = serving._reachable_species(al)
+    assert keep == ({"A"} if alpha == "diag" else {"A", "B1", "B2", "B3"})
+    rng = np.random.default_rng(11)
+    for jmax in (7, 16):
+        a, b, c = np.sort(rng.uniform(0.05, 30.0, 3))[::-1]
+        m0 = lambda J, m: m == 0
+        full = rot.asymtop_basis(b, c, a, jmax, m_filter=m0)
+        A1, E1, idx1 = rot.zfield_m0_operator(full, al, start=(0, "A"))
+        part = rot.asymtop_basis(b, c, a, jmax, m_filter=m0, species=keep)
+        A2, E2, sub = rot.zfield_m0_operator(part, al, start=(0, "A"))
+        idx2 = serving._m0_positions(part, jmax)[sub]
+        assert A1.shape == A2.shape and abs(A1 - A2).max() == 0.0
+        assert np.array_equal(E1, E2) and np.array_equal(idx1, idx2)
