@@ -136,6 +136,7 @@ struct rm_plan {
   int64_t* sub_rp = nullptr;     // [n+1]
   int32_t* sub_ccol = nullptr;   // [nnz] permuted column
   double2* sub_cval = nullptr;   // [nfield][nnz]
+  float2* sub_cval32 = nullptr;  // [nfield][nnz] the same in FP32 (FP32-mode matvecs read it)
   std::vector<int64_t> h_sub_rp;
   // per-call scratch of the subspace path
   int32_t* dom = nullptr;        // 2 per column: domain lo, hi
@@ -158,6 +159,7 @@ struct SubOp {
   const int64_t* rp;       // the same B_s as permuted CSR: row pointers [n+1],
   const int32_t* ccol;     //   columns [nnz] and
   const double2* cval;     //   values [nslot][nnz]
+  const float2* cval32;    //   and their FP32 rounding
   int64_t nnz;
   const int32_t* perm;     // permuted row -> original row
   const double2* phase;    // original order, or NULL

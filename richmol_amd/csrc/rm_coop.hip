@@ -483,7 +483,11 @@ __global__ __launch_bounds__(ct_of<MODE>()) void k_long(LongArgs a) {
           };
           auto eval = [&](int64_t q, int64_t e1) __attribute__((always_inline)) -> XT {
             if (q >= e1) return to_x<F32>(make_double2(0.0, 0.0));
-            return direct ? to_x<F32>(op.cval[q]) : Ast[q - e_lo];
+            if (!direct) return Ast[q - e_lo];
+            if constexpr (F32)
+              return op.cval32[q];   // the FP32 rounding of B_0, prepared with the plan
+            else
+              return op.cval[q];
           };
           // MG row groups per pass, their loads in flight together; UMG entries
           // per lane and row at a time

@@ -676,9 +676,12 @@ int csr_build(rm_plan* p, const std::vector<double2>& B) {
         cval[(size_t)s * nnz + (size_t)(rp[ip] + j)] = B[(size_t)s * nnz + (size_t)(r0 + j)];
     }
   }
+  std::vector<float2> cval32(cval.size());
+  for (size_t q = 0; q < cval.size(); ++q) cval32[q] = make_float2((float)cval[q].x, (float)cval[q].y);
   int rc = RM_OK;
   if ((rc = upload(&p->sub_rp, rp.data(), n + 1)) || (rc = upload(&p->sub_ccol, ccol.data(), nnz)) ||
-      (rc = upload(&p->sub_cval, cval.data(), (int64_t)nf * nnz)))
+      (rc = upload(&p->sub_cval, cval.data(), (int64_t)nf * nnz)) ||
+      (rc = upload(&p->sub_cval32, cval32.data(), (int64_t)nf * nnz)))
     return rc;
   p->h_sub_rp.swap(rp);
   return RM_OK;
@@ -1127,6 +1130,7 @@ SubOp plan_op(const rm_plan* p) {
   op.rp = p->sub_rp;
   op.ccol = p->sub_ccol;
   op.cval = p->sub_cval;
+  op.cval32 = p->sub_cval32;
   op.nnz = p->nnz;
   op.perm = p->sub_perm;
   op.n = (int32_t)p->n;

@@ -38,7 +38,7 @@ static void plan_free(rm_plan* p) {
   void* bufs[] = {p->row_ptr, p->col, p->pair_ptr, p->pair_term, p->pair_kval, p->term_ptr,
                   p->term_field, p->term_mval, p->term_leaf, p->mf, p->aval, p->flag, p->ws,
                   p->ell_col, p->ell_val, p->ell_pos, p->sub_perm, p->sub_label, p->sub_lo,
-                  p->sub_hi, p->sub_col, p->sub_val, p->sub_rp, p->sub_ccol, p->sub_cval, p->dom,
+                  p->sub_hi, p->sub_col, p->sub_val, p->sub_rp, p->sub_ccol, p->sub_cval, p->sub_cval32, p->dom,
                   p->sflag};
   sub_scratch_release(p->sub_scratch);
   for (void* b : bufs)
