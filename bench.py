@@ -308,6 +308,8 @@ def main():
                     help="matvec form of the large asymmetric-top leg")
     ap.add_argument("--kron-only", action="store_true",
                     help="only the large asymmetric-top leg (for profiling its kernels) and exit")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="only the serving leg (for profiling its kernels) and exit")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-sized device buffer 3 times (reference for the HBM "
                          "byte counters) and exit")
@@ -326,6 +328,13 @@ def main():
     if args.kron_only:
         print(json.dumps({"large_top_j80": kron_leg(args.kron_steps, kernel=args.kron_kernel,
                                                     accuracy_steps=args.kron_accuracy_steps)}), flush=True)
+        return
+    if args.serve_only:
+        import torch
+        from richmol_amd import serving
+        torch.cuda.set_device(0)
+        print(json.dumps({"serving": serving.benchmark(n_requests=args.serve_requests, seed=args.seed)}),
+              flush=True)
         return
     if args.asym_only:
         print(json.dumps({"asymtop_water_j20": asymtop_leg(args.asym_steps, cpu_steps=0)}), flush=True)
