@@ -144,6 +144,8 @@ struct rm_plan {
   int32_t* sflag = nullptr;      // per step: 1 if any MF element is non-zero
   int64_t sflag_cap = 0;
   std::vector<int32_t> h_dom;
+  int32_t* h_dom_pin = nullptr;  // pinned landing buffer of the domains' copy to the host
+  int64_t h_dom_pin_cap = 0;
   void* sub_scratch = nullptr;   // launch staging and workspace (rm_subspace.hip)
   // optional HIP-event timing of the Krylov step launches
   int timing = 0;

@@ -1108,9 +1108,17 @@ int sub_domains(rm_plan* p, const void* psi, int64_t ld, int64_t ncols, hipStrea
   hipLaunchKernelGGL(k_col_domain, dim3(grid_for(n, 256), (unsigned)ncols), dim3(256), 0, s, n, ld,
                      (const double2*)psi, p->sub_label, p->sub_lo, p->sub_hi, p->dom);
   HIPCHK(hipGetLastError());
-  p->h_dom.resize((size_t)(2 * ncols));
-  HIPCHK(hipMemcpyAsync(p->h_dom.data(), p->dom, sizeof(int32_t) * 2 * ncols, hipMemcpyDeviceToHost, s));
+  if (p->h_dom_pin_cap < 2 * ncols) {
+    if (p->h_dom_pin) HIPCHK(hipHostFree(p->h_dom_pin));
+    p->h_dom_pin = nullptr;
+    p->h_dom_pin_cap = 0;
+    const int64_t cap = std::max<int64_t>(2 * ncols, 16384);
+    HIPCHK(hipHostMalloc((void**)&p->h_dom_pin, sizeof(int32_t) * (size_t)cap, hipHostMallocDefault));
+    p->h_dom_pin_cap = cap;
+  }
+  HIPCHK(hipMemcpyAsync(p->h_dom_pin, p->dom, sizeof(int32_t) * 2 * ncols, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
+  p->h_dom.assign(p->h_dom_pin, p->h_dom_pin + 2 * ncols);
   return RM_OK;
 }
 
@@ -1198,6 +1206,15 @@ int sub_propagate(rm_plan* p, void* psi, int64_t ld, int64_t ncols, int64_t nste
   if (nsteps > INT32_MAX) return RM_OK;
   int rc = sub_build(p);
   if (rc) return rc;
+  // step flags and output fills first: they run while the host waits for the domains
+  if ((rc = grow(&p->sflag, &p->sflag_cap, nsteps))) return rc;
+  step_flags(p, nsteps, fprod_dev, p->sflag, s);
+  if (kused) {
+    const int64_t nk = nsteps * ncols;
+    hipLaunchKernelGGL(k_sub_fill, dim3(grid_for(nk, 256)), dim3(256), 0, s, kused, nk, 0, 0);
+  }
+  if (status) hipLaunchKernelGGL(k_sub_fill, dim3(grid_for(ncols, 256)), dim3(256), 0, s, status, ncols, 0, 0);
+  HIPCHK(hipGetLastError());
   if ((rc = sub_domains(p, psi, ld, ncols, s))) return rc;
   std::vector<int4> cls[NCLASS];
   std::vector<int4> big;
@@ -1219,14 +1236,6 @@ int sub_propagate(rm_plan* p, void* psi, int64_t ld, int64_t ncols, int64_t nste
   sort_long(blk);
   for (auto& v : cls)
     std::stable_sort(v.begin(), v.end(), [](const int4& x, const int4& y) { return x.z > y.z; });
-  if ((rc = grow(&p->sflag, &p->sflag_cap, nsteps))) return rc;
-  step_flags(p, nsteps, fprod_dev, p->sflag, s);
-  if (kused) {
-    const int64_t nk = nsteps * ncols;
-    hipLaunchKernelGGL(k_sub_fill, dim3(grid_for(nk, 256)), dim3(256), 0, s, kused, nk, 0, 0);
-  }
-  if (status) hipLaunchKernelGGL(k_sub_fill, dim3(grid_for(ncols, 256)), dim3(256), 0, s, status, ncols, 0, 0);
-  HIPCHK(hipGetLastError());
   *done = true;
 
   std::vector<SubOp> ops(1, plan_op(p));

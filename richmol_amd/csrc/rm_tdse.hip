@@ -41,6 +41,7 @@ static void plan_free(rm_plan* p) {
                   p->sub_hi, p->sub_col, p->sub_val, p->sub_rp, p->sub_ccol, p->sub_cval, p->sub_cval32, p->dom,
                   p->sflag};
   sub_scratch_release(p->sub_scratch);
+  if (p->h_dom_pin) (void)hipHostFree(p->h_dom_pin);
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (hipEvent_t e : p->ev0) (void)hipEventDestroy(e);
