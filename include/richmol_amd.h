@@ -34,6 +34,7 @@ extern "C" {
 #define RM_OK              0
 #define RM_NOT_CONVERGED   1    /* a column hit maxorder (tdse.py:480-484) */
 #define RM_DEVICE_TIMEOUT  2    /* status bit: a cooperative launch's grid barrier timed out */
+#define RM_STALE_DOMAIN    4    /* status bit: reused column domains no longer held; column untouched */
 #define RM_ERR_ARG        -1
 #define RM_ERR_HIP        -2
 #define RM_ERR_ALLOC      -3
@@ -48,7 +49,7 @@ typedef struct rm_plan rm_plan;
 const char* rm_last_error(void);
 
 /* Library ABI version (RM_ABI_VERSION of the header it was built from). */
-#define RM_ABI_VERSION 2
+#define RM_ABI_VERSION 3
 int rm_version(void);
 
 /*
@@ -114,6 +115,8 @@ int rm_step(rm_plan* plan, const void* psi_in, void* psi_out, int64_t ld,
             int64_t ncols, double f_re, double f_im, const void* phase,
             int32_t krylov_fixed, int32_t maxorder, double tol,
             int32_t* kused, int32_t* status, void* stream);
+/* rm_step runs the full-vector kernels, which compute in FP64 only: a plan
+ * set to RM_PREC_FP32 (rm_plan_set_precision) is refused with RM_ERR_ARG. */
 
 /*
  * nsteps consecutive steps of the same wavepackets (the user time loop of
@@ -122,9 +125,15 @@ int rm_step(rm_plan* plan, const void* psi_in, void* psi_out, int64_t ld,
  * nsteps x nfield complex) and advances every column as rm_step does.
  *   psi: device, in/out; work: device scratch of the same shape (ping-pong).
  *   kused: device int32[nsteps*ncols] or NULL (Krylov dimension per step).
- *   status: device int32[ncols] or NULL: RM_NOT_CONVERGED if any step of the
- *   column hit maxorder (zeroed at the start).
+ *   status: device int32[ncols] or NULL (zeroed at the start), per column
+ *   the bits RM_NOT_CONVERGED (some step hit maxorder), RM_DEVICE_TIMEOUT (a
+ *   cooperative launch's bounded wait gave up: the column is invalid) and
+ *   RM_STALE_DOMAIN (domain reuse only, below: the column was left as it was).
+ *   With status == NULL the call reports a cooperative timeout by returning
+ *   RM_DEVICE_TIMEOUT (it then waits for the stream).
  * thresh: host, nleaf doubles (< 0 = none), the same for every step.
+ * The full-vector kernels (taken when thresholds are given) compute in FP64
+ * only: a plan set to RM_PREC_FP32 gets RM_ERR_ARG there, as from rm_step.
  */
 int rm_propagate(rm_plan* plan, void* psi, void* work, int64_t ld, int64_t ncols,
                  int64_t nsteps, const double* fprod_dev, const double* thresh,
@@ -206,6 +215,25 @@ int rm_plan_set_precision(rm_plan* plan, int32_t precision);
  */
 int rm_plan_domains(rm_plan* plan, const void* psi, int64_t ld, int64_t ncols,
                     int32_t* lohi, void* stream);
+
+/*
+ * Domain reuse (no reference counterpart: removes the per-call device-to-host
+ * read of the column domains).  With on != 0, rm_propagate keeps the domains
+ * and launch lists it computed and reuses them on its next call with the same
+ * psi, ld and ncols -- propagation keeps every column inside its domain.  A
+ * reused call first checks on the device that every non-zero entry of psi
+ * still lies in its column's domain; where it does not (psi was rewritten
+ * between the calls), no column is propagated, psi is left as it was and the
+ * columns get RM_STALE_DOMAIN in status.  Every call of this function, with
+ * either value, forgets the kept domains.  Default: off.
+ */
+int rm_plan_set_domain_reuse(rm_plan* plan, int32_t on);
+
+/* Test hook (no reference counterpart): with preset != 0 every cooperative
+ * launch of the plan starts with its abort flag raised, so its bounded waits
+ * give up at once, the launch runs to its end and its columns get
+ * RM_DEVICE_TIMEOUT -- the timeout path without a timeout.  Default 0. */
+int rm_plan_set_coop_abort(rm_plan* plan, int32_t preset);
 
 /* sflag[s] = 1 if step s (row s of fprod_dev, nsteps x nfield complex,
  * device) makes any MF element non-zero, else 0: the `len(H.mfmat) > 0`

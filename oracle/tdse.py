@@ -60,6 +60,9 @@ class OracleTDSE:
 
     def init_state(self, H, temp=None, thresh=1e-3, zpe=None):
         """tdse.py:177-262 (dense output)."""
+        if temp is not None:
+            assert temp >= 0                                       # tdse.py:206-207
+        assert thresh >= 0                                         # tdse.py:210-211
         is_diag = False
         if getattr(H, "cart", [None])[0] == "0":
             H.field([0, 0, 1])
@@ -70,7 +73,9 @@ class OracleTDSE:
         else:
             enrs, v = np.linalg.eigh(H.tomat(form="full", repres="dense"))
             vecs = csr_matrix(v)
-        if zpe is None:
+        if zpe is not None:
+            assert zpe <= abs(enrs[0])                             # tdse.py:235-238
+        else:
             zpe = enrs[0]
         enrs = enrs - zpe
         vecs = vecs.transpose()

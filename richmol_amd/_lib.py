@@ -3,14 +3,17 @@
 The library is built in-tree by ``python -m richmol_amd._build``.  Loading
 fails loudly when it is missing: there is no CPU fallback.
 """
+import atexit
 import ctypes
 import os
+import weakref
 
 from ._build import LIB_PATH
 
 RM_OK = 0
 RM_NOT_CONVERGED = 1
 RM_DEVICE_TIMEOUT = 2
+RM_STALE_DOMAIN = 4
 RM_MAX_FIELD = 32
 RM_MAX_LEAF = 8
 RM_MAX_KRYLOV = 128
@@ -47,6 +50,8 @@ SIGNATURES = {
     "rm_expect": (ctypes.c_int, [_vp, _vp, ctypes.c_int64, ctypes.c_int64, _vp, _vp]),
     "rm_plan_subspaces": (ctypes.c_int, [_vp, _c_i64p, _c_i64p]),
     "rm_plan_set_precision": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "rm_plan_set_domain_reuse": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "rm_plan_set_coop_abort": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "rm_plan_domains": (ctypes.c_int, [_vp, _vp, ctypes.c_int64, ctypes.c_int64, _c_i32p, _vp]),
     "rm_plan_step_flags": (ctypes.c_int, [_vp, ctypes.c_int64, _vp, _vp, _vp]),
     "rm_batch_create": (ctypes.c_int, [ctypes.POINTER(_vp)]),
@@ -85,6 +90,38 @@ class RichmolAmdError(RuntimeError):
     pass
 
 
+# Objects holding library handles (plans, batches, Kronecker plans), closed
+# newest first by an exit hook while the HIP runtime is still up -- not by
+# finalizers that interpreter shutdown runs in an arbitrary order.
+_LIVE = []
+
+
+def track(obj):
+    """Register an object with a close() method for the exit-time release."""
+    _LIVE.append(weakref.ref(obj))
+    if len(_LIVE) > 4096:
+        _LIVE[:] = [r for r in _LIVE if r() is not None]
+
+
+def release_all():
+    """Close every tracked handle (newest first) and drop the module caches."""
+    import sys
+    for name in ("richmol_amd.plan", "richmol_amd.tdse"):
+        mod = sys.modules.get(name)
+        if mod is not None:
+            for attr in ("_CACHE", "_KRON_CACHE"):
+                cache = getattr(mod, attr, None)
+                if cache is not None:
+                    cache.clear()
+    while _LIVE:
+        obj = _LIVE.pop()()
+        if obj is not None:
+            try:
+                obj.close()
+            except Exception:
+                pass
+
+
 def load():
     """Load the library (once) and attach the ctypes signatures."""
     global _LIB
@@ -102,6 +139,8 @@ def load():
         fn.restype = res
         fn.argtypes = args
     _LIB = lib
+    # registered after torch's import: runs before torch's own exit hooks
+    atexit.register(release_all)
     return lib
 
 

@@ -71,7 +71,7 @@ def tune(ens, fields, steps=6, reps=2, save=True, verbose=False):
             continue
         best = None
         for _ in range(reps):
-            ens.psi.copy_(psi0)
+            ens.load(psi0)
             torch.cuda.synchronize()
             plan.timing(True)
             try:
@@ -88,7 +88,7 @@ def tune(ens, fields, steps=6, reps=2, save=True, verbose=False):
             results[cand] = best
             if verbose:
                 print(f"  {cand:8s} {best * 1e3:9.1f} us/launch", flush=True)
-    ens.psi.copy_(psi0)
+    ens.load(psi0)
     choice = min(results, key=results.get)
     plan.set_kernel(choice)
     ens.kernel = choice

@@ -147,6 +147,21 @@ struct rm_plan {
   int32_t* h_dom_pin = nullptr;  // pinned landing buffer of the domains' copy to the host
   int64_t h_dom_pin_cap = 0;
   void* sub_scratch = nullptr;   // launch staging and workspace (rm_subspace.hip)
+  // domain reuse (rm_plan_set_domain_reuse): the domains of the last
+  // rm_propagate call, kept for its next call on the same wavepackets; each
+  // such call checks them on the device and stores its call id into `stale`
+  // when they no longer hold
+  int dom_reuse = 0;
+  int32_t* dom_keep = nullptr;   // device: the kept domains, 2 per column
+  int64_t dom_keep_cap = 0;
+  int32_t* stale = nullptr;      // device int32[1]
+  int32_t call_id = 0;
+  // status of calls whose caller passed none (cooperative timeouts are reported)
+  int32_t* int_status = nullptr;
+  int64_t int_status_cap = 0;
+  int32_t* h_status_pin = nullptr;
+  int64_t h_status_pin_cap = 0;
+  int32_t coop_abort = 0;        // test hook: cooperative launches start with the abort flag raised
   // optional HIP-event timing of the Krylov step launches
   int timing = 0;
   int nrec = 0;
@@ -172,8 +187,9 @@ struct SubOp {
   int32_t* kused;          // kused[step*kused_ld + b - kcol0] or NULL
   int64_t kused_ld;
   int32_t* status;         // status[b - kcol0] or NULL: failures only
+  const int32_t* stale;    // device flag: == call_id when the reused domains failed their check
   double2 f;
-  int32_t n, w, nslot, nsteps, kcol0;
+  int32_t n, w, nslot, nsteps, kcol0, call_id;
 };
 
 template <typename T>
@@ -219,10 +235,11 @@ struct LongItem {
 };
 // Long-domain wavepackets (rm_coop.hip): blk one workgroup each in one launch,
 // coop the whole device each (one cooperative launch each).  d_ops, d_blk,
-// d_coop: device copies of the jobs and of the two item lists.
+// d_coop: device copies of the jobs and of the two item lists.  abort_preset:
+// test hook, the cooperative launches start with their abort flag raised.
 int long_run(void** scratch, const SubOp* d_ops, const int4* d_blk, const std::vector<LongItem>& blk,
              const int4* d_coop, const std::vector<LongItem>& coop, int32_t kcap, int32_t fixed_m,
-             int32_t maxorder, double tol, bool f32, int n_cu, hipStream_t s);
+             int32_t maxorder, double tol, bool f32, int n_cu, hipStream_t s, int32_t abort_preset = 0);
 // Event-bracketed launch helpers for the plan's optional timing.
 hipError_t timing_begin(rm_plan* p, hipStream_t s);
 hipError_t timing_end(rm_plan* p, hipStream_t s);

@@ -345,6 +345,12 @@ __global__ __launch_bounds__(ct_of<MODE>()) void k_long(LongArgs a) {
     const int4 item = a.items[itm];
     const SubOp& op = a.ops[item.w];
     const int b = item.x, lo = item.y, len = item.z;
+    // reused domains that failed their check: the wavepacket is left as it is
+    // (the same decision in every thread of the cooperating set)
+    if (op.stale && *op.stale == op.call_id) {
+      if (t0 == 0 && op.status) op.status[b - op.kcol0] |= RM_STALE_DOMAIN;
+      continue;
+    }
     const int64_t e_lo = op.rp[lo];
     const int ngroups = (len + RPW - 1) / RPW;
     // per-row loops: REG runs each row on its matvec owner (lane sub == 0 of
@@ -690,9 +696,10 @@ __global__ __launch_bounds__(ct_of<MODE>()) void k_long(LongArgs a) {
   }
 }
 
-__global__ void k_long_reset(unsigned long long* bar, int32_t* abort_flag, unsigned long long* lpart, int n_lpart) {
+__global__ void k_long_reset(unsigned long long* bar, int32_t* abort_flag, unsigned long long* lpart, int n_lpart,
+                             int32_t abort_preset) {
   if (threadIdx.x < BAR_LINES) bar[BAR_STRIDE * threadIdx.x] = 0ull;
-  if (threadIdx.x == 0) *abort_flag = 0;
+  if (threadIdx.x == 0) *abort_flag = abort_preset ? 1 : 0;
   for (int i = threadIdx.x; i < n_lpart; i += blockDim.x) lpart[i] = LSENT;
 }
 
@@ -769,7 +776,7 @@ void coop_scratch_release(void* sc) {
 // copies of the jobs and of the two item lists (same order as blk / coop).
 int long_run(void** scratch, const SubOp* d_ops, const int4* d_blk, const std::vector<LongItem>& blk,
              const int4* d_coop, const std::vector<LongItem>& coop, int32_t kcap, int32_t fixed_m,
-             int32_t maxorder, double tol, bool f32, int n_cu, hipStream_t s) {
+             int32_t maxorder, double tol, bool f32, int n_cu, hipStream_t s, int32_t abort_preset) {
   if (blk.empty() && coop.empty()) return RM_OK;
   int gres_r = 1, gres_m = 1, gres_b = 1, gres_x = 1;
   HIPCHK((max_resident<GRID_REG>(n_cu, 0, &gres_r)));
@@ -886,7 +893,7 @@ int long_run(void** scratch, const SubOp* d_ops, const int4* d_blk, const std::v
     const int wpg = reg ? wr : wm;
     const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((groups + wpg - 1) / wpg, reg ? gres_r : gres_m));
     hipLaunchKernelGGL(k_long_reset, dim3(1), dim3(256), 0, s, bar, abort_flag, (unsigned long long*)lpart,
-                       LRING * 4 * gmax);
+                       LRING * 4 * gmax, abort_preset);
     HIPCHK(hipGetLastError());
     if (reg)
       HIPCHK((launch_m<GRID_REG>(ac, grid, f32, 0, s)));

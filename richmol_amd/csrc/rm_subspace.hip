@@ -76,6 +76,11 @@ __global__ __launch_bounds__(64) void k_sub_prop(SubArgs a) {
     const int4 item = a.items[itm];
     const int b = item.x, lo = item.y, len = item.z;
     const SubOp op = a.ops[item.w];
+    if (op.stale && *op.stale == op.call_id) {
+      // reused domains that failed their check: the wavepacket is left as it is
+      if (l == 0 && op.status) op.status[b - op.kcol0] |= RM_STALE_DOMAIN;
+      continue;
+    }
     const int n = op.n, W = op.w;
     const bool breg = WM > 0 && op.nslot <= NSR;
     // domain within lanes 0..31: the upper half of every sum is zero
@@ -427,6 +432,11 @@ __global__ __launch_bounds__(64) void k_sub_pair(SubArgs a) {
     const int4 item = itm < a.nitems ? a.items[itm] : make_int4(-1, 0, 0, a.items[2 * pr].w);
     const int b = item.x, lo = item.y, len = b >= 0 ? item.z : 0;
     const SubOp op = a.ops[item.w];   // the same for both halves
+    if (op.stale && *op.stale == op.call_id) {
+      // reused domains that failed their check: both wavepackets are left as they are
+      if (i == 0 && b >= 0 && op.status) op.status[b - op.kcol0] |= RM_STALE_DOMAIN;
+      continue;
+    }
     const int n = op.n, W = op.w;
     const bool breg = WM > 0 && op.nslot <= NSR;
     const bool ok = i < len;
@@ -625,6 +635,72 @@ __global__ void k_step_flags(int64_t nterm, int64_t nsteps, int32_t nfield,
     double2 acc = make_double2(0.0, 0.0);
     for (int64_t q = term_ptr[t]; q < term_ptr[t + 1]; ++q) acc = cfma(fp[term_field[q]], term_mval[q], acc);
     if (acc.x != 0.0 || acc.y != 0.0 || acc.x != acc.x || acc.y != acc.y) sflag[st] = 1;
+  }
+}
+
+// The device work of an rm_propagate call ahead of its propagation launches,
+// in one launch: blocks [0, nb_flag) the step flags (one block per step,
+// as k_step_flags), blocks [nb_flag, nb_flag + nb_fill) zero kused and
+// status, the remaining blocks (domain reuse) check every entry of psi
+// against its column's kept domain and store call_id into *stale on a miss.
+struct Prologue {
+  int64_t nterm, nsteps;
+  int32_t nfield;
+  const int64_t* term_ptr;
+  const int32_t* term_field;
+  const double2* term_mval;
+  const double2* table;
+  int32_t* sflag;
+  int32_t* kused;
+  int64_t nk;
+  int32_t* status;
+  int64_t ncols;
+  int64_t n, ld;
+  const double2* psi;
+  const int32_t* label;   // original row -> subspace
+  const int32_t* clo;     // subspace -> first permuted row
+  const int32_t* chi;     // subspace -> one past its last
+  const int32_t* dom;     // kept domains (NULL: no check)
+  int32_t* stale;
+  int32_t call_id;
+  int32_t nb_flag, nb_fill, cpb;   // cpb: check blocks per column
+};
+
+__global__ __launch_bounds__(256) void k_sub_prologue(Prologue a) {
+  const int tid = threadIdx.x;
+  const int64_t bid = blockIdx.x;
+  if (bid < a.nb_flag) {
+    for (int64_t st = bid; st < a.nsteps; st += a.nb_flag) {
+      const double2* fp = a.table + st * a.nfield;
+      int any = 0;
+      for (int64_t t = tid; t < a.nterm; t += 256) {
+        double2 acc = make_double2(0.0, 0.0);
+        for (int64_t q = a.term_ptr[t]; q < a.term_ptr[t + 1]; ++q)
+          acc = cfma(fp[a.term_field[q]], a.term_mval[q], acc);
+        any |= (acc.x != 0.0 || acc.y != 0.0 || acc.x != acc.x || acc.y != acc.y) ? 1 : 0;
+      }
+      any = __syncthreads_or(any);
+      if (tid == 0) a.sflag[st] = any ? 1 : 0;
+    }
+  } else if (bid < (int64_t)a.nb_flag + a.nb_fill) {
+    const int64_t stride = (int64_t)a.nb_fill * 256;
+    for (int64_t i = (bid - a.nb_flag) * 256 + tid; i < a.nk; i += stride) a.kused[i] = 0;
+    for (int64_t i = (bid - a.nb_flag) * 256 + tid; i < a.ncols; i += stride) a.status[i] = 0;
+  } else {
+    const int64_t j = bid - a.nb_flag - a.nb_fill;
+    const int64_t b = j / a.cpb;
+    const int64_t lo = a.dom[2 * b], hi = a.dom[2 * b + 1];
+    bool miss = false;
+    for (int64_t i = (j % a.cpb) * 256 + tid; i < a.n; i += (int64_t)a.cpb * 256) {
+      const double2 v = a.psi[b * a.ld + i];
+      if (!(isfinite(v.x) && isfinite(v.y))) {
+        miss |= lo > 0 || hi < a.n;
+      } else if (v.x != 0.0 || v.y != 0.0) {
+        const int32_t c = a.label[i];
+        miss |= a.clo[c] < lo || a.chi[c] > hi;
+      }
+    }
+    if (miss) *a.stale = a.call_id;
   }
 }
 
@@ -900,6 +976,13 @@ struct SubScratch {
   hipStream_t aux[NCLASS + 1] = {};
   hipEvent_t fork = nullptr;
   hipEvent_t join[NCLASS + 1] = {};
+  // launch lists of the last rm_propagate call, kept for domain reuse
+  bool kept = false;
+  const void* kept_psi = nullptr;
+  int64_t kept_ld = 0, kept_ncols = 0;
+  int kept_force = 0;
+  std::vector<int4> kept_cls[NCLASS];
+  std::vector<LongItem> kept_blk, kept_coop;
 };
 
 void scratch_release(SubScratch* sc) {
@@ -930,7 +1013,7 @@ void scratch_release(SubScratch* sc) {
 int sub_run(SubScratch* sc, const std::vector<SubOp>& ops, std::vector<int4> (&cls)[NCLASS],
             const std::vector<LongItem>& blk, const std::vector<LongItem>& coop, int32_t kcap,
             int32_t fixed_m, int32_t maxorder, double tol, bool f32, int n_cu, hipStream_t s,
-            rm_plan* timed) {
+            rm_plan* timed, int32_t abort_preset = 0) {
   // the pair classes: class-0 items of <= 32 rows under a fixed Krylov
   // dimension <= 16 (the banded ones, placed in PAIR_BAND_CLASS by the
   // caller, keep their own class), grouped by operator, each group padded to
@@ -1075,7 +1158,8 @@ int sub_run(SubScratch* sc, const std::vector<SubOp>& ops, std::vector<int4> (&c
   hipStream_t bs = s;
   if (!blk.empty() && (rc = next_stream(&bs))) return rc;
   rc = long_run(&sc->long_ws, (const SubOp*)sc->dev[bi], (const int4*)(sc->dev[bi] + off_blk), blk,
-                (const int4*)(sc->dev[bi] + off_coop), coop, kcap, fixed_m, maxorder, tol, f32, n_cu, bs);
+                (const int4*)(sc->dev[bi] + off_coop), coop, kcap, fixed_m, maxorder, tol, f32, n_cu, bs,
+                abort_preset);
   if (rc) return rc;
   if (fork && !blk.empty()) HIPCHK(hipEventRecord(sc->join[nused - 1], bs));
   for (int i = 0; i < (fork ? nused : 0); ++i) HIPCHK(hipStreamWaitEvent(s, sc->join[i], 0));
@@ -1160,6 +1244,19 @@ extern "C" int rm_plan_subspaces(rm_plan* p, int64_t* count, int64_t* max_rows) 
   return RM_OK;
 }
 
+extern "C" int rm_plan_set_domain_reuse(rm_plan* p, int32_t on) {
+  if (!p) return fail(RM_ERR_ARG, "rm_plan_set_domain_reuse: NULL plan");
+  p->dom_reuse = on ? 1 : 0;
+  if (p->sub_scratch) ((SubScratch*)p->sub_scratch)->kept = false;
+  return RM_OK;
+}
+
+extern "C" int rm_plan_set_coop_abort(rm_plan* p, int32_t preset) {
+  if (!p) return fail(RM_ERR_ARG, "rm_plan_set_coop_abort: NULL plan");
+  p->coop_abort = preset ? 1 : 0;
+  return RM_OK;
+}
+
 extern "C" int rm_plan_set_precision(rm_plan* p, int32_t precision) {
   if (!p) return fail(RM_ERR_ARG, "rm_plan_set_precision: NULL plan");
   if (precision != RM_PREC_FP64 && precision != RM_PREC_FP32)
@@ -1206,37 +1303,93 @@ int sub_propagate(rm_plan* p, void* psi, int64_t ld, int64_t ncols, int64_t nste
   if (nsteps > INT32_MAX) return RM_OK;
   int rc = sub_build(p);
   if (rc) return rc;
-  // step flags and output fills first: they run while the host waits for the domains
-  if ((rc = grow(&p->sflag, &p->sflag_cap, nsteps))) return rc;
-  step_flags(p, nsteps, fprod_dev, p->sflag, s);
-  if (kused) {
-    const int64_t nk = nsteps * ncols;
-    hipLaunchKernelGGL(k_sub_fill, dim3(grid_for(nk, 256)), dim3(256), 0, s, kused, nk, 0, 0);
+  if (!p->sub_scratch) p->sub_scratch = new (std::nothrow) SubScratch();
+  if (!p->sub_scratch) return fail(RM_ERR_ALLOC, "rm_propagate: host allocation");
+  SubScratch* sc = (SubScratch*)p->sub_scratch;
+  // the caller passed no status: an internal one, so that a cooperative
+  // timeout can still be reported
+  int32_t* st = status;
+  if (!st) {
+    if ((rc = grow(&p->int_status, &p->int_status_cap, ncols))) return rc;
+    st = p->int_status;
   }
-  if (status) hipLaunchKernelGGL(k_sub_fill, dim3(grid_for(ncols, 256)), dim3(256), 0, s, status, ncols, 0, 0);
+  const bool reuse = p->dom_reuse && sc->kept && sc->kept_psi == psi && sc->kept_ld == ld &&
+                     sc->kept_ncols == ncols && sc->kept_force == force;
+  if ((rc = grow(&p->sflag, &p->sflag_cap, nsteps))) return rc;
+  if (!p->stale) {
+    if ((rc = upload(&p->stale, (const int32_t*)nullptr, 1))) return rc;
+    hipLaunchKernelGGL(k_sub_fill, dim3(1), dim3(64), 0, s, p->stale, (int64_t)1, 0, 0);
+  }
+  p->call_id = p->call_id == INT32_MAX ? 1 : p->call_id + 1;
+  // step flags, output fills and (reused domains) the check in one launch;
+  // without reuse it runs while the host waits for the domains
+  Prologue pa;
+  memset(&pa, 0, sizeof(pa));
+  pa.nterm = p->nterm;
+  pa.nsteps = nsteps;
+  pa.nfield = p->nfield;
+  pa.term_ptr = p->term_ptr;
+  pa.term_field = p->term_field;
+  pa.term_mval = p->term_mval;
+  pa.table = (const double2*)fprod_dev;
+  pa.sflag = p->sflag;
+  pa.kused = kused;
+  pa.nk = kused ? nsteps * ncols : 0;
+  pa.status = st;
+  pa.ncols = ncols;
+  pa.n = p->n;
+  pa.ld = ld;
+  pa.psi = (const double2*)psi;
+  pa.label = p->sub_label;
+  pa.clo = p->sub_lo;
+  pa.chi = p->sub_hi;
+  pa.dom = reuse ? p->dom_keep : nullptr;
+  pa.stale = p->stale;
+  pa.call_id = p->call_id;
+  pa.nb_flag = (int32_t)std::min<int64_t>(nsteps, 4096);
+  pa.nb_fill = (int32_t)std::max<int64_t>(1, std::min<int64_t>(grid_for(std::max(pa.nk, ncols), 256), 1024));
+  pa.cpb = (int32_t)std::max<int64_t>(1, (int64_t)grid_for(p->n, 1024));
+  const int64_t nb = (int64_t)pa.nb_flag + pa.nb_fill + (reuse ? pa.cpb * ncols : 0);
+  hipLaunchKernelGGL(k_sub_prologue, dim3((unsigned)nb), dim3(256), 0, s, pa);
   HIPCHK(hipGetLastError());
-  if ((rc = sub_domains(p, psi, ld, ncols, s))) return rc;
-  std::vector<int4> cls[NCLASS];
-  std::vector<int4> big;
-  for (int64_t b = 0; b < ncols; ++b) {
-    const int32_t lo = p->h_dom[2 * b], hi = p->h_dom[2 * b + 1];
-    if (hi <= lo) continue;   // zero column: stays zero (kused 0)
-    if (force || hi - lo > RM_SUBSPACE_MAX_ROWS || !p->sub_col) {
-      big.push_back(make_int4((int)b, lo, hi - lo, 0));
-    } else {
-      const int c = sub_class(p->sub_w, hi - lo);
-      cls[c == 0 && hi - lo <= 32 && p->nfield <= NSR && sub_banded(p, lo, hi) ? PAIR_BAND_CLASS : c]
-          .push_back(make_int4((int)b, lo, hi - lo, 0));
+  if (!reuse) {
+    sc->kept = false;
+    if ((rc = sub_domains(p, psi, ld, ncols, s))) return rc;
+    for (auto& v : sc->kept_cls) v.clear();
+    sc->kept_blk.clear();
+    sc->kept_coop.clear();
+    std::vector<int4> big;
+    for (int64_t b = 0; b < ncols; ++b) {
+      const int32_t lo = p->h_dom[2 * b], hi = p->h_dom[2 * b + 1];
+      if (hi <= lo) continue;   // zero column: stays zero (kused 0)
+      if (force || hi - lo > RM_SUBSPACE_MAX_ROWS || !p->sub_col) {
+        big.push_back(make_int4((int)b, lo, hi - lo, 0));
+      } else {
+        const int c = sub_class(p->sub_w, hi - lo);
+        sc->kept_cls[c == 0 && hi - lo <= 32 && p->nfield <= NSR && sub_banded(p, lo, hi) ? PAIR_BAND_CLASS : c]
+            .push_back(make_int4((int)b, lo, hi - lo, 0));
+      }
+    }
+    if (!big.empty() && !p->sub_rp && (rc = csr_build(p, entry_values(p)))) return rc;
+    const bool to_coop = force == RM_KERNEL_COOP || (force == 0 && (int64_t)big.size() <= COOP_MAX_COLS);
+    for (const int4& it : big) (to_coop ? sc->kept_coop : sc->kept_blk).push_back(long_item(p, it.x, it.y, it.z, 0));
+    sort_long(sc->kept_blk);
+    for (auto& v : sc->kept_cls)
+      std::stable_sort(v.begin(), v.end(), [](const int4& x, const int4& y) { return x.z > y.z; });
+    if (p->dom_reuse) {
+      // a copy of their own: rm_plan_domains rewrites p->dom
+      if ((rc = grow(&p->dom_keep, &p->dom_keep_cap, 2 * ncols))) return rc;
+      HIPCHK(hipMemcpyAsync(p->dom_keep, p->dom, sizeof(int32_t) * 2 * (size_t)ncols, hipMemcpyDeviceToDevice, s));
+      sc->kept = true;
+      sc->kept_psi = psi;
+      sc->kept_ld = ld;
+      sc->kept_ncols = ncols;
+      sc->kept_force = force;
     }
   }
-  if (!big.empty() && !p->sub_rp && (rc = csr_build(p, entry_values(p)))) return rc;
-  std::vector<LongItem> blk, coop;
-  const bool to_coop = force == RM_KERNEL_COOP || (force == 0 && (int64_t)big.size() <= COOP_MAX_COLS);
-  for (const int4& it : big) (to_coop ? coop : blk).push_back(long_item(p, it.x, it.y, it.z, 0));
-  sort_long(blk);
-  for (auto& v : cls)
-    std::stable_sort(v.begin(), v.end(), [](const int4& x, const int4& y) { return x.z > y.z; });
   *done = true;
+  std::vector<int4> cls[NCLASS];
+  for (int c = 0; c < NCLASS; ++c) cls[c] = sc->kept_cls[c];   // sub_run regroups its copy
 
   std::vector<SubOp> ops(1, plan_op(p));
   SubOp& op = ops[0];
@@ -1247,14 +1400,30 @@ int sub_propagate(rm_plan* p, void* psi, int64_t ld, int64_t ncols, int64_t nste
   op.ld = ld;
   op.kused = kused;
   op.kused_ld = ncols;
-  op.status = status;
+  op.status = st;
+  op.stale = reuse ? p->stale : nullptr;
+  op.call_id = p->call_id;
   op.f = make_double2(f_re, f_im);
   op.nsteps = (int32_t)nsteps;
-  if (!p->sub_scratch) p->sub_scratch = new (std::nothrow) SubScratch();
-  if (!p->sub_scratch) return fail(RM_ERR_ALLOC, "rm_propagate: host allocation");
   const int32_t kcap = krylov_fixed > 0 ? krylov_fixed : maxorder;
-  return sub_run((SubScratch*)p->sub_scratch, ops, cls, blk, coop, kcap, krylov_fixed, maxorder, tol,
-                 p->precision == RM_PREC_FP32, p->n_cu, s, p);
+  rc = sub_run(sc, ops, cls, sc->kept_blk, sc->kept_coop, kcap, krylov_fixed, maxorder, tol,
+               p->precision == RM_PREC_FP32, p->n_cu, s, p, p->coop_abort);
+  if (rc || status || sc->kept_coop.empty()) return rc;
+  // no status array from the caller: read the internal one for timeouts
+  if (p->h_status_pin_cap < ncols) {
+    if (p->h_status_pin) HIPCHK(hipHostFree(p->h_status_pin));
+    p->h_status_pin = nullptr;
+    p->h_status_pin_cap = 0;
+    HIPCHK(hipHostMalloc((void**)&p->h_status_pin, sizeof(int32_t) * (size_t)ncols, hipHostMallocDefault));
+    p->h_status_pin_cap = ncols;
+  }
+  HIPCHK(hipMemcpyAsync(p->h_status_pin, st, sizeof(int32_t) * (size_t)ncols, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  for (int64_t b = 0; b < ncols; ++b)
+    if (p->h_status_pin[b] & RM_DEVICE_TIMEOUT)
+      return fail(RM_DEVICE_TIMEOUT, "rm_propagate: a cooperative launch's bounded wait gave up (column %lld); "
+                  "the result is invalid", (long long)b);
+  return RM_OK;
 }
 
 // ---------------------------------------------------------------------------

@@ -39,9 +39,10 @@ static void plan_free(rm_plan* p) {
                   p->term_field, p->term_mval, p->term_leaf, p->mf, p->aval, p->flag, p->ws,
                   p->ell_col, p->ell_val, p->ell_pos, p->sub_perm, p->sub_label, p->sub_lo,
                   p->sub_hi, p->sub_col, p->sub_val, p->sub_rp, p->sub_ccol, p->sub_cval, p->sub_cval32, p->dom,
-                  p->sflag};
+                  p->sflag, p->dom_keep, p->stale, p->int_status};
   sub_scratch_release(p->sub_scratch);
   if (p->h_dom_pin) (void)hipHostFree(p->h_dom_pin);
+  if (p->h_status_pin) (void)hipHostFree(p->h_status_pin);
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (hipEvent_t e : p->ev0) (void)hipEventDestroy(e);
@@ -1439,7 +1440,7 @@ extern "C" int rm_plan_kernel_time(rm_plan* p, double* total_ms, int64_t* launch
 // Validate the step parameters and fill StepArgs (workspace sized for ncols).
 static int step_setup(rm_plan* p, const char* who, int64_t ld, int64_t ncols, double f_re,
                       double f_im, const void* phase, int32_t krylov_fixed, int32_t maxorder,
-                      double tol, StepArgs* a) {
+                      double tol, StepArgs* a, bool workspace = true) {
   if (!p) return fail(RM_ERR_ARG, "%s: NULL plan", who);
   if (ncols < 0 || ld < p->n)
     return fail(RM_ERR_ARG, "%s: bad ld=%lld or ncols=%lld", who, (long long)ld, (long long)ncols);
@@ -1451,7 +1452,7 @@ static int step_setup(rm_plan* p, const char* who, int64_t ld, int64_t ncols, do
   if (krylov_fixed == 0 && !(tol > 0.0 && tol <= 1.0)) return fail(RM_ERR_ARG, "%s: tol must be in (0, 1]", who);
   const int kcap = krylov_fixed > 0 ? krylov_fixed : maxorder;
   const size_t per_col = (size_t)kcap * p->n + (size_t)p->n + (size_t)kcap * kcap + 2 * (size_t)kcap;
-  if (ncols > 0 && p->n > 0) {
+  if (workspace && ncols > 0 && p->n > 0) {
     int rc = ensure_ws(p, sizeof(double2) * per_col * (size_t)ncols);
     if (rc) return rc;
   }
@@ -1484,6 +1485,8 @@ static int step_setup(rm_plan* p, const char* who, int64_t ld, int64_t ncols, do
 extern "C" int rm_step(rm_plan* p, const void* psi_in, void* psi_out, int64_t ld, int64_t ncols,
                        double f_re, double f_im, const void* phase, int32_t krylov_fixed,
                        int32_t maxorder, double tol, int32_t* kused, int32_t* status, void* stream) {
+  if (p && p->precision != RM_PREC_FP64)
+    return fail(RM_ERR_ARG, "rm_step: the full-vector kernels compute in FP64 only (plan set to FP32)");
   StepArgs a;
   int rc = step_setup(p, "rm_step", ld, ncols, f_re, f_im, phase, krylov_fixed, maxorder, tol, &a);
   if (rc) return rc;
@@ -1521,7 +1524,8 @@ extern "C" int rm_propagate(rm_plan* p, void* psi, void* work, int64_t ld, int64
                             int32_t maxorder, double tol, int32_t* kused, int32_t* status,
                             void* stream) {
   StepArgs a;
-  int rc = step_setup(p, "rm_propagate", ld, ncols, f_re, f_im, phase, krylov_fixed, maxorder, tol, &a);
+  // the full-vector workspace is allocated below, only when those kernels run
+  int rc = step_setup(p, "rm_propagate", ld, ncols, f_re, f_im, phase, krylov_fixed, maxorder, tol, &a, false);
   if (rc) return rc;
   if (nsteps < 0) return fail(RM_ERR_ARG, "rm_propagate: nsteps < 0");
   if (ncols > 65535) return fail(RM_ERR_ARG, "rm_propagate: ncols > 65535");
@@ -1552,6 +1556,10 @@ extern "C" int rm_propagate(rm_plan* p, void* psi, void* work, int64_t ld, int64
   if (sub_kernel)
     return fail(RM_ERR_ARG, "rm_propagate: the subspace kernels do not apply (%s)",
                 any_thresh ? "MF thresholds given" : "more than 2^31 steps");
+  if (p->precision != RM_PREC_FP64)
+    return fail(RM_ERR_ARG, "rm_propagate: %s -- the full-vector kernels compute in FP64 only (plan set to FP32)",
+                any_thresh ? "MF thresholds send this call to the full-vector kernels" : "the subspace path does not apply");
+  if ((rc = step_setup(p, "rm_propagate", ld, ncols, f_re, f_im, phase, krylov_fixed, maxorder, tol, &a))) return rc;
   if (status) hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(ncols, 256)), dim3(256), 0, s, status, ncols, 0);
   a.status = status;
   a.status_sticky = 1;

@@ -129,16 +129,43 @@ class Ensemble:
         if self.plan.precision != self.precision:
             self.plan.set_precision(self.precision)
 
-    def propagate(self, fields, thresh=None, table=None, kused=None):
+    def propagate(self, fields, thresh=None, table=None, kused=None, check=True):
         """Advance every row through len(fields) steps with one device call
-        (rm_propagate).  ``table`` may hold a prepared field table."""
+        (rm_propagate).  ``table`` may hold a prepared field table.
+
+        The plan keeps the rows' domains from one call to the next (the
+        rows never leave them); a call on rows rewritten in between is
+        caught on the device and leaves them untouched.  check=True (the
+        default) reads the status back: such a call is then repeated with
+        fresh domains, and a cooperative timeout or (adaptive rule) maxorder
+        raises.  check=False returns without waiting (call check() later)."""
         if table is None:
             table = self.prepare(fields, thresh)
         th = None if thresh is None else [thresh]
         self._apply_kernel()
-        self.plan.propagate(self.psi, self.out, table, self.f, phase=self.phase, thresh=th,
-                            krylov_fixed=self.krylov_fixed, maxorder=self.maxorder, tol=self.tol,
-                            kused=kused, status=self.status, stream=self.stream)
+        if self.plan.domain_owner != id(self):
+            self.plan.set_domain_reuse(True, owner=id(self))
+        for attempt in (0, 1):
+            self.plan.propagate(self.psi, self.out, table, self.f, phase=self.phase, thresh=th,
+                                krylov_fixed=self.krylov_fixed, maxorder=self.maxorder, tol=self.tol,
+                                kused=kused, status=self.status, stream=self.stream)
+            if not check:
+                return
+            from . import _lib
+            if attempt == 0 and np.any(self.status.cpu().numpy() & _lib.RM_STALE_DOMAIN):
+                self.plan.set_domain_reuse(True, owner=id(self))   # forget the kept domains
+                continue
+            break
+        self.check()
+
+    def load(self, psi0):
+        """Replace the rows (same shape) by psi0 (numpy or device array)."""
+        import torch
+        src = torch.as_tensor(np.ascontiguousarray(psi0, dtype=np.complex128)) \
+            if not isinstance(psi0, torch.Tensor) else psi0
+        self.psi.copy_(src.to(self.psi.device))
+        if self.plan is not None and self.plan.domain_owner == id(self):
+            self.plan.set_domain_reuse(True, owner=id(self))
 
     def launch(self):
         """The Krylov step of all rows with the field already on the device."""
@@ -157,10 +184,16 @@ class Ensemble:
         return out
 
     def check(self):
+        """Raise on the status bits of the last call: a cooperative timeout,
+        rows rewritten since the domains were kept, or (adaptive rule) a row
+        that reached maxorder."""
         from . import _lib
         st = self.status.cpu().numpy()
         if np.any(st & _lib.RM_DEVICE_TIMEOUT):
             raise RuntimeError("a cooperative propagation launch timed out at a grid barrier; "
                                "its wavepackets are invalid")
+        if np.any(st & _lib.RM_STALE_DOMAIN):
+            raise RuntimeError("the rows changed since the last propagate call kept their domains "
+                               "(use Ensemble.load); they were not propagated")
         if self.krylov_fixed == 0 and bool(np.any(st & _lib.RM_NOT_CONVERGED)):
             raise ValueError(f"Lanczos reached maximum order of '{self.maxorder}' without convergence")

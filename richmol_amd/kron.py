@@ -344,6 +344,7 @@ class DeviceKronPlan:
             ctypes.byref(h)), "rm_kplan_create")
         self._keep = None
         self.handle = h
+        _lib.track(self)
         n = ctypes.c_int64()
         fl = ctypes.c_double()
         _lib.check(lib.rm_kplan_info(h, ctypes.byref(n), ctypes.byref(fl)), "rm_kplan_info")
@@ -351,11 +352,18 @@ class DeviceKronPlan:
         self.gemm_flops_per_col = float(fl.value)
         self.kernel = "gemm"
 
-    def __del__(self):
+    def close(self):
+        """Destroy the device plan (idempotent)."""
         h = getattr(self, "handle", None)
         if h is not None and h.value:
             self._lib.load().rm_kplan_destroy(h)
-            self.handle = None
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
     KERNELS = {"fused": 0, "gemm": 1}
 

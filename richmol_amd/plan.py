@@ -216,6 +216,8 @@ class DevicePlan:
             _lib.ptr(arr["term_mval"].view(np.float64), c_dbl), _lib.ptr(arr["term_leaf"], c_i32),
             self.nfield, self.nleaf, ctypes.byref(h)), "rm_plan_create")
         self.handle = h
+        self.domain_owner = None
+        _lib.track(self)
 
     @classmethod
     def from_matrix(cls, A, cart):
@@ -224,14 +226,18 @@ class DevicePlan:
         term, every entry of the sparse matrix A its K value."""
         return cls(None, active=[[cart]], arrays=matrix_arrays(A, cart))
 
-    def __del__(self):
+    def close(self):
+        """Destroy the device plan (idempotent)."""
         h = getattr(self, "handle", None)
         if h is not None and h.value:
-            try:
-                self.lib.rm_plan_destroy(h)
-            except Exception:
-                pass
-            self.handle = None
+            self.lib.rm_plan_destroy(h)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
     KERNELS = {"auto": 0, "global": 1, "resident": 2, "ell": 3, "subspace": 4, "coop": 5, "block": 6}
     ELL_CFG0 = 16
@@ -257,6 +263,18 @@ class DevicePlan:
         _lib.check(self.lib.rm_plan_set_precision(self.handle, _lib.PRECISIONS[precision]),
                    "rm_plan_set_precision")
         self.precision = precision
+
+    def set_domain_reuse(self, on=True, owner=None):
+        """Keep rm_propagate's column domains for its next call on the same
+        wavepackets (checked on the device); every call forgets the kept
+        ones.  ``owner`` records which caller's wavepackets they belong to."""
+        _lib.check(self.lib.rm_plan_set_domain_reuse(self.handle, 1 if on else 0),
+                   "rm_plan_set_domain_reuse")
+        self.domain_owner = owner if on else None
+
+    def set_coop_abort(self, preset):
+        """Test hook: cooperative launches start with their abort flag raised."""
+        _lib.check(self.lib.rm_plan_set_coop_abort(self.handle, 1 if preset else 0), "rm_plan_set_coop_abort")
 
     def domain_sizes(self, X):
         """Rows and stored entries of each column's domain (host, from the
@@ -378,7 +396,7 @@ class DevicePlan:
         if thresh is not None:
             th[: len(thresh)] = [t if (t is not None and t > 0) else -1.0 for t in thresh]
         vp = ctypes.c_void_p
-        _lib.check(self.lib.rm_propagate(
+        rc = _lib.check(self.lib.rm_propagate(
             self.handle, vp(psi.data_ptr()), vp(work.data_ptr()), psi.stride(0), psi.shape[0],
             table.shape[0], vp(table.data_ptr()), _lib.ptr(th, ctypes.c_double),
             float(f.real), float(f.imag), vp(phase.data_ptr() if phase is not None else None),
@@ -386,6 +404,8 @@ class DevicePlan:
             vp(kused.data_ptr() if kused is not None else None),
             vp(status.data_ptr() if status is not None else None),
             vp(_stream_handle(stream))), "rm_propagate")
+        if rc == _lib.RM_DEVICE_TIMEOUT:
+            raise RuntimeError("rm_propagate: " + self.lib.rm_last_error().decode(errors="replace"))
 
     def expect(self, psi, out, stream=None):
         import ctypes

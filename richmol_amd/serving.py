@@ -299,6 +299,7 @@ class Server:
         h = ctypes.c_void_p()
         _lib.check(self.lib.rm_batch_create(ctypes.byref(h)), "rm_batch_create")
         self.handle = h
+        _lib.track(self)
         if block_rows is not None:
             _lib.check(self.lib.rm_batch_set_block_rows(h, int(block_rows)), "rm_batch_set_block_rows")
         self.block_rows = 1024 if block_rows is None else int(block_rows)
@@ -307,13 +308,18 @@ class Server:
         self.models = {}
         self.used = 0
 
-    def __del__(self):
+    def close(self):
+        """Destroy the device batch object (idempotent)."""
         h = getattr(self, "handle", None)
         if h is not None and h.value:
-            try:
-                self.lib.rm_batch_destroy(h)
-            except Exception:
-                pass
+            self.lib.rm_batch_destroy(h)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
     def model(self, jmax):
         m = self.models.get(jmax)

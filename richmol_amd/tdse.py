@@ -223,9 +223,12 @@ class TDSE:
             if precision != "fp64":
                 raise ValueError("precision applies to the Kronecker-block form; use Ensemble for FP32 matvecs")
             plan = _plan.plan_for(H)
-            # plans are shared with ensembles that may have picked another kernel
+            # plans are shared with ensembles that may have picked another
+            # kernel or precision
             if plan.kernel != "auto":
                 plan.set_kernel("auto")
+            if plan.precision != "fp64":
+                plan.set_precision("fp64")
 
         phase = None
         if H0 is not None:
