@@ -281,6 +281,30 @@ int rm_batch_propagate(rm_batch* batch, const rm_job* jobs, int32_t njobs, int32
                        int32_t krylov_fixed, int32_t maxorder, double tol, int32_t precision,
                        void* stream);
 
+/*
+ * rm_batch_propagate without allocating (no counterpart in the reference: for
+ * callers that cannot allocate device memory -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The same
+ * jobs, kernels and results as rm_batch_propagate, with the job descriptors
+ * and the Krylov workspace in a fixed device array of the library
+ * (RM_STATIC_SCRATCH_BYTES per GPU, held while the process runs) instead of
+ * memory the call allocates: no hipMalloc/hipFree, no pinned memory, no
+ * streams or events.  Limits, each refused with RM_ERR_ARG and a message:
+ * at most RM_STATIC_MAX_JOBS jobs; every job's domain on the one-wave kernels
+ * (<= RM_SUBSPACE_MAX_ROWS rows, no block-form jobs); every plan's subspace
+ * arrays built beforehand (rm_plan_subspaces, or any earlier propagate call on
+ * the plan).  The launch classes run one after another on `stream`, with as
+ * many waves as the workspace holds (fewer than rm_batch_propagate may use
+ * for Krylov bases outside LDS or the adaptive rule's Gram matrices; the
+ * results do not depend on the wave count).  Every call on a GPU shares that
+ * GPU's array: calls that overlap on one GPU (other streams, other threads)
+ * race and must be serialised by the caller.
+ */
+#define RM_STATIC_SCRATCH_BYTES (32 << 20)
+#define RM_STATIC_MAX_JOBS 4096
+int rm_batch_propagate_static(const rm_job* jobs, int32_t njobs, int32_t job_bytes, int32_t krylov_fixed,
+                              int32_t maxorder, double tol, int32_t precision, void* stream);
+
 /* psi_out[:, b] = (psi_in[:, b] * phase) * phase: the step when the operator
  * carries no field at all (tdse.py:375, 394 with the Lanczos skipped). */
 int rm_phase_apply(int64_t n, const void* psi_in, void* psi_out, int64_t ld, int64_t ncols,

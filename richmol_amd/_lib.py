@@ -59,6 +59,8 @@ SIGNATURES = {
     "rm_batch_set_block_rows": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "rm_batch_propagate": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                           ctypes.c_int32, ctypes.c_double, ctypes.c_int32, _vp]),
+    "rm_batch_propagate_static": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                                 ctypes.c_int32, ctypes.c_double, ctypes.c_int32, _vp]),
     "rm_kplan_create": (ctypes.c_int, [ctypes.c_int32, _c_i32p, ctypes.c_int32, ctypes.c_int32, _c_i32p,
                                        _c_i32p, _c_i32p, _c_i32p, _c_i64p, _c_dblp, _c_i64p, _c_i32p,
                                        _c_i64p, _c_dblp, ctypes.POINTER(_vp)]),

@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <mutex>
 #include <numeric>
 
 #include "rm_common.h"
@@ -40,6 +41,7 @@ struct SubArgs {
   double2* cscr;           // per wave 2 * kcap (adaptive)
   int32_t kcap, fixed_m, maxorder;
   double tol;
+  int32_t max_waves;       // > 0: at most this many waves (the launch's workspace holds no more)
 };
 
 // One wave per wavepacket, R rows per lane (domain <= 64 R rows).  WM > 0:
@@ -896,6 +898,7 @@ hipError_t launch_pair(const SubArgs& a0, int n_cu, hipStream_t s, int* waves_ou
   if (occ < 1) occ = 1;
   const int64_t npair = ((int64_t)a0.nitems + 1) / 2;
   int waves = (int)std::min<int64_t>(npair, (int64_t)occ * n_cu);
+  if (a0.max_waves > 0) waves = std::min(waves, a0.max_waves);
   if (waves < 1) waves = 1;
   *waves_out = waves;
   if (dry) return hipSuccess;
@@ -920,6 +923,7 @@ hipError_t launch_sub_k(const SubArgs& a0, int n_cu, hipStream_t s, int* waves_o
   if (e != hipSuccess) return e;
   if (occ < 1) occ = 1;
   int waves = std::min<int64_t>(a0.nitems, (int64_t)occ * n_cu);
+  if (a0.max_waves > 0) waves = std::min(waves, a0.max_waves);
   if (waves < 1) waves = 1;
   *waves_out = waves;
   if (dry) return hipSuccess;
@@ -1006,18 +1010,11 @@ void scratch_release(SubScratch* sc) {
   delete sc;
 }
 
-// Stage ops and items (the wave classes, then the block and cooperative
-// lists) and launch one kernel per non-empty wave class, the block kernel
-// and the cooperative launches on s; with timing on, the launches are
-// bracketed as one.
-int sub_run(SubScratch* sc, const std::vector<SubOp>& ops, std::vector<int4> (&cls)[NCLASS],
-            const std::vector<LongItem>& blk, const std::vector<LongItem>& coop, int32_t kcap,
-            int32_t fixed_m, int32_t maxorder, double tol, bool f32, int n_cu, hipStream_t s,
-            rm_plan* timed, int32_t abort_preset = 0) {
-  // the pair classes: class-0 items of <= 32 rows under a fixed Krylov
-  // dimension <= 16 (the banded ones, placed in PAIR_BAND_CLASS by the
-  // caller, keep their own class), grouped by operator, each group padded to
-  // an even count; other Krylov settings send the banded items back to class 0
+// The pair classes: class-0 items of <= 32 rows under a fixed Krylov
+// dimension <= 16 (the banded ones, placed in PAIR_BAND_CLASS by the caller,
+// keep their own class), grouped by operator, each group padded to an even
+// count; other Krylov settings send the banded items back to class 0.
+void group_pairs(std::vector<int4> (&cls)[NCLASS], int32_t fixed_m) {
   const bool pairs = fixed_m >= 1 && fixed_m <= PAIR_MAX_M;
   if (!pairs && !cls[PAIR_BAND_CLASS].empty()) {
     cls[0].insert(cls[0].end(), cls[PAIR_BAND_CLASS].begin(), cls[PAIR_BAND_CLASS].end());
@@ -1042,6 +1039,17 @@ int sub_run(SubScratch* sc, const std::vector<SubOp>& ops, std::vector<int4> (&c
     pair_up(cls[PAIR_CLASS]);
     pair_up(cls[PAIR_BAND_CLASS]);
   }
+}
+
+// Stage ops and items (the wave classes, then the block and cooperative
+// lists) and launch one kernel per non-empty wave class, the block kernel
+// and the cooperative launches on s; with timing on, the launches are
+// bracketed as one.
+int sub_run(SubScratch* sc, const std::vector<SubOp>& ops, std::vector<int4> (&cls)[NCLASS],
+            const std::vector<LongItem>& blk, const std::vector<LongItem>& coop, int32_t kcap,
+            int32_t fixed_m, int32_t maxorder, double tol, bool f32, int n_cu, hipStream_t s,
+            rm_plan* timed, int32_t abort_preset = 0) {
+  group_pairs(cls, fixed_m);
   size_t nitems = blk.size() + coop.size();
   for (int c = 0; c < NCLASS; ++c) nitems += cls[c].size();
   if (nitems == 0) return RM_OK;
@@ -1493,45 +1501,48 @@ extern "C" int rm_plan_step_flags(rm_plan* p, int64_t nsteps, const double* fpro
   return RM_OK;
 }
 
-extern "C" int rm_batch_propagate(rm_batch* bt, const rm_job* jobs, int32_t njobs, int32_t job_bytes,
-                                  int32_t krylov_fixed, int32_t maxorder, double tol, int32_t precision,
-                                  void* stream) {
-  if (!bt) return fail(RM_ERR_ARG, "rm_batch_propagate: NULL batch");
+// Jobs of a batch call as launch lists: validated, one SubOp each, wave
+// classes and block-form items, longest jobs first.  fixed_plans: every plan
+// must already hold its subspace arrays (built by an earlier call; nothing
+// is uploaded here) and every job must fit the one-wave kernels.
+static int collect_jobs(const char* who, const rm_job* jobs, int32_t njobs, int32_t job_bytes, int32_t block_rows,
+                 bool fixed_plans, std::vector<SubOp>& ops, std::vector<int4> (&cls)[NCLASS],
+                 std::vector<LongItem>& blk) {
   if (job_bytes != (int32_t)sizeof(rm_job))
-    return fail(RM_ERR_ARG, "rm_batch_propagate: job records of %d bytes, the ABI's rm_job has %d",
-                job_bytes, (int)sizeof(rm_job));
-  if (njobs < 0 || (njobs > 0 && !jobs)) return fail(RM_ERR_ARG, "rm_batch_propagate: bad job list");
-  if (krylov_fixed < 0 || krylov_fixed > RM_MAX_KRYLOV)
-    return fail(RM_ERR_ARG, "rm_batch_propagate: krylov_fixed=%d outside [0, %d]", krylov_fixed, RM_MAX_KRYLOV);
-  if (krylov_fixed == 0 && (maxorder < 1 || maxorder > RM_MAX_KRYLOV))
-    return fail(RM_ERR_ARG, "rm_batch_propagate: maxorder=%d outside [1, %d]", maxorder, RM_MAX_KRYLOV);
-  if (krylov_fixed == 0 && !(tol > 0.0 && tol <= 1.0)) return fail(RM_ERR_ARG, "rm_batch_propagate: tol must be in (0, 1]");
-  if (precision != RM_PREC_FP64 && precision != RM_PREC_FP32)
-    return fail(RM_ERR_ARG, "rm_batch_propagate: unknown precision %d", precision);
-  std::vector<SubOp> ops;
+    return fail(RM_ERR_ARG, "%s: job records of %d bytes, the ABI's rm_job has %d", who, job_bytes,
+                (int)sizeof(rm_job));
+  if (njobs < 0 || (njobs > 0 && !jobs)) return fail(RM_ERR_ARG, "%s: bad job list", who);
   ops.reserve((size_t)njobs);
-  std::vector<int4> cls[NCLASS];
   std::vector<int64_t> work[NCLASS];
-  std::vector<LongItem> blk;
   std::vector<int64_t> blk_work;
   for (int32_t j = 0; j < njobs; ++j) {
     const rm_job& jb = jobs[j];
     rm_plan* p = jb.plan;
-    if (!p) return fail(RM_ERR_ARG, "rm_batch_propagate: job %d has no plan", j);
-    if (jb.nsteps < 0) return fail(RM_ERR_ARG, "rm_batch_propagate: job %d nsteps < 0", j);
+    if (!p) return fail(RM_ERR_ARG, "%s: job %d has no plan", who, j);
+    if (jb.nsteps < 0) return fail(RM_ERR_ARG, "%s: job %d nsteps < 0", who, j);
     if (jb.nsteps == 0) continue;
-    int rc = sub_build(p);
-    if (rc) return rc;
+    int rc = RM_OK;
+    if (fixed_plans) {
+      if (!p->sub_ready || !p->sub_col)
+        return fail(RM_ERR_ARG, "%s: job %d: the plan's subspace arrays are not built (call rm_plan_subspaces "
+                    "on it once beforehand)", who, j);
+    } else if ((rc = sub_build(p))) {
+      return rc;
+    }
     if (!jb.psi || jb.ld < p->n || jb.col < 0 || !jb.sflag_dev || (p->nfield > 0 && !jb.fprod_dev))
-      return fail(RM_ERR_ARG, "rm_batch_propagate: job %d: bad wavepacket, field table or flags", j);
+      return fail(RM_ERR_ARG, "%s: job %d: bad wavepacket, field table or flags", who, j);
     // the domain must be a union of whole subspaces, or gathers would leave it
     const int32_t lo = jb.lo, len = jb.len;
     if (len < 1 || len > RM_BLOCK_MAX_ROWS || lo < 0 || (int64_t)lo + len > p->n ||
         p->h_sub_start[p->h_sub_label_perm[lo]] != lo ||
         p->h_sub_end[p->h_sub_label_perm[lo + len - 1]] != lo + len)
-      return fail(RM_ERR_ARG, "rm_batch_propagate: job %d: [%d, %d) is not a domain of %d rows at most "
-                  "(use rm_plan_domains)", j, lo, lo + len, RM_BLOCK_MAX_ROWS);
-    const bool longdom = len > bt->block_rows || !p->sub_col;
+      return fail(RM_ERR_ARG, "%s: job %d: [%d, %d) is not a domain of %d rows at most "
+                  "(use rm_plan_domains)", who, j, lo, lo + len, RM_BLOCK_MAX_ROWS);
+    const bool longdom = len > block_rows || !p->sub_col;
+    if (longdom && fixed_plans)
+      return fail(RM_ERR_ARG, "%s: job %d: a domain of %d rows needs the block kernel, whose workspace this "
+                  "entry point does not hold (at most %d rows; use rm_batch_propagate)", who, j, len,
+                  RM_SUBSPACE_MAX_ROWS);
     if (longdom && !p->sub_rp && (rc = csr_build(p, entry_values(p)))) return rc;
     SubOp op = plan_op(p);
     op.phase = (const double2*)jb.phase;
@@ -1566,19 +1577,153 @@ extern "C" int rm_batch_propagate(rm_batch* bt, const rm_job* jobs, int32_t njob
   }
   // long domains: one workgroup each (block kernel), LDS-staged domains
   // first, most work first
-  {
-    std::vector<size_t> ix(blk.size());
-    std::iota(ix.begin(), ix.end(), 0);
-    std::stable_sort(ix.begin(), ix.end(), [&](size_t x, size_t y) {
-      const bool lx = blk[x].item.z <= RM_XL_ROWS, ly = blk[y].item.z <= RM_XL_ROWS;
-      if (lx != ly) return lx;
-      return blk_work[x] > blk_work[y];
-    });
-    std::vector<LongItem> sorted(ix.size());
-    for (size_t t = 0; t < ix.size(); ++t) sorted[t] = blk[ix[t]];
-    blk.swap(sorted);
-  }
+  std::vector<size_t> ix(blk.size());
+  std::iota(ix.begin(), ix.end(), 0);
+  std::stable_sort(ix.begin(), ix.end(), [&](size_t x, size_t y) {
+    const bool lx = blk[x].item.z <= RM_XL_ROWS, ly = blk[y].item.z <= RM_XL_ROWS;
+    if (lx != ly) return lx;
+    return blk_work[x] > blk_work[y];
+  });
+  std::vector<LongItem> sorted(ix.size());
+  for (size_t t = 0; t < ix.size(); ++t) sorted[t] = blk[ix[t]];
+  blk.swap(sorted);
+  return RM_OK;
+}
+
+static int check_krylov(const char* who, int32_t krylov_fixed, int32_t maxorder, double tol, int32_t precision) {
+  if (krylov_fixed < 0 || krylov_fixed > RM_MAX_KRYLOV)
+    return fail(RM_ERR_ARG, "%s: krylov_fixed=%d outside [0, %d]", who, krylov_fixed, RM_MAX_KRYLOV);
+  if (krylov_fixed == 0 && (maxorder < 1 || maxorder > RM_MAX_KRYLOV))
+    return fail(RM_ERR_ARG, "%s: maxorder=%d outside [1, %d]", who, maxorder, RM_MAX_KRYLOV);
+  if (krylov_fixed == 0 && !(tol > 0.0 && tol <= 1.0)) return fail(RM_ERR_ARG, "%s: tol must be in (0, 1]", who);
+  if (precision != RM_PREC_FP64 && precision != RM_PREC_FP32)
+    return fail(RM_ERR_ARG, "%s: unknown precision %d", who, precision);
+  return RM_OK;
+}
+
+extern "C" int rm_batch_propagate(rm_batch* bt, const rm_job* jobs, int32_t njobs, int32_t job_bytes,
+                                  int32_t krylov_fixed, int32_t maxorder, double tol, int32_t precision,
+                                  void* stream) {
+  if (!bt) return fail(RM_ERR_ARG, "rm_batch_propagate: NULL batch");
+  int rc = check_krylov("rm_batch_propagate", krylov_fixed, maxorder, tol, precision);
+  if (rc) return rc;
+  std::vector<SubOp> ops;
+  std::vector<int4> cls[NCLASS];
+  std::vector<LongItem> blk;
+  if ((rc = collect_jobs("rm_batch_propagate", jobs, njobs, job_bytes, bt->block_rows, false, ops, cls, blk)))
+    return rc;
   const std::vector<LongItem> none;
   return sub_run(bt->sc, ops, cls, blk, none, krylov_fixed > 0 ? krylov_fixed : maxorder, krylov_fixed,
                  maxorder, tol, precision == RM_PREC_FP32, bt->n_cu, (hipStream_t)stream, nullptr);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free batch calls (rm_batch_propagate_static).  The descriptors
+// and the Krylov workspace live in `scratch`, one copy per device (the
+// library holds it while the process runs); host code reaches it only
+// through the address hipGetSymbolAddress returns for the current device.
+
+static constexpr size_t STATIC_STAGE_BYTES = (size_t)RM_STATIC_MAX_JOBS * (sizeof(SubOp) + sizeof(int4)) + 512;
+static_assert(STATIC_STAGE_BYTES < (size_t)RM_STATIC_SCRATCH_BYTES / 4, "descriptor staging exceeds its share");
+static __device__ __align__(256) char scratch[RM_STATIC_SCRATCH_BYTES];
+
+static constexpr int STATIC_MAX_DEVICES = 64;
+static std::mutex g_static_mu;
+static char* g_static_ptr[STATIC_MAX_DEVICES] = {};
+static int g_static_cu[STATIC_MAX_DEVICES] = {};
+
+// The current device's copy of `scratch` and its CU count (looked up once per device).
+static int static_scratch(char** ptr, int* n_cu) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= STATIC_MAX_DEVICES) return fail(RM_ERR_ARG, "rm_batch_propagate_static: device %d", dev);
+  std::lock_guard<std::mutex> lock(g_static_mu);
+  if (!g_static_ptr[dev]) {
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(RM_ERR_HIP, "rm_batch_propagate_static: no device address for the scratch array");
+    int cu = 0;
+    if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cu <= 0) cu = 256;
+    g_static_cu[dev] = cu;
+    g_static_ptr[dev] = (char*)p;
+  }
+  *ptr = g_static_ptr[dev];
+  *n_cu = g_static_cu[dev];
+  return RM_OK;
+}
+
+// sub_run for the wave classes with everything in [dev, dev + bytes): the
+// descriptors first, the Krylov workspace after them; the classes run one
+// after another on s, each with as many waves as the workspace holds.
+static int sub_run_static(char* dev, size_t bytes, const std::vector<SubOp>& ops, std::vector<int4> (&cls)[NCLASS],
+                   int32_t kcap, int32_t fixed_m, int32_t maxorder, double tol, bool f32, int n_cu, hipStream_t s) {
+  group_pairs(cls, fixed_m);
+  size_t nitems = 0;
+  for (int c = 0; c < NCLASS; ++c) nitems += cls[c].size();
+  if (nitems == 0) return RM_OK;
+  const size_t ops_bytes = (ops.size() * sizeof(SubOp) + 255) & ~(size_t)255;
+  const size_t stage = (ops_bytes + nitems * sizeof(int4) + 255) & ~(size_t)255;
+  if (stage > STATIC_STAGE_BYTES) return fail(RM_ERR_ARG, "rm_batch_propagate_static: %zu descriptor bytes", stage);
+  static thread_local std::vector<char> host;   // host side of the staging (pageable)
+  if (host.size() < stage) host.resize(STATIC_STAGE_BYTES);
+  memcpy(host.data(), ops.data(), ops.size() * sizeof(SubOp));
+  size_t off[NCLASS];
+  size_t pos = ops_bytes;
+  for (int c = 0; c < NCLASS; ++c) {
+    off[c] = pos;
+    if (!cls[c].empty()) memcpy(host.data() + pos, cls[c].data(), cls[c].size() * sizeof(int4));
+    pos += cls[c].size() * sizeof(int4);
+  }
+  HIPCHK(hipMemcpyAsync(dev, host.data(), stage, hipMemcpyHostToDevice, s));
+  char* ws = dev + stage;
+  const size_t ws_bytes = bytes - stage;
+  SubArgs a;
+  memset(&a, 0, sizeof(a));
+  a.ops = (const SubOp*)dev;
+  a.kcap = kcap;
+  a.fixed_m = fixed_m;
+  a.maxorder = maxorder;
+  a.tol = tol;
+  for (int c = 0; c < NCLASS; ++c) {
+    if (cls[c].empty()) continue;
+    a.items = (const int4*)(dev + off[c]);
+    a.nitems = (int32_t)cls[c].size();
+    const bool bl = class_basis_lds(c, kcap);
+    const size_t per_wave = sizeof(double2) * ((bl ? 0 : (size_t)kcap * 64 * CLASS_R[c]) +
+                                               (fixed_m > 0 ? 0 : (size_t)kcap * kcap + 2 * (size_t)kcap));
+    a.max_waves = 0;
+    if (per_wave > 0) {
+      const size_t cap = ws_bytes / per_wave;
+      if (cap < 1)
+        return fail(RM_ERR_ARG, "rm_batch_propagate_static: one wave's Krylov workspace (%zu bytes) exceeds "
+                    "the %zu bytes left in the scratch array", per_wave, ws_bytes);
+      a.max_waves = (int32_t)std::min<size_t>(cap, INT32_MAX);
+    }
+    int waves = 0;
+    HIPCHK(launch_class(c, a, f32, n_cu, s, &waves, true));
+    a.basis = (double2*)ws;
+    a.gram = a.basis + (bl ? 0 : (size_t)waves * kcap * 64 * CLASS_R[c]);
+    a.cscr = a.gram + (fixed_m > 0 ? 0 : (size_t)waves * kcap * kcap);
+    HIPCHK(launch_class(c, a, f32, n_cu, s, &waves, false));
+  }
+  return RM_OK;
+}
+
+extern "C" int rm_batch_propagate_static(const rm_job* jobs, int32_t njobs, int32_t job_bytes, int32_t krylov_fixed,
+                                         int32_t maxorder, double tol, int32_t precision, void* stream) {
+  int rc = check_krylov("rm_batch_propagate_static", krylov_fixed, maxorder, tol, precision);
+  if (rc) return rc;
+  if (njobs > RM_STATIC_MAX_JOBS)
+    return fail(RM_ERR_ARG, "rm_batch_propagate_static: %d jobs exceed the maximum of %d", njobs, RM_STATIC_MAX_JOBS);
+  char* dev = nullptr;
+  int n_cu = 0;
+  if ((rc = static_scratch(&dev, &n_cu))) return rc;
+  std::vector<SubOp> ops;
+  std::vector<int4> cls[NCLASS];
+  std::vector<LongItem> blk;
+  if ((rc = collect_jobs("rm_batch_propagate_static", jobs, njobs, job_bytes, RM_SUBSPACE_MAX_ROWS, true, ops, cls,
+                         blk)))
+    return rc;
+  return sub_run_static(dev, RM_STATIC_SCRATCH_BYTES, ops, cls, krylov_fixed > 0 ? krylov_fixed : maxorder,
+                        krylov_fixed, maxorder, tol, precision == RM_PREC_FP32, n_cu, (hipStream_t)stream);
 }
