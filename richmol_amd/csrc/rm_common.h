@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "../../include/richmol_amd.h"
@@ -143,6 +144,10 @@ struct rm_plan {
   int64_t dom_cap = 0;
   int32_t* sflag = nullptr;      // per step: 1 if any MF element is non-zero
   int64_t sflag_cap = 0;
+  int32_t* sflag_part = nullptr; // per step and chunk of terms: partial flags
+  int64_t sflag_part_cap = 0;
+  uint32_t* sflag_arrive = nullptr;   // per step: chunk arrivals (only grows)
+  int64_t sflag_arrive_cap = 0;
   std::vector<int32_t> h_dom;
   int32_t* h_dom_pin = nullptr;  // pinned landing buffer of the domains' copy to the host
   int64_t h_dom_pin_cap = 0;
@@ -192,6 +197,9 @@ struct SubOp {
   int32_t n, w, nslot, nsteps, kcol0, call_id;
 };
 
+// Whether step `step` runs the expv (the `len(H.mfmat) > 0` test of tdse.py:377).
+__device__ __forceinline__ bool step_on(const SubOp& op, int step) { return op.sflag[step] != 0; }
+
 template <typename T>
 static int upload(T** dst, const T* src, int64_t count) {
   *dst = nullptr;
@@ -220,6 +228,27 @@ static int grow(T** buf, int64_t* cap, int64_t count) {
   return RM_OK;
 }
 
+// Workgroups of `kern` a CU holds with `lds` bytes of dynamic LDS, asked
+// once per (kernel, LDS size) -- the query costs more than a launch.
+static inline hipError_t occupancy(const void* kern, int threads, size_t lds, int* occ) {
+  struct Entry { const void* k; size_t lds; int occ; };
+  static std::mutex mu;
+  static std::vector<Entry> cache;
+  std::lock_guard<std::mutex> lock(mu);
+  for (const Entry& e : cache)
+    if (e.k == kern && e.lds == lds) {
+      *occ = e.occ;
+      return hipSuccess;
+    }
+  int o = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, kern, threads, lds);
+  if (e != hipSuccess) return e;
+  if (o < 1) o = 1;
+  cache.push_back({kern, lds, o});
+  *occ = o;
+  return hipSuccess;
+}
+
 // Krylov workspace of the plan, at least `bytes` (defined with the step kernels).
 int ensure_ws(rm_plan* p, size_t bytes);
 // Free a plan's subspace-launch scratch (rm_subspace.hip).
@@ -235,11 +264,13 @@ struct LongItem {
 };
 // Long-domain wavepackets (rm_coop.hip): blk one workgroup each in one launch,
 // coop the whole device each (one cooperative launch each).  d_ops, d_blk,
-// d_coop: device copies of the jobs and of the two item lists.  abort_preset:
-// test hook, the cooperative launches start with their abort flag raised.
+// d_coop: device copies of the jobs and of the two item lists; op0 (host,
+// may be NULL): the one job by value instead of d_ops.  abort_preset: test
+// hook, the cooperative launches start with their abort flag raised.
 int long_run(void** scratch, const SubOp* d_ops, const int4* d_blk, const std::vector<LongItem>& blk,
              const int4* d_coop, const std::vector<LongItem>& coop, int32_t kcap, int32_t fixed_m,
-             int32_t maxorder, double tol, bool f32, int n_cu, hipStream_t s, int32_t abort_preset = 0);
+             int32_t maxorder, double tol, bool f32, int n_cu, hipStream_t s, int32_t abort_preset = 0,
+             const SubOp* op0 = nullptr);
 // Event-bracketed launch helpers for the plan's optional timing.
 hipError_t timing_begin(rm_plan* p, hipStream_t s);
 hipError_t timing_end(rm_plan* p, hipStream_t s);
@@ -326,6 +357,26 @@ __device__ __forceinline__ double half_allsum32(double v) {
   double e, o;
   swap16_d(v, &e, &o);
   return e + o;
+}
+
+// sqrt(s) and 1/sqrt(s) for s > 0 without a branch: s is first scaled by an
+// exact power of two (2^1000 below 2^-900, 2^-1000 above 2^900) so that the
+// hardware reciprocal square root and its two Newton steps stay in range,
+// and the results are scaled back exactly.  s == 0 gives no meaningful
+// result (the caller's breakdown path replaces it).
+__device__ __forceinline__ void sqrt_rcp_nb(double s, double* beta, double* ib) {
+  const bool tiny = s < 0x1p-900, huge = s > 0x1p+900;
+  const double up = tiny ? 0x1p+1000 : (huge ? 0x1p-1000 : 1.0);
+  const double hup = tiny ? 0x1p+500 : (huge ? 0x1p-500 : 1.0);     // sqrt(up)
+  const double hdn = tiny ? 0x1p-500 : (huge ? 0x1p+500 : 1.0);     // 1 / sqrt(up)
+  const double t = s * up;
+  double r = __builtin_amdgcn_rsq(t);
+  double e = fma(-t * r, r, 1.0);
+  r = fma(0.5 * r, e, r);
+  e = fma(-t * r, r, 1.0);
+  r = fma(0.5 * r, e, r);
+  *beta = (t * r) * hdn;
+  *ib = r * hup;
 }
 
 __device__ __forceinline__ double wave_max(double v) {

@@ -63,7 +63,8 @@ constexpr int LRING = 4;   // slot sums use LRING sets of slots in turn
 enum { SUM_BARRIER = 0, SUM_SCALAR = 1, SUM_FENCED = 2 };
 
 struct LongArgs {
-  const SubOp* ops;
+  const SubOp* ops;        // the jobs, or NULL: the one job is op0
+  SubOp op0;               // the job of a single-job launch, by value
   const int4* items;       // (column, lo, len, op)
   int32_t nitems;
   char* ws;                // workspace: one region (cooperative) or one per workgroup (block)
@@ -343,7 +344,7 @@ __global__ __launch_bounds__(ct_of<MODE>()) void k_long(LongArgs a) {
 
   for (int itm = MODE == BLOCK ? blockIdx.x : 0; itm < a.nitems; itm += MODE == BLOCK ? gridDim.x : 1) {
     const int4 item = a.items[itm];
-    const SubOp& op = a.ops[item.w];
+    const SubOp& op = a.ops ? a.ops[item.w] : a.op0;
     const int b = item.x, lo = item.y, len = item.z;
     // reused domains that failed their check: the wavepacket is left as it is
     // (the same decision in every thread of the cooperating set)
@@ -395,7 +396,7 @@ __global__ __launch_bounds__(ct_of<MODE>()) void k_long(LongArgs a) {
     int built = -1;   // step whose operator the registers / Ast hold
 
     for (int step = 0; step < op.nsteps; ++step) {
-      if (op.sflag[step] == 0) {
+      if (!step_on(op, step)) {
         // no field: the expv is skipped (tdse.py:377), phases only
         if (op.phase)
           for (int64_t i = r_beg; i < len; i += r_step) S[i] = cmul(cmul(S[i], P[i]), P[i]);
@@ -736,7 +737,7 @@ hipError_t max_resident(int n_cu, size_t lds, int* grid) {
   int occ = INT32_MAX;
   for (const void* k : ks) {
     int o = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k, ct_of<MODE>(), lds);
+    hipError_t e = occupancy(k, ct_of<MODE>(), lds, &o);
     if (e != hipSuccess) return e;
     occ = std::min(occ, o);
   }
@@ -776,7 +777,8 @@ void coop_scratch_release(void* sc) {
 // copies of the jobs and of the two item lists (same order as blk / coop).
 int long_run(void** scratch, const SubOp* d_ops, const int4* d_blk, const std::vector<LongItem>& blk,
              const int4* d_coop, const std::vector<LongItem>& coop, int32_t kcap, int32_t fixed_m,
-             int32_t maxorder, double tol, bool f32, int n_cu, hipStream_t s, int32_t abort_preset) {
+             int32_t maxorder, double tol, bool f32, int n_cu, hipStream_t s, int32_t abort_preset,
+             const SubOp* op0) {
   if (blk.empty() && coop.empty()) return RM_OK;
   int gres_r = 1, gres_m = 1, gres_b = 1, gres_x = 1;
   HIPCHK((max_resident<GRID_REG>(n_cu, 0, &gres_r)));
@@ -855,7 +857,8 @@ int long_run(void** scratch, const SubOp* d_ops, const int4* d_blk, const std::v
   unsigned long long* bar = (unsigned long long*)w;
   int32_t* abort_flag = (int32_t*)(w + b_bar);
   for (LongArgs* a : {&ab, &ac}) {
-    a->ops = d_ops;
+    a->ops = op0 ? nullptr : d_ops;
+    if (op0) a->op0 = *op0;
     a->part = part;
     a->lpart = lpart;
     a->gram = gram;

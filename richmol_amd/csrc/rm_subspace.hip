@@ -33,7 +33,8 @@ constexpr size_t SUB_LDS_BASIS = 32 * 1024;    // LDS budget of a wave's Krylov 
 constexpr int64_t COOP_MAX_COLS = 8;           // long-domain wavepackets per call for the cooperative kernel
 
 struct SubArgs {
-  const SubOp* ops;
+  const SubOp* ops;        // the jobs, or NULL: the one job is op0
+  SubOp op0;               // the job of a single-job launch, by value (no staging)
   const int4* items;       // (column, lo, len, op), one wavepacket each
   int32_t nitems;
   double2* basis;          // per wave kcap * 64R (global-basis variant)
@@ -77,7 +78,7 @@ __global__ __launch_bounds__(64) void k_sub_prop(SubArgs a) {
   for (int itm = gw; itm < a.nitems; itm += nw) {
     const int4 item = a.items[itm];
     const int b = item.x, lo = item.y, len = item.z;
-    const SubOp op = a.ops[item.w];
+    const SubOp op = a.ops ? a.ops[item.w] : a.op0;
     if (op.stale && *op.stale == op.call_id) {
       // reused domains that failed their check: the wavepacket is left as it is
       if (l == 0 && op.status) op.status[b - op.kcol0] |= RM_STALE_DOMAIN;
@@ -120,7 +121,7 @@ __global__ __launch_bounds__(64) void k_sub_prop(SubArgs a) {
       double2 vk[R];
 #pragma unroll
       for (int r = 0; r < R; ++r) vk[r] = op.phase ? cmul(st[r], ph[r]) : st[r];
-      if (op.sflag[step] == 0) {
+      if (!step_on(op, step)) {
         // no field: the expv is skipped (tdse.py:377), phases only
 #pragma unroll
         for (int r = 0; r < R; ++r) st[r] = op.phase ? cmul(vk[r], ph[r]) : vk[r];
@@ -433,7 +434,7 @@ __global__ __launch_bounds__(64) void k_sub_pair(SubArgs a) {
     const int itm = 2 * pr + h;
     const int4 item = itm < a.nitems ? a.items[itm] : make_int4(-1, 0, 0, a.items[2 * pr].w);
     const int b = item.x, lo = item.y, len = b >= 0 ? item.z : 0;
-    const SubOp op = a.ops[item.w];   // the same for both halves
+    const SubOp op = a.ops ? a.ops[item.w] : a.op0;   // the same for both halves
     if (op.stale && *op.stale == op.call_id) {
       // reused domains that failed their check: both wavepackets are left as they are
       if (i == 0 && b >= 0 && op.status) op.status[b - op.kcol0] |= RM_STALE_DOMAIN;
@@ -477,9 +478,22 @@ __global__ __launch_bounds__(64) void k_sub_pair(SubArgs a) {
     double2* sa = s_alpha[h];
     double* sb = s_beta[h];
     double2* sc = s_cf[h];
+    // the step flag and the first two field products of the next step are
+    // loaded while the current step computes (their latency is off the chain)
+    int sf_next = op.nsteps > 0 ? op.sflag[0] : 0;
+    double2 f0_next = op.nsteps > 0 && op.nslot > 0 ? op.table[0] : zero;
+    double2 f1_next = op.nsteps > 0 && op.nslot > 1 ? op.table[1] : zero;
     for (int step = 0; step < op.nsteps; ++step) {
+      const int sf = sf_next;
+      const double2 f0 = f0_next, f1 = f1_next;
+      if (step + 1 < op.nsteps) {
+        const double2* q = op.table + (int64_t)(step + 1) * op.nslot;
+        sf_next = op.sflag[step + 1];
+        if (op.nslot > 0) f0_next = q[0];
+        if (op.nslot > 1) f1_next = q[1];
+      }
       double2 vk = op.phase ? cmul(st, ph) : st;
-      if (op.sflag[step] == 0) {
+      if (sf == 0) {
         st = op.phase ? cmul(vk, ph) : vk;
         if (i == 0 && b >= 0 && op.kused) op.kused[(int64_t)step * op.kused_ld + b - op.kcol0] = 0;
         continue;
@@ -488,15 +502,9 @@ __global__ __launch_bounds__(64) void k_sub_pair(SubArgs a) {
       XT A[WMR];
       XT Ab[3];
       if (BAND) {
-        const double2 f0 = fp[0], f1 = op.nslot > 1 ? fp[1] : zero;
 #pragma unroll
         for (int o = 0; o < 3; ++o) Ab[o] = to_x<F32>(cfma(f1, Bo[1][o], cmul(f0, Bo[0][o])));
       } else if (WM > 0) {
-        double2 f0 = zero, f1 = zero;
-        if (breg) {
-          f0 = fp[0];
-          if (op.nslot > 1) f1 = fp[1];
-        }
 #pragma unroll
         for (int j = 0; j < WMR; ++j) {
           double2 v;
@@ -539,12 +547,13 @@ __global__ __launch_bounds__(64) void k_sub_pair(SubArgs a) {
       sa[0] = al;
       int k = 1;
       for (int it = 1; it < a.fixed_m; ++it) {
-        const double beta = sqrt(half_allsum32(Wr.x * Wr.x + Wr.y * Wr.y));
-        const bool brk = !(beta != 0.0);
+        // beta = |W| and 1/beta from the reciprocal square root (one chain
+        // instead of a square root and a quotient)
+        const double ww = half_allsum32(Wr.x * Wr.x + Wr.y * Wr.y);
+        double beta = 0.0, ib = 0.0;
+        if (ww != 0.0) sqrt_rcp_nb(ww, &beta, &ib);
+        const bool brk = !(ww != 0.0);
         if (!brk) {
-          double ib = __builtin_amdgcn_rcp(beta);
-          ib = fma(fma(-beta, ib, 1.0), ib, ib);
-          ib = fma(fma(-beta, ib, 1.0), ib, ib);
           vp = vk;
           vk = cscale(ib, Wr);
         } else {
@@ -641,9 +650,12 @@ __global__ void k_step_flags(int64_t nterm, int64_t nsteps, int32_t nfield,
 }
 
 // The device work of an rm_propagate call ahead of its propagation launches,
-// in one launch: blocks [0, nb_flag) the step flags (one block per step,
-// as k_step_flags), blocks [nb_flag, nb_flag + nb_fill) zero kused and
-// status, the remaining blocks (domain reuse) check every entry of psi
+// in one launch: blocks [0, nb_flag) the step flags -- block (step, chunk)
+// tests the MF elements of its chunk of terms (no block walks all terms) and
+// stores its partial flag; the last block of a step to finish (a per-step
+// arrival counter that only grows, so it needs no clearing) combines them
+// into the step's flag; blocks [nb_flag, nb_flag + nb_fill) zero kused and
+// status; the remaining blocks (domain reuse) check every entry of psi
 // against its column's kept domain and store call_id into *stale on a miss.
 struct Prologue {
   int64_t nterm, nsteps;
@@ -666,23 +678,42 @@ struct Prologue {
   int32_t* stale;
   int32_t call_id;
   int32_t nb_flag, nb_fill, cpb;   // cpb: check blocks per column
+  int32_t parts;                    // flag chunks per step (nb_flag = nsteps * parts)
+  int64_t chunk;                    // terms per chunk
+  int32_t* part;                    // [nsteps][parts] partial flags (parts > 1)
+  uint32_t* arrive;                 // [nsteps] arrivals of a step's chunk blocks, multiples of parts between calls
 };
 
 __global__ __launch_bounds__(256) void k_sub_prologue(Prologue a) {
   const int tid = threadIdx.x;
   const int64_t bid = blockIdx.x;
   if (bid < a.nb_flag) {
-    for (int64_t st = bid; st < a.nsteps; st += a.nb_flag) {
-      const double2* fp = a.table + st * a.nfield;
-      int any = 0;
-      for (int64_t t = tid; t < a.nterm; t += 256) {
-        double2 acc = make_double2(0.0, 0.0);
-        for (int64_t q = a.term_ptr[t]; q < a.term_ptr[t + 1]; ++q)
-          acc = cfma(fp[a.term_field[q]], a.term_mval[q], acc);
-        any |= (acc.x != 0.0 || acc.y != 0.0 || acc.x != acc.x || acc.y != acc.y) ? 1 : 0;
+    const int64_t st = bid / a.parts, c = bid % a.parts;
+    const double2* fp = a.table + st * a.nfield;
+    const int64_t t1 = std::min(a.nterm, (c + 1) * a.chunk);
+    int any = 0;
+    for (int64_t t = c * a.chunk + tid; t < t1; t += 256) {
+      double2 acc = make_double2(0.0, 0.0);
+      for (int64_t q = a.term_ptr[t]; q < a.term_ptr[t + 1]; ++q)
+        acc = cfma(fp[a.term_field[q]], a.term_mval[q], acc);
+      any |= (acc.x != 0.0 || acc.y != 0.0 || acc.x != acc.x || acc.y != acc.y) ? 1 : 0;
+    }
+    any = __syncthreads_or(any);
+    if (tid == 0) {
+      if (a.parts == 1) {
+        a.sflag[st] = any ? 1 : 0;
+      } else {
+        __hip_atomic_store(a.part + bid, any ? 1 : 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+        const uint32_t old = atomicAdd(a.arrive + st, 1u);
+        if ((old + 1) % (uint32_t)a.parts == 0) {
+          __threadfence();
+          int all = 0;
+          for (int64_t c = 0; c < a.parts; ++c)
+            all |= __hip_atomic_load(a.part + st * a.parts + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          a.sflag[st] = all ? 1 : 0;
+        }
       }
-      any = __syncthreads_or(any);
-      if (tid == 0) a.sflag[st] = any ? 1 : 0;
     }
   } else if (bid < (int64_t)a.nb_flag + a.nb_fill) {
     const int64_t stride = (int64_t)a.nb_fill * 256;
@@ -893,9 +924,8 @@ template <bool F32, bool BAND>
 hipError_t launch_pair(const SubArgs& a0, int n_cu, hipStream_t s, int* waves_out, bool dry) {
   auto kern = k_sub_pair<4, F32, BAND>;
   int occ = 0;
-  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)kern, 64, 0);
+  hipError_t e = occupancy((const void*)kern, 64, 0, &occ);
   if (e != hipSuccess) return e;
-  if (occ < 1) occ = 1;
   const int64_t npair = ((int64_t)a0.nitems + 1) / 2;
   int waves = (int)std::min<int64_t>(npair, (int64_t)occ * n_cu);
   if (a0.max_waves > 0) waves = std::min(waves, a0.max_waves);
@@ -919,9 +949,8 @@ hipError_t launch_sub_k(const SubArgs& a0, int n_cu, hipStream_t s, int* waves_o
     attr = true;
   }
   int occ = 0;
-  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)kern, 64, lds);
+  hipError_t e = occupancy((const void*)kern, 64, lds, &occ);
   if (e != hipSuccess) return e;
-  if (occ < 1) occ = 1;
   int waves = std::min<int64_t>(a0.nitems, (int64_t)occ * n_cu);
   if (a0.max_waves > 0) waves = std::min(waves, a0.max_waves);
   if (waves < 1) waves = 1;
@@ -985,8 +1014,11 @@ struct SubScratch {
   const void* kept_psi = nullptr;
   int64_t kept_ld = 0, kept_ncols = 0;
   int kept_force = 0;
+  bool kept_pairs = false;             // grouped for two wavepackets per wave
   std::vector<int4> kept_cls[NCLASS];
   std::vector<LongItem> kept_blk, kept_coop;
+  int4* kept_dev = nullptr;            // the kept lists' items on the device (classes, block, cooperative)
+  size_t kept_dev_cap = 0;             // in items
 };
 
 void scratch_release(SubScratch* sc) {
@@ -998,6 +1030,7 @@ void scratch_release(SubScratch* sc) {
     if (sc->done[i]) (void)hipEventDestroy(sc->done[i]);
   }
   if (sc->ws) (void)hipFree(sc->ws);
+  if (sc->kept_dev) (void)hipFree(sc->kept_dev);
   coop_scratch_release(sc->long_ws);
   for (int i = 0; i <= NCLASS; ++i) {
     if (sc->aux[i]) {
@@ -1045,6 +1078,90 @@ void group_pairs(std::vector<int4> (&cls)[NCLASS], int32_t fixed_m) {
 // lists) and launch one kernel per non-empty wave class, the block kernel
 // and the cooperative launches on s; with timing on, the launches are
 // bracketed as one.
+// The launches of one call: every non-empty wave class, the block kernel and
+// the cooperative launches, on s (or, several launches and no cooperative
+// one, each on an auxiliary stream forked from s and joined back into it,
+// so they run side by side); each class with a workspace region of its own.
+// a0: the jobs (a0.ops, or one job by value in a0.op0); items[c]: device
+// work items of class c; d_blk, d_coop: device items of the long lists.
+int launch_all(SubScratch* sc, const SubArgs& a0, const int4* const (&items)[NCLASS],
+               const std::vector<int4> (&cls)[NCLASS], const int4* d_blk, const std::vector<LongItem>& blk,
+               const int4* d_coop, const std::vector<LongItem>& coop, int32_t kcap, int32_t fixed_m,
+               int32_t maxorder, double tol, bool f32, int n_cu, hipStream_t s, rm_plan* timed,
+               int32_t abort_preset) {
+  SubArgs a = a0;
+  size_t need = 0, wsoff[NCLASS] = {};
+  int ngroups = blk.empty() ? 0 : 1;
+  for (int c = 0; c < NCLASS; ++c) {
+    if (cls[c].empty()) continue;
+    ++ngroups;
+    a.nitems = (int32_t)cls[c].size();
+    int waves = 0;
+    HIPCHK(launch_class(c, a, f32, n_cu, s, &waves, true));
+    const size_t per_wave = (class_basis_lds(c, kcap) ? 0 : (size_t)kcap * 64 * CLASS_R[c]) +
+                            (fixed_m > 0 ? 0 : (size_t)kcap * kcap + 2 * (size_t)kcap);
+    wsoff[c] = need;
+    need += (sizeof(double2) * std::max<size_t>(per_wave, 1) * (size_t)waves + 255) & ~(size_t)255;
+  }
+  if (sc->ws_bytes < need) {
+    if (sc->ws) {
+      HIPCHK(hipDeviceSynchronize());
+      HIPCHK(hipFree(sc->ws));
+      sc->ws = nullptr;
+      sc->ws_bytes = 0;
+    }
+    HIPCHK(hipMalloc(&sc->ws, need));
+    sc->ws_bytes = need;
+  }
+  if (timed) HIPCHK(timing_begin(timed, s));
+  const bool fork = ngroups > 1 && coop.empty();
+  int nused = 0;
+  auto next_stream = [&](hipStream_t* out) -> int {
+    if (!fork) {
+      *out = s;
+      return RM_OK;
+    }
+    const int i = nused++;
+    if (!sc->aux[i]) HIPCHK(hipStreamCreateWithFlags(&sc->aux[i], hipStreamNonBlocking));
+    if (!sc->join[i]) HIPCHK(hipEventCreateWithFlags(&sc->join[i], hipEventDisableTiming));
+    HIPCHK(hipStreamWaitEvent(sc->aux[i], sc->fork, 0));
+    *out = sc->aux[i];
+    return RM_OK;
+  };
+  if (fork) {
+    if (!sc->fork) HIPCHK(hipEventCreateWithFlags(&sc->fork, hipEventDisableTiming));
+    HIPCHK(hipEventRecord(sc->fork, s));
+  }
+  int rc = RM_OK;
+  for (int c = 0; c < NCLASS; ++c) {
+    if (cls[c].empty()) continue;
+    a.items = items[c];
+    a.nitems = (int32_t)cls[c].size();
+    int waves = 0;
+    HIPCHK(launch_class(c, a, f32, n_cu, s, &waves, true));
+    const bool bl = class_basis_lds(c, kcap);
+    a.basis = (double2*)((char*)sc->ws + wsoff[c]);
+    a.gram = a.basis + (bl ? 0 : (size_t)waves * kcap * 64 * CLASS_R[c]);
+    a.cscr = a.gram + (fixed_m > 0 ? 0 : (size_t)waves * kcap * kcap);
+    hipStream_t cs;
+    if ((rc = next_stream(&cs))) return rc;
+    HIPCHK(launch_class(c, a, f32, n_cu, cs, &waves, false));
+    if (fork) HIPCHK(hipEventRecord(sc->join[nused - 1], cs));
+  }
+  hipStream_t bs = s;
+  if (!blk.empty() && (rc = next_stream(&bs))) return rc;
+  rc = long_run(&sc->long_ws, a0.ops, d_blk, blk, d_coop, coop, kcap, fixed_m, maxorder, tol, f32, n_cu, bs,
+                abort_preset, a0.ops ? nullptr : &a0.op0);
+  if (rc) return rc;
+  if (fork && !blk.empty()) HIPCHK(hipEventRecord(sc->join[nused - 1], bs));
+  for (int i = 0; i < (fork ? nused : 0); ++i) HIPCHK(hipStreamWaitEvent(s, sc->join[i], 0));
+  if (timed) HIPCHK(timing_end(timed, s));
+  return RM_OK;
+}
+
+// Stage ops and items (the wave classes, then the block and cooperative
+// lists) through the double-buffered pinned staging and launch them
+// (launch_all); with timing on, the launches are bracketed as one.
 int sub_run(SubScratch* sc, const std::vector<SubOp>& ops, std::vector<int4> (&cls)[NCLASS],
             const std::vector<LongItem>& blk, const std::vector<LongItem>& coop, int32_t kcap,
             int32_t fixed_m, int32_t maxorder, double tol, bool f32, int n_cu, hipStream_t s,
@@ -1076,10 +1193,10 @@ int sub_run(SubScratch* sc, const std::vector<SubOp>& ops, std::vector<int4> (&c
     }
   }
   memcpy(sc->host[bi], ops.data(), ops.size() * sizeof(SubOp));
-  size_t off[NCLASS];
+  const int4* items[NCLASS];
   size_t pos = ops_bytes;
   for (int c = 0; c < NCLASS; ++c) {
-    off[c] = pos;
+    items[c] = (const int4*)(sc->dev[bi] + pos);
     if (!cls[c].empty()) memcpy(sc->host[bi] + pos, cls[c].data(), cls[c].size() * sizeof(int4));
     pos += cls[c].size() * sizeof(int4);
   }
@@ -1094,7 +1211,6 @@ int sub_run(SubScratch* sc, const std::vector<SubOp>& ops, std::vector<int4> (&c
     pos += sizeof(int4);
   }
   HIPCHK(hipMemcpyAsync(sc->dev[bi], sc->host[bi], bytes, hipMemcpyHostToDevice, s));
-
   SubArgs a;
   memset(&a, 0, sizeof(a));
   a.ops = (const SubOp*)sc->dev[bi];
@@ -1102,76 +1218,10 @@ int sub_run(SubScratch* sc, const std::vector<SubOp>& ops, std::vector<int4> (&c
   a.fixed_m = fixed_m;
   a.maxorder = maxorder;
   a.tol = tol;
-  // one workspace region per class (the classes may run at the same time)
-  size_t need = 0, wsoff[NCLASS] = {};
-  int ngroups = blk.empty() ? 0 : 1;
-  for (int c = 0; c < NCLASS; ++c) {
-    if (cls[c].empty()) continue;
-    ++ngroups;
-    a.nitems = (int32_t)cls[c].size();
-    int waves = 0;
-    HIPCHK(launch_class(c, a, f32, n_cu, s, &waves, true));
-    const size_t per_wave = (class_basis_lds(c, kcap) ? 0 : (size_t)kcap * 64 * CLASS_R[c]) +
-                            (fixed_m > 0 ? 0 : (size_t)kcap * kcap + 2 * (size_t)kcap);
-    wsoff[c] = need;
-    need += (sizeof(double2) * std::max<size_t>(per_wave, 1) * (size_t)waves + 255) & ~(size_t)255;
-  }
-  if (sc->ws_bytes < need) {
-    if (sc->ws) {
-      HIPCHK(hipDeviceSynchronize());
-      HIPCHK(hipFree(sc->ws));
-      sc->ws = nullptr;
-      sc->ws_bytes = 0;
-    }
-    HIPCHK(hipMalloc(&sc->ws, need));
-    sc->ws_bytes = need;
-  }
-  if (timed) HIPCHK(timing_begin(timed, s));
-  // several launches and no cooperative one: each on an auxiliary stream
-  // forked from s and joined back into it, so they run side by side
-  const bool fork = ngroups > 1 && coop.empty();
-  int nused = 0;
-  auto next_stream = [&](hipStream_t* out) -> int {
-    if (!fork) {
-      *out = s;
-      return RM_OK;
-    }
-    const int i = nused++;
-    if (!sc->aux[i]) HIPCHK(hipStreamCreateWithFlags(&sc->aux[i], hipStreamNonBlocking));
-    if (!sc->join[i]) HIPCHK(hipEventCreateWithFlags(&sc->join[i], hipEventDisableTiming));
-    HIPCHK(hipStreamWaitEvent(sc->aux[i], sc->fork, 0));
-    *out = sc->aux[i];
-    return RM_OK;
-  };
-  if (fork) {
-    if (!sc->fork) HIPCHK(hipEventCreateWithFlags(&sc->fork, hipEventDisableTiming));
-    HIPCHK(hipEventRecord(sc->fork, s));
-  }
-  int rc = RM_OK;
-  for (int c = 0; c < NCLASS; ++c) {
-    if (cls[c].empty()) continue;
-    a.items = (const int4*)(sc->dev[bi] + off[c]);
-    a.nitems = (int32_t)cls[c].size();
-    int waves = 0;
-    HIPCHK(launch_class(c, a, f32, n_cu, s, &waves, true));
-    const bool bl = class_basis_lds(c, kcap);
-    a.basis = (double2*)((char*)sc->ws + wsoff[c]);
-    a.gram = a.basis + (bl ? 0 : (size_t)waves * kcap * 64 * CLASS_R[c]);
-    a.cscr = a.gram + (fixed_m > 0 ? 0 : (size_t)waves * kcap * kcap);
-    hipStream_t cs;
-    if ((rc = next_stream(&cs))) return rc;
-    HIPCHK(launch_class(c, a, f32, n_cu, cs, &waves, false));
-    if (fork) HIPCHK(hipEventRecord(sc->join[nused - 1], cs));
-  }
-  hipStream_t bs = s;
-  if (!blk.empty() && (rc = next_stream(&bs))) return rc;
-  rc = long_run(&sc->long_ws, (const SubOp*)sc->dev[bi], (const int4*)(sc->dev[bi] + off_blk), blk,
-                (const int4*)(sc->dev[bi] + off_coop), coop, kcap, fixed_m, maxorder, tol, f32, n_cu, bs,
-                abort_preset);
+  const int rc = launch_all(sc, a, items, cls, (const int4*)(sc->dev[bi] + off_blk), blk,
+                            (const int4*)(sc->dev[bi] + off_coop), coop, kcap, fixed_m, maxorder, tol, f32,
+                            n_cu, s, timed, abort_preset);
   if (rc) return rc;
-  if (fork && !blk.empty()) HIPCHK(hipEventRecord(sc->join[nused - 1], bs));
-  for (int i = 0; i < (fork ? nused : 0); ++i) HIPCHK(hipStreamWaitEvent(s, sc->join[i], 0));
-  if (timed) HIPCHK(timing_end(timed, s));
   if (!sc->done[bi]) HIPCHK(hipEventCreateWithFlags(&sc->done[bi], hipEventDisableTiming));
   HIPCHK(hipEventRecord(sc->done[bi], s));
   return RM_OK;
@@ -1321,9 +1371,22 @@ int sub_propagate(rm_plan* p, void* psi, int64_t ld, int64_t ncols, int64_t nste
     if ((rc = grow(&p->int_status, &p->int_status_cap, ncols))) return rc;
     st = p->int_status;
   }
+  const bool pairs = krylov_fixed >= 1 && krylov_fixed <= PAIR_MAX_M;
   const bool reuse = p->dom_reuse && sc->kept && sc->kept_psi == psi && sc->kept_ld == ld &&
-                     sc->kept_ncols == ncols && sc->kept_force == force;
+                     sc->kept_ncols == ncols && sc->kept_force == force && sc->kept_pairs == pairs;
+  // step flags in chunks of <= 1024 terms, at most 16 per step
+  const int32_t parts = (int32_t)std::max<int64_t>(1, std::min<int64_t>(16, (p->nterm + 1023) / 1024));
+  const int64_t chunk = std::max<int64_t>(1, (p->nterm + parts - 1) / parts);
+  if (nsteps * parts > INT32_MAX) return RM_OK;
   if ((rc = grow(&p->sflag, &p->sflag_cap, nsteps))) return rc;
+  if (parts > 1) {
+    if ((rc = grow(&p->sflag_part, &p->sflag_part_cap, nsteps * parts))) return rc;
+    const int64_t cap0 = p->sflag_arrive_cap;
+    if ((rc = grow(&p->sflag_arrive, &p->sflag_arrive_cap, nsteps))) return rc;
+    if (p->sflag_arrive_cap != cap0)   // a new counter array starts at zero
+      hipLaunchKernelGGL(k_sub_fill, dim3(grid_for(p->sflag_arrive_cap, 256)), dim3(256), 0, s,
+                         (int32_t*)p->sflag_arrive, p->sflag_arrive_cap, 0, 0);
+  }
   if (!p->stale) {
     if ((rc = upload(&p->stale, (const int32_t*)nullptr, 1))) return rc;
     hipLaunchKernelGGL(k_sub_fill, dim3(1), dim3(64), 0, s, p->stale, (int64_t)1, 0, 0);
@@ -1354,7 +1417,11 @@ int sub_propagate(rm_plan* p, void* psi, int64_t ld, int64_t ncols, int64_t nste
   pa.dom = reuse ? p->dom_keep : nullptr;
   pa.stale = p->stale;
   pa.call_id = p->call_id;
-  pa.nb_flag = (int32_t)std::min<int64_t>(nsteps, 4096);
+  pa.parts = parts;
+  pa.chunk = chunk;
+  pa.part = p->sflag_part;
+  pa.arrive = p->sflag_arrive;
+  pa.nb_flag = (int32_t)(nsteps * parts);
   pa.nb_fill = (int32_t)std::max<int64_t>(1, std::min<int64_t>(grid_for(std::max(pa.nk, ncols), 256), 1024));
   pa.cpb = (int32_t)std::max<int64_t>(1, (int64_t)grid_for(p->n, 1024));
   const int64_t nb = (int64_t)pa.nb_flag + pa.nb_fill + (reuse ? pa.cpb * ncols : 0);
@@ -1384,10 +1451,31 @@ int sub_propagate(rm_plan* p, void* psi, int64_t ld, int64_t ncols, int64_t nste
     sort_long(sc->kept_blk);
     for (auto& v : sc->kept_cls)
       std::stable_sort(v.begin(), v.end(), [](const int4& x, const int4& y) { return x.z > y.z; });
+    group_pairs(sc->kept_cls, krylov_fixed);
+    sc->kept_pairs = pairs;
     if (p->dom_reuse) {
       // a copy of their own: rm_plan_domains rewrites p->dom
       if ((rc = grow(&p->dom_keep, &p->dom_keep_cap, 2 * ncols))) return rc;
       HIPCHK(hipMemcpyAsync(p->dom_keep, p->dom, sizeof(int32_t) * 2 * (size_t)ncols, hipMemcpyDeviceToDevice, s));
+      // the launch lists' items uploaded once (classes in order, then the
+      // block and the cooperative items); later calls launch from them
+      std::vector<int4> all;
+      for (int c = 0; c < NCLASS; ++c) all.insert(all.end(), sc->kept_cls[c].begin(), sc->kept_cls[c].end());
+      for (const LongItem& it : sc->kept_blk) all.push_back(it.item);
+      for (const LongItem& it : sc->kept_coop) all.push_back(it.item);
+      if (sc->kept_dev_cap < all.size() || !sc->kept_dev) {
+        if (sc->kept_dev) {
+          HIPCHK(hipStreamSynchronize(s));
+          HIPCHK(hipFree(sc->kept_dev));
+          sc->kept_dev = nullptr;
+        }
+        sc->kept_dev_cap = std::max<size_t>(all.size(), 4096);
+        HIPCHK(hipMalloc((void**)&sc->kept_dev, sizeof(int4) * sc->kept_dev_cap));
+      }
+      if (!all.empty()) {
+        HIPCHK(hipMemcpyAsync(sc->kept_dev, all.data(), sizeof(int4) * all.size(), hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));   // `all` is released at the end of this block
+      }
       sc->kept = true;
       sc->kept_psi = psi;
       sc->kept_ld = ld;
@@ -1396,9 +1484,6 @@ int sub_propagate(rm_plan* p, void* psi, int64_t ld, int64_t ncols, int64_t nste
     }
   }
   *done = true;
-  std::vector<int4> cls[NCLASS];
-  for (int c = 0; c < NCLASS; ++c) cls[c] = sc->kept_cls[c];   // sub_run regroups its copy
-
   std::vector<SubOp> ops(1, plan_op(p));
   SubOp& op = ops[0];
   op.phase = (const double2*)phase;
@@ -1414,8 +1499,33 @@ int sub_propagate(rm_plan* p, void* psi, int64_t ld, int64_t ncols, int64_t nste
   op.f = make_double2(f_re, f_im);
   op.nsteps = (int32_t)nsteps;
   const int32_t kcap = krylov_fixed > 0 ? krylov_fixed : maxorder;
-  rc = sub_run(sc, ops, cls, sc->kept_blk, sc->kept_coop, kcap, krylov_fixed, maxorder, tol,
-               p->precision == RM_PREC_FP32, p->n_cu, s, p, p->coop_abort);
+  const bool f32 = p->precision == RM_PREC_FP32;
+  if (sc->kept) {
+    // kept lists: nothing to stage, the job goes by value
+    SubArgs a;
+    memset(&a, 0, sizeof(a));
+    a.ops = nullptr;
+    a.op0 = op;
+    a.kcap = kcap;
+    a.fixed_m = krylov_fixed;
+    a.maxorder = maxorder;
+    a.tol = tol;
+    const int4* items[NCLASS];
+    size_t pos = 0;
+    for (int c = 0; c < NCLASS; ++c) {
+      items[c] = sc->kept_dev + pos;
+      pos += sc->kept_cls[c].size();
+    }
+    const int4* d_blk = sc->kept_dev + pos;
+    const int4* d_coop = d_blk + sc->kept_blk.size();
+    rc = launch_all(sc, a, items, sc->kept_cls, d_blk, sc->kept_blk, d_coop, sc->kept_coop, kcap, krylov_fixed,
+                    maxorder, tol, f32, p->n_cu, s, p, p->coop_abort);
+  } else {
+    std::vector<int4> cls[NCLASS];
+    for (int c = 0; c < NCLASS; ++c) cls[c] = sc->kept_cls[c];
+    rc = sub_run(sc, ops, cls, sc->kept_blk, sc->kept_coop, kcap, krylov_fixed, maxorder, tol, f32, p->n_cu, s, p,
+                 p->coop_abort);
+  }
   if (rc || status || sc->kept_coop.empty()) return rc;
   // no status array from the caller: read the internal one for timeouts
   if (p->h_status_pin_cap < ncols) {

@@ -39,7 +39,7 @@ static void plan_free(rm_plan* p) {
                   p->term_field, p->term_mval, p->term_leaf, p->mf, p->aval, p->flag, p->ws,
                   p->ell_col, p->ell_val, p->ell_pos, p->sub_perm, p->sub_label, p->sub_lo,
                   p->sub_hi, p->sub_col, p->sub_val, p->sub_rp, p->sub_ccol, p->sub_cval, p->sub_cval32, p->dom,
-                  p->sflag, p->dom_keep, p->stale, p->int_status};
+                  p->sflag, p->sflag_part, p->sflag_arrive, p->dom_keep, p->stale, p->int_status};
   sub_scratch_release(p->sub_scratch);
   if (p->h_dom_pin) (void)hipHostFree(p->h_dom_pin);
   if (p->h_status_pin) (void)hipHostFree(p->h_status_pin);
