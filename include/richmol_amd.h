@@ -49,7 +49,7 @@ typedef struct rm_plan rm_plan;
 const char* rm_last_error(void);
 
 /* Library ABI version (RM_ABI_VERSION of the header it was built from). */
-#define RM_ABI_VERSION 3
+#define RM_ABI_VERSION 4
 int rm_version(void);
 
 /*
@@ -140,6 +140,30 @@ int rm_propagate(rm_plan* plan, void* psi, void* work, int64_t ld, int64_t ncols
                  double f_re, double f_im, const void* phase, int32_t krylov_fixed,
                  int32_t maxorder, double tol, int32_t* kused, int32_t* status,
                  void* stream);
+
+/*
+ * rm_propagate with an observable read after every step: the loop of
+ * examples/ocs_alignment.py:89-103 (TDSE.update, then <psi|O|psi> of every
+ * vector and their sum) with the host out of it.  obs: a plan of the same n
+ * holding the observable's operator -- the matrix of its last
+ * rm_plan_set_field (e.g. cos^2 theta, tomat(form="full", cart="0")); not
+ * `plan` itself.  After step s (phases and expv applied, as TDSE.update
+ * returns it):
+ *   expect: device complex [nsteps][ncols] (or NULL),
+ *     expect[s*ncols + b] = <psi_b|O|psi_b>;
+ *   expect_sum: device complex [nsteps] (or NULL), the sum over the columns
+ *     (the ensemble value of ocs_alignment.py:99-100, without its +1/3).
+ * At least one of them non-NULL.  The observable is evaluated in FP64 on the
+ * FP64 wavepackets whatever the plan's precision.  A column whose status has
+ * RM_STALE_DOMAIN gets no values (the call is to be repeated); a zero column
+ * gets zeros.  obs's matrix is read on `stream`: set its field on that stream
+ * (or synchronise) before the call.
+ */
+int rm_propagate_expect(rm_plan* plan, void* psi, void* work, int64_t ld, int64_t ncols,
+                        int64_t nsteps, const double* fprod_dev, const double* thresh,
+                        double f_re, double f_im, const void* phase, int32_t krylov_fixed,
+                        int32_t maxorder, double tol, int32_t* kused, int32_t* status,
+                        rm_plan* obs, void* expect, void* expect_sum, void* stream);
 
 /* Measurement hooks (no reference counterpart).  With timing enabled, every
  * Krylov-step launch of rm_step / rm_propagate is bracketed by HIP events on

@@ -124,6 +124,7 @@ struct rm_plan {
   int32_t sub_w = 0;             // ELL width in permuted order
   int32_t* sub_perm = nullptr;   // permuted row -> original row
   int32_t* sub_label = nullptr;  // original row -> subspace
+  int32_t* sub_iperm = nullptr;  // original row -> permuted row
   int32_t* sub_lo = nullptr;     // subspace -> first permuted row
   int32_t* sub_hi = nullptr;     // subspace -> one past its last permuted row
   int32_t* sub_col = nullptr;    // [sub_w][n] permuted column of each entry
@@ -167,6 +168,19 @@ struct rm_plan {
   int32_t* h_status_pin = nullptr;
   int64_t h_status_pin_cap = 0;
   int32_t coop_abort = 0;        // test hook: cooperative launches start with the abort flag raised
+  // the matrix of the last rm_plan_set_field: an id unique over every plan
+  // and call (observable caches compare it)
+  int64_t field_gen = 0;
+  // observable of rm_propagate_expect in this plan's permuted ELL form,
+  // built from plan obs_src's matrix obs_gen
+  const rm_plan* obs_src = nullptr;
+  int64_t obs_gen = -1;
+  int32_t obs_w = 0;
+  int32_t* obs_col = nullptr;
+  double2* obs_val = nullptr;
+  int64_t obs_cap = 0, obs_vcap = 0;
+  double2* ex_buf = nullptr;     // per-column values when the caller wants only the sums
+  int64_t ex_cap = 0;
   // optional HIP-event timing of the Krylov step launches
   int timing = 0;
   int nrec = 0;
@@ -193,6 +207,14 @@ struct SubOp {
   int64_t kused_ld;
   int32_t* status;         // status[b - kcol0] or NULL: failures only
   const int32_t* stale;    // device flag: == call_id when the reused domains failed their check
+  // observable read after every step (ow == 0: none): the operator in the
+  // permuted ELL form [ow][n] (permuted columns; padding: own row, value 0);
+  // expect[step * expect_ld + b - kcol0] = <psi_b|O|psi_b> after that step
+  const int32_t* ocol;
+  const double2* oval;
+  double2* expect;
+  int64_t expect_ld;
+  int32_t ow;
   double2 f;
   int32_t n, w, nslot, nsteps, kcol0, call_id;
 };

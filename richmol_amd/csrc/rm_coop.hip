@@ -394,6 +394,27 @@ __global__ __launch_bounds__(ct_of<MODE>()) void k_long(LongArgs a) {
     // one field slot (memory mode): the matvec reads B_0 and scales the row sums
     const bool direct = !REG && op.nslot == 1;
     int built = -1;   // step whose operator the registers / Ast hold
+    // <psi|O|psi> after step `step` (the state in S): a barrier publishes S,
+    // every thread takes rows of the domain, one sum (a uniform branch: only
+    // jobs with an observable)
+    auto observe = [&](int step) __attribute__((always_inline)) {
+      if (op.ow <= 0) return;
+      sync_all<MODE>(a, ss, &s_ab);
+      double d[2] = {0.0, 0.0};
+      for (int64_t i = t0; i < len; i += nthr) {
+        double2 y = zero;
+        for (int j = 0; j < op.ow; ++j) {
+          const int64_t q = (int64_t)j * op.n + lo + i;
+          const int c = op.ocol[q] - lo;
+          if (c >= 0 && c < len) y = cfma(op.oval[q], S[c], y);
+        }
+        const double2 v = S[i];
+        d[0] += v.x * y.x + v.y * y.y;
+        d[1] += v.x * y.y - v.y * y.x;
+      }
+      coop_sum<2, CWV, MODE>(d, a, ss, s_red, s_tot, &s_ab);
+      if (t0 == 0 && op.expect) op.expect[(int64_t)step * op.expect_ld + b - op.kcol0] = make_double2(d[0], d[1]);
+    };
 
     for (int step = 0; step < op.nsteps; ++step) {
       if (!step_on(op, step)) {
@@ -401,6 +422,7 @@ __global__ __launch_bounds__(ct_of<MODE>()) void k_long(LongArgs a) {
         if (op.phase)
           for (int64_t i = r_beg; i < len; i += r_step) S[i] = cmul(cmul(S[i], P[i]), P[i]);
         if (t0 == 0 && op.kused) op.kused[(int64_t)step * op.kused_ld + b - op.kcol0] = 0;
+        observe(step);
         continue;
       }
       const double2* fp = op.table + (int64_t)step * op.nslot;
@@ -684,6 +706,7 @@ __global__ __launch_bounds__(ct_of<MODE>()) void k_long(LongArgs a) {
         if (op.kused) op.kused[(int64_t)step * op.kused_ld + b - op.kcol0] = k;
         if (!FIXED && k >= a.maxorder && op.status) op.status[b - op.kcol0] |= RM_NOT_CONVERGED;
       }
+      observe(step);
       // the next step's V_0 is written by the same threads; s_cf and the LDS
       // Gram matrix are rewritten only after the next step's first barrier
     }

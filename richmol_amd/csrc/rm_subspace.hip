@@ -43,6 +43,7 @@ struct SubArgs {
   int32_t kcap, fixed_m, maxorder;
   double tol;
   int32_t max_waves;       // > 0: at most this many waves (the launch's workspace holds no more)
+  int32_t obs;             // 1: some job reads an observable (k_sub_prop: LDS for the state)
 };
 
 // One wave per wavepacket, R rows per lane (domain <= 64 R rows).  WM > 0:
@@ -104,6 +105,32 @@ __global__ __launch_bounds__(64) void k_sub_prop(SubArgs a) {
       st[r] = ok[r] ? op.psi[(int64_t)b * op.ld + orig[r]] : make_double2(0.0, 0.0);
       ph[r] = op.phase ? op.phase[orig[r]] : make_double2(1.0, 0.0);
     }
+    // <psi|O|psi> after step `step` (the state in st): the state staged in
+    // LDS past the matvec input and the basis, the observable's entries read
+    // from memory (a uniform branch: only launches whose job has one)
+    auto observe = [&](int step) __attribute__((always_inline)) {
+      if (!(a.obs && op.ow > 0)) return;
+      double2* s_o = (double2*)(s_raw + NR * sizeof(XT) + (BL ? (size_t)kc * NR * sizeof(double2) : 0));
+#pragma unroll
+      for (int r = 0; r < R; ++r) s_o[l + 64 * r] = st[r];
+      wave_sync();
+      double e0 = 0.0, e1 = 0.0;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        double2 y = zero;
+        for (int j = 0; j < op.ow; ++j) {
+          const int64_t q = (int64_t)j * n + gq[r];
+          const int c = op.ocol[q] - lo;
+          if (ok[r] && c >= 0 && c < len) y = cfma(op.oval[q], s_o[c], y);
+        }
+        e0 += st[r].x * y.x + st[r].y * y.y;
+        e1 += st[r].x * y.y - st[r].y * y.x;
+      }
+      e0 = rsum(e0);
+      e1 = rsum(e1);
+      if (l == 0 && op.expect) op.expect[(int64_t)step * op.expect_ld + b - op.kcol0] = make_double2(e0, e1);
+      wave_sync();   // s_o is rewritten at the next step only after these reads
+    };
     int cidx[WMR][R];
     double2 Bv[NSR][WMR][R];
 #pragma unroll
@@ -126,6 +153,7 @@ __global__ __launch_bounds__(64) void k_sub_prop(SubArgs a) {
 #pragma unroll
         for (int r = 0; r < R; ++r) st[r] = op.phase ? cmul(vk[r], ph[r]) : vk[r];
         if (l == 0 && op.kused) op.kused[(int64_t)step * op.kused_ld + b - op.kcol0] = 0;
+        observe(step);
         continue;
       }
       const double2* fp = op.table + (int64_t)step * op.nslot;
@@ -373,6 +401,7 @@ __global__ __launch_bounds__(64) void k_sub_prop(SubArgs a) {
         if (op.kused) op.kused[(int64_t)step * op.kused_ld + b - op.kcol0] = k;
         if (!FIXED && k >= a.maxorder && op.status) op.status[b - op.kcol0] = RM_NOT_CONVERGED;
       }
+      observe(step);
       // the next step rewrites s_cf / s_x / the basis only after this wave's reads above
       wave_sync();
     }
@@ -417,7 +446,7 @@ __device__ __forceinline__ float2 wave_shift<float2>(float2 v, bool up) {
                      __int_as_float(wave_shift_i(__float_as_int(v.y), up)));
 }
 
-template <int WM, bool F32, bool BAND = false>
+template <int WM, bool F32, bool BAND = false, bool OBS = false>
 __global__ __launch_bounds__(64) void k_sub_pair(SubArgs a) {
   using XT = typename XType<F32>::T;
   constexpr int WMR = WM > 0 ? WM : 1;
@@ -475,6 +504,42 @@ __global__ __launch_bounds__(64) void k_sub_pair(SubArgs a) {
             if (o == oo) Bo[sl][oo] = cadd(Bo[sl][oo], Bv[sl][j]);
       }
     }
+    // OBS: the observable's entries of my row as (source lane, value); a
+    // column outside the domain (where the wavepacket is zero) or a row past
+    // it gives none.  Entries beyond OWR are read from memory at every step.
+    constexpr int OWR = 4;
+    int osrc[OBS ? OWR : 1];
+    double2 oval[OBS ? OWR : 1];
+    if constexpr (OBS) {
+#pragma unroll
+      for (int j = 0; j < OWR; ++j) {
+        const bool e = ok && j < op.ow;
+        const int c = e ? op.ocol[(int64_t)j * n + gq] - lo : -1;
+        const bool in = c >= 0 && c < len;
+        osrc[j] = in ? 32 * h + c : l;
+        oval[j] = in ? op.oval[(int64_t)j * n + gq] : zero;
+      }
+    }
+    // <psi|O|psi> of the pair's wavepackets after step `step` (the state in st)
+    auto observe = [&](int step) __attribute__((always_inline)) {
+      if constexpr (OBS) {
+        double2 y = zero;
+#pragma unroll
+        for (int j = 0; j < OWR; ++j)
+          y = cfma(oval[j], make_double2(__shfl(st.x, osrc[j]), __shfl(st.y, osrc[j])), y);
+        for (int j = OWR; j < op.ow; ++j) {
+          const int c = ok ? op.ocol[(int64_t)j * n + gq] - lo : -1;
+          const bool in = c >= 0 && c < len;
+          const double2 v = in ? op.oval[(int64_t)j * n + gq] : zero;
+          const int src = in ? 32 * h + c : l;
+          y = cfma(v, make_double2(__shfl(st.x, src), __shfl(st.y, src)), y);
+        }
+        const double e0 = half_allsum32(st.x * y.x + st.y * y.y);
+        const double e1 = half_allsum32(st.x * y.y - st.y * y.x);
+        if (i == 0 && b >= 0 && op.expect)
+          op.expect[(int64_t)step * op.expect_ld + b - op.kcol0] = make_double2(e0, e1);
+      }
+    };
     double2* sa = s_alpha[h];
     double* sb = s_beta[h];
     double2* sc = s_cf[h];
@@ -496,6 +561,7 @@ __global__ __launch_bounds__(64) void k_sub_pair(SubArgs a) {
       if (sf == 0) {
         st = op.phase ? cmul(vk, ph) : vk;
         if (i == 0 && b >= 0 && op.kused) op.kused[(int64_t)step * op.kused_ld + b - op.kcol0] = 0;
+        observe(step);
         continue;
       }
       const double2* fp = op.table + (int64_t)step * op.nslot;
@@ -603,6 +669,7 @@ __global__ __launch_bounds__(64) void k_sub_pair(SubArgs a) {
       const double2 uu = cfma(sc[k - 1], vk, cadd(u0, u1));
       st = ok ? (op.phase ? cmul(uu, ph) : uu) : zero;
       if (i == 0 && b >= 0 && op.kused) op.kused[(int64_t)step * op.kused_ld + b - op.kcol0] = k;
+      observe(step);
       wave_sync();
     }
     if (ok) op.psi[(int64_t)b * op.ld + orig] = st;
@@ -682,6 +749,8 @@ struct Prologue {
   int64_t chunk;                    // terms per chunk
   int32_t* part;                    // [nsteps][parts] partial flags (parts > 1)
   uint32_t* arrive;                 // [nsteps] arrivals of a step's chunk blocks, multiples of parts between calls
+  double2* ex;                      // observable values [nsteps][ncols] (zero columns keep the fill), or NULL
+  int64_t nex;
 };
 
 __global__ __launch_bounds__(256) void k_sub_prologue(Prologue a) {
@@ -719,6 +788,7 @@ __global__ __launch_bounds__(256) void k_sub_prologue(Prologue a) {
     const int64_t stride = (int64_t)a.nb_fill * 256;
     for (int64_t i = (bid - a.nb_flag) * 256 + tid; i < a.nk; i += stride) a.kused[i] = 0;
     for (int64_t i = (bid - a.nb_flag) * 256 + tid; i < a.ncols; i += stride) a.status[i] = 0;
+    for (int64_t i = (bid - a.nb_flag) * 256 + tid; i < a.nex; i += stride) a.ex[i] = make_double2(0.0, 0.0);
   } else {
     const int64_t j = bid - a.nb_flag - a.nb_fill;
     const int64_t b = j / a.cpb;
@@ -849,7 +919,8 @@ int sub_build(rm_plan* p) {
   for (int64_t ip = 0; ip < n; ++ip) p->h_sub_label_perm[ip] = label[perm[ip]];
   int rc = RM_OK;
   if ((rc = upload(&p->sub_perm, perm.data(), n)) || (rc = upload(&p->sub_label, label.data(), n)) ||
-      (rc = upload(&p->sub_lo, lo.data(), nc)) || (rc = upload(&p->sub_hi, hi.data(), nc)))
+      (rc = upload(&p->sub_iperm, iperm.data(), n)) || (rc = upload(&p->sub_lo, lo.data(), nc)) ||
+      (rc = upload(&p->sub_hi, hi.data(), nc)))
     return rc;
   // ELL (column-major) for the one-wave kernel: only if some subspace fits a wave
   const int32_t W = p->ell_w;
@@ -920,9 +991,9 @@ bool class_basis_lds(int c, int kcap) {
   return (c <= 2 || c >= 8) && (size_t)kcap * 64 * CLASS_R[c] * sizeof(double2) <= SUB_LDS_BASIS;
 }
 
-template <bool F32, bool BAND>
+template <bool F32, bool BAND, bool OBS>
 hipError_t launch_pair(const SubArgs& a0, int n_cu, hipStream_t s, int* waves_out, bool dry) {
-  auto kern = k_sub_pair<4, F32, BAND>;
+  auto kern = k_sub_pair<4, F32, BAND, OBS>;
   int occ = 0;
   hipError_t e = occupancy((const void*)kern, 64, 0, &occ);
   if (e != hipSuccess) return e;
@@ -940,7 +1011,8 @@ template <int R, int WM, bool FIXED, bool F32, bool BL>
 hipError_t launch_sub_k(const SubArgs& a0, int n_cu, hipStream_t s, int* waves_out, bool dry) {
   auto kern = k_sub_prop<R, WM, FIXED, F32, BL>;
   const size_t lds = (size_t)64 * R * (F32 ? sizeof(float2) : sizeof(double2)) +
-                     (BL ? (size_t)a0.kcap * 64 * R * sizeof(double2) : 0);
+                     (BL ? (size_t)a0.kcap * 64 * R * sizeof(double2) : 0) +
+                     (a0.obs ? (size_t)64 * R * sizeof(double2) : 0);
   static bool attr = false;
   if (!attr) {
     hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -981,10 +1053,18 @@ hipError_t launch_class_p(int c, const SubArgs& a, int n_cu, hipStream_t s, int*
 }
 
 hipError_t launch_class(int c, const SubArgs& a, bool f32, int n_cu, hipStream_t s, int* waves, bool dry) {
-  if (c == PAIR_CLASS)
-    return f32 ? launch_pair<true, false>(a, n_cu, s, waves, dry) : launch_pair<false, false>(a, n_cu, s, waves, dry);
-  if (c == PAIR_BAND_CLASS)
-    return f32 ? launch_pair<true, true>(a, n_cu, s, waves, dry) : launch_pair<false, true>(a, n_cu, s, waves, dry);
+  if (c == PAIR_CLASS || c == PAIR_BAND_CLASS) {
+    const bool band = c == PAIR_BAND_CLASS, obs = a.obs != 0;
+    if (f32)
+      return band ? (obs ? launch_pair<true, true, true>(a, n_cu, s, waves, dry)
+                         : launch_pair<true, true, false>(a, n_cu, s, waves, dry))
+                  : (obs ? launch_pair<true, false, true>(a, n_cu, s, waves, dry)
+                         : launch_pair<true, false, false>(a, n_cu, s, waves, dry));
+    return band ? (obs ? launch_pair<false, true, true>(a, n_cu, s, waves, dry)
+                       : launch_pair<false, true, false>(a, n_cu, s, waves, dry))
+                : (obs ? launch_pair<false, false, true>(a, n_cu, s, waves, dry)
+                       : launch_pair<false, false, false>(a, n_cu, s, waves, dry));
+  }
   const bool fixed = a.fixed_m > 0;
   if (fixed && f32) return launch_class_p<true, true>(c, a, n_cu, s, waves, dry);
   if (fixed) return launch_class_p<true, false>(c, a, n_cu, s, waves, dry);
@@ -1218,6 +1298,7 @@ int sub_run(SubScratch* sc, const std::vector<SubOp>& ops, std::vector<int4> (&c
   a.fixed_m = fixed_m;
   a.maxorder = maxorder;
   a.tol = tol;
+  for (const SubOp& op : ops) a.obs |= op.ow > 0 ? 1 : 0;
   const int rc = launch_all(sc, a, items, cls, (const int4*)(sc->dev[bi] + off_blk), blk,
                             (const int4*)(sc->dev[bi] + off_coop), coop, kcap, fixed_m, maxorder, tol, f32,
                             n_cu, s, timed, abort_preset);
@@ -1353,14 +1434,51 @@ void sort_long(std::vector<LongItem>& v) {
 // the block kernel.  force: RM_KERNEL_COOP / RM_KERNEL_BLOCK send every
 // wavepacket to that kernel (0: the rule above).  *done = false when the
 // path does not apply; nothing has then been written to psi.
+// The observable plan's matrix in p's permuted ELL form (p->obs_col,
+// p->obs_val, width p->obs_w), rebuilt when the plan or its matrix changed.
+__global__ void k_obs_ell(int64_t n, int32_t ow, const int32_t* __restrict__ perm,
+                          const int32_t* __restrict__ iperm, const int64_t* __restrict__ rp,
+                          const int32_t* __restrict__ col, const double2* __restrict__ val,
+                          int32_t* __restrict__ ocol, double2* __restrict__ oval) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int32_t r = perm[q];
+  const int64_t e0 = rp[r], e1 = rp[r + 1];
+  for (int32_t j = 0; j < ow; ++j) {
+    const int64_t e = e0 + j;
+    ocol[(int64_t)j * n + q] = e < e1 ? iperm[col[e]] : (int32_t)q;
+    oval[(int64_t)j * n + q] = e < e1 ? val[e] : make_double2(0.0, 0.0);
+  }
+}
+
+int obs_prepare(rm_plan* p, const rm_plan* obs, hipStream_t s) {
+  if (p->obs_src == obs && p->obs_gen == obs->field_gen && p->obs_col) return RM_OK;
+  int64_t w = 1;
+  for (int64_t r = 0; r < obs->n; ++r) w = std::max<int64_t>(w, obs->h_row_ptr[r + 1] - obs->h_row_ptr[r]);
+  if (w * p->n > INT32_MAX) return fail(RM_ERR_ARG, "rm_propagate_expect: observable too wide (%lld entries in a row)",
+                                        (long long)w);
+  int rc;
+  if ((rc = grow(&p->obs_col, &p->obs_cap, w * p->n)) || (rc = grow(&p->obs_val, &p->obs_vcap, w * p->n)))
+    return rc;
+  hipLaunchKernelGGL(k_obs_ell, dim3(grid_for(p->n, 256)), dim3(256), 0, s, p->n, (int32_t)w, p->sub_perm,
+                     p->sub_iperm, obs->row_ptr, obs->col, obs->aval, p->obs_col, p->obs_val);
+  HIPCHK(hipGetLastError());
+  p->obs_src = obs;
+  p->obs_gen = obs->field_gen;
+  p->obs_w = (int32_t)w;
+  return RM_OK;
+}
+
 int sub_propagate(rm_plan* p, void* psi, int64_t ld, int64_t ncols, int64_t nsteps,
                   const double* fprod_dev, double f_re, double f_im, const void* phase,
                   int32_t krylov_fixed, int32_t maxorder, double tol, int32_t* kused,
-                  int32_t* status, int force, hipStream_t s, bool* done) {
+                  int32_t* status, int force, hipStream_t s, bool* done, const rm_plan* obs,
+                  double2* expect) {
   *done = false;
   if (nsteps > INT32_MAX) return RM_OK;
   int rc = sub_build(p);
   if (rc) return rc;
+  if (obs && (rc = obs_prepare(p, obs, s))) return rc;
   if (!p->sub_scratch) p->sub_scratch = new (std::nothrow) SubScratch();
   if (!p->sub_scratch) return fail(RM_ERR_ALLOC, "rm_propagate: host allocation");
   SubScratch* sc = (SubScratch*)p->sub_scratch;
@@ -1422,7 +1540,10 @@ int sub_propagate(rm_plan* p, void* psi, int64_t ld, int64_t ncols, int64_t nste
   pa.part = p->sflag_part;
   pa.arrive = p->sflag_arrive;
   pa.nb_flag = (int32_t)(nsteps * parts);
-  pa.nb_fill = (int32_t)std::max<int64_t>(1, std::min<int64_t>(grid_for(std::max(pa.nk, ncols), 256), 1024));
+  pa.ex = obs ? expect : nullptr;
+  pa.nex = obs ? nsteps * ncols : 0;
+  pa.nb_fill = (int32_t)std::max<int64_t>(
+      1, std::min<int64_t>(grid_for(std::max(std::max(pa.nk, pa.nex), ncols), 256), 1024));
   pa.cpb = (int32_t)std::max<int64_t>(1, (int64_t)grid_for(p->n, 1024));
   const int64_t nb = (int64_t)pa.nb_flag + pa.nb_fill + (reuse ? pa.cpb * ncols : 0);
   hipLaunchKernelGGL(k_sub_prologue, dim3((unsigned)nb), dim3(256), 0, s, pa);
@@ -1498,6 +1619,13 @@ int sub_propagate(rm_plan* p, void* psi, int64_t ld, int64_t ncols, int64_t nste
   op.call_id = p->call_id;
   op.f = make_double2(f_re, f_im);
   op.nsteps = (int32_t)nsteps;
+  if (obs) {
+    op.ocol = p->obs_col;
+    op.oval = p->obs_val;
+    op.ow = p->obs_w;
+    op.expect = expect;
+    op.expect_ld = ncols;
+  }
   const int32_t kcap = krylov_fixed > 0 ? krylov_fixed : maxorder;
   const bool f32 = p->precision == RM_PREC_FP32;
   if (sc->kept) {
@@ -1506,6 +1634,7 @@ int sub_propagate(rm_plan* p, void* psi, int64_t ld, int64_t ncols, int64_t nste
     memset(&a, 0, sizeof(a));
     a.ops = nullptr;
     a.op0 = op;
+    a.obs = op.ow > 0 ? 1 : 0;
     a.kcap = kcap;
     a.fixed_m = krylov_fixed;
     a.maxorder = maxorder;

@@ -13,6 +13,8 @@
 
 #include <string>
 
+#include <atomic>
+
 #include "rm_common.h"
 
 // ---------------------------------------------------------------------------
@@ -39,7 +41,8 @@ static void plan_free(rm_plan* p) {
                   p->term_field, p->term_mval, p->term_leaf, p->mf, p->aval, p->flag, p->ws,
                   p->ell_col, p->ell_val, p->ell_pos, p->sub_perm, p->sub_label, p->sub_lo,
                   p->sub_hi, p->sub_col, p->sub_val, p->sub_rp, p->sub_ccol, p->sub_cval, p->sub_cval32, p->dom,
-                  p->sflag, p->sflag_part, p->sflag_arrive, p->dom_keep, p->stale, p->int_status};
+                  p->sflag, p->sflag_part, p->sflag_arrive, p->dom_keep, p->stale, p->int_status,
+                  p->sub_iperm, p->obs_col, p->obs_val, p->ex_buf};
   sub_scratch_release(p->sub_scratch);
   if (p->h_dom_pin) (void)hipHostFree(p->h_dom_pin);
   if (p->h_status_pin) (void)hipHostFree(p->h_status_pin);
@@ -289,6 +292,8 @@ extern "C" int rm_plan_set_field(rm_plan* p, const double* fprod, const double* 
     for (int i = 0; i < p->nleaf; ++i) fa.thresh[i] = thresh[i];
   hipStream_t s = (hipStream_t)stream;
   p->step_id = (p->step_id + 1) & 0x3fffffff;
+  static std::atomic<int64_t> gen{0};
+  p->field_gen = ++gen;
   launch_field(p, fa, nullptr, s);
   HIPCHK(hipGetLastError());
   return RM_OK;
@@ -1516,13 +1521,69 @@ __global__ void k_copy_cols(int64_t n, int64_t ld, const double2* __restrict__ s
 int sub_propagate(rm_plan* p, void* psi, int64_t ld, int64_t ncols, int64_t nsteps,
                   const double* fprod_dev, double f_re, double f_im, const void* phase,
                   int32_t krylov_fixed, int32_t maxorder, double tol, int32_t* kused,
-                  int32_t* status, int force, hipStream_t s, bool* done);
+                  int32_t* status, int force, hipStream_t s, bool* done, const rm_plan* obs,
+                  double2* expect);
+
+__global__ __launch_bounds__(NT) void k_expect(int64_t n, int64_t ld, const int64_t* __restrict__ rp,
+                                               const int32_t* __restrict__ col,
+                                               const double2* __restrict__ a,
+                                               const double2* __restrict__ psi,
+                                               double2* __restrict__ out);
+
+// sum[s] = sum over b of ex[s * ncols + b], one wave per step (fixed order)
+__global__ __launch_bounds__(64) void k_expect_sum(int64_t nsteps, int64_t ncols, const double2* __restrict__ ex,
+                                                   double2* __restrict__ sum) {
+  for (int64_t st = blockIdx.x; st < nsteps; st += gridDim.x) {
+    double re = 0.0, im = 0.0;
+    for (int64_t b = threadIdx.x; b < ncols; b += 64) {
+      const double2 v = ex[st * ncols + b];
+      re += v.x;
+      im += v.y;
+    }
+    re = wave_allsum(re);
+    im = wave_allsum(im);
+    if (threadIdx.x == 0) sum[st] = make_double2(re, im);
+  }
+}
+
+static int propagate_impl(rm_plan* p, void* psi, void* work, int64_t ld, int64_t ncols, int64_t nsteps,
+                          const double* fprod_dev, const double* thresh, double f_re, double f_im,
+                          const void* phase, int32_t krylov_fixed, int32_t maxorder, double tol,
+                          int32_t* kused, int32_t* status, rm_plan* obs, double2* expect,
+                          double2* expect_sum, hipStream_t s);
 
 extern "C" int rm_propagate(rm_plan* p, void* psi, void* work, int64_t ld, int64_t ncols,
                             int64_t nsteps, const double* fprod_dev, const double* thresh,
                             double f_re, double f_im, const void* phase, int32_t krylov_fixed,
                             int32_t maxorder, double tol, int32_t* kused, int32_t* status,
                             void* stream) {
+  return propagate_impl(p, psi, work, ld, ncols, nsteps, fprod_dev, thresh, f_re, f_im, phase, krylov_fixed,
+                        maxorder, tol, kused, status, nullptr, nullptr, nullptr, (hipStream_t)stream);
+}
+
+extern "C" int rm_propagate_expect(rm_plan* p, void* psi, void* work, int64_t ld, int64_t ncols,
+                                   int64_t nsteps, const double* fprod_dev, const double* thresh,
+                                   double f_re, double f_im, const void* phase, int32_t krylov_fixed,
+                                   int32_t maxorder, double tol, int32_t* kused, int32_t* status,
+                                   rm_plan* obs, void* expect, void* expect_sum, void* stream) {
+  if (!p || !obs) return fail(RM_ERR_ARG, "rm_propagate_expect: NULL plan or observable");
+  if (obs == p) return fail(RM_ERR_ARG, "rm_propagate_expect: the observable must be a plan of its own");
+  if (obs->n != p->n)
+    return fail(RM_ERR_ARG, "rm_propagate_expect: observable of %lld states for a plan of %lld",
+                (long long)obs->n, (long long)p->n);
+  if (obs->field_gen == 0)
+    return fail(RM_ERR_ARG, "rm_propagate_expect: the observable plan has no matrix yet (rm_plan_set_field)");
+  if (!expect && !expect_sum) return fail(RM_ERR_ARG, "rm_propagate_expect: NULL expect and expect_sum");
+  return propagate_impl(p, psi, work, ld, ncols, nsteps, fprod_dev, thresh, f_re, f_im, phase, krylov_fixed,
+                        maxorder, tol, kused, status, obs, (double2*)expect, (double2*)expect_sum,
+                        (hipStream_t)stream);
+}
+
+static int propagate_impl(rm_plan* p, void* psi, void* work, int64_t ld, int64_t ncols, int64_t nsteps,
+                          const double* fprod_dev, const double* thresh, double f_re, double f_im,
+                          const void* phase, int32_t krylov_fixed, int32_t maxorder, double tol,
+                          int32_t* kused, int32_t* status, rm_plan* obs, double2* expect,
+                          double2* expect_sum, hipStream_t s) {
   StepArgs a;
   // the full-vector workspace is allocated below, only when those kernels run
   int rc = step_setup(p, "rm_propagate", ld, ncols, f_re, f_im, phase, krylov_fixed, maxorder, tol, &a, false);
@@ -1530,10 +1591,24 @@ extern "C" int rm_propagate(rm_plan* p, void* psi, void* work, int64_t ld, int64
   if (nsteps < 0) return fail(RM_ERR_ARG, "rm_propagate: nsteps < 0");
   if (ncols > 65535) return fail(RM_ERR_ARG, "rm_propagate: ncols > 65535");
   if (ncols == 0 || p->n == 0 || nsteps == 0) return RM_OK;
+  // per-column observable values: the caller's array, or the plan's own when
+  // the caller wants only the sums
+  double2* ex = expect;
+  if (obs && !ex) {
+    if ((rc = grow(&p->ex_buf, &p->ex_cap, nsteps * ncols))) return rc;
+    ex = p->ex_buf;
+  }
+  auto finish = [&]() -> int {
+    if (obs && expect_sum) {
+      hipLaunchKernelGGL(k_expect_sum, dim3((unsigned)std::min<int64_t>(nsteps, 65535)), dim3(64), 0, s, nsteps,
+                         ncols, (const double2*)ex, expect_sum);
+      HIPCHK(hipGetLastError());
+    }
+    return RM_OK;
+  };
   if (!psi || !work) return fail(RM_ERR_ARG, "rm_propagate: NULL psi or work");
   if (psi == work) return fail(RM_ERR_ARG, "rm_propagate: psi and work must differ");
   if (p->nfield > 0 && !fprod_dev) return fail(RM_ERR_ARG, "rm_propagate: NULL field table");
-  hipStream_t s = (hipStream_t)stream;
   FieldArgs fa;
   memset(&fa, 0, sizeof(fa));
   bool any_thresh = false;
@@ -1549,9 +1624,9 @@ extern "C" int rm_propagate(rm_plan* p, void* psi, void* work, int64_t ld, int64
     bool done = false;
     const int force = p->variant == RM_KERNEL_COOP || p->variant == RM_KERNEL_BLOCK ? p->variant : 0;
     rc = sub_propagate(p, psi, ld, ncols, nsteps, fprod_dev, f_re, f_im, phase, krylov_fixed,
-                       maxorder, tol, kused, status, force, s, &done);
+                       maxorder, tol, kused, status, force, s, &done, obs, ex);
     if (rc) return rc;
-    if (done) return RM_OK;
+    if (done) return finish();
   }
   if (sub_kernel)
     return fail(RM_ERR_ARG, "rm_propagate: the subspace kernels do not apply (%s)",
@@ -1573,6 +1648,9 @@ extern "C" int rm_propagate(rm_plan* p, void* psi, void* work, int64_t ld, int64
     a.psi_out = nxt;
     a.kused = kused ? kused + st * ncols : nullptr;
     HIPCHK(timed_launch_step(p, a, (unsigned)ncols, s));
+    if (obs)
+      hipLaunchKernelGGL(k_expect, dim3((unsigned)ncols), dim3(NT), 0, s, obs->n, ld, obs->row_ptr, obs->col,
+                         obs->aval, (const double2*)nxt, ex + st * ncols);
     double2* t = cur;
     cur = nxt;
     nxt = t;
@@ -1581,7 +1659,7 @@ extern "C" int rm_propagate(rm_plan* p, void* psi, void* work, int64_t ld, int64
     hipLaunchKernelGGL(k_copy_cols, dim3(grid_for(p->n, 256), (unsigned)ncols), dim3(256), 0, s,
                        p->n, ld, (const double2*)cur, (double2*)psi);
   HIPCHK(hipGetLastError());
-  return RM_OK;
+  return finish();
 }
 
 // ---------------------------------------------------------------------------

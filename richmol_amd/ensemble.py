@@ -129,9 +129,17 @@ class Ensemble:
         if self.plan.precision != self.precision:
             self.plan.set_precision(self.precision)
 
-    def propagate(self, fields, thresh=None, table=None, kused=None, check=True):
+    def propagate(self, fields, thresh=None, table=None, kused=None, check=True,
+                  obs=None, expect=None, expect_sum=None):
         """Advance every row through len(fields) steps with one device call
         (rm_propagate).  ``table`` may hold a prepared field table.
+
+        obs: an observable's device plan (``observable_plan``); after every
+        step <psi_b|O|psi_b> goes to ``expect`` (device (nsteps, B)) and their
+        sum over the rows to ``expect_sum`` (device (nsteps,)), both complex128
+        -- the observable loop of the reference's examples inside the launch
+        (rm_propagate_expect).  Pass True for either to have it allocated;
+        the two are returned as (expect, expect_sum).
 
         The plan keeps the rows' domains from one call to the next (the
         rows never leave them); a call on rows rewritten in between is
@@ -139,24 +147,34 @@ class Ensemble:
         default) reads the status back: such a call is then repeated with
         fresh domains, and a cooperative timeout or (adaptive rule) maxorder
         raises.  check=False returns without waiting (call check() later)."""
+        import torch
         if table is None:
             table = self.prepare(fields, thresh)
         th = None if thresh is None else [thresh]
         self._apply_kernel()
         if self.plan.domain_owner != id(self):
             self.plan.set_domain_reuse(True, owner=id(self))
+        if obs is not None:
+            nsteps, B, dev = table.shape[0], self.psi.shape[0], self.psi.device
+            if expect is True or (expect is None and expect_sum is None):
+                expect = torch.empty((nsteps, B), dtype=torch.complex128, device=dev)
+            if expect_sum is True:
+                expect_sum = torch.empty(nsteps, dtype=torch.complex128, device=dev)
         for attempt in (0, 1):
             self.plan.propagate(self.psi, self.out, table, self.f, phase=self.phase, thresh=th,
                                 krylov_fixed=self.krylov_fixed, maxorder=self.maxorder, tol=self.tol,
-                                kused=kused, status=self.status, stream=self.stream)
+                                kused=kused, status=self.status, stream=self.stream,
+                                obs=obs, expect=expect, expect_sum=expect_sum)
             if not check:
-                return
+                return (expect, expect_sum) if obs is not None else None
             from . import _lib
             if attempt == 0 and np.any(self.status.cpu().numpy() & _lib.RM_STALE_DOMAIN):
                 self.plan.set_domain_reuse(True, owner=id(self))   # forget the kept domains
                 continue
             break
         self.check()
+        if obs is not None:
+            return expect, expect_sum
 
     def load(self, psi0):
         """Replace the rows (same shape) by psi0 (numpy or device array)."""
@@ -174,6 +192,17 @@ class Ensemble:
                        maxorder=self.maxorder, tol=self.tol, kused=self.kused, status=self.status,
                        stream=self.stream)
         self.psi, self.out = self.out, self.psi
+
+    @staticmethod
+    def observable_plan(O, F=(0.0, 0.0, 1.0)):
+        """Device plan of an observable CarTens O with its matrix formed for
+        field F (default: the "0"/Z component as tomat(cart="0") takes it,
+        e.g. cos^2 theta); for Ensemble.propagate(obs=...) and expect()."""
+        from . import plan as _plan
+        O.field(list(F))
+        p = _plan.plan_for(O)
+        p.set_field(O)
+        return p
 
     def expect(self, obs_plan, out=None):
         """<psi_b|O|psi_b> for every row (complex, device)."""

@@ -385,10 +385,17 @@ class DevicePlan:
         return tab
 
     def propagate(self, psi, work, table, f, phase=None, thresh=None, krylov_fixed=0,
-                  maxorder=100, tol=1e-15, kused=None, status=None, stream=None):
+                  maxorder=100, tol=1e-15, kused=None, status=None, stream=None,
+                  obs=None, expect=None, expect_sum=None):
         """len(table) steps of every row of psi (in place); table: device
-        (nsteps, nfield) complex128 tensor of field products."""
+        (nsteps, nfield) complex128 tensor of field products.
+
+        obs: a DevicePlan of an observable (its field set), read after every
+        step into expect (device (nsteps, ncols) complex128) and/or
+        expect_sum (device (nsteps,) complex128, the sum over the rows) --
+        rm_propagate_expect."""
         import ctypes
+        import torch
         _check_cols(psi, self.n); _check_cols(work, self.n)
         if psi.stride(0) != work.stride(0):
             raise ValueError("psi and work need the same row stride")
@@ -396,16 +403,31 @@ class DevicePlan:
         if thresh is not None:
             th[: len(thresh)] = [t if (t is not None and t > 0) else -1.0 for t in thresh]
         vp = ctypes.c_void_p
-        rc = _lib.check(self.lib.rm_propagate(
-            self.handle, vp(psi.data_ptr()), vp(work.data_ptr()), psi.stride(0), psi.shape[0],
-            table.shape[0], vp(table.data_ptr()), _lib.ptr(th, ctypes.c_double),
-            float(f.real), float(f.imag), vp(phase.data_ptr() if phase is not None else None),
-            int(krylov_fixed), int(maxorder), float(tol),
-            vp(kused.data_ptr() if kused is not None else None),
-            vp(status.data_ptr() if status is not None else None),
-            vp(_stream_handle(stream))), "rm_propagate")
+        args = (self.handle, vp(psi.data_ptr()), vp(work.data_ptr()), psi.stride(0), psi.shape[0],
+                table.shape[0], vp(table.data_ptr()), _lib.ptr(th, ctypes.c_double),
+                float(f.real), float(f.imag), vp(phase.data_ptr() if phase is not None else None),
+                int(krylov_fixed), int(maxorder), float(tol),
+                vp(kused.data_ptr() if kused is not None else None),
+                vp(status.data_ptr() if status is not None else None))
+        if obs is None:
+            if expect is not None or expect_sum is not None:
+                raise ValueError("expect / expect_sum need an observable plan")
+            name = "rm_propagate"
+            rc = _lib.check(self.lib.rm_propagate(*args, vp(_stream_handle(stream))), name)
+        else:
+            nsteps, ncols = table.shape[0], psi.shape[0]
+            for t, shape in ((expect, (nsteps, ncols)), (expect_sum, (nsteps,))):
+                if t is None:
+                    continue
+                if t.dtype != torch.complex128 or not t.is_cuda or tuple(t.shape) != shape or not t.is_contiguous():
+                    raise ValueError(f"expected a contiguous device complex128 tensor of shape {shape}")
+            name = "rm_propagate_expect"
+            rc = _lib.check(self.lib.rm_propagate_expect(
+                *args, obs.handle, vp(expect.data_ptr() if expect is not None else None),
+                vp(expect_sum.data_ptr() if expect_sum is not None else None),
+                vp(_stream_handle(stream))), name)
         if rc == _lib.RM_DEVICE_TIMEOUT:
-            raise RuntimeError("rm_propagate: " + self.lib.rm_last_error().decode(errors="replace"))
+            raise RuntimeError(name + ": " + self.lib.rm_last_error().decode(errors="replace"))
 
     def expect(self, psi, out, stream=None):
         import ctypes

@@ -281,6 +281,61 @@ class TDSE:
             return out
         return out.cpu().numpy()
 
+    def run(self, H, vecs, field, H0=None, obs=None, thresh=None, nsteps=None, **kwargs):
+        """The reference's time loop (examples/ocs_alignment.py:89-103) as one
+        device call::
+
+            for i, t in enumerate(tdse.time_grid()):
+                H.field(field(t), thresh=thresh)
+                vecs, t_ = tdse.update(H, H0=H0, vecs=vecs)
+                expval[i] = sum(vdot(vecs[b], O @ vecs[b]) for b in rows)
+
+        field: a callable t -> [Fx, Fy, Fz] taken at the step centres of the
+        time grid (from the step update() would take next), or an
+        (nsteps, 3) array.  obs: an observable CarTens O (its matrix for the
+        field [0, 0, 1], as tomat(form="full", cart="0") takes it, e.g.
+        cos^2 theta) or None.  nsteps: default, the rest of the grid.
+        Kwargs as update(): tol, maxorder, krylov_fixed, stream, precision.
+        Returns (vecs, t, expval): t the end of the last step (update()'s
+        call counter advances by nsteps), expval the complex (nsteps,) sums
+        (None without obs).  Runs every step in one rm_propagate_expect call
+        on the flattened operator."""
+        import torch
+        from .ensemble import Ensemble
+        if "_time_grid" not in self.__dict__:
+            self.time_grid()
+        icall = self._ncalls.get("update", 0)
+        t2, tc = self._time_grid[1], self._time_grid[2]
+        if nsteps is None:
+            nsteps = len(tc) - icall
+        if nsteps < 1 or icall + nsteps > len(tc):
+            raise ValueError(f"{nsteps} steps from step {icall} run past the time grid ({len(tc)} steps)")
+        if callable(field):
+            fields = np.array([np.asarray(field(t), dtype=np.float64) for t in tc[icall:icall + nsteps]])
+        else:
+            fields = np.asarray(field, dtype=np.float64)
+            if fields.shape != (nsteps, 3):
+                raise ValueError(f"field array of shape {fields.shape}, expected ({nsteps}, 3)")
+        if H0 is not None:
+            h0 = H0.tomat(form="full", cart="0")
+            assert (h0 - diags(h0.diagonal())).nnz == 0, \
+                "field-free Hamiltonian `H0` is not diagonal -> split-operator approach not implemented"
+        on_device = isinstance(vecs, torch.Tensor)
+        ens = Ensemble(H, H0, self.exp_fac, vecs if on_device else np.asarray(vecs),
+                       krylov_fixed=int(kwargs.get("krylov_fixed", 0)),
+                       maxorder=int(kwargs.get("maxorder", 100)), tol=float(kwargs.get("tol", 1e-15)),
+                       stream=kwargs.get("stream", None))
+        ens.set_precision(kwargs.get("precision", "fp64"))
+        expval = None
+        if obs is None:
+            ens.propagate(fields, thresh=thresh)
+        else:
+            _, s = ens.propagate(fields, thresh=thresh, obs=Ensemble.observable_plan(obs), expect_sum=True)
+            expval = s.cpu().numpy()
+        self._ncalls["update"] = icall + nsteps
+        out = ens.psi if on_device else ens.psi.cpu().numpy()
+        return out, t2[icall + nsteps - 1], expval
+
 
 # 'auto' operator form: bases at least this large with every J carrying all m
 # and the full k space keep the operator factored (a flattened asymmetric top
