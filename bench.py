@@ -89,37 +89,42 @@ def usable_cores():
 
 
 def _oracle_run(args):
-    """One process's share of the CPU baseline: the oracle's TDSE.update
-    (row loop, block matvec, Lanczos) on its rows, one thread."""
-    h0, H, rows, fields, krylov = args
+    """One process's share of the CPU baseline: the reference's time loop
+    (examples/ocs_alignment.py:89-103) on the oracle -- TDSE.update (row
+    loop, block matvec, Lanczos), then <cos^2 theta> of every row -- on its
+    rows, one thread."""
+    h0, H, C, rows, fields, krylov = args
     from threadpoolctl import threadpool_limits
-    from oracle.cartens import OracleCarTens
+    from oracle.cartens import OracleCarTens, flatten, unflatten
     from oracle.tdse import OracleTDSE
-    Oh0, OH = OracleCarTens.from_attrs(h0), OracleCarTens.from_attrs(H)
+    Oh0, OH, OC = (OracleCarTens.from_attrs(T) for T in (h0, H, C))
+    OC.field([0.0, 0.0, 1.0])
     td = OracleTDSE(t_end=DT_FS * (len(fields) + 1), dt=DT_FS, t_units="fs")
     td.time_grid()
     X = rows.copy()
+    c2 = np.zeros(len(fields), dtype=np.complex128)
     with threadpool_limits(limits=1):
         t0 = time.perf_counter()
-        for F in fields:
+        for i, F in enumerate(fields):
             OH.field(F)
             X, _ = td.update(OH, X, H0=Oh0, fixed_m=krylov)
+            c2[i] = sum(np.vdot(x, flatten(OC, OC.vec(unflatten(OC, x)))) for x in X)
         el = time.perf_counter() - t0
-    return X, el
+    return X, el, c2
 
 
-def cpu_baselines(h0, H, psi0, fields, krylov, ncols, ncols_all):
+def cpu_baselines(h0, H, C, psi0, fields, krylov, ncols, ncols_all):
     """The oracle (a port of richmol's CPU path) on the host, before the GPU
     is touched: ncols rows on one core, then ncols_all rows split over every
     usable core (one process per core, as richmol's MPI notebook splits rows)."""
-    X1, el1 = _oracle_run((h0, H, psi0[:ncols], fields, krylov))
-    one = {"value": ncols * len(fields) / el1, "seconds": el1, "rows": ncols, "X": X1}
+    X1, el1, c2 = _oracle_run((h0, H, C, psi0[:ncols], fields, krylov))
+    one = {"value": ncols * len(fields) / el1, "seconds": el1, "rows": ncols, "X": X1, "cos2": c2}
     cores = usable_cores()
     allc = None
     if cores > 1 and ncols_all > 0:
         import multiprocessing as mp
         parts = np.array_split(np.arange(ncols_all), cores)
-        jobs = [(h0, H, psi0[p], fields, krylov) for p in parts if len(p)]
+        jobs = [(h0, H, C, psi0[p], fields, krylov) for p in parts if len(p)]
         ctx = mp.get_context("fork")
         t0 = time.perf_counter()
         with ctx.Pool(len(jobs)) as pool:
@@ -130,29 +135,39 @@ def cpu_baselines(h0, H, psi0, fields, krylov, ncols, ncols_all):
     return one, allc
 
 
-def timed_run(ens, warm, timed, precision="fp64"):
+def timed_run(ens, warm, timed, precision="fp64", obs=None):
     """Warm-up run, then the timed run (no host read inside it):
-    (wall s, summed kernel ms, launches)."""
+    (wall s, summed kernel ms, launches, per-step ensemble sums of the
+    observable or None).  obs: the observable read after every step inside
+    the launch (rm_propagate_expect), as the reference's loop reads it."""
     import torch
-    # both field tables first, so that the warm-up call directly precedes the
-    # timed one; timing on from the warm-up call, so the timed call creates
-    # no events
+    # both field tables and the output first, so that the warm-up call
+    # directly precedes the timed one; timing on from the warm-up call, so
+    # the timed call creates no events
     tw = ens.prepare(warm)
     tt = ens.prepare(timed)
+    sums = None
+    if obs is not None:
+        # the caller's output arrays (per wavepacket and summed), allocated
+        # before the timed call like the wavepackets
+        dev, B = ens.psi.device, ens.psi.shape[0]
+        sums = [torch.empty(t.shape[0], dtype=torch.complex128, device=dev) for t in (tw, tt)]
+        exs = [torch.empty((t.shape[0], B), dtype=torch.complex128, device=dev) for t in (tw, tt)]
+    kw = lambda i: {} if obs is None else {"obs": obs, "expect": exs[i], "expect_sum": sums[i]}
     ens.set_precision(precision)
     ens.plan.timing(True)
-    ens.propagate(None, table=tw, check=False)
+    ens.propagate(None, table=tw, check=False, **kw(0))
     ens.plan.kernel_time()
     ens.check()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    ens.propagate(None, table=tt, check=False)
+    ens.propagate(None, table=tt, check=False, **kw(1))
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms, nl = ens.plan.kernel_time()
     ens.plan.timing(False)
     ens.check()
-    return wall, ms, nl
+    return wall, ms, nl, (sums[1].cpu().numpy() if sums else None)
 
 
 def cos2_of(ens, cplan):
@@ -161,8 +176,8 @@ def cos2_of(ens, cplan):
 
 def accuracy_window(H, h0, f, psi0, m, cplan, precisions=("fp64", "fp32")):
     """Every step of the window 0..ACCURACY_STEPS-1 (across the pulse peak)
-    for each precision, <cos^2 theta> of every wavepacket after each step:
-    {precision: (nsteps, ncols) array, final rows}."""
+    for each precision in one launch, <cos^2 theta> of every wavepacket read
+    inside it after each step: {precision: (nsteps, ncols) array, final rows}."""
     from richmol_amd import ensemble
     out = {}
     t = (np.arange(ACCURACY_STEPS) + 0.5) * DT_FS
@@ -171,12 +186,8 @@ def accuracy_window(H, h0, f, psi0, m, cplan, precisions=("fp64", "fp32")):
         e = ensemble.Ensemble(H, h0, f, psi0, krylov_fixed=m)
         e.set_kernel("subspace")
         e.set_precision(prec)
-        tab = e.prepare(fields)
-        c2 = np.zeros((ACCURACY_STEPS, psi0.shape[0]))
-        for i in range(ACCURACY_STEPS):
-            e.propagate(None, table=tab[i:i + 1])
-            c2[i] = cos2_of(e, cplan)
-        out[prec] = (c2, e.psi.cpu().numpy())
+        ex, _ = e.propagate(fields, obs=cplan)
+        out[prec] = (ex.real.cpu().numpy() + 1.0 / 3.0, e.psi.cpu().numpy())
     return out
 
 
@@ -430,7 +441,7 @@ def main():
     cpu_one = cpu_all = None
     if not args.no_cpu:
         # before the GPU is initialised: the all-core leg forks worker processes
-        cpu_one, cpu_all = cpu_baselines(h0, H, psi0, peak, m, args.cpu_cols, args.cpu_cols_all)
+        cpu_one, cpu_all = cpu_baselines(h0, H, cos2, psi0, peak, m, args.cpu_cols, args.cpu_cols_all)
         print(json.dumps({"progress": "cpu baseline", "one_core": cpu_one["value"],
                           "all_cores": cpu_all and cpu_all["value"]}), file=sys.stderr, flush=True)
 
@@ -448,7 +459,7 @@ def main():
     # ---- headline: production path
     ens = ensemble.Ensemble(H, h0, f, psi0, krylov_fixed=m)
     ens.set_kernel(args.kernel)
-    wall, kms, nl = timed_run(ens, warm, timed)
+    wall, kms, nl, c2_sums = timed_run(ens, warm, timed, obs=cplan)
     value = args.ncols * args.steps / wall
     p = ens.plan
     used = "subspace" if (nl == 1 and args.kernel in ("auto", "subspace")) else p.kernel
@@ -504,6 +515,10 @@ def main():
                      "note": "serial Krylov recurrence of each wavepacket in registers and LDS; "
                              "no MFMA instruction, HBM traffic ~1e-3 of the 8 TB/s roofline"},
         "cos2_ensemble": float(c2_fp64.mean()),
+        "cos2_ensemble_timed_steps": [float(c2_sums.real.min() / args.ncols + 1.0 / 3.0),
+                                      float(c2_sums.real.max() / args.ncols + 1.0 / 3.0)],
+        "observable": "<cos^2 theta> of every wavepacket and their sum read after every step "
+                      "inside the timed launch (rm_propagate_expect), as the reference's loop does",
     }
     print(json.dumps({"progress": "headline", "value": value, "kernel_used": used}), file=sys.stderr,
           flush=True)
@@ -513,7 +528,7 @@ def main():
         # ---- full-vector kernel on the same workload
         e2 = ensemble.Ensemble(H, h0, f, psi0, krylov_fixed=m)
         e2.set_kernel(args.full_kernel)
-        w2, k2, n2 = timed_run(e2, warm, timed)
+        w2, k2, n2, _ = timed_run(e2, warm, timed, obs=cplan)
         flops_full = args.ncols * (8.0 * m * e2.plan.nnz + 8.0 * m * n * 7)
         k2l = k2 / max(n2, 1)
         meas2 = MEASURED_TRAFFIC_FULL.get(("ell", args.jmax, args.ncols, m))
@@ -527,7 +542,7 @@ def main():
         # across the pulse peak (independent of --steps)
         e3 = ensemble.Ensemble(H, h0, f, psi0, krylov_fixed=m)
         e3.set_kernel("subspace")
-        w3, k3, n3 = timed_run(e3, warm, timed, precision="fp32")
+        w3, k3, n3, _ = timed_run(e3, warm, timed, precision="fp32", obs=cplan)
         acc = accuracy_window(H, h0, f, psi0, m, cplan)
         c64, c32 = acc["fp64"][0], acc["fp32"][0]
         p64, p32 = acc["fp64"][1], acc["fp32"][1]
@@ -584,13 +599,15 @@ def main():
     if cpu_one is not None:
         # the same sample on the device, from the same initial rows
         ens2 = ensemble.Ensemble(H, h0, f, psi0[:args.cpu_cols], krylov_fixed=m)
-        ens2.propagate(peak)
+        _, c2s = ens2.propagate(peak, obs=cplan, expect_sum=True)
         dev_err = float(np.abs(ens2.psi.cpu().numpy() - cpu_one["X"]).max())
+        c2_err = float(np.abs(c2s.cpu().numpy() - cpu_one["cos2"]).max())
         result["cpu_baseline"] = {
             "value": cpu_one["value"], "unit": "wavepacket-steps/s", "cores": 1, "kind": "port",
             "sample": f"{args.cpu_cols} wavepackets x {args.cpu_steps} steps of the same workload at the "
                       f"pulse peak ({cpu_one['seconds']:.1f} s), FP64 oracle (numpy/scipy restatement of "
-                      "richmol's TDSE.update row loop and block matvec), 1 thread"}
+                      "richmol's TDSE.update row loop and block matvec, then <cos^2 theta> of every row, as "
+                      "examples/ocs_alignment.py:89-103 loops), 1 thread"}
         if cpu_all is not None:
             result["cpu_baseline_all_cores"] = {
                 "value": cpu_all["value"], "unit": "wavepacket-steps/s", "cores": cpu_all["cores"],
@@ -600,6 +617,7 @@ def main():
         result["host_cores"] = {"affinity": len(os.sched_getaffinity(0)), "cpu_count": os.cpu_count(),
                                 "used_all_cores_leg": cpu_all["cores"] if cpu_all else 1}
         result["parity_vs_oracle_max_abs"] = dev_err
+        result["parity_vs_oracle_cos2_sum_max_abs"] = c2_err
     print(json.dumps(result), flush=True)
 
 

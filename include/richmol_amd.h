@@ -165,6 +165,23 @@ int rm_propagate_expect(rm_plan* plan, void* psi, void* work, int64_t ld, int64_
                         int32_t maxorder, double tol, int32_t* kused, int32_t* status,
                         rm_plan* obs, void* expect, void* expect_sum, void* stream);
 
+/*
+ * Propagator of rm_propagate / rm_propagate_expect (TDSE.update's `propag`,
+ * tdse.py:377-412): RM_PROPAG_LANCZOS (default, 'internal': the Lanczos
+ * expv of tdse.py:417-486) or RM_PROPAG_EXPOKIT ('external': Expokit's
+ * ZHEXPV, expokit/expokit.f:2783-3170, as tdse.py:383-391 calls it -- the
+ * operator iH(F), t = Im(f), the whole matrix's 1-norm for the first step
+ * size).  With Expokit, krylov_fixed is ZHEXPV's Krylov size m (the
+ * reference passes 12; 1 <= m <= RM_EXPOKIT_MAX_M, m < n), tol its
+ * tolerance (maxorder unused), kused receives the matvecs of each step, and
+ * a column that needed more than 500 sub-steps gets RM_NOT_CONVERGED (the
+ * reference raises).  FP64 only; no MF thresholds.
+ */
+#define RM_PROPAG_LANCZOS 0
+#define RM_PROPAG_EXPOKIT 1
+#define RM_EXPOKIT_MAX_M 16
+int rm_plan_set_propagator(rm_plan* plan, int32_t propag);
+
 /* Measurement hooks (no reference counterpart).  With timing enabled, every
  * Krylov-step launch of rm_step / rm_propagate is bracketed by HIP events on
  * its stream; rm_plan_kernel_time waits for the last one, returns the summed
@@ -402,6 +419,24 @@ int rm_kplan_propagate(rm_kplan* plan, void* psi, int64_t ld, int32_t ncols, int
                        const double* fprod, double f_re, double f_im, const void* phase,
                        int32_t krylov_fixed, int32_t maxorder, double tol, int32_t* kused,
                        int32_t* status, int32_t precision, void* stream);
+
+/*
+ * rm_kplan_propagate with an observable read after every step (the loop of
+ * examples/ocs_alignment.py:89-103 on the Kronecker-block form): obs, a
+ * Kronecker-block plan over the same J list, with obs_fprod (HOST, obs's
+ * nslot complex products, fixed for the run, e.g. those of the field
+ * [0, 0, 1] for cos^2 theta).  After step s, Y = O psi (FP64 products,
+ * whatever `precision`) and
+ *   expect: device complex [nsteps][ncols] (or NULL), <psi_b|O|psi_b>;
+ *   expect_sum: device complex [nsteps] (or NULL), their sum over b;
+ * at least one of them non-NULL.  obs holds a workspace of its own for the
+ * products (as large as a matvec's of this many wavepackets).
+ */
+int rm_kplan_propagate_expect(rm_kplan* plan, void* psi, int64_t ld, int32_t ncols, int64_t nsteps,
+                              const double* fprod, double f_re, double f_im, const void* phase,
+                              int32_t krylov_fixed, int32_t maxorder, double tol, int32_t* kused,
+                              int32_t* status, int32_t precision, rm_kplan* obs, const double* obs_fprod,
+                              void* expect, void* expect_sum, void* stream);
 
 #ifdef __cplusplus
 }

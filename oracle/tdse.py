@@ -93,17 +93,29 @@ class OracleTDSE:
         return np.asarray(vecs.toarray()).astype(np.complex128)
 
     @update_counter
-    def update(self, H, vecs, H0=None, tol=1e-15, maxorder=100, fixed_m=None, stats=None):
-        """tdse.py:265-414 with propag='internal', matvec_lib='scipy'.
+    def update(self, H, vecs, H0=None, tol=1e-15, maxorder=100, fixed_m=None, stats=None,
+               propag="internal"):
+        """tdse.py:265-414 with matvec_lib='scipy'.
 
         ``fixed_m`` (not in the reference) runs a fixed Krylov dimension
         instead of the adaptive stopping rule; ``stats`` (a list) receives the
-        Krylov dimension used for each row.
+        Krylov dimension used for each row.  propag='external' runs Expokit's
+        ZHEXPV per row as tdse.py:383-391 / 404-412 call it: Krylov size 12,
+        t = Im(exp_fac), the matvec v -> i H v, with the 1-norm of the full
+        matrix (the reference passes scipy's onenormest estimate of it,
+        which only sets the first step size; here the exact norm).
         """
         f = self.exp_fac
 
         def matvec(v):
             return flatten(H, H.vec(unflatten(H, v)))
+
+        if propag == "external":
+            from .expokit import zhexpv
+            anorm = float(abs(H.tomat(form="full")).sum(axis=0).max())
+            prop = lambda v: zhexpv(v, anorm, 12, f.imag, lambda x: matvec(x) * 1j, tol=tol)
+        else:
+            prop = lambda v: expmv_lanczos(v, f, matvec, maxorder, tol, fixed_m, stats)
 
         if H0 is not None:
             if "_exp_fac_H0" not in self.__dict__:
@@ -113,12 +125,12 @@ class OracleTDSE:
             vecs2 = vecs * self._exp_fac_H0
             if hasattr(H, "mfmat") and len(H.mfmat) > 0:
                 for i, v in enumerate(vecs2):
-                    vecs2[i] = expmv_lanczos(v, f, matvec, maxorder, tol, fixed_m, stats)
+                    vecs2[i] = prop(v)
             vecs2 *= self._exp_fac_H0
         else:
             vecs2 = np.empty(vecs.shape, dtype=vecs.dtype)
             for i, v in enumerate(vecs):
-                vecs2[i] = expmv_lanczos(v, f, matvec, maxorder, tol, fixed_m, stats)
+                vecs2[i] = prop(v)
         return vecs2
 
 

@@ -9,7 +9,8 @@ LIB_PATH = os.path.join(PKG_DIR, LIB_NAME)
 SOURCES = [os.path.join(PKG_DIR, "csrc", "rm_tdse.hip"),
            os.path.join(PKG_DIR, "csrc", "rm_subspace.hip"),
            os.path.join(PKG_DIR, "csrc", "rm_coop.hip"),
-           os.path.join(PKG_DIR, "csrc", "rm_kron.hip")]
+           os.path.join(PKG_DIR, "csrc", "rm_kron.hip"),
+           os.path.join(PKG_DIR, "csrc", "rm_expokit.hip")]
 
 HIPCC_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -46,6 +46,7 @@ SIGNATURES = {
                                            _vp, _c_dblp, ctypes.c_double, ctypes.c_double, _vp,
                                            ctypes.c_int32, ctypes.c_int32, ctypes.c_double, _vp, _vp,
                                            _vp, _vp, _vp, _vp]),
+    "rm_plan_set_propagator": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "rm_plan_timing": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "rm_plan_kernel_time": (ctypes.c_int, [_vp, _c_dblp, _c_i64p]),
     "rm_plan_set_kernel": (ctypes.c_int, [_vp, ctypes.c_int32]),
@@ -76,6 +77,10 @@ SIGNATURES = {
     "rm_kplan_propagate": (ctypes.c_int, [_vp, _vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int64, _c_dblp,
                                           ctypes.c_double, ctypes.c_double, _vp, ctypes.c_int32,
                                           ctypes.c_int32, ctypes.c_double, _vp, _vp, ctypes.c_int32, _vp]),
+    "rm_kplan_propagate_expect": (ctypes.c_int, [_vp, _vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int64,
+                                                 _c_dblp, ctypes.c_double, ctypes.c_double, _vp, ctypes.c_int32,
+                                                 ctypes.c_int32, ctypes.c_double, _vp, _vp, ctypes.c_int32,
+                                                 _vp, _c_dblp, _vp, _vp, _vp]),
 }
 
 

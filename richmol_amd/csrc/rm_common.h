@@ -181,6 +181,15 @@ struct rm_plan {
   int64_t obs_cap = 0, obs_vcap = 0;
   double2* ex_buf = nullptr;     // per-column values when the caller wants only the sums
   int64_t ex_cap = 0;
+  // propagator of rm_propagate (rm_plan_set_propagator) and the Expokit
+  // path's scratch: workgroup regions, per-step norms, work items
+  int32_t propag = RM_PROPAG_LANCZOS;
+  void* expo_ws = nullptr;
+  size_t expo_ws_bytes = 0;
+  double* expo_anorm = nullptr;
+  int64_t expo_anorm_cap = 0;
+  int4* expo_items = nullptr;
+  int64_t expo_items_cap = 0;
   // optional HIP-event timing of the Krylov step launches
   int timing = 0;
   int nrec = 0;
@@ -293,6 +302,13 @@ int long_run(void** scratch, const SubOp* d_ops, const int4* d_blk, const std::v
              const int4* d_coop, const std::vector<LongItem>& coop, int32_t kcap, int32_t fixed_m,
              int32_t maxorder, double tol, bool f32, int n_cu, hipStream_t s, int32_t abort_preset = 0,
              const SubOp* op0 = nullptr);
+// Expokit propagation (rm_expokit.hip) of the items (column, lo, len) of one
+// job with Krylov size m and tolerance tol; scratch grown in place.
+int expokit_run(const SubOp& op, const std::vector<int4>& items, int32_t m, double tol, int n_cu, void** ws,
+                size_t* ws_bytes, double** anorm, int64_t* anorm_cap, int4** d_items, int64_t* items_cap,
+                hipStream_t s);
+// sum[s] = sum over b of ex[s * ncols + b] (device, fixed order), on s.
+int launch_expect_sum(int64_t nsteps, int64_t ncols, const double2* ex, double2* sum, hipStream_t s);
 // Event-bracketed launch helpers for the plan's optional timing.
 hipError_t timing_begin(rm_plan* p, hipStream_t s);
 hipError_t timing_end(rm_plan* p, hipStream_t s);

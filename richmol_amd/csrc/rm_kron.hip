@@ -102,6 +102,8 @@ struct rm_kplan {
   std::vector<int64_t> h_meta;          // host copy (its source must outlive the async upload)
   double2* d_fp = nullptr;              // field products of a call's steps
   int64_t fp_cap = 0;
+  double2* d_ex = nullptr;              // as an observable: per-wavepacket values when only sums are wanted
+  int64_t ex_cap = 0;
 };
 
 namespace {
@@ -625,6 +627,53 @@ __global__ void k_kp_fill_i(int32_t* __restrict__ x, int32_t n, int32_t v) {
   if (i < n) x[i] = v;
 }
 
+// Observable partials per (J, wavepacket): conj(S) . Y over the J block, S
+// the wavepackets and Y = O S, both in the split layout
+__global__ __launch_bounds__(256) void k_kp_odot(const int64_t* __restrict__ voff, const int32_t* __restrict__ Js,
+                                                 const double* __restrict__ S, const double* __restrict__ Y,
+                                                 double* __restrict__ part, int32_t nJ) {
+  const int j = blockIdx.x, b = blockIdx.y;
+  const int64_t d = jdim(Js[j]), len = 2 * d * d;
+  const int64_t base = voff[j] * 2 * gridDim.y + (int64_t)b * len;
+  double a0 = 0.0, a1 = 0.0;
+  for (int64_t i = threadIdx.x; i < d * d; i += blockDim.x) {
+    const int64_t row = i / d, k = i % d;
+    const int64_t pr = base + row * 2 * d + k, pi = pr + d;
+    const double sr = S[pr], si = S[pi], yr = Y[pr], yi = Y[pi];
+    a0 += sr * yr + si * yi;
+    a1 += sr * yi - si * yr;
+  }
+  __shared__ double red[2][4];
+  a0 = wave_allsum(a0);
+  a1 = wave_allsum(a1);
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = a0;
+    red[1][threadIdx.x >> 6] = a1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    part[((int64_t)b * nJ + j) * 2] = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+    part[((int64_t)b * nJ + j) * 2 + 1] = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
+  }
+}
+
+// ex[b] = <S_b|O|S_b>: the J partials of wavepacket b added in order
+__global__ void k_kp_ofin(const double* __restrict__ part, int32_t nJ, int32_t ncols, double2* __restrict__ ex) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= ncols) return;
+  double re = 0.0, im = 0.0;
+  for (int j = 0; j < nJ; ++j) {
+    re += part[((int64_t)b * nJ + j) * 2];
+    im += part[((int64_t)b * nJ + j) * 2 + 1];
+  }
+  ex[b] = make_double2(re, im);
+}
+
+__global__ void k_kp_fill_d(double* __restrict__ x, int32_t n, double v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = v;
+}
+
 // S = P sum_j c_j V_j (kf != NULL: wavepacket b uses its own dimension kf[b])
 __global__ void k_kp_comb(const int64_t* __restrict__ voff, const int32_t* __restrict__ Js,
                           const double2* __restrict__ ph, const double* __restrict__ V, int64_t vstride,
@@ -693,7 +742,8 @@ __global__ void k_kp_out(const int64_t* __restrict__ voff, const int32_t* __rest
 
 void kplan_free(rm_kplan* p) {
   if (!p) return;
-  void* bufs[] = {p->d_mdiag, p->d_sig, p->d_blk, p->d_mf, p->ws, p->d_tasks, p->d_voff, p->d_Js, p->d_fp, p->d_meta, p->d_kf64, p->d_kf32, p->d_colmap, p->d_fz};
+  void* bufs[] = {p->d_mdiag, p->d_sig, p->d_blk, p->d_mf, p->ws, p->d_tasks, p->d_voff, p->d_Js, p->d_fp, p->d_meta, p->d_kf64, p->d_kf32, p->d_colmap, p->d_fz,
+                  p->d_ex};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (double* k : p->kbig64)
@@ -1209,10 +1259,41 @@ extern "C" int rm_kplan_matvec(rm_kplan* p, const void* x, void* y, int64_t ld, 
 // nsteps split-operator Lanczos steps (tdse.py:365-414, 417-486) for ncols
 // wavepackets: a fixed Krylov dimension, or the reference's adaptive rule
 // (krylov_fixed == 0); see the header for the arguments.
+static int kplan_propagate(rm_kplan* p, void* psi, int64_t ld, int32_t ncols, int64_t nsteps,
+                           const double* fprod, double f_re, double f_im, const void* phase,
+                           int32_t krylov_fixed, int32_t maxorder, double tol, int32_t* kused,
+                           int32_t* status, int32_t precision, rm_kplan* obs, const double* obs_fprod,
+                           double2* expect, double2* expect_sum, hipStream_t s);
+
 extern "C" int rm_kplan_propagate(rm_kplan* p, void* psi, int64_t ld, int32_t ncols, int64_t nsteps,
                                   const double* fprod, double f_re, double f_im, const void* phase,
                                   int32_t krylov_fixed, int32_t maxorder, double tol, int32_t* kused,
                                   int32_t* status, int32_t precision, void* stream) {
+  return kplan_propagate(p, psi, ld, ncols, nsteps, fprod, f_re, f_im, phase, krylov_fixed, maxorder, tol, kused,
+                         status, precision, nullptr, nullptr, nullptr, nullptr, (hipStream_t)stream);
+}
+
+extern "C" int rm_kplan_propagate_expect(rm_kplan* p, void* psi, int64_t ld, int32_t ncols, int64_t nsteps,
+                                         const double* fprod, double f_re, double f_im, const void* phase,
+                                         int32_t krylov_fixed, int32_t maxorder, double tol, int32_t* kused,
+                                         int32_t* status, int32_t precision, rm_kplan* obs,
+                                         const double* obs_fprod, void* expect, void* expect_sum, void* stream) {
+  if (!p || !obs) return fail(RM_ERR_ARG, "rm_kplan_propagate_expect: NULL plan or observable");
+  if (obs == p) return fail(RM_ERR_ARG, "rm_kplan_propagate_expect: the observable must be a plan of its own");
+  if (obs->n != p->n || obs->Js != p->Js)
+    return fail(RM_ERR_ARG, "rm_kplan_propagate_expect: the observable's basis differs from the plan's");
+  if (obs->nslot > 0 && !obs_fprod) return fail(RM_ERR_ARG, "rm_kplan_propagate_expect: NULL obs_fprod");
+  if (!expect && !expect_sum) return fail(RM_ERR_ARG, "rm_kplan_propagate_expect: NULL expect and expect_sum");
+  return kplan_propagate(p, psi, ld, ncols, nsteps, fprod, f_re, f_im, phase, krylov_fixed, maxorder, tol, kused,
+                         status, precision, obs, obs_fprod, (double2*)expect, (double2*)expect_sum,
+                         (hipStream_t)stream);
+}
+
+static int kplan_propagate(rm_kplan* p, void* psi, int64_t ld, int32_t ncols, int64_t nsteps,
+                           const double* fprod, double f_re, double f_im, const void* phase,
+                           int32_t krylov_fixed, int32_t maxorder, double tol, int32_t* kused,
+                           int32_t* status, int32_t precision, rm_kplan* obs, const double* obs_fprod,
+                           double2* expect, double2* expect_sum, hipStream_t s) {
   if (!p) return fail(RM_ERR_ARG, "rm_kplan_propagate: NULL plan");
   if (ncols < 1 || ncols > 65535 || ld < p->n || !psi || nsteps < 0 || (nsteps > 0 && p->nslot > 0 && !fprod))
     return fail(RM_ERR_ARG, "rm_kplan_propagate: bad arguments");
@@ -1223,7 +1304,6 @@ extern "C" int rm_kplan_propagate(rm_kplan* p, void* psi, int64_t ld, int32_t nc
     return fail(RM_ERR_ARG, "rm_kplan_propagate: maxorder must be in [1, %d] and tol > 0", KMAX);
   if (precision != RM_PREC_FP64 && precision != RM_PREC_FP32)
     return fail(RM_ERR_ARG, "rm_kplan_propagate: unknown precision %d", precision);
-  hipStream_t s = (hipStream_t)stream;
   const bool f32 = precision == RM_PREC_FP32;
   const int32_t m = adaptive ? maxorder : krylov_fixed;   // Krylov vectors held
   if (status)
@@ -1246,6 +1326,39 @@ extern "C" int rm_kplan_propagate(rm_kplan* p, void* psi, int64_t ld, int32_t nc
   const dim3 fin_grid((unsigned)((ncols + 3) / 4)), fin_block(256);
   const int64_t vs = w.vreal;
   hipLaunchKernelGGL(k_kp_pack, cg, cb, 0, s, p->d_voff, p->d_Js, (const double2*)psi, ld, w.S, 1);
+  // observable: its operator (fixed products) formed once; after every step
+  // Y = O S in FP64 on its own workspace, then <S_b|Y_b> per wavepacket
+  KWork wo;
+  int64_t ntask_o = 0;
+  std::vector<int64_t> zoff_o;
+  double2* ex = expect;
+  if (obs) {
+    if ((rc = kwork(obs, ncols, 0, false, &wo))) return rc;
+    if ((rc = ktasks(obs, ncols, &ntask_o, zoff_o, s))) return rc;
+    if (!ex) {
+      if ((rc = grow(&obs->d_ex, &obs->ex_cap, nsteps * ncols))) return rc;
+      ex = obs->d_ex;
+    }
+    if ((rc = grow(&obs->d_fp, &obs->fp_cap, std::max<int32_t>(obs->nslot, 1)))) return rc;
+    if (obs->nslot)
+      HIPCHK(hipMemcpyAsync(obs->d_fp, obs_fprod, sizeof(double2) * obs->nslot, hipMemcpyHostToDevice, s));
+    if (obs->nblk)
+      hipLaunchKernelGGL(k_kp_mf, dim3(8, (unsigned)obs->nblk), dim3(256), 0, s, obs->d_blk, obs->d_mdiag,
+                         obs->d_fp, obs->d_mf);
+    hipLaunchKernelGGL(k_kp_fill_d, dim3(grid_for(ncols, 256)), dim3(256), 0, s, wo.ib, ncols, 1.0);
+    HIPCHK(hipGetLastError());
+  }
+  auto observe = [&](int64_t st) -> int {
+    if (!obs) return RM_OK;
+    int r = kmatvec(obs, w.S, wo.ib, wo.Y, wo.Z, ncols, ntask_o, zoff_o, false, s);
+    if (r) return r;
+    hipLaunchKernelGGL(k_kp_odot, cg, cb, 0, s, p->d_voff, p->d_Js, (const double*)w.S, (const double*)wo.Y,
+                       wo.part, p->nJ);
+    hipLaunchKernelGGL(k_kp_ofin, dim3(grid_for(ncols, 256)), dim3(256), 0, s, (const double*)wo.part, p->nJ,
+                       ncols, ex + st * ncols);
+    HIPCHK(hipGetLastError());
+    return RM_OK;
+  };
   for (int64_t st = 0; st < nsteps; ++st) {
     bool field = false;
     for (int32_t c = 0; c < p->nslot && !field; ++c)
@@ -1254,6 +1367,7 @@ extern "C" int rm_kplan_propagate(rm_kplan* p, void* psi, int64_t ld, int32_t nc
       if (ph) hipLaunchKernelGGL(k_kp_phase2, cg, cb, 0, s, p->d_voff, p->d_Js, ph, w.S);
       if (kused)
         hipLaunchKernelGGL(k_kp_izero, dim3(grid_for(ncols, 256)), dim3(256), 0, s, kused + st * ncols, ncols);
+      if ((rc = observe(st))) return rc;
       continue;
     }
     if (p->nblk)
@@ -1317,9 +1431,11 @@ extern "C" int rm_kplan_propagate(rm_kplan* p, void* psi, int64_t ld, int32_t nc
                            RM_NOT_CONVERGED);
     }
     HIPCHK(hipGetLastError());
+    if ((rc = observe(st))) return rc;
   }
   hipLaunchKernelGGL(k_kp_pack, cg, cb, 0, s, p->d_voff, p->d_Js, (const double2*)psi, ld, w.S, 0);
   HIPCHK(hipGetLastError());
+  if (obs && expect_sum) return launch_expect_sum(nsteps, ncols, ex, expect_sum, s);
   return RM_OK;
 }
 

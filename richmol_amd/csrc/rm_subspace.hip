@@ -446,6 +446,13 @@ __device__ __forceinline__ float2 wave_shift<float2>(float2 v, bool up) {
                      __int_as_float(wave_shift_i(__float_as_int(v.y), up)));
 }
 
+// v of lane src (ds_bpermute on the two halves of the double, no memory)
+__device__ __forceinline__ double lane_read(double v, int src) {
+  const int addr = src << 2;
+  return __hiloint2double(__builtin_amdgcn_ds_bpermute(addr, __double2hiint(v)),
+                          __builtin_amdgcn_ds_bpermute(addr, __double2loint(v)));
+}
+
 template <int WM, bool F32, bool BAND = false, bool OBS = false>
 __global__ __launch_bounds__(64) void k_sub_pair(SubArgs a) {
   using XT = typename XType<F32>::T;
@@ -455,6 +462,8 @@ __global__ __launch_bounds__(64) void k_sub_pair(SubArgs a) {
   __shared__ double2 s_alpha[2][16];
   __shared__ double s_beta[2][17];
   __shared__ double2 s_cf[2][16];
+  constexpr int OBK = 16;                  // observable terms buffered per flush (OBS)
+  __shared__ double2 s_ob[OBS ? OBK : 1][64];
   const int l = threadIdx.x;
   const int h = l >> 5, i = l & 31;
   const double2 zero = make_double2(0.0, 0.0);
@@ -504,40 +513,78 @@ __global__ __launch_bounds__(64) void k_sub_pair(SubArgs a) {
             if (o == oo) Bo[sl][oo] = cadd(Bo[sl][oo], Bv[sl][j]);
       }
     }
-    // OBS: the observable's entries of my row as (source lane, value); a
-    // column outside the domain (where the wavepacket is zero) or a row past
-    // it gives none.  Entries beyond OWR are read from memory at every step.
+    // OBS: the observable's entries of my row as (source lane, value), and
+    // summed per offset -1, 0, +1 when every entry inside the domain lies
+    // within one row of the diagonal (then the rows come by DPP shifts, as
+    // the banded matvec takes them); a column outside the domain (where the
+    // wavepacket is zero) or a row past it gives none.  Entries beyond OWR
+    // are read from memory at every step.  Each lane's term conj(psi_i)
+    // (O psi)_i goes to LDS; every OBK steps (and after the last) lane i of
+    // each half adds the terms of buffered step i over its rows in order.
     constexpr int OWR = 4;
     int osrc[OBS ? OWR : 1];
     double2 oval[OBS ? OWR : 1];
+    double2 o3[3] = {zero, zero, zero};
+    bool oband = false;
     if constexpr (OBS) {
+      bool far = false;
 #pragma unroll
       for (int j = 0; j < OWR; ++j) {
         const bool e = ok && j < op.ow;
         const int c = e ? op.ocol[(int64_t)j * n + gq] - lo : -1;
         const bool in = c >= 0 && c < len;
+        const double2 v = in ? op.oval[(int64_t)j * n + gq] : zero;
         osrc[j] = in ? 32 * h + c : l;
-        oval[j] = in ? op.oval[(int64_t)j * n + gq] : zero;
+        oval[j] = v;
+        const int d = c - i;
+#pragma unroll
+        for (int o = 0; o < 3; ++o)
+          if (in && d == o - 1) o3[o] = cadd(o3[o], v);
+        far |= in && (d < -1 || d > 1);
       }
+      for (int j = OWR; j < op.ow; ++j) {
+        const int c = ok ? op.ocol[(int64_t)j * n + gq] - lo : -1;
+        const bool in = c >= 0 && c < len;
+        const double2 v = in ? op.oval[(int64_t)j * n + gq] : zero;
+        const int d = c - i;
+#pragma unroll
+        for (int o = 0; o < 3; ++o)
+          if (in && d == o - 1) o3[o] = cadd(o3[o], v);
+        far |= in && (d < -1 || d > 1);
+      }
+      oband = !__any(far);
     }
     // <psi|O|psi> of the pair's wavepackets after step `step` (the state in st)
     auto observe = [&](int step) __attribute__((always_inline)) {
       if constexpr (OBS) {
         double2 y = zero;
+        if (oband) {
+          y = cmul(o3[1], st);
+          y = cfma(o3[0], wave_shift<double2>(st, false), y);
+          y = cfma(o3[2], wave_shift<double2>(st, true), y);
+        } else {
 #pragma unroll
-        for (int j = 0; j < OWR; ++j)
-          y = cfma(oval[j], make_double2(__shfl(st.x, osrc[j]), __shfl(st.y, osrc[j])), y);
-        for (int j = OWR; j < op.ow; ++j) {
-          const int c = ok ? op.ocol[(int64_t)j * n + gq] - lo : -1;
-          const bool in = c >= 0 && c < len;
-          const double2 v = in ? op.oval[(int64_t)j * n + gq] : zero;
-          const int src = in ? 32 * h + c : l;
-          y = cfma(v, make_double2(__shfl(st.x, src), __shfl(st.y, src)), y);
+          for (int j = 0; j < OWR; ++j)
+            y = cfma(oval[j], make_double2(lane_read(st.x, osrc[j]), lane_read(st.y, osrc[j])), y);
+          for (int j = OWR; j < op.ow; ++j) {
+            const int c = ok ? op.ocol[(int64_t)j * n + gq] - lo : -1;
+            const bool in = c >= 0 && c < len;
+            const double2 v = in ? op.oval[(int64_t)j * n + gq] : zero;
+            const int src = in ? 32 * h + c : l;
+            y = cfma(v, make_double2(lane_read(st.x, src), lane_read(st.y, src)), y);
+          }
         }
-        const double e0 = half_allsum32(st.x * y.x + st.y * y.y);
-        const double e1 = half_allsum32(st.x * y.y - st.y * y.x);
-        if (i == 0 && b >= 0 && op.expect)
-          op.expect[(int64_t)step * op.expect_ld + b - op.kcol0] = make_double2(e0, e1);
+        s_ob[step % OBK][l] = make_double2(st.x * y.x + st.y * y.y, st.x * y.y - st.y * y.x);
+        if ((step + 1) % OBK == 0 || step + 1 == op.nsteps) {
+          wave_sync();
+          const int nk = step % OBK + 1, s0 = step + 1 - nk;
+          if (i < nk && b >= 0 && op.expect) {
+            double2 acc = zero;
+            for (int r = 0; r < len; ++r) acc = cadd(acc, s_ob[i][32 * h + r]);
+            op.expect[(int64_t)(s0 + i) * op.expect_ld + b - op.kcol0] = acc;
+          }
+          wave_sync();   // the buffer is rewritten after these reads
+        }
       }
     };
     double2* sa = s_alpha[h];
@@ -760,13 +807,23 @@ __global__ __launch_bounds__(256) void k_sub_prologue(Prologue a) {
     const int64_t st = bid / a.parts, c = bid % a.parts;
     const double2* fp = a.table + st * a.nfield;
     const int64_t t1 = std::min(a.nterm, (c + 1) * a.chunk);
+    // all products of the step zero: every MF element is zero (the M values
+    // are finite); otherwise the chunk is scanned until its first non-zero
+    // element (a uniform exit after each pass of 256 terms)
+    bool anyf = false;
+    for (int f = 0; f < a.nfield; ++f) anyf |= !(fp[f].x == 0.0 && fp[f].y == 0.0);
     int any = 0;
-    for (int64_t t = c * a.chunk + tid; t < t1; t += 256) {
-      double2 acc = make_double2(0.0, 0.0);
-      for (int64_t q = a.term_ptr[t]; q < a.term_ptr[t + 1]; ++q)
-        acc = cfma(fp[a.term_field[q]], a.term_mval[q], acc);
-      any |= (acc.x != 0.0 || acc.y != 0.0 || acc.x != acc.x || acc.y != acc.y) ? 1 : 0;
-    }
+    if (anyf)
+      for (int64_t t0 = c * a.chunk; t0 < t1; t0 += 256) {
+        const int64_t t = t0 + tid;
+        if (t < t1) {
+          double2 acc = make_double2(0.0, 0.0);
+          for (int64_t q = a.term_ptr[t]; q < a.term_ptr[t + 1]; ++q)
+            acc = cfma(fp[a.term_field[q]], a.term_mval[q], acc);
+          any |= (acc.x != 0.0 || acc.y != 0.0 || acc.x != acc.x || acc.y != acc.y) ? 1 : 0;
+        }
+        if (__syncthreads_or(any)) break;
+      }
     any = __syncthreads_or(any);
     if (tid == 0) {
       if (a.parts == 1) {
@@ -1390,6 +1447,14 @@ extern "C" int rm_plan_set_domain_reuse(rm_plan* p, int32_t on) {
   return RM_OK;
 }
 
+extern "C" int rm_plan_set_propagator(rm_plan* p, int32_t propag) {
+  if (!p) return fail(RM_ERR_ARG, "rm_plan_set_propagator: NULL plan");
+  if (propag != RM_PROPAG_LANCZOS && propag != RM_PROPAG_EXPOKIT)
+    return fail(RM_ERR_ARG, "rm_plan_set_propagator: unknown propagator %d", propag);
+  p->propag = propag;
+  return RM_OK;
+}
+
 extern "C" int rm_plan_set_coop_abort(rm_plan* p, int32_t preset) {
   if (!p) return fail(RM_ERR_ARG, "rm_plan_set_coop_abort: NULL plan");
   p->coop_abort = preset ? 1 : 0;
@@ -1490,7 +1555,10 @@ int sub_propagate(rm_plan* p, void* psi, int64_t ld, int64_t ncols, int64_t nste
     st = p->int_status;
   }
   const bool pairs = krylov_fixed >= 1 && krylov_fixed <= PAIR_MAX_M;
-  const bool reuse = p->dom_reuse && sc->kept && sc->kept_psi == psi && sc->kept_ld == ld &&
+  // the Expokit propagator takes fresh domains at every call (its items are
+  // not kept)
+  const bool expo = p->propag == RM_PROPAG_EXPOKIT;
+  const bool reuse = !expo && p->dom_reuse && sc->kept && sc->kept_psi == psi && sc->kept_ld == ld &&
                      sc->kept_ncols == ncols && sc->kept_force == force && sc->kept_pairs == pairs;
   // step flags in chunks of <= 1024 terms, at most 16 per step
   const int32_t parts = (int32_t)std::max<int64_t>(1, std::min<int64_t>(16, (p->nterm + 1023) / 1024));
@@ -1548,9 +1616,50 @@ int sub_propagate(rm_plan* p, void* psi, int64_t ld, int64_t ncols, int64_t nste
   const int64_t nb = (int64_t)pa.nb_flag + pa.nb_fill + (reuse ? pa.cpb * ncols : 0);
   hipLaunchKernelGGL(k_sub_prologue, dim3((unsigned)nb), dim3(256), 0, s, pa);
   HIPCHK(hipGetLastError());
+  // the job of this call (one operator, every column)
+  auto make_op = [&]() {
+    SubOp op = plan_op(p);
+    op.phase = (const double2*)phase;
+    op.table = (const double2*)fprod_dev;
+    op.sflag = p->sflag;
+    op.psi = (double2*)psi;
+    op.ld = ld;
+    op.kused = kused;
+    op.kused_ld = ncols;
+    op.status = st;
+    op.stale = reuse ? p->stale : nullptr;
+    op.call_id = p->call_id;
+    op.f = make_double2(f_re, f_im);
+    op.nsteps = (int32_t)nsteps;
+    if (obs) {
+      op.ocol = p->obs_col;
+      op.oval = p->obs_val;
+      op.ow = p->obs_w;
+      op.expect = expect;
+      op.expect_ld = ncols;
+    }
+    return op;
+  };
   if (!reuse) {
     sc->kept = false;
     if ((rc = sub_domains(p, psi, ld, ncols, s))) return rc;
+    if (expo) {
+      // Expokit: one workgroup per column on its domain, any length
+      std::vector<int4> items;
+      for (int64_t b = 0; b < ncols; ++b) {
+        const int32_t lo = p->h_dom[2 * b], hi = p->h_dom[2 * b + 1];
+        if (hi > lo) items.push_back(make_int4((int)b, lo, hi - lo, 0));
+      }
+      if (!p->sub_rp && (rc = csr_build(p, entry_values(p)))) return rc;
+      *done = true;
+      const SubOp op = make_op();
+      if (p->timing) HIPCHK(timing_begin(p, s));
+      rc = expokit_run(op, items, krylov_fixed, tol, p->n_cu, &p->expo_ws, &p->expo_ws_bytes, &p->expo_anorm,
+                       &p->expo_anorm_cap, &p->expo_items, &p->expo_items_cap, s);
+      if (rc) return rc;
+      if (p->timing) HIPCHK(timing_end(p, s));
+      return RM_OK;
+    }
     for (auto& v : sc->kept_cls) v.clear();
     sc->kept_blk.clear();
     sc->kept_coop.clear();
@@ -1605,27 +1714,8 @@ int sub_propagate(rm_plan* p, void* psi, int64_t ld, int64_t ncols, int64_t nste
     }
   }
   *done = true;
-  std::vector<SubOp> ops(1, plan_op(p));
-  SubOp& op = ops[0];
-  op.phase = (const double2*)phase;
-  op.table = (const double2*)fprod_dev;
-  op.sflag = p->sflag;
-  op.psi = (double2*)psi;
-  op.ld = ld;
-  op.kused = kused;
-  op.kused_ld = ncols;
-  op.status = st;
-  op.stale = reuse ? p->stale : nullptr;
-  op.call_id = p->call_id;
-  op.f = make_double2(f_re, f_im);
-  op.nsteps = (int32_t)nsteps;
-  if (obs) {
-    op.ocol = p->obs_col;
-    op.oval = p->obs_val;
-    op.ow = p->obs_w;
-    op.expect = expect;
-    op.expect_ld = ncols;
-  }
+  std::vector<SubOp> ops(1, make_op());
+  const SubOp& op = ops[0];
   const int32_t kcap = krylov_fixed > 0 ? krylov_fixed : maxorder;
   const bool f32 = p->precision == RM_PREC_FP32;
   if (sc->kept) {

@@ -42,7 +42,8 @@ static void plan_free(rm_plan* p) {
                   p->ell_col, p->ell_val, p->ell_pos, p->sub_perm, p->sub_label, p->sub_lo,
                   p->sub_hi, p->sub_col, p->sub_val, p->sub_rp, p->sub_ccol, p->sub_cval, p->sub_cval32, p->dom,
                   p->sflag, p->sflag_part, p->sflag_arrive, p->dom_keep, p->stale, p->int_status,
-                  p->sub_iperm, p->obs_col, p->obs_val, p->ex_buf};
+                  p->sub_iperm, p->obs_col, p->obs_val, p->ex_buf, p->expo_ws, p->expo_anorm,
+                  p->expo_items};
   sub_scratch_release(p->sub_scratch);
   if (p->h_dom_pin) (void)hipHostFree(p->h_dom_pin);
   if (p->h_status_pin) (void)hipHostFree(p->h_status_pin);
@@ -1546,6 +1547,13 @@ __global__ __launch_bounds__(64) void k_expect_sum(int64_t nsteps, int64_t ncols
   }
 }
 
+int launch_expect_sum(int64_t nsteps, int64_t ncols, const double2* ex, double2* sum, hipStream_t s) {
+  hipLaunchKernelGGL(k_expect_sum, dim3((unsigned)std::min<int64_t>(nsteps, 65535)), dim3(64), 0, s, nsteps, ncols,
+                     ex, sum);
+  HIPCHK(hipGetLastError());
+  return RM_OK;
+}
+
 static int propagate_impl(rm_plan* p, void* psi, void* work, int64_t ld, int64_t ncols, int64_t nsteps,
                           const double* fprod_dev, const double* thresh, double f_re, double f_im,
                           const void* phase, int32_t krylov_fixed, int32_t maxorder, double tol,
@@ -1599,12 +1607,7 @@ static int propagate_impl(rm_plan* p, void* psi, void* work, int64_t ld, int64_t
     ex = p->ex_buf;
   }
   auto finish = [&]() -> int {
-    if (obs && expect_sum) {
-      hipLaunchKernelGGL(k_expect_sum, dim3((unsigned)std::min<int64_t>(nsteps, 65535)), dim3(64), 0, s, nsteps,
-                         ncols, (const double2*)ex, expect_sum);
-      HIPCHK(hipGetLastError());
-    }
-    return RM_OK;
+    return obs && expect_sum ? launch_expect_sum(nsteps, ncols, ex, expect_sum, s) : RM_OK;
   };
   if (!psi || !work) return fail(RM_ERR_ARG, "rm_propagate: NULL psi or work");
   if (psi == work) return fail(RM_ERR_ARG, "rm_propagate: psi and work must differ");
@@ -1618,6 +1621,23 @@ static int propagate_impl(rm_plan* p, void* psi, void* work, int64_t ld, int64_t
       fa.thresh[i] = thresh[i];
       any_thresh |= thresh[i] > 0.0;
     }
+  if (p->propag == RM_PROPAG_EXPOKIT) {
+    // Expokit's ZHEXPV on the invariant-subspace domains (rm_expokit.hip)
+    if (krylov_fixed < 1 || krylov_fixed > RM_EXPOKIT_MAX_M || krylov_fixed >= p->n)
+      return fail(RM_ERR_ARG, "rm_propagate: Expokit Krylov size m=%d must be in [1, %d] and below n=%lld",
+                  krylov_fixed, RM_EXPOKIT_MAX_M, (long long)p->n);
+    if (any_thresh) return fail(RM_ERR_ARG, "rm_propagate: the Expokit propagator takes no MF thresholds");
+    if (p->precision != RM_PREC_FP64)
+      return fail(RM_ERR_ARG, "rm_propagate: the Expokit propagator computes in FP64 only (plan set to FP32)");
+    if (nsteps > INT32_MAX)
+      return fail(RM_ERR_ARG, "rm_propagate: more than 2^31 steps");
+    bool done = false;
+    rc = sub_propagate(p, psi, ld, ncols, nsteps, fprod_dev, f_re, f_im, phase, krylov_fixed, maxorder, tol,
+                       kused, status, 0, s, &done, obs, ex);
+    if (rc) return rc;
+    if (!done) return fail(RM_ERR_ARG, "rm_propagate: the Expokit propagator does not apply");
+    return finish();
+  }
   const bool sub_kernel = p->variant == RM_KERNEL_SUBSPACE || p->variant == RM_KERNEL_COOP ||
                           p->variant == RM_KERNEL_BLOCK;
   if ((p->variant == RM_KERNEL_AUTO || sub_kernel) && !any_thresh) {

@@ -10,6 +10,11 @@ import numpy as np
 from . import units
 
 
+class ExpokitError(ValueError):
+    """The external propagator's failures, as richmol/pyexpokit.py:5-14
+    names them."""
+
+
 def thermal_sample(basis, temp, nsamp, seed=0):
     """Indices of nsamp basis states |J, m, k> drawn with probability
     proportional to exp(-E/kT) (all m equally weighted), seeded."""
@@ -224,5 +229,8 @@ class Ensemble:
         if np.any(st & _lib.RM_STALE_DOMAIN):
             raise RuntimeError("the rows changed since the last propagate call kept their domains "
                                "(use Ensemble.load); they were not propagated")
-        if self.krylov_fixed == 0 and bool(np.any(st & _lib.RM_NOT_CONVERGED)):
-            raise ValueError(f"Lanczos reached maximum order of '{self.maxorder}' without convergence")
+        if bool(np.any(st & _lib.RM_NOT_CONVERGED)):
+            if self.plan is not None and getattr(self.plan, "propag", "internal") == "external":
+                raise ExpokitError("maximum number of steps reached without convergence")
+            if self.krylov_fixed == 0:
+                raise ValueError(f"Lanczos reached maximum order of '{self.maxorder}' without convergence")

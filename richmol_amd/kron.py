@@ -396,18 +396,41 @@ class DeviceKronPlan:
         return out
 
     def propagate(self, psi, fprod, f, phase=None, krylov_fixed=12, precision="fp64", stream=None,
-                  maxorder=100, tol=1e-15, kused=None, status=None):
+                  maxorder=100, tol=1e-15, kused=None, status=None, obs=None, obs_fprod=None,
+                  expect=None, expect_sum=None):
         """len(fprod) steps of TDSE.update on the rows of psi (in place);
         fprod: host (nsteps, nslot) complex from slot_products.  krylov_fixed
         0: the reference's adaptive rule with maxorder and tol; kused (device
-        int32 (nsteps, ncols)) and status (device int32 (ncols,)) optional."""
+        int32 (nsteps, ncols)) and status (device int32 (ncols,)) optional.
+
+        obs: a DeviceKronPlan of an observable over the same J list, with
+        obs_fprod (host (nslot,) complex, its fixed products); after every
+        step <psi_b|O|psi_b> goes to expect (device (nsteps, ncols)) and the
+        sums over the rows to expect_sum (device (nsteps,)), complex128
+        (rm_kplan_propagate_expect)."""
         import ctypes
+        import torch
         from .plan import _check_cols, _stream_handle
         _check_cols(psi, self.n)
         fp = np.ascontiguousarray(fprod, dtype=np.complex128).reshape(-1, max(self.nslot, 1))
         vp = lambda t: ctypes.c_void_p(t.data_ptr() if t is not None else 0)
-        self._lib.check(self._lib.load().rm_kplan_propagate(
-            self.handle, ctypes.c_void_p(psi.data_ptr()), psi.stride(0), psi.shape[0], fp.shape[0],
-            self._lib.ptr(fp.view(np.float64), ctypes.c_double), float(complex(f).real), float(complex(f).imag),
-            vp(phase), int(krylov_fixed), int(maxorder), float(tol), vp(kused), vp(status),
-            self._lib.PRECISIONS[precision], ctypes.c_void_p(_stream_handle(stream))), "rm_kplan_propagate")
+        args = (self.handle, ctypes.c_void_p(psi.data_ptr()), psi.stride(0), psi.shape[0], fp.shape[0],
+                self._lib.ptr(fp.view(np.float64), ctypes.c_double), float(complex(f).real),
+                float(complex(f).imag), vp(phase), int(krylov_fixed), int(maxorder), float(tol), vp(kused),
+                vp(status), self._lib.PRECISIONS[precision])
+        lib = self._lib.load()
+        if obs is None:
+            self._lib.check(lib.rm_kplan_propagate(*args, ctypes.c_void_p(_stream_handle(stream))),
+                            "rm_kplan_propagate")
+            return
+        nsteps, ncols = fp.shape[0], psi.shape[0]
+        for t, shape in ((expect, (nsteps, ncols)), (expect_sum, (nsteps,))):
+            if t is not None and (t.dtype != torch.complex128 or not t.is_cuda or tuple(t.shape) != shape
+                                  or not t.is_contiguous()):
+                raise ValueError(f"expected a contiguous device complex128 tensor of shape {shape}")
+        ofp = np.ascontiguousarray(obs_fprod, dtype=np.complex128).reshape(-1)
+        if ofp.size != max(obs.nslot, 1) and obs.nslot:
+            raise ValueError(f"obs_fprod: {obs.nslot} products expected, got {ofp.size}")
+        self._lib.check(lib.rm_kplan_propagate_expect(
+            *args, obs.handle, self._lib.ptr(ofp.view(np.float64), ctypes.c_double), vp(expect),
+            vp(expect_sum), ctypes.c_void_p(_stream_handle(stream))), "rm_kplan_propagate_expect")

@@ -188,6 +188,7 @@ class DevicePlan:
         self.lib = _lib.load()
         self.kernel = "auto"
         self.precision = "fp64"
+        self.propag = "internal"
         if arrays is None:
             self.active = active_carts(T) if active is None else active
             arr = flatten_operator(T, self.active)
@@ -256,6 +257,16 @@ class DevicePlan:
             code = self.KERNELS[variant]
         _lib.check(self.lib.rm_plan_set_kernel(self.handle, code), "rm_plan_set_kernel")
         self.kernel = variant
+
+    PROPAGATORS = {"internal": 0, "external": 1}
+
+    def set_propagator(self, propag):
+        """'internal' (the Lanczos expv, default) or 'external' (Expokit's
+        ZHEXPV as tdse.py:383-391 calls it; krylov_fixed is its Krylov size
+        m, 12 in the reference) for propagate()."""
+        _lib.check(self.lib.rm_plan_set_propagator(self.handle, self.PROPAGATORS[propag]),
+                   "rm_plan_set_propagator")
+        self.propag = propag
 
     def set_precision(self, precision):
         """'fp64' (default) or 'fp32': complex64 bulk matvec in the subspace

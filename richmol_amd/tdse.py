@@ -177,9 +177,6 @@ class TDSE:
         assert type(propag) is str, f"propagator 'propag' has bad type: '{type(propag)}', (must be 'str')"
         assert propag in ["external", "internal", "hip"], \
             f"propagator 'propag' has bad value: '{propag}', (use 'external', 'internal')"
-        if propag == "external":
-            raise NotImplementedError("propag='external' (Expokit zhexpv) is not provided; "
-                                      "use the Lanczos propagator ('internal')")
         if "tol" in kwargs:
             assert type(kwargs["tol"]) in [int, float], \
                 f"tolerance `tol` has bad type: '{type(kwargs['tol'])}', (must be 'int', 'float')"
@@ -192,6 +189,8 @@ class TDSE:
         krylov_fixed = int(kwargs.get("krylov_fixed", 0))
         stream = kwargs.get("stream", None)
         operator_form = kwargs.get("operator_form", "auto")
+        if propag == "external":
+            operator_form = "flat"
         if operator_form not in ("auto", "flat", "kron"):
             raise ValueError(f"operator_form has bad value: '{operator_form}' (use 'auto', 'flat', 'kron')")
         precision = kwargs.get("precision", "fp64")
@@ -268,6 +267,27 @@ class TDSE:
         elif plan is None:
             # no field on the operator: only the two H0 phases (tdse.py:375, 377, 394)
             _phase_apply(psi, out, phase, stream)
+        elif propag == "external":
+            # Expokit's ZHEXPV as tdse.py:383-391 / 404-412 call it: Krylov
+            # size 12, t = Im(f), the operator iH; one rm_propagate step
+            fprod, thresh = plan.field_arrays(H)
+            if np.any(thresh > 0):
+                raise ValueError("propag='external' takes no field thresholds on the device")
+            table = torch.from_numpy(np.ascontiguousarray(fprod[None, :], dtype=np.complex128)).to(psi.device)
+            status = torch.zeros(ncols, dtype=torch.int32, device=psi.device)
+            kused = torch.zeros(ncols, dtype=torch.int32, device=psi.device)
+            out.copy_(psi)
+            work = torch.empty_like(psi)
+            plan.set_propagator("external")
+            try:
+                plan.propagate(out, work, table, f, phase=phase, krylov_fixed=int(kwargs.get("krylov_fixed", 12) or 12),
+                               tol=tol, kused=kused, status=status, stream=stream)
+            finally:
+                plan.set_propagator("internal")
+            self.last_krylov_dims = kused
+            if bool((status != 0).any()):
+                from .ensemble import ExpokitError
+                raise ExpokitError("maximum number of steps reached without convergence")
         else:
             status = torch.zeros(ncols, dtype=torch.int32, device=psi.device)
             kused = torch.zeros(ncols, dtype=torch.int32, device=psi.device)
@@ -299,7 +319,9 @@ class TDSE:
         Returns (vecs, t, expval): t the end of the last step (update()'s
         call counter advances by nsteps), expval the complex (nsteps,) sums
         (None without obs).  Runs every step in one rm_propagate_expect call
-        on the flattened operator."""
+        on the flattened operator, or (operator_form as in update(), 'auto'
+        for large bases) one rm_kplan_propagate_expect call on the
+        Kronecker-block form."""
         import torch
         from .ensemble import Ensemble
         if "_time_grid" not in self.__dict__:
@@ -321,6 +343,12 @@ class TDSE:
             assert (h0 - diags(h0.diagonal())).nnz == 0, \
                 "field-free Hamiltonian `H0` is not diagonal -> split-operator approach not implemented"
         on_device = isinstance(vecs, torch.Tensor)
+        n = vecs.shape[1]
+        form = kwargs.get("operator_form", "auto")
+        if form == "kron" or (form == "auto" and n >= KRON_MIN_STATES and _kron_eligible(H)):
+            out, expval = self._run_kron(H, vecs, fields, H0, obs, thresh, kwargs)
+            self._ncalls["update"] = icall + nsteps
+            return (out if on_device else out.cpu().numpy()), t2[icall + nsteps - 1], expval
         ens = Ensemble(H, H0, self.exp_fac, vecs if on_device else np.asarray(vecs),
                        krylov_fixed=int(kwargs.get("krylov_fixed", 0)),
                        maxorder=int(kwargs.get("maxorder", 100)), tol=float(kwargs.get("tol", 1e-15)),
@@ -335,6 +363,55 @@ class TDSE:
         self._ncalls["update"] = icall + nsteps
         out = ens.psi if on_device else ens.psi.cpu().numpy()
         return out, t2[icall + nsteps - 1], expval
+
+    def _run_kron(self, H, vecs, fields, H0, obs, thresh, kwargs):
+        """run() on the Kronecker-block form: per-step slot products from
+        H.field(F) on the host, one rm_kplan_propagate_expect call."""
+        import torch
+        if thresh is not None:
+            raise ValueError("field thresholds are not supported by the Kronecker-block form")
+        H.field(list(fields[0]))
+        kop, dp, perm = _kron_plan(H)
+        fp = []
+        for F in fields:
+            H.field(list(F))
+            fp.append(kop.slot_row([q for _, q, _ in H.terms()]))
+        fp = np.array(fp)
+        src = vecs if isinstance(vecs, torch.Tensor) else torch.from_numpy(
+            np.ascontiguousarray(vecs, dtype=np.complex128)).cuda()
+        if src.shape[1] != kop.n:
+            raise ValueError("wavepacket length does not match the operator's basis")
+        work = torch.empty_like(src)
+        work[:, perm] = src
+        ph = None
+        if H0 is not None:
+            d = torch.from_numpy(np.ascontiguousarray(H0.tomat(form="full", cart="0").diagonal(),
+                                                      dtype=np.complex128)).cuda()
+            phase = torch.empty_like(d)
+            _plan_phase(d, self.exp_fac, phase, kwargs.get("stream"))
+            ph = torch.empty_like(phase)
+            ph[perm] = phase
+        krylov_fixed = int(kwargs.get("krylov_fixed", 0))
+        maxorder = int(kwargs.get("maxorder", 100))
+        status = torch.zeros(src.shape[0], dtype=torch.int32, device=src.device)
+        common = dict(phase=ph, krylov_fixed=krylov_fixed, precision=kwargs.get("precision", "fp64"),
+                      stream=kwargs.get("stream"), maxorder=maxorder, tol=float(kwargs.get("tol", 1e-15)),
+                      status=status)
+        expval = None
+        if obs is None:
+            dp.propagate(work, fp, self.exp_fac, **common)
+        else:
+            obs.field([0.0, 0.0, 1.0])
+            okop, odp, operm = _kron_plan(obs)
+            if okop.Jlist != kop.Jlist or not torch.equal(operm, perm):
+                raise ValueError("the observable's basis differs from the Hamiltonian's")
+            sums = torch.empty(len(fields), dtype=torch.complex128, device=src.device)
+            dp.propagate(work, fp, self.exp_fac, obs=odp,
+                         obs_fprod=okop.slot_row([q for _, q, _ in obs.terms()]), expect_sum=sums, **common)
+            expval = sums.cpu().numpy()
+        if krylov_fixed == 0 and bool((status != 0).any()):
+            raise ValueError(f"Lanczos reached maximum order of '{maxorder}' without convergence")
+        return work[:, perm], expval
 
 
 # 'auto' operator form: bases at least this large with every J carrying all m
@@ -371,8 +448,12 @@ def _kron_plan(H):
     if hit is None:
         kop, _ = KronOperator.from_cartens(H)
         hit = (kop, DeviceKronPlan(kop), torch.from_numpy(kop.perm).cuda())
-        _KRON_CACHE.clear()
+        # two plans at most: an operator and its observable
+        while len(_KRON_CACHE) >= 2:
+            _KRON_CACHE.pop(next(iter(_KRON_CACHE)))
         _KRON_CACHE[key] = hit
+    else:
+        _KRON_CACHE[key] = _KRON_CACHE.pop(key)
     return hit
 
 
