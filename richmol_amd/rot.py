@@ -174,6 +174,65 @@ def rigid_rotor_matrix(J, Bx, By, Bz):
     return H
 
 
+# Watson's centrifugal-distortion terms (rot/solution.py:288-386): the
+# reference's attribute names of the constants per reduction.
+WATSON_A = ("DeltaJ", "DeltaJK", "DeltaK", "deltaJ", "deltaK", "HJ", "HJK", "HKJ", "HK",
+            "phiJ", "phiJK", "phiK")
+WATSON_S = ("DeltaJ", "DeltaJK", "DeltaK", "d1", "d2", "HJ", "HJK", "HKJ", "HK", "h1", "h2", "h3")
+
+
+def _ladder2(J):
+    """J+^2 + J-^2 over k = -J..J: <k+2|.|k> = <k|.|k+2> =
+    sqrt((J(J+1) - k(k+1)) (J(J+1) - (k+1)(k+2))) -- symmetric in the raising
+    and lowering convention of the molecule-fixed J+-, as every Watson term
+    uses it."""
+    ks = np.arange(-J, J + 1, dtype=np.float64)
+    jj = J * (J + 1.0)
+    U = np.zeros((2 * J + 1, 2 * J + 1))
+    if J >= 1:
+        k = ks[:-2]
+        i = np.arange(k.size)
+        U[i + 2, i] = np.sqrt((jj - k * (k + 1)) * (jj - (k + 1) * (k + 2)))
+    return U
+
+
+def watson_matrix(J, reduction, consts):
+    """The distortion terms of Watson's A- or S-reduced Hamiltonian at J over
+    k = -J..J (rot/solution.py:288-386), constants by the reference's names
+    (missing ones are zero):
+      A: -DJ J^4 - DJK J^2 Jz^2 - DK Jz^4 - 1/2 {dJ J^2 + dK Jz^2, J+^2 + J-^2}
+         + HJ J^6 + HJK J^4 Jz^2 + HKJ J^2 Jz^4 + HK Jz^6
+         + 1/2 {phiJ J^4 + phiJK J^2 Jz^2 + phiK Jz^4, J+^2 + J-^2}
+      S: -DJ J^4 - DJK J^2 Jz^2 - DK Jz^4 + d1 J^2 (J+^2 + J-^2) + d2 (J+^4 + J-^4)
+         + HJ J^6 + HJK J^4 Jz^2 + HKJ J^2 Jz^4 + HK Jz^6
+         + h1 J^4 (J+^2 + J-^2) + h2 J^2 (J+^4 + J-^4) + h3 (J+^6 + J-^6)."""
+    names = {"A": WATSON_A, "watson_a": WATSON_A, "S": WATSON_S, "watson_s": WATSON_S}.get(reduction)
+    if names is None:
+        raise ValueError(f"Hamiltonian '{reduction}' is not available (use 'watson_a' or 'watson_s')")
+    unknown = set(consts) - set(names)
+    if unknown:
+        raise ValueError(f"constants {sorted(unknown)} are not terms of the {reduction} reduction")
+    c = {k: float(consts.get(k, 0.0)) for k in names}
+    ks = np.arange(-J, J + 1, dtype=np.float64)
+    j2 = J * (J + 1.0)
+    Z2 = np.diag(ks ** 2)
+    U = _ladder2(J)
+    P2 = U + U.T
+    anti = lambda D, X: 0.5 * (D @ X + X @ D)
+    H = np.diag(-c["DeltaJ"] * j2 ** 2 - c["DeltaJK"] * j2 * ks ** 2 - c["DeltaK"] * ks ** 4
+                + c["HJ"] * j2 ** 3 + c["HJK"] * j2 ** 2 * ks ** 2 + c["HKJ"] * j2 * ks ** 4 + c["HK"] * ks ** 6)
+    if names is WATSON_A:
+        H = H - anti(c["deltaJ"] * j2 * np.eye(ks.size) + c["deltaK"] * Z2, P2)
+        H = H + anti(c["phiJ"] * j2 ** 2 * np.eye(ks.size) + c["phiJK"] * j2 * Z2 + c["phiK"] * Z2 @ Z2, P2)
+    else:
+        U2 = U @ U
+        P4 = U2 + U2.T
+        U3 = U2 @ U
+        P6 = U3 + U3.T
+        H = H + c["d1"] * j2 * P2 + c["d2"] * P4 + c["h1"] * j2 ** 2 * P2 + c["h2"] * j2 * P4 + c["h3"] * P6
+    return H
+
+
 @functools.lru_cache(maxsize=None)
 def _wang(J):
     """Wang functions of J as columns over k = -J..J (K = 0, then K = 1..J
@@ -193,7 +252,8 @@ def _wang(J):
     return W, labels
 
 
-def asymtop_basis(Bx, By, Bz, Jmax, Jmin=0, m_filter=None, J_filter=None, species=None):
+def asymtop_basis(Bx, By, Bz, Jmax, Jmin=0, m_filter=None, J_filter=None, species=None,
+                  watson=None, distortion=None):
     """Rigid asymmetric-top basis (rot/solution.py:160-391, rot/basis.py:702-801):
     per J the eigenstates of Bx Jx^2 + By Jy^2 + Bz Jz^2 [cm^-1], with the
     rotational constants along the molecule-frame axes of the tensors that
@@ -201,33 +261,75 @@ def asymtop_basis(Bx, By, Bz, Jmax, Jmin=0, m_filter=None, J_filter=None, specie
     species (the Hamiltonian couples Delta k = 0, +-2 and commutes with
     k -> -k); a species block is diagonalised on its own and its states are
     stored as coefficient columns over primitive k = -J..J, ordered by energy.
-    species: the D2 species to keep (all four by default)."""
+    species: the D2 species to keep (all four by default).
+    watson: 'watson_a' or 'watson_s' adds Watson's centrifugal-distortion
+    terms with the constants of ``distortion`` (a dict by the reference's
+    names, rot/solution.py:288-386; see watson_matrix): they couple
+    Delta k = 0, +-2, +-4, +-6 and keep the D2 species."""
     jl, ml, kf, en = [], {}, {}, {}
+    blocks = []   # (J, species, Wang columns, species block of H)
     for J in range(Jmin, Jmax + 1):
         if J_filter is not None and not J_filter(J):
             continue
         ms = [m for m in range(-J, J + 1) if m_filter is None or m_filter(J, m)]
         if not ms:
             continue
-        ks = list(range(-J, J + 1))
-        H = rigid_rotor_matrix(J, Bx, By, Bz)
-        # Wang functions (columns over primitive k) and their D2 species
-        W, labels = _wang(J)
-        kj, ej = {}, {}
-        for sym in ("A", "B1", "B2", "B3"):
-            sel = labels == sym
-            if not np.any(sel) or (species is not None and sym not in species):
+        H = None
+        if watson is not None:
+            H = rigid_rotor_matrix(J, Bx, By, Bz) + watson_matrix(J, watson, distortion or {})
+        any_kept = False
+        for sym, Wm, (X, Y, Z) in _species_blocks(J):
+            if species is not None and sym not in species:
                 continue
-            Wm = W[:, sel]
-            e, v = np.linalg.eigh(Wm.T @ H @ Wm)
-            kj[sym] = (ks, (Wm @ v).astype(np.complex128))
-            ej[sym] = e
-        if not kj:   # no kept species at this J
+            # the rigid rotor's species block from the cached blocks of Jx^2, Jy^2, Jz^2
+            Hs = Bx * X + By * Y + Bz * Z if H is None else Wm.T @ H @ Wm
+            blocks.append((J, sym, Wm, Hs))
+            any_kept = True
+        if not any_kept:   # no kept species at this J
             continue
         jl.append(J)
         ml[J] = ms
-        kf[J], en[J] = kj, ej
+        kf[J], en[J] = {}, {}
+    # every species block diagonalised in one batched call: blocks padded to
+    # a common order with decoupled diagonal entries above every block's
+    # Gershgorin bound, so each block's eigenpairs come first in its slice
+    if blocks:
+        nmax = max(b[3].shape[0] for b in blocks)
+        big = max(float(np.abs(b[3]).sum(axis=1).max()) for b in blocks) + 1.0
+        S = np.zeros((len(blocks), nmax, nmax))
+        for i, (_, _, _, Hs) in enumerate(blocks):
+            ns = Hs.shape[0]
+            S[i, :ns, :ns] = Hs
+            pad = np.arange(ns, nmax)
+            S[i, pad, pad] = big * (2.0 + pad)
+        E, V = np.linalg.eigh(S)
+        for i, (J, sym, Wm, Hs) in enumerate(blocks):
+            ns = Hs.shape[0]
+            kf[J][sym] = (list(range(-J, J + 1)), (Wm @ V[i, :ns, :ns]).astype(np.complex128))
+            en[J][sym] = E[i, :ns].copy()
     return RotorBasis(jl, ml, kf, en)
+
+
+@functools.lru_cache(maxsize=None)
+def _species_blocks(J):
+    """Per D2 species of J (in the order A, B1, B2, B3, present ones only):
+    its Wang columns and the species blocks of Jx^2, Jy^2, Jz^2 (cached, read
+    only): the rigid rotor's block is their combination with Bx, By, Bz."""
+    W, labels = _wang(J)
+    X = rigid_rotor_matrix(J, 1.0, 0.0, 0.0)
+    Y = rigid_rotor_matrix(J, 0.0, 1.0, 0.0)
+    Z = rigid_rotor_matrix(J, 0.0, 0.0, 1.0)
+    out = []
+    for sym in ("A", "B1", "B2", "B3"):
+        sel = labels == sym
+        if not np.any(sel):
+            continue
+        Wm = W[:, sel]
+        mats = tuple(Wm.T @ M @ Wm for M in (X, Y, Z))
+        for a in (Wm,) + mats:
+            a.setflags(write=False)
+        out.append((sym, Wm, mats))
+    return tuple(out)
 
 
 @functools.lru_cache(maxsize=None)

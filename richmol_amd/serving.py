@@ -231,31 +231,51 @@ def _m0_positions(bas, jmax):
     return np.concatenate(pos).astype(np.int64)
 
 
-class _AsymModel:
-    """Operator and device plan of one asymmetric-top request: H(F) =
-    F_z^2 A on the invariant subspace of the J = 0 state inside the m = 0
-    block (rot.zfield_m0_operator from block (0, 'A'); the polarizability
-    ASYM_ALPHA unless the request gives its own, -1/2 alpha F F), energies
-    of the rigid rotor with A, B, C along z, x, y.  index: the m = 0 basis
-    position of every subspace state (the J = 0 state is the first)."""
+def _m0_only(J, m):
+    return m == 0
 
-    def __init__(self, req):
+
+def asym_host_build(abc, jmax, alpha=None):
+    """The host part of an asymmetric-top request's model -- everything but
+    the device plan, so that it can run in a builder process while the
+    server's thread admits and launches: H(F) = F_z^2 A on the invariant
+    subspace of the J = 0 state inside the m = 0 block
+    (rot.zfield_m0_operator from block (0, 'A'); the polarizability
+    ASYM_ALPHA unless given, -1/2 alpha F F), the rigid rotor's energies with
+    A, B, C along z, x, y, the m = 0 basis position of every subspace state
+    (the J = 0 state first) and the domain's extent and statistics.
+    Returns plain arrays: ((data, indices, indptr, shape), E, index, lo,
+    len, stats)."""
+    a, b, c = abc
+    alpha = np.diag(ASYM_ALPHA) if alpha is None else np.asarray(alpha, dtype=np.float64)
+    # only the species the J = 0 state reaches are diagonalised and coupled
+    keep = _reachable_species(alpha)
+    bas = rot.asymtop_basis(b, c, a, jmax, m_filter=_m0_only, species=keep)
+    A0, E, sub = rot.zfield_m0_operator(bas, alpha, start=(0, "A"))
+    index = _m0_positions(bas, jmax)[sub]
+    H = (A0 * (-0.5 * units.AUpol_x_Vm_to_invcm())).tocsr()
+    H.sort_indices()
+    n = H.shape[0]
+    lo, ln = _domain(H, n)
+    return (H.data, H.indices, H.indptr, H.shape), E, index, lo, ln, _domain_stats(H, n)
+
+
+class _AsymModel:
+    """Operator and device plan of one asymmetric-top request (the host part
+    from asym_host_build, given or computed here)."""
+
+    def __init__(self, req, built=None):
+        from scipy.sparse import csr_matrix
         from .plan import DevicePlan
-        a, b, c = req.abc
-        alpha = np.diag(ASYM_ALPHA) if req.alpha is None else np.asarray(req.alpha, dtype=np.float64)
-        # only the species the J = 0 state reaches are diagonalised and coupled
-        keep = _reachable_species(alpha)
-        bas = rot.asymtop_basis(b, c, a, req.jmax, m_filter=lambda J, m: m == 0, species=keep)
-        A0, self.E, sub = rot.zfield_m0_operator(bas, alpha, start=(0, "A"))
-        self.index = _m0_positions(bas, req.jmax)[sub]
+        if built is None:
+            built = asym_host_build(req.abc, req.jmax, req.alpha)
+        (data, indices, indptr, shape), self.E, self.index, self.lo, self.len, self.stats = built
+        self.H = csr_matrix((data, indices, indptr), shape=shape)
         self.nfull = req.n
-        self.H = A0 * (-0.5 * units.AUpol_x_Vm_to_invcm())
-        self.n = A0.shape[0]
+        self.n = self.H.shape[0]
         self.plan = DevicePlan.from_matrix(self.H, "zz")
         self.plan.subspaces()   # the subspace arrays now, during the running tick
         self.nslot = 1
-        self.lo, self.len = _domain(self.H, self.n)
-        self.stats = _domain_stats(self.H, self.n)
 
     def energies(self, req):
         return self.E
@@ -279,11 +299,13 @@ class Server:
     by expensive ones (a result does not depend on how its steps are split);
     krylov_fixed: fixed Krylov dimension (0: adaptive with tol, maxorder);
     block_rows: jobs on domains longer than this run one workgroup each
-    (default: only those longer than a wave holds).
+    (default: only those longer than a wave holds);
+    builders: processes that build asymmetric tops' operators on the host
+    from their arrival on (0: built at admission, on the server's thread).
     """
 
     def __init__(self, hbm_budget=8 << 30, max_jobs=8192, chunk=64, krylov_fixed=12, maxorder=100,
-                 tol=1e-15, precision="fp64", jmax_master=60, block_rows=64, chunk_max=4096):
+                 tol=1e-15, precision="fp64", jmax_master=60, block_rows=64, chunk_max=4096, builders=0):
         import torch
         self.torch = torch
         self.lib = _lib.load()
@@ -307,13 +329,24 @@ class Server:
         self.master = rot.tensor(np.diag(LINEAR_ALPHA), bas)
         self.models = {}
         self.used = 0
+        self.pool = None
+        if builders > 0:
+            import multiprocessing
+            from concurrent.futures import ProcessPoolExecutor
+            # spawned (not forked) workers: they never touch the device
+            self.pool = ProcessPoolExecutor(max_workers=int(builders),
+                                            mp_context=multiprocessing.get_context("spawn"))
 
     def close(self):
-        """Destroy the device batch object (idempotent)."""
+        """Destroy the device batch object and stop the builders (idempotent)."""
         h = getattr(self, "handle", None)
         if h is not None and h.value:
             self.lib.rm_batch_destroy(h)
         self.handle = None
+        pool = getattr(self, "pool", None)
+        if pool is not None:
+            pool.shutdown(wait=True, cancel_futures=True)
+        self.pool = None
 
     def __del__(self):
         try:
@@ -363,7 +396,9 @@ class Server:
         if req.kind == "linear":
             m = self.model(req.jmax)
         else:
-            m = _AsymModel(req)
+            fut = getattr(req, "build", None)
+            m = _AsymModel(req, built=fut.result() if fut is not None else None)
+            req.build = None
             m.cost = self.step_cost(m)
         dev = "cuda"
         E = m.energies(req)
@@ -424,7 +459,11 @@ class Server:
             nonlocal jobs
             changed = False
             while pending and pending[0].arrival <= now:
-                queue.append(pending.popleft())
+                r = pending.popleft()
+                if self.pool is not None and r.kind != "linear":
+                    # the operator is built while the request waits for a slot
+                    r.build = self.pool.submit(asym_host_build, r.abc, r.jmax, r.alpha)
+                queue.append(r)
             while queue and len(active) < self.max_jobs and self.used + self.footprint(queue[0]) <= self.budget:
                 if busy is not None and changed and busy.query():
                     break
@@ -524,9 +563,11 @@ def summarize(done, wall):
             "latency_max_ms": float(lat.max() * 1e3), "mean_steps": steps / len(done)}
 
 
-def benchmark(n_requests=200, seed=0, load=0.8, chunk=64, asym_fraction=0.5, precision="fp32", **kw):
-    """Closed-loop capacity, then a Poisson stream at ``load`` x capacity."""
-    srv = Server(chunk=chunk, precision=precision, **kw)
+def benchmark(n_requests=200, seed=0, load=0.8, chunk=64, asym_fraction=0.5, precision="fp32", builders=8,
+              **kw):
+    """Closed-loop capacity, then a Poisson stream at ``load`` x capacity;
+    asymmetric tops' operators built by ``builders`` host processes."""
+    srv = Server(chunk=chunk, precision=precision, builders=builders, **kw)
     reqs = make_requests(n_requests, seed=seed, asym_fraction=asym_fraction)
     for j in sorted({r.jmax for r in reqs if r.kind == "linear"}):   # linear operators (shared per Jmax)
         srv.model(j)
@@ -549,5 +590,7 @@ def benchmark(n_requests=200, seed=0, load=0.8, chunk=64, asym_fraction=0.5, pre
                 "stream": f"linear rotors / asymmetric tops ({asym_fraction:.0%} asym), rotational constants "
                           "U(0.05,30) cm^-1, Jmax U{10..60}, FWHM U(100,1000) fs, I log-U(1e10,1e12) W/cm^2, "
                           "200-2000 steps of 10 fs, Z pulse, psi0 = J=0 state, "
-                          f"Poisson arrivals at {load} x closed-loop capacity (seeded)"})
+                          f"Poisson arrivals at {load} x closed-loop capacity (seeded)",
+                "host_builders": builders})
+    srv.close()
     return rep
