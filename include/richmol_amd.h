@@ -438,6 +438,24 @@ int rm_kplan_propagate_expect(rm_kplan* plan, void* psi, int64_t ld, int32_t nco
                               int32_t* status, int32_t precision, rm_kplan* obs, const double* obs_fprod,
                               void* expect, void* expect_sum, void* stream);
 
+/*
+ * Rotational densities of wavepackets on a product grid of Euler angles (the
+ * at-scale use of richmol/rotdens.py:10-283): with a^J_mk the wavepacket's
+ * coefficients over the primitive symmetric-top functions (including
+ * sqrt((2J+1)/(8 pi^2))),
+ *   psi(alpha, beta, gamma) = sum_J sum_mk a^J_mk e^{i m alpha} d^J_mk(beta) e^{i k gamma},
+ * rho = |psi|^2 sin(beta).  m, k run over -jmax..jmax (M = 2 jmax + 1):
+ *   coef:   device complex [nwp][jmax+1][M][M] (zero where |m| or |k| > J);
+ *   ealpha: device complex [na][M], e^{i m alpha_a};
+ *   dtab:   device double [jmax+1][nb][M][M], d^J_mk(beta_b);
+ *   sinb:   device double [nb];  egamma: device complex [ng][M], e^{i k gamma_g};
+ *   rho:    device double, [na][nb][ng] summed over the wavepackets (the
+ *           ensemble's density; per_wavepacket 0) or [nwp][na][nb][ng].
+ */
+int rm_rotdens(int32_t jmax, int32_t nwp, const void* coef, int32_t na, const void* ealpha, int32_t nb,
+               const double* dtab, const double* sinb, int32_t ng, const void* egamma,
+               int32_t per_wavepacket, void* rho, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -10,7 +10,8 @@ SOURCES = [os.path.join(PKG_DIR, "csrc", "rm_tdse.hip"),
            os.path.join(PKG_DIR, "csrc", "rm_subspace.hip"),
            os.path.join(PKG_DIR, "csrc", "rm_coop.hip"),
            os.path.join(PKG_DIR, "csrc", "rm_kron.hip"),
-           os.path.join(PKG_DIR, "csrc", "rm_expokit.hip")]
+           os.path.join(PKG_DIR, "csrc", "rm_expokit.hip"),
+           os.path.join(PKG_DIR, "csrc", "rm_rotdens.hip")]
 
 HIPCC_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared",

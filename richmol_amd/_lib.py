@@ -47,6 +47,8 @@ SIGNATURES = {
                                            ctypes.c_int32, ctypes.c_int32, ctypes.c_double, _vp, _vp,
                                            _vp, _vp, _vp, _vp]),
     "rm_plan_set_propagator": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "rm_rotdens": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, _vp, ctypes.c_int32, _vp, ctypes.c_int32,
+                                  _vp, _vp, ctypes.c_int32, _vp, ctypes.c_int32, _vp, _vp]),
     "rm_plan_timing": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "rm_plan_kernel_time": (ctypes.c_int, [_vp, _c_dblp, _c_i64p]),
     "rm_plan_set_kernel": (ctypes.c_int, [_vp, ctypes.c_int32]),

@@ -77,7 +77,7 @@ class Request:
     diag(ASYM_ALPHA))."""
 
     __slots__ = ("rid", "arrival", "kind", "B", "abc", "alpha", "jmax", "fwhm", "intensity", "nsteps", "dt",
-                 "t_admit", "t_done", "psi", "steps_done", "kused", "index", "cumfield")
+                 "t_admit", "t_done", "psi", "steps_done", "kused", "index", "cumfield", "build")
 
     def __init__(self, rid, arrival, B, jmax, fwhm, intensity, nsteps, kind="linear", dt=DT_FS, abc=None,
                  alpha=None):
@@ -86,6 +86,7 @@ class Request:
         if kind == "asym" and (abc is None or len(abc) != 3):
             raise ValueError("an asymmetric-top request needs abc = (A, B, C)")
         self.rid, self.arrival, self.kind = rid, float(arrival), kind
+        self.build = None   # future of the operator build (a server with builder processes)
         self.B, self.jmax, self.fwhm, self.intensity = float(B), int(jmax), float(fwhm), float(intensity)
         self.abc = None if abc is None else tuple(float(x) for x in abc)
         # molecular-frame polarizability of an asymmetric top (default diag(ASYM_ALPHA))
@@ -396,7 +397,7 @@ class Server:
         if req.kind == "linear":
             m = self.model(req.jmax)
         else:
-            fut = getattr(req, "build", None)
+            fut = req.build
             m = _AsymModel(req, built=fut.result() if fut is not None else None)
             req.build = None
             m.cost = self.step_cost(m)
