@@ -59,9 +59,13 @@ class CarTens:
     def __init__(self, filename=None, name=None, **kwargs):
         self._uid = next(_uid)
         if filename is not None:
-            raise NotImplementedError(
-                "reading CarTens HDF5 files needs h5py, which this package does not use; "
-                "build the tensor with richmol_amd.rot or richmol_amd.trove instead")
+            from . import h5
+            if h5.is_hdf5(filename):
+                # read tensor from an HDF5 file (field.py:173-184)
+                if name is None:
+                    raise ValueError(
+                        f"please specify the name of the data group in HDF5 file '{filename}'") from None
+                self.read(filename, name=name, **kwargs)
 
     # ------------------------------------------------------------------ state
     def _touch(self):
@@ -155,6 +159,230 @@ class CarTens:
         x = flatten(self, vec, side=2)
         y = _plan.apply(self, x[None, :])[0]
         return unflatten(self, y, side=1)
+
+    # ------------------------------------------------------ field.py:187-401
+    def filter(self, thresh=None, bra=lambda **kw: True, ket=lambda **kw: True):
+        """State selection: keeps the bra (ket) states for which ``bra(J=,
+        sym=, k=, enr=)`` and ``bra(J=, sym=, m=)`` hold, drops (J, symmetry)
+        blocks left empty, and cuts the K and M tensors to what is kept (with
+        |element| < ``thresh`` removed).  In place; new K/M trees, so copies
+        that share the old ones are not changed."""
+        if self.is_sum:
+            raise AttributeError("a sum of tensors has no M tensor to filter "
+                                 "(field.py:1012-1015 deletes it)")
+        for side, flt in ((1, bra), (2, ket)):
+            Jl, sl = getattr(self, f"Jlist{side}"), getattr(self, f"symlist{side}")
+            qk, qm = getattr(self, f"quanta_k{side}"), getattr(self, f"quanta_m{side}")
+            ik = {J: {s: [i for i, (q, e) in enumerate(qk[J][s]) if flt(J=J, sym=s, k=q, enr=e)]
+                      for s in sl[J]} for J in Jl}
+            im = {J: {s: [i for i, q in enumerate(qm[J][s]) if flt(J=J, sym=s, m=q)]
+                      for s in sl[J]} for J in Jl}
+            qk = {J: {s: [qk[J][s][i] for i in ik[J][s]] for s in sl[J]} for J in Jl}
+            qm = {J: {s: [qm[J][s][i] for i in im[J][s]] for s in sl[J]} for J in Jl}
+            # a (J, symmetry) block stays when it keeps k and m states
+            keep = {J: [s for s in sl[J] if ik[J][s] and im[J][s]] for J in Jl}
+            Jl = sorted(J for J in Jl if keep[J])
+            sl = {J: sorted(keep[J]) for J in Jl}
+            sub = lambda d: {J: {s: d[J][s] for s in sl[J]} for J in Jl}   # noqa: E731
+            ik, im, qk, qm = sub(ik), sub(im), sub(qk), sub(qm)
+            setattr(self, f"Jlist{side}", Jl)
+            setattr(self, f"symlist{side}", sl)
+            setattr(self, f"ind_k{side}", ik)
+            setattr(self, f"ind_m{side}", im)
+            setattr(self, f"quanta_k{side}", qk)
+            setattr(self, f"quanta_m{side}", qm)
+            setattr(self, f"dim_k{side}", {J: {s: len(ik[J][s]) for s in sl[J]} for J in Jl})
+            setattr(self, f"dim_m{side}", {J: {s: len(im[J][s]) for s in sl[J]} for J in Jl})
+            setattr(self, f"dim{side}", {J: {s: len(ik[J][s]) * len(im[J][s]) for s in sl[J]} for J in Jl})
+
+        def cut(mat, rows, cols):
+            m = mat.tocsr()[rows, :].tocsc()[:, cols].tocsr()
+            if thresh is not None:
+                m.data[np.abs(m.data) < thresh] = 0
+                m.eliminate_zeros()
+            return m
+
+        kmat, mmat = {}, {}
+        for Jp in set(self.mmat.keys()) & set(self.kmat.keys()):
+            J1, J2 = Jp
+            if J1 not in self.Jlist1 or J2 not in self.Jlist2:
+                continue
+            for sp in set(self.mmat[Jp].keys()) & set(self.kmat[Jp].keys()):
+                s1, s2 = sp
+                if s1 not in self.symlist1[J1] or s2 not in self.symlist2[J2]:
+                    continue
+                ik1, ik2 = self.ind_k1[J1][s1], self.ind_k2[J2][s2]
+                im1, im2 = self.ind_m1[J1][s1], self.ind_m2[J2][s2]
+                kd, md = {}, {}
+                for w in set(self.kmat[Jp][sp].keys()) & set(self.mmat[Jp][sp].keys()):
+                    k = cut(self.kmat[Jp][sp][w], ik1, ik2)
+                    m = {c: cut(v, im1, im2) for c, v in self.mmat[Jp][sp][w].items()}
+                    m = {c: v for c, v in m.items() if v.nnz > 0}
+                    if k.nnz > 0 and m:
+                        kd[w], md[w] = k, m
+                if kd:
+                    kmat.setdefault(Jp, {})[sp] = kd
+                    mmat.setdefault(Jp, {})[sp] = md
+        self.kmat, self.mmat = kmat, mmat
+        self._touch()
+
+    # ---------------------------------------------------- field.py:1304-1848
+    def class_name(self):
+        """Type tag of the tensor's HDF5 group: the reference's class path
+        (``richmol.field.CarTens`` for this class), so that each package reads
+        the other's files (field.py:1845-1849, 1633-1638)."""
+        mod = self.__module__.split(".")
+        return ".".join(["richmol"] + mod[1:] + [self.__class__.__name__])
+
+    def store(self, filename, name=None, comment=None, replace=False, replace_k=False, replace_m=False,
+              store_k=True, store_m=True, thresh=None, store_basis=True):
+        """Write the tensor into the HDF5 group ``name`` of ``filename``
+        (field.py:1304-1597): the basis attributes (natively, or as JSON
+        under ``<name>__json``), and per coupled (J1, J2) and (sym1, sym2)
+        the K and M tensors as concatenated CSR arrays with their split
+        sizes.  Elements below ``thresh`` are not written (the tensor itself
+        is not changed).  ``store_basis`` has no effect: this package's
+        tensors carry no symmetric-top basis group."""
+        from . import h5
+        if self.is_sum:
+            raise AttributeError("a sum of tensors has no M tensor to store "
+                                 "(field.py:1012-1015 deletes it)")
+        if name is None:
+            name = _retrieve_name(self)
+        with h5.File(filename, "a") as fl:
+            if name in fl and replace:
+                del fl[name]
+            group = fl.require_group(name)
+            group.attrs["__class_name__"] = self.class_name()
+            doc = group.attrs["__doc__"] if "__doc__" in group.attrs else "Cartesian tensor operator"
+            import datetime
+            doc += ", store date: " + datetime.datetime.now().strftime("%Y-%m-%d %H:%M:%S")
+            if comment is not None:
+                doc += ", comment: " + " ".join(comment.split())
+            group.attrs["__doc__"] = doc
+            exclude = {"mmat", "kmat", "molecule", "basis", "eigvec", "symtop_basis"}
+            exclude |= set(getattr(self, "store_exclude", ()))
+            for attr, val in self.__dict__.items():
+                if attr in exclude or attr.startswith("_"):
+                    continue
+                try:
+                    group.attrs[attr] = val
+                except TypeError:
+                    group.attrs[attr + "__json"] = h5.json_dumps(val)
+
+            def drop(m):
+                m = m.tocsr(copy=True)
+                if thresh is not None:
+                    m.data[np.abs(m.data) < thresh] = 0
+                    m.eliminate_zeros()
+                return m
+
+            def put(gsym, what, mats, keys, replace_):
+                if replace_:
+                    for key in ("_data", "_indices", "_indptr"):
+                        if what + key in gsym:
+                            del gsym[what + key]
+                if any(what + key in gsym for key in ("_data", "_indices", "_indptr")):
+                    raise RuntimeError(
+                        f"found existing {what[0].upper()}-tensor dataset for `(J1, J2)` = '{(J1, J2)}' and "
+                        f"`(sym1, sym2)` = '{(s1, s2)}'  in file '{filename}', use `replace_{what[0]}` = True "
+                        f"to replace {what[0].upper()}-tensor datasets") from None
+                gsym.create_dataset(what + "_data", np.concatenate([m.data for m in mats]).astype(np.complex128))
+                gsym.create_dataset(what + "_indices", np.concatenate([m.indices for m in mats]).astype(np.int32))
+                gsym.create_dataset(what + "_indptr", np.concatenate([m.indptr for m in mats]).astype(np.int32))
+                gsym.attrs[what + "_nnz"] = np.array([len(m.data) for m in mats], dtype=np.int64)
+                gsym.attrs[what + "_nind"] = np.array([len(m.indices) for m in mats], dtype=np.int64)
+                gsym.attrs[what + "_nptr"] = np.array([len(m.indptr) for m in mats], dtype=np.int64)
+                gsym.attrs[what + "_shape"] = np.array([m.shape for m in mats], dtype=np.int64)
+                gsym.attrs[what + ("_irreps" if what == "kmat" else "_irrep_cart")] = keys
+
+            for (J1, J2) in sorted(set(self.mmat.keys()) & set(self.kmat.keys())):
+                for (s1, s2) in sorted(set(self.mmat[(J1, J2)].keys()) & set(self.kmat[(J1, J2)].keys())):
+                    gsym = None
+                    if store_k:
+                        kd = {w: drop(m) for w, m in self.kmat[(J1, J2)][(s1, s2)].items()}
+                        kd = {w: m for w, m in kd.items() if m.nnz > 0}
+                        if kd:
+                            gsym = group.require_group(J_group_key(J1, J2)).require_group(sym_group_key(s1, s2))
+                            put(gsym, "kmat", list(kd.values()), np.array(list(kd.keys()), dtype=np.int64),
+                                replace_k)
+                    if store_m:
+                        md = {(w, c): drop(m) for w, mw in self.mmat[(J1, J2)][(s1, s2)].items()
+                              for c, m in mw.items()}
+                        md = {key: m for key, m in md.items() if m.nnz > 0}
+                        if md:
+                            gsym = gsym or group.require_group(J_group_key(J1, J2)).require_group(
+                                sym_group_key(s1, s2))
+                            put(gsym, "mmat", list(md.values()),
+                                np.array([(str(w), c) for w, c in md.keys()], dtype="S"), replace_m)
+
+    def read(self, filename, name=None, thresh=None, **kwargs):
+        """Read the tensor of the HDF5 group ``name`` (field.py:1600-1842):
+        attributes first, then the state filters ``bra``/``ket`` of kwargs,
+        then the K and M tensors of the kept states (elements below
+        ``thresh`` dropped).  J quanta that are whole numbers come back as
+        ints; the symmetric-top basis, eigenvector and rotational-density
+        groups the reference may also hold are not read (this package builds
+        densities from its own basis)."""
+        import re
+        from . import h5
+        if name is None:
+            name = _retrieve_name(self)
+        with h5.File(filename, "r") as fl:
+            if name not in fl:
+                raise KeyError(f"file '{filename}' has no dataset with the name '{name}'") from None
+            group = fl[name]
+            cls = group.attrs["__class_name__"]
+            if cls != self.class_name():
+                raise TypeError(f"dataset with the name '{name}' in file '{filename}' "
+                                f"has different type: '{cls}'") from None
+            attrs = {}
+            for key, val in group.attrs.items():
+                if key.find("__json") == -1:
+                    attrs[key] = val
+                else:
+                    attrs[key.replace("__json", "")] = h5.json_loads(val)
+            self.__dict__.update(_normalise_attrs(attrs))
+            self.kmat, self.mmat = {}, {}
+            self.filter(**kwargs)
+            Jre = re.compile(r"^J:(\d+\.\d+),(\d+\.\d+)$")
+            sre = re.compile(r"^sym:(\w+),(\w+)$")
+            for key in group.keys():
+                mj = Jre.match(key)
+                if not mj:
+                    continue
+                J1, J2 = (_jnum(round(float(x), 1)) for x in mj.groups())
+                if J1 not in self.Jlist1 or J2 not in self.Jlist2:
+                    continue
+                gj = group[key]
+                for key2 in gj.keys():
+                    ms = sre.match(key2)
+                    if not ms:
+                        continue
+                    s1, s2 = ms.groups()
+                    if s1 not in self.symlist1[J1] or s2 not in self.symlist2[J2]:
+                        continue
+                    gs = gj[key2]
+                    ik1, ik2 = self.ind_k1[J1][s1], self.ind_k2[J2][s2]
+                    im1, im2 = self.ind_m1[J1][s1], self.ind_m2[J2][s2]
+                    kd = _read_csr(gs, "kmat")
+                    if kd is not None:
+                        mats, keys = kd
+                        kd = {int(w): _sel(m, ik1, ik2, thresh) for w, m in zip(keys, mats)}
+                        kd = {w: m for w, m in kd.items() if m.nnz > 0}
+                        if kd:
+                            self.kmat.setdefault((J1, J2), {})[(s1, s2)] = kd
+                    md = _read_csr(gs, "mmat")
+                    if md is not None:
+                        mats, keys = md
+                        out = {}
+                        for (w, c), m in zip(keys, mats):
+                            m = _sel(m, im1, im2, thresh)
+                            if m.nnz > 0:
+                                out.setdefault(int(w), {})[c] = m
+                        if out:
+                            self.mmat.setdefault((J1, J2), {})[(s1, s2)] = out
+        self.filter(thresh=thresh, **kwargs)
 
     # ------------------------------------------------ field.py:449-569, 659-692
     def tomat(self, form="block", repres="csr_matrix", thresh=None, cart=None):
@@ -284,6 +512,97 @@ class CarTens:
 
     __rmul__ = __mul__
     __radd__ = __add__
+
+
+def filter(obj, bra, ket, thresh=None):
+    """Copy of a tensor with the state selection applied (field.py:1859-1883)."""
+    if not isinstance(obj, CarTens):
+        raise TypeError(f"bad argument type for 'obj': '{type(obj)}'") from None
+    res = copy.deepcopy(obj)
+    res.filter(bra=bra, ket=ket, thresh=thresh)
+    return res
+
+
+def J_group_key(J1, J2):
+    """HDF5 group of the (J1, J2) block (field.py:1886-1890)."""
+    return "J:" + str(round(float(J1), 1)) + "," + str(round(float(J2), 1))
+
+
+def sym_group_key(sym1, sym2):
+    """HDF5 group of the (sym1, sym2) block (field.py:1893-1897)."""
+    return "sym:" + str(sym1) + "," + str(sym2)
+
+
+def _retrieve_name(var):
+    """Name a variable bound to ``var`` has in the calling frames, outermost
+    first (the reference's default group name)."""
+    import inspect
+    for fi in reversed(inspect.stack()):
+        for n, v in fi.frame.f_locals.items():
+            if v is var and not n.startswith("_") and n != "self":
+                return n
+    raise ValueError("no name for the tensor: pass `name`")
+
+
+def _jnum(J):
+    """J quantum as an int when it is a whole number."""
+    f = float(J)
+    return int(f) if f.is_integer() else round(f, 1)
+
+
+_JKEYED = ("symlist1", "symlist2", "dim1", "dim2", "dim_k1", "dim_k2", "dim_m1", "dim_m2",
+           "ind_k1", "ind_k2", "ind_m1", "ind_m2", "quanta_k1", "quanta_k2", "quanta_m1", "quanta_m2")
+
+
+def _normalise_attrs(attrs):
+    """Attributes as read from a file in the types the tensor code uses:
+    J quanta and J keys as ints when whole, lists for arrays of labels,
+    (assignment, energy) pairs as tuples."""
+    out = dict(attrs)
+    for k in ("Jlist1", "Jlist2"):
+        if k in out:
+            out[k] = [_jnum(J) for J in np.atleast_1d(out[k])]
+    for k in _JKEYED:
+        if k in out and isinstance(out[k], dict):
+            out[k] = {_jnum(J): dict(v) if isinstance(v, dict) else v for J, v in out[k].items()}
+    for k in ("quanta_k1", "quanta_k2"):
+        if k in out:
+            out[k] = {J: {s: [tuple(q) for q in v] for s, v in d.items()} for J, d in out[k].items()}
+    if "cart" in out:
+        out["cart"] = [str(c) for c in np.atleast_1d(out["cart"])]
+    if "os" in out:
+        out["os"] = [tuple(int(x) for x in r) for r in np.atleast_2d(out["os"])]
+    if "rank" in out:
+        out["rank"] = int(out["rank"])
+    return out
+
+
+def _read_csr(gs, what):
+    """CSR matrices and their keys of one block (field.py:1745-1835)."""
+    if what + "_nnz" not in gs.attrs or what + "_data" not in gs:
+        return None
+    nnz, nind, nptr = (np.asarray(gs.attrs[what + k]).ravel() for k in ("_nnz", "_nind", "_nptr"))
+    shape = np.asarray(gs.attrs[what + "_shape"]).reshape(-1, 2)
+    data = np.split(np.asarray(gs[what + "_data"].read()), np.cumsum(nnz))[:-1]
+    ind = np.split(np.asarray(gs[what + "_indices"].read()), np.cumsum(nind))[:-1]
+    ptr = np.split(np.asarray(gs[what + "_indptr"].read()), np.cumsum(nptr))[:-1]
+    mats = [csr_matrix((d.astype(np.complex128), i, p), shape=tuple(int(x) for x in sh))
+            for d, i, p, sh in zip(data, ind, ptr, shape)]
+    if what == "kmat":
+        keys = [int(w) for w in np.atleast_1d(gs.attrs["kmat_irreps"])]
+    else:
+        raw = np.asarray(gs.attrs["mmat_irrep_cart"]).reshape(-1, 2)
+        dec = lambda x: x.decode("utf-8") if isinstance(x, bytes) else str(x)   # noqa: E731
+        keys = [(int(dec(w)), dec(c)) for w, c in raw]
+    return mats, keys
+
+
+def _sel(m, rows, cols, thresh):
+    m = m[rows, :].tocsc()[:, cols].tocsr()
+    if thresh is not None:
+        m.data[np.abs(m.data) < thresh] = 0
+        m.eliminate_zeros()
+    return m
 
 
 def _first_cart(t):
