@@ -48,7 +48,7 @@ ACCURACY_STEPS = int(round(2 * T0_FS / DT_FS)) + 1
 # at two step counts: bytes = fixed + per_step * steps.  Key: (kernel, jmax,
 # wavepackets, Krylov dimension).
 TRAFFIC_MODEL = {
-    ("pair", 40, 512, 12): (2.0084e6, 159.0, "profiles/k_sub_pair_linear_j40_b512_m12.md"),
+    ("pair", 40, 512, 12): (2.515e6, 13.2e3, "profiles/k_sub_pair_linear_j40_b512_m12.md"),
 }
 MEASURED_TRAFFIC_FULL = {
     ("ell", 40, 512, 12): (351.1e6, "profiles/k_lanczos_ell_linear_j40_b512_m12_pmc.md"),
@@ -557,25 +557,36 @@ def main():
             "gate": 1e-6,
             "note": "complex64 operator and gathered vector, FP32 accumulation; Krylov recurrence, "
                     "norms, exp(fT), phases and the wavepacket in FP64"}
-        # ---- plain single-wavepacket FP64 path: one TDSE.update call per step
+        # ---- plain single-wavepacket FP64 path: one TDSE.update call per step,
+        # the default (one-step rm_propagate on the subspace kernels) and the
+        # full-vector rm_step
         from richmol_amd.tdse import TDSE
         nsw = min(60, args.steps)
-        td = TDSE(t_end=DT_FS * (nst + 1), dt=DT_FS, t_units="fs")
-        td.time_grid()
-        v = torch.from_numpy(psi0[:1].copy()).to(dev)
-        for F in warm:
-            H.field(F)
-            v, _ = td.update(H, v, H0=h0, krylov_fixed=m)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for F in timed[:nsw]:
-            H.field(F)
-            v, _ = td.update(H, v, H0=h0, krylov_fixed=m)
-        torch.cuda.synchronize()
-        single = nsw / (time.perf_counter() - t0)
+
+        def per_step(kernel):
+            td = TDSE(t_end=DT_FS * (nst + 1), dt=DT_FS, t_units="fs")
+            td.time_grid()
+            v = torch.from_numpy(psi0[:1].copy()).to(dev)
+            for F in warm:
+                H.field(F)
+                v, _ = td.update(H, v, H0=h0, krylov_fixed=m, kernel=kernel)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for F in timed[:nsw]:
+                H.field(F)
+                v, _ = td.update(H, v, H0=h0, krylov_fixed=m, kernel=kernel)
+            torch.cuda.synchronize()
+            return nsw / (time.perf_counter() - t0), v
+
+        single, v_sub = per_step("auto")
+        single_ell, v_ell = per_step("ell")
         result["single_wavepacket_fp64"] = {
             "value": single, "steps": nsw,
-            "path": "TDSE.update per step, one wavepacket, FP64 (host loop, rm_step)",
+            "path": "TDSE.update per step, one wavepacket, FP64 (host loop; one-step rm_propagate on the "
+                    "subspace kernels, domains kept between calls)",
+            "value_rm_step": single_ell,
+            "path_rm_step": "the same loop with kernel='ell' (rm_step, full-vector ELL kernel)",
+            "max_abs_diff_paths": float((v_sub - v_ell).abs().max()),
             "speedup_of_value": value / single}
         print(json.dumps({"progress": "extras"}), file=sys.stderr, flush=True)
 

@@ -49,7 +49,7 @@ typedef struct rm_plan rm_plan;
 const char* rm_last_error(void);
 
 /* Library ABI version (RM_ABI_VERSION of the header it was built from). */
-#define RM_ABI_VERSION 4
+#define RM_ABI_VERSION 5
 int rm_version(void);
 
 /*
@@ -227,6 +227,27 @@ int rm_plan_kernel_time(rm_plan* plan, double* total_ms, int64_t* launches);
  * (rm_ell_config() describes it); the autotuner's search space. */
 #define RM_KERNEL_ELL_CFG0  16
 int rm_plan_set_kernel(rm_plan* plan, int32_t variant);
+
+/* Launch choices of rm_propagate's invariant-subspace path, for an autotuner
+ * over the production kernels (no counterpart in richmol, whose step has one
+ * code path; callers that do not tune leave the defaults):
+ *  RM_TUNE_PAIR      1 (default): two wavepackets per wave on domains of
+ *                    <= 32 rows under a fixed Krylov dimension <= 16;
+ *                    0: one wave each;
+ *  RM_TUNE_COOP_MAX  long-domain wavepackets per call up to which the
+ *                    cooperative form runs (default 8; more go one workgroup
+ *                    each);
+ *  RM_TUNE_COOP_GRID workgroups of a cooperative launch at most (default 0:
+ *                    one per 64-row group of the domain, up to one per CU);
+ *  RM_TUNE_FORK      1 (default): the launch classes of a call side by side
+ *                    on auxiliary streams; 0: one after another.
+ * Results do not depend on these choices beyond summation order.  A change
+ * drops the launch lists kept for domain reuse. */
+#define RM_TUNE_PAIR       0
+#define RM_TUNE_COOP_MAX   1
+#define RM_TUNE_COOP_GRID  2
+#define RM_TUNE_FORK       3
+int rm_plan_set_tuning(rm_plan* plan, int32_t knob, int32_t value);
 
 /* Launch configuration i of the ELL kernel: threads per workgroup, rows per
  * thread, wavepackets per workgroup, operator staged in LDS (1/0).

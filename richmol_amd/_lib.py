@@ -14,6 +14,9 @@ RM_OK = 0
 RM_NOT_CONVERGED = 1
 RM_DEVICE_TIMEOUT = 2
 RM_STALE_DOMAIN = 4
+# rm_plan_set_tuning knobs and their defaults
+TUNE_KNOBS = {"pair": 0, "coop_max": 1, "coop_grid": 2, "fork": 3}
+TUNE_DEFAULTS = {"pair": 1, "coop_max": 8, "coop_grid": 0, "fork": 1}
 RM_MAX_FIELD = 32
 RM_MAX_LEAF = 8
 RM_MAX_KRYLOV = 128
@@ -58,6 +61,7 @@ SIGNATURES = {
     "rm_plan_subspaces": (ctypes.c_int, [_vp, _c_i64p, _c_i64p]),
     "rm_plan_set_precision": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "rm_plan_set_domain_reuse": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "rm_plan_set_tuning": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32]),
     "rm_plan_set_coop_abort": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "rm_plan_domains": (ctypes.c_int, [_vp, _vp, ctypes.c_int64, ctypes.c_int64, _c_i32p, _vp]),
     "rm_plan_step_flags": (ctypes.c_int, [_vp, ctypes.c_int64, _vp, _vp, _vp]),

@@ -1,13 +1,24 @@
-"""Launch-shape autotuner for the Krylov step kernel.
+"""Launch autotuner for the propagation kernels.
 
-The ELL kernel has 12 launch configurations (threads per workgroup, rows per
-thread, wavepackets per workgroup, operator staged in LDS or read from L2;
-``plan.ell_configs()``).  Which is fastest depends on the basis size N, the
-ELL width, the number of wavepackets and the Krylov mode, so it is measured:
-every valid configuration runs a few steps of the caller's own workload
-(HIP-event timing of the step launches), and the winner is cached per key
-(N, ELL width, wavepackets, Krylov mode, device) in memory and in
-``tuned_kernels.json`` next to this file.
+Measured on the caller's own workload (its rows, operator and field steps),
+per key -- device, basis size N, operator width, number of wavepackets,
+Krylov mode and the shape of the wavepackets' domains:
+
+* the production invariant-subspace path of rm_propagate ('auto'): two
+  wavepackets per wave or one (``pair``), the launch classes side by side
+  on auxiliary streams or in order (``fork``), and for domains longer than
+  a wave holds the cooperative / one-workgroup split (``coop_max``) and the
+  cooperative grid (``coop_grid``) -- rm_plan_set_tuning;
+* the full-vector ELL kernel's launch configurations (threads per
+  workgroup, rows per thread, wavepackets per workgroup, operator staged in
+  LDS or read from L2; ``plan.ell_configs()``).
+
+Every candidate runs the same few steps from the same rows (restored after
+each); a call is timed with HIP events on the caller's stream around the
+whole rm_propagate call (host enqueue included), after one untimed call that
+sets up its launch lists.  The fastest is kept in memory and in
+``tuned_kernels.json`` next to this file, as {"kernel": ..., "tuning": {...}};
+``Ensemble(kernel="tuned")`` applies it.
 """
 import json
 import os
@@ -16,6 +27,7 @@ import numpy as np
 
 CACHE_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned_kernels.json")
 _MEM = None
+SUBSPACE_MAX_ROWS = 1024   # RM_SUBSPACE_MAX_ROWS
 
 
 def _load():
@@ -35,69 +47,114 @@ def _device_name():
     return getattr(p, "gcnArchName", p.name).split(":")[0]
 
 
-def key(plan, ncols, krylov_fixed):
+def domain_shape(plan, X):
+    """(pairable domains, one-wave domains, long domains, rows of the
+    longest) of the rows X: the part of the key that decides the launch
+    classes."""
+    rows, _ = plan.domain_sizes(np.asarray(X))
+    rows = rows[rows > 0]
+    return (int(np.sum(rows <= 32)), int(np.sum((rows > 32) & (rows <= SUBSPACE_MAX_ROWS))),
+            int(np.sum(rows > SUBSPACE_MAX_ROWS)), int(rows.max(initial=0)))
+
+
+def key(plan, ncols, krylov_fixed, shape=None):
     mode = f"m{krylov_fixed}" if krylov_fixed else "adaptive"
     width = int(plan._arrays["row_ptr"].size and np.diff(plan._arrays["row_ptr"]).max(initial=0))
-    return f"{_device_name()}|n{plan.n}|w{width}|b{ncols}|{mode}"
+    k = f"{_device_name()}|n{plan.n}|w{width}|b{ncols}|{mode}"
+    if shape is not None:
+        k += "|d" + ",".join(str(x) for x in shape)
+    return k
 
 
-def candidates(plan, ncols):
+def candidates(plan, ncols, krylov_fixed=0, shape=None):
+    """[(kernel, tuning)] to time: the subspace path's choices that apply to
+    this shape, then the ELL configurations that fit N."""
     from .plan import ell_configs
-    out = ["auto"]
+    out = [("auto", {})]
+    if shape is not None:
+        npair, nwave, nlong, lmax = shape
+        if npair and 1 <= krylov_fixed <= 16:
+            out.append(("auto", {"pair": 0}))
+        if (npair > 0) + (nwave > 0) + (nlong > 0) > 1 or npair > 0 and nwave > 0:
+            out.append(("auto", {"fork": 0}))
+        if nlong:
+            groups = (lmax + 63) // 64
+            if nlong <= 8:
+                out.append(("auto", {"coop_max": 0}))                      # one workgroup each
+                for g in (16, 32, 64):
+                    if g < groups:
+                        out.append(("auto", {"coop_grid": g}))
+            else:
+                out.append(("auto", {"coop_max": nlong}))                  # cooperative, in turn
     for i, (ntr, r, c, opl) in enumerate(ell_configs()):
         if plan.n <= ntr * r and (c == 1 or ncols >= 2):
-            out.append(f"ell:{i}")
+            out.append((f"ell:{i}", {}))
     if len(out) == 1:
-        out.append("global")
+        out.append(("global", {}))
     return out
 
 
-def lookup(plan, ncols, krylov_fixed):
-    return _load().get(key(plan, ncols, krylov_fixed))
+def lookup(plan, ncols, krylov_fixed, shape=None):
+    """The cached choice as (kernel, tuning), or None."""
+    mem = _load()
+    v = mem.get(key(plan, ncols, krylov_fixed, shape)) if shape is not None else None
+    if v is None:
+        v = mem.get(key(plan, ncols, krylov_fixed))
+    if v is None:
+        return None
+    if isinstance(v, str):
+        return v, {}
+    return v["kernel"], dict(v.get("tuning", {}))
 
 
 def tune(ens, fields, steps=6, reps=2, save=True, verbose=False):
-    """Time every candidate kernel on ``steps`` steps of ``fields`` for the
-    ensemble's own rows (restored afterwards) and select the fastest."""
+    """Time every candidate on ``steps`` steps of ``fields`` for the
+    ensemble's own rows (restored afterwards), select the fastest and keep
+    it for the ensemble's shape.  Returns ((kernel, tuning), {label: ms})."""
     import torch
     table = ens.prepare(fields[:steps])
     plan = ens.plan
     psi0 = ens.psi.clone()
+    shape = domain_shape(plan, psi0.cpu().numpy())
+    stream = ens.stream or torch.cuda.current_stream()
     results = {}
-    for cand in candidates(plan, ens.psi.shape[0]):
+    for kern, tun in candidates(plan, ens.psi.shape[0], ens.krylov_fixed, shape):
+        label = kern + "".join(f" {k}={v}" for k, v in sorted(tun.items()))
         try:
-            plan.set_kernel(cand)
+            ens.kernel, ens.tuning, ens._tuned = kern, dict(tun), None
+            ens._apply_kernel()
         except Exception:
             continue
         best = None
-        for _ in range(reps):
-            ens.load(psi0)
-            torch.cuda.synchronize()
-            plan.timing(True)
-            try:
-                ens.propagate(None, table=table)
-            except Exception:
-                best = None
-                plan.timing(False)
-                break
-            ms, nl = plan.kernel_time()
-            plan.timing(False)
-            t = ms / max(nl, 1)
-            best = t if best is None else min(best, t)
+        try:
+            for rep in range(reps + 1):
+                ens.load(psi0)
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                ens.propagate(None, table=table, check=False)
+                e1.record(stream)
+                ens.check()
+                e1.synchronize()
+                if rep > 0:   # the first call sets up the launch lists
+                    t = e0.elapsed_time(e1)
+                    best = t if best is None else min(best, t)
+        except Exception:
+            best = None
         if best is not None:
-            results[cand] = best
+            results[label] = (best, kern, tun)
             if verbose:
-                print(f"  {cand:8s} {best * 1e3:9.1f} us/launch", flush=True)
+                print(f"  {label:28s} {best * 1e3:9.1f} us/call", flush=True)
     ens.load(psi0)
-    choice = min(results, key=results.get)
-    plan.set_kernel(choice)
-    ens.kernel = choice
-    k = key(plan, ens.psi.shape[0], ens.krylov_fixed)
-    _load()[k] = choice
+    label = min(results, key=lambda x: results[x][0])
+    _, kern, tun = results[label]
+    _load()[key(plan, ens.psi.shape[0], ens.krylov_fixed, shape)] = {"kernel": kern, "tuning": tun}
     if save:
         try:
             with open(CACHE_PATH, "w") as f:
                 json.dump(_MEM, f, indent=1, sort_keys=True)
         except OSError:
             pass
-    return choice, results
+    ens.kernel, ens.tuning = "tuned", {}
+    ens._tuned = (kern, dict(tun))
+    ens._apply_kernel()
+    return (kern, tun), {k: v[0] for k, v in results.items()}

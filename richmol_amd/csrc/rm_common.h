@@ -79,6 +79,15 @@ __device__ __forceinline__ float2 xfma(float2 a, float2 b, float2 acc) { return 
 // ---------------------------------------------------------------------------
 // plan
 
+// launch choices of rm_propagate's invariant-subspace path
+// (rm_plan_set_tuning); the defaults are the hand-set ones
+struct RmTuning {
+  int32_t pair = 1;        // two wavepackets per wave on domains <= 32 rows (fixed Krylov dimension <= 16)
+  int32_t coop_max = 8;    // long-domain wavepackets per call up to which the cooperative form runs
+  int32_t coop_grid = 0;   // workgroups of a cooperative launch at most (0: the domain's row groups, one per CU)
+  int32_t fork = 1;        // launch classes of a call side by side on auxiliary streams
+};
+
 struct rm_plan {
   int64_t n = 0, nnz = 0, npair = 0, nterm = 0, nq = 0;
   int32_t nfield = 0, nleaf = 0;
@@ -109,6 +118,7 @@ struct rm_plan {
   size_t ws_bytes = 0;
   int32_t variant = RM_KERNEL_AUTO;
   int32_t precision = RM_PREC_FP64;
+  RmTuning tune;
   // ELL copy of the matrix: width = max entries per row, column-major
   // (entry j of row i at j*n + i), padding entries (value 0, column i)
   int32_t ell_w = 0;
@@ -301,7 +311,7 @@ struct LongItem {
 int long_run(void** scratch, const SubOp* d_ops, const int4* d_blk, const std::vector<LongItem>& blk,
              const int4* d_coop, const std::vector<LongItem>& coop, int32_t kcap, int32_t fixed_m,
              int32_t maxorder, double tol, bool f32, int n_cu, hipStream_t s, int32_t abort_preset = 0,
-             const SubOp* op0 = nullptr);
+             const SubOp* op0 = nullptr, int32_t coop_grid = 0);
 // Expokit propagation (rm_expokit.hip) of the items (column, lo, len) of one
 // job with Krylov size m and tolerance tol; scratch grown in place.
 int expokit_run(const SubOp& op, const std::vector<int4>& items, int32_t m, double tol, int n_cu, void** ws,
@@ -551,6 +561,27 @@ __device__ __forceinline__ double row_max16(double v) {
   return v;
 }
 
+// Terms J.. of a factor of expm_e0_16, unrolled at compile time (1/J a
+// constant of the term): term <- h/J (f T) term, acc += term on the lanes
+// whose own count reaches J; nmax (uniform) ends the sequence.  The same
+// operations in the same order as a loop over j <= nterm.
+template <int J>
+__device__ __forceinline__ void cplx_terms(int nmax, int nterm, double h, double2 a, double2 bl, double2 bu,
+                                           double2& term, double2& acc) {
+  if constexpr (J <= TAYLOR_TERMS) {
+    if (J > nmax) return;
+    const double2 prev = make_double2(dpp_row_shift0<0x111>(term.x), dpp_row_shift0<0x111>(term.y));
+    const double2 next = make_double2(dpp_row_shift0<0x101>(term.x), dpp_row_shift0<0x101>(term.y));
+    double2 nt = cmul(a, term);
+    nt = cfma(bl, prev, nt);
+    nt = cfma(bu, next, nt);
+    constexpr double r = 1.0 / J;
+    term = cscale(h * r, nt);
+    if (J <= nterm) acc = cadd(acc, term);
+    cplx_terms<J + 1>(nmax, nterm, h, a, bl, bu, term, acc);
+  }
+}
+
 // exp(f*T) e_0 as expm_e0, for s <= 16: lane i (< s) of each 16-lane row owns
 // row i and its neighbours arrive by DPP row shifts (no LDS traffic); the
 // Taylor sum of each factor stops at the first term below 2^-60 of the
@@ -578,19 +609,97 @@ static __device__ void expm_e0_16(int s, const double2* alpha, const double* bet
   if (!(nsub >= 1) || nsub > (1 << 20)) nsub = 1;
   const double h = 1.0 / (double)nsub;
   const int nterm = taylor_terms(h * norm1);
+  // the larger count of the two halves runs (uniform, unrolled); each lane
+  // adds only the terms of its own count
+  const int nmax = max(__builtin_amdgcn_readlane(nterm, 0), __builtin_amdgcn_readlane(nterm, 32));
   double2 y = make_double2(i == 0 ? 1.0 : 0.0, 0.0);
   for (int sub = 0; sub < nsub; ++sub) {
     double2 term = y, acc = y;
-    for (int j = 1; j <= nterm; ++j) {
-      const double2 prev = make_double2(dpp_row_shift0<0x111>(term.x), dpp_row_shift0<0x111>(term.y));
-      const double2 next = make_double2(dpp_row_shift0<0x101>(term.x), dpp_row_shift0<0x101>(term.y));
-      double2 nt = cmul(a, term);
-      nt = cfma(bl, prev, nt);
-      nt = cfma(bu, next, nt);
-      term = cscale(h * taylor_rinv(j), nt);
-      acc = cadd(acc, term);
-    }
+    cplx_terms<1>(nmax, nterm, h, a, bl, bu, term, acc);
     y = acc;
+  }
+  if (PAIR) {
+    if ((l & 31) < s) c[l & 31] = y;
+  } else if (l < s) {
+    c[l] = y;
+  }
+}
+
+// exp(i phi T) e0 for a real symmetric tridiagonal T (diagonal alpha[].x,
+// off-diagonal beta) and real phi: the case f = i phi of expm_e0_16 with a
+// real Krylov matrix.  Same scaling rule, number of terms per problem and
+// output lanes as expm_e0_16.  Without sub-steps (|phi T|_1 <= 1/2) the
+// powers w_j = (phi T)^j e0 are real, and the sum is
+// y = sum_j i^j w_j / j!: one real recursion, one DPP shift per neighbour and
+// two FMAs on the serial chain per term, the weights 1/j! compile-time
+// constants (the loop is unrolled); both halves run the larger number of
+// terms and each adds only its own.  A half with |phi T|_1 > 1/2 takes
+// sub-steps instead, each applying i h phi T to a complex vector with real
+// coefficients; what one half computes never depends on the other.
+// 1/j! rounded at compile time
+constexpr double rfact_ct(int j) { return j <= 1 ? 1.0 : rfact_ct(j - 1) / (double)j; }
+
+// Terms J.. of the real recursion of expm_e0_16_herm, unrolled at compile
+// time (the weight 1/J! and the power of i are constants of the term): w <-
+// (phi T) w, and y += i^J w / J! on the lanes whose own count reaches J;
+// nmax (uniform) ends the sequence.
+template <int J>
+__device__ __forceinline__ void herm_terms(int nmax, int nterm, double a, double bl, double bu, double& w,
+                                           double2& y) {
+  if constexpr (J <= TAYLOR_TERMS) {
+    if (J > nmax) return;
+    const double prev = dpp_row_shift0<0x111>(w), next = dpp_row_shift0<0x101>(w);
+    w = fma(bu, next, fma(bl, prev, a * w));
+    constexpr double rf = rfact_ct(J);
+    const double t = J <= nterm ? rf * w : 0.0;
+    if constexpr ((J & 3) == 1) y.y += t;
+    else if constexpr ((J & 3) == 2) y.x -= t;
+    else if constexpr ((J & 3) == 3) y.y -= t;
+    else y.x += t;
+    herm_terms<J + 1>(nmax, nterm, a, bl, bu, w, y);
+  }
+}
+
+template <bool PAIR = false>
+static __device__ void expm_e0_16_herm(int s, const double2* alpha, const double* beta, double phi, double2* c) {
+  const int l = threadIdx.x & 63;
+  const int i = l & 15;
+  double a = 0.0, bl = 0.0, bu = 0.0, col1 = 0.0;
+  if (i < s) {
+    a = phi * alpha[i].x;
+    if (i >= 1) bl = phi * beta[i];
+    if (i + 1 < s) bu = phi * beta[i + 1];
+    col1 = fabs(a) + fabs(bl) + fabs(bu);
+  }
+  const double norm1 = row_max16(col1);
+  double2 y = make_double2(i == 0 ? 1.0 : 0.0, 0.0);
+  // each half decides for itself (the rows of a half are the rows of its
+  // DPP rows, so the two halves may take different branches below)
+  const bool small = !(norm1 > 0.5);
+  const int nterm_r = small ? taylor_terms(norm1) : 0;
+  const int nmax = max(__builtin_amdgcn_readlane(nterm_r, 0), __builtin_amdgcn_readlane(nterm_r, 32));
+  if (nmax > 0) {
+    double w = y.x;
+    herm_terms<1>(nmax, nterm_r, a, bl, bu, w, y);
+  }
+  if (!small) {
+    int nsub = (int)ceil(2.0 * norm1);
+    if (!(nsub >= 1) || nsub > (1 << 20)) nsub = 1;
+    const double h = 1.0 / (double)nsub;
+    const int nterm = taylor_terms(h * norm1);
+    for (int sub = 0; sub < nsub; ++sub) {
+      double2 term = y, acc = y;
+      for (int j = 1; j <= nterm; ++j) {
+        const double px = dpp_row_shift0<0x111>(term.x), py = dpp_row_shift0<0x111>(term.y);
+        const double nx = dpp_row_shift0<0x101>(term.x), ny = dpp_row_shift0<0x101>(term.y);
+        const double tx = fma(bu, nx, fma(bl, px, a * term.x));
+        const double ty = fma(bu, ny, fma(bl, py, a * term.y));
+        const double sc = h * taylor_rinv(j);
+        term = make_double2(-ty * sc, tx * sc);
+        acc = cadd(acc, term);
+      }
+      y = acc;
+    }
   }
   if (PAIR) {
     if ((l & 31) < s) c[l & 31] = y;

@@ -801,7 +801,7 @@ void coop_scratch_release(void* sc) {
 int long_run(void** scratch, const SubOp* d_ops, const int4* d_blk, const std::vector<LongItem>& blk,
              const int4* d_coop, const std::vector<LongItem>& coop, int32_t kcap, int32_t fixed_m,
              int32_t maxorder, double tol, bool f32, int n_cu, hipStream_t s, int32_t abort_preset,
-             const SubOp* op0) {
+             const SubOp* op0, int32_t coop_grid) {
   if (blk.empty() && coop.empty()) return RM_OK;
   int gres_r = 1, gres_m = 1, gres_b = 1, gres_x = 1;
   HIPCHK((max_resident<GRID_REG>(n_cu, 0, &gres_r)));
@@ -914,10 +914,13 @@ int long_run(void** scratch, const SubOp* d_ops, const int4* d_blk, const std::v
     const int64_t groups = (it.item.z + RPW - 1) / RPW;
     // registers hold a lane's entries when every wave has one row group and
     // no row of the domain is longer than LPR * UR entries
+    // (coop_grid > 0 caps the grid; the memory form loops over the row groups)
     constexpr int wr = ct_of<GRID_REG>() / 64, wm = ct_of<GRID_MEM>() / 64;
-    const bool reg = groups <= (int64_t)gres_r * wr && it.maxrow <= LPR * UR;
+    const int64_t cap_r = coop_grid > 0 ? std::min<int64_t>(gres_r, coop_grid) : gres_r;
+    const int64_t cap_m = coop_grid > 0 ? std::min<int64_t>(gres_m, coop_grid) : gres_m;
+    const bool reg = groups <= cap_r * wr && it.maxrow <= LPR * UR;
     const int wpg = reg ? wr : wm;
-    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((groups + wpg - 1) / wpg, reg ? gres_r : gres_m));
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((groups + wpg - 1) / wpg, reg ? cap_r : cap_m));
     hipLaunchKernelGGL(k_long_reset, dim3(1), dim3(256), 0, s, bar, abort_flag, (unsigned long long*)lpart,
                        LRING * 4 * gmax, abort_preset);
     HIPCHK(hipGetLastError());

@@ -30,7 +30,6 @@ namespace {
 
 constexpr int NSR = 2;                         // field slots held in registers
 constexpr size_t SUB_LDS_BASIS = 32 * 1024;    // LDS budget of a wave's Krylov basis
-constexpr int64_t COOP_MAX_COLS = 8;           // long-domain wavepackets per call for the cooperative kernel
 
 struct SubArgs {
   const SubOp* ops;        // the jobs, or NULL: the one job is op0
@@ -670,14 +669,22 @@ __global__ __launch_bounds__(64) void k_sub_pair(SubArgs a) {
         s_x[l] = to_x<F32>(vk);
         wave_sync();
       }
-      // Hermitian domain operator and real field products: alpha is real
-      const bool realal = herm && f0.y == 0.0 && f1.y == 0.0;
+      // Hermitian domain operator and real field products: alpha is real,
+      // and with a real-time exponent f = i phi the Krylov exponential takes
+      // the real recursion (a wave-wide vote: both halves share the operator)
+      const bool realal = herm && __all(f0.y == 0.0 && f1.y == 0.0);
+      const bool realexp = realal && __all(op.f.x == 0.0);
       double2 Wr = matvec(), vp = zero;
       double acc3[2];
       acc3[0] = Wr.x * vk.x + Wr.y * vk.y;
       acc3[1] = Wr.x * vk.y - Wr.y * vk.x;
-      double2 al = make_double2(half_allsum32(acc3[0]), realal ? 0.0 : half_allsum32(acc3[1]));
-      Wr = csub(Wr, cmul(al, vk));
+      double2 al = make_double2(half_allsum32(acc3[0]), 0.0);
+      if (realal) {
+        Wr = csub(Wr, cscale(al.x, vk));
+      } else {
+        al.y = half_allsum32(acc3[1]);
+        Wr = csub(Wr, cmul(al, vk));
+      }
       sa[0] = al;
       int k = 1;
       for (int it = 1; it < a.fixed_m; ++it) {
@@ -715,14 +722,23 @@ __global__ __launch_bounds__(64) void k_sub_pair(SubArgs a) {
         }
         Wr = matvec();
         const double a0 = Wr.x * vk.x + Wr.y * vk.y, a1 = Wr.x * vk.y - Wr.y * vk.x;
-        al = make_double2(half_allsum32(a0), realal ? 0.0 : half_allsum32(a1));
-        Wr = csub(csub(Wr, cmul(al, vk)), cscale(beta, vp));
+        al = make_double2(half_allsum32(a0), 0.0);
+        if (realal) {
+          Wr = csub(csub(Wr, cscale(al.x, vk)), cscale(beta, vp));
+        } else {
+          al.y = half_allsum32(a1);
+          Wr = csub(csub(Wr, cmul(al, vk)), cscale(beta, vp));
+        }
         sa[it] = al;
         k = it + 1;
       }
       wave_sync();
-      if (k > 1) expm_e0_16<true>(k, sa, sb, op.f, sc);
-      else if (i == 0) sc[0] = make_double2(1.0, 0.0);
+      if (k > 1) {
+        if (realexp) expm_e0_16_herm<true>(k, sa, sb, op.f.y, sc);
+        else expm_e0_16<true>(k, sa, sb, op.f, sc);
+      } else if (i == 0) {
+        sc[0] = make_double2(1.0, 0.0);
+      }
       __threadfence_block();
       wave_sync();
       // two interleaved partial sums (even and odd j) halve the dependent chain
@@ -1205,8 +1221,8 @@ void scratch_release(SubScratch* sc) {
 // dimension <= 16 (the banded ones, placed in PAIR_BAND_CLASS by the caller,
 // keep their own class), grouped by operator, each group padded to an even
 // count; other Krylov settings send the banded items back to class 0.
-void group_pairs(std::vector<int4> (&cls)[NCLASS], int32_t fixed_m) {
-  const bool pairs = fixed_m >= 1 && fixed_m <= PAIR_MAX_M;
+void group_pairs(std::vector<int4> (&cls)[NCLASS], int32_t fixed_m, bool allow = true) {
+  const bool pairs = allow && fixed_m >= 1 && fixed_m <= PAIR_MAX_M;
   if (!pairs && !cls[PAIR_BAND_CLASS].empty()) {
     cls[0].insert(cls[0].end(), cls[PAIR_BAND_CLASS].begin(), cls[PAIR_BAND_CLASS].end());
     cls[PAIR_BAND_CLASS].clear();
@@ -1246,7 +1262,7 @@ int launch_all(SubScratch* sc, const SubArgs& a0, const int4* const (&items)[NCL
                const std::vector<int4> (&cls)[NCLASS], const int4* d_blk, const std::vector<LongItem>& blk,
                const int4* d_coop, const std::vector<LongItem>& coop, int32_t kcap, int32_t fixed_m,
                int32_t maxorder, double tol, bool f32, int n_cu, hipStream_t s, rm_plan* timed,
-               int32_t abort_preset) {
+               int32_t abort_preset, const RmTuning& tn) {
   SubArgs a = a0;
   size_t need = 0, wsoff[NCLASS] = {};
   int ngroups = blk.empty() ? 0 : 1;
@@ -1272,7 +1288,7 @@ int launch_all(SubScratch* sc, const SubArgs& a0, const int4* const (&items)[NCL
     sc->ws_bytes = need;
   }
   if (timed) HIPCHK(timing_begin(timed, s));
-  const bool fork = ngroups > 1 && coop.empty();
+  const bool fork = tn.fork && ngroups > 1 && coop.empty();
   int nused = 0;
   auto next_stream = [&](hipStream_t* out) -> int {
     if (!fork) {
@@ -1309,7 +1325,7 @@ int launch_all(SubScratch* sc, const SubArgs& a0, const int4* const (&items)[NCL
   hipStream_t bs = s;
   if (!blk.empty() && (rc = next_stream(&bs))) return rc;
   rc = long_run(&sc->long_ws, a0.ops, d_blk, blk, d_coop, coop, kcap, fixed_m, maxorder, tol, f32, n_cu, bs,
-                abort_preset, a0.ops ? nullptr : &a0.op0);
+                abort_preset, a0.ops ? nullptr : &a0.op0, tn.coop_grid);
   if (rc) return rc;
   if (fork && !blk.empty()) HIPCHK(hipEventRecord(sc->join[nused - 1], bs));
   for (int i = 0; i < (fork ? nused : 0); ++i) HIPCHK(hipStreamWaitEvent(s, sc->join[i], 0));
@@ -1323,8 +1339,8 @@ int launch_all(SubScratch* sc, const SubArgs& a0, const int4* const (&items)[NCL
 int sub_run(SubScratch* sc, const std::vector<SubOp>& ops, std::vector<int4> (&cls)[NCLASS],
             const std::vector<LongItem>& blk, const std::vector<LongItem>& coop, int32_t kcap,
             int32_t fixed_m, int32_t maxorder, double tol, bool f32, int n_cu, hipStream_t s,
-            rm_plan* timed, int32_t abort_preset = 0) {
-  group_pairs(cls, fixed_m);
+            rm_plan* timed, int32_t abort_preset = 0, const RmTuning& tn = RmTuning()) {
+  group_pairs(cls, fixed_m, tn.pair != 0);
   size_t nitems = blk.size() + coop.size();
   for (int c = 0; c < NCLASS; ++c) nitems += cls[c].size();
   if (nitems == 0) return RM_OK;
@@ -1379,7 +1395,7 @@ int sub_run(SubScratch* sc, const std::vector<SubOp>& ops, std::vector<int4> (&c
   for (const SubOp& op : ops) a.obs |= op.ow > 0 ? 1 : 0;
   const int rc = launch_all(sc, a, items, cls, (const int4*)(sc->dev[bi] + off_blk), blk,
                             (const int4*)(sc->dev[bi] + off_coop), coop, kcap, fixed_m, maxorder, tol, f32,
-                            n_cu, s, timed, abort_preset);
+                            n_cu, s, timed, abort_preset, tn);
   if (rc) return rc;
   if (!sc->done[bi]) HIPCHK(hipEventCreateWithFlags(&sc->done[bi], hipEventDisableTiming));
   HIPCHK(hipEventRecord(sc->done[bi], s));
@@ -1461,6 +1477,30 @@ extern "C" int rm_plan_subspaces(rm_plan* p, int64_t* count, int64_t* max_rows) 
   return RM_OK;
 }
 
+extern "C" int rm_plan_set_tuning(rm_plan* p, int32_t knob, int32_t value) {
+  if (!p) return fail(RM_ERR_ARG, "rm_plan_set_tuning: NULL plan");
+  switch (knob) {
+    case RM_TUNE_PAIR:
+    case RM_TUNE_FORK:
+      if (value != 0 && value != 1) return fail(RM_ERR_ARG, "rm_plan_set_tuning: knob %d takes 0 or 1", knob);
+      (knob == RM_TUNE_PAIR ? p->tune.pair : p->tune.fork) = value;
+      break;
+    case RM_TUNE_COOP_MAX:
+      if (value < 0) return fail(RM_ERR_ARG, "rm_plan_set_tuning: negative cooperative column limit %d", value);
+      p->tune.coop_max = value;
+      break;
+    case RM_TUNE_COOP_GRID:
+      if (value < 0) return fail(RM_ERR_ARG, "rm_plan_set_tuning: negative cooperative grid %d", value);
+      p->tune.coop_grid = value;
+      break;
+    default:
+      return fail(RM_ERR_ARG, "rm_plan_set_tuning: unknown knob %d", knob);
+  }
+  // launch lists kept for domain reuse were grouped under the old choices
+  if (p->sub_scratch) ((SubScratch*)p->sub_scratch)->kept = false;
+  return RM_OK;
+}
+
 extern "C" int rm_plan_set_domain_reuse(rm_plan* p, int32_t on) {
   if (!p) return fail(RM_ERR_ARG, "rm_plan_set_domain_reuse: NULL plan");
   p->dom_reuse = on ? 1 : 0;
@@ -1516,8 +1556,8 @@ void sort_long(std::vector<LongItem>& v) {
 // The subspace path of rm_propagate.  Arguments are validated by the caller
 // (ld, ncols, Krylov parameters, pointers).  Domains of up to
 // RM_SUBSPACE_MAX_ROWS rows go to the one-wave kernel; longer ones to the
-// cooperative kernel when there are at most COOP_MAX_COLS of them, else to
-// the block kernel.  force: RM_KERNEL_COOP / RM_KERNEL_BLOCK send every
+// cooperative kernel when there are at most p->tune.coop_max of them (8
+// unless tuned), else to the block kernel.  force: RM_KERNEL_COOP / RM_KERNEL_BLOCK send every
 // wavepacket to that kernel (0: the rule above).  *done = false when the
 // path does not apply; nothing has then been written to psi.
 // The observable plan's matrix in p's permuted ELL form (p->obs_col,
@@ -1575,14 +1615,18 @@ int sub_propagate(rm_plan* p, void* psi, int64_t ld, int64_t ncols, int64_t nste
     if ((rc = grow(&p->int_status, &p->int_status_cap, ncols))) return rc;
     st = p->int_status;
   }
-  const bool pairs = krylov_fixed >= 1 && krylov_fixed <= PAIR_MAX_M;
+  const bool pairs = p->tune.pair && krylov_fixed >= 1 && krylov_fixed <= PAIR_MAX_M;
   // the Expokit propagator takes fresh domains at every call (its items are
   // not kept)
   const bool expo = p->propag == RM_PROPAG_EXPOKIT;
   const bool reuse = !expo && p->dom_reuse && sc->kept && sc->kept_psi == psi && sc->kept_ld == ld &&
                      sc->kept_ncols == ncols && sc->kept_force == force && sc->kept_pairs == pairs;
-  // step flags in chunks of <= 1024 terms, at most 16 per step
-  const int32_t parts = (int32_t)std::max<int64_t>(1, std::min<int64_t>(16, (p->nterm + 1023) / 1024));
+  // step flags in chunks of <= 1024 terms, at most 16 per step; from 64
+  // steps on one block per step (a step with a field ends its scan at the
+  // first non-zero element, in its first pass: more blocks per step would
+  // only repeat that first pass)
+  const int32_t parts = nsteps >= 64 ? 1
+                                     : (int32_t)std::max<int64_t>(1, std::min<int64_t>(16, (p->nterm + 1023) / 1024));
   const int64_t chunk = std::max<int64_t>(1, (p->nterm + parts - 1) / parts);
   if (nsteps * parts > INT32_MAX) return RM_OK;
   if ((rc = grow(&p->sflag, &p->sflag_cap, nsteps))) return rc;
@@ -1697,12 +1741,12 @@ int sub_propagate(rm_plan* p, void* psi, int64_t ld, int64_t ncols, int64_t nste
       }
     }
     if (!big.empty() && !p->sub_rp && (rc = csr_build(p, entry_values(p)))) return rc;
-    const bool to_coop = force == RM_KERNEL_COOP || (force == 0 && (int64_t)big.size() <= COOP_MAX_COLS);
+    const bool to_coop = force == RM_KERNEL_COOP || (force == 0 && (int64_t)big.size() <= p->tune.coop_max);
     for (const int4& it : big) (to_coop ? sc->kept_coop : sc->kept_blk).push_back(long_item(p, it.x, it.y, it.z, 0));
     sort_long(sc->kept_blk);
     for (auto& v : sc->kept_cls)
       std::stable_sort(v.begin(), v.end(), [](const int4& x, const int4& y) { return x.z > y.z; });
-    group_pairs(sc->kept_cls, krylov_fixed);
+    group_pairs(sc->kept_cls, krylov_fixed, p->tune.pair != 0);
     sc->kept_pairs = pairs;
     if (p->dom_reuse) {
       // a copy of their own: rm_plan_domains rewrites p->dom
@@ -1759,12 +1803,12 @@ int sub_propagate(rm_plan* p, void* psi, int64_t ld, int64_t ncols, int64_t nste
     const int4* d_blk = sc->kept_dev + pos;
     const int4* d_coop = d_blk + sc->kept_blk.size();
     rc = launch_all(sc, a, items, sc->kept_cls, d_blk, sc->kept_blk, d_coop, sc->kept_coop, kcap, krylov_fixed,
-                    maxorder, tol, f32, p->n_cu, s, p, p->coop_abort);
+                    maxorder, tol, f32, p->n_cu, s, p, p->coop_abort, p->tune);
   } else {
     std::vector<int4> cls[NCLASS];
     for (int c = 0; c < NCLASS; ++c) cls[c] = sc->kept_cls[c];
     rc = sub_run(sc, ops, cls, sc->kept_blk, sc->kept_coop, kcap, krylov_fixed, maxorder, tol, f32, p->n_cu, s, p,
-                 p->coop_abort);
+                 p->coop_abort, p->tune);
   }
   if (rc || status || sc->kept_coop.empty()) return rc;
   // no status array from the caller: read the internal one for timeouts

@@ -55,6 +55,8 @@ class Ensemble:
         self.plan = None
         self._active = None
         self.kernel = "auto"
+        self.tuning = {}
+        self._tuned = None
         self.precision = "fp64"
         dev = torch.device("cuda", torch.cuda.current_device())
         self.psi = torch.as_tensor(np.ascontiguousarray(psi0, dtype=np.complex128)).to(dev)
@@ -82,8 +84,17 @@ class Ensemble:
         self.plan.set_field(self.H, self.stream)
 
     def set_kernel(self, kernel):
-        """'auto', 'tuned', 'global', 'resident', 'ell', 'ell:<i>' or 'subspace'."""
+        """'auto', 'tuned' (the autotuner's choice for these rows, 'auto' when
+        never tuned), 'global', 'resident', 'ell', 'ell:<i>' or 'subspace'."""
         self.kernel = kernel
+        self._tuned = None
+        if self.plan is not None:
+            self._apply_kernel()
+
+    def set_tuning(self, **choices):
+        """Launch choices of the subspace path (pair, coop_max, coop_grid,
+        fork; see Plan.set_tuning) for this ensemble's calls."""
+        self.tuning = dict(choices)
         if self.plan is not None:
             self._apply_kernel()
 
@@ -125,12 +136,16 @@ class Ensemble:
         cached choice for this shape, or 'auto' when the shape was never tuned.
         Plans are shared between ensembles over the same operator, so this
         runs before every launch."""
-        k = self.kernel
+        k, tun = self.kernel, getattr(self, "tuning", {})
         if k == "tuned":
-            from . import autotune
-            k = autotune.lookup(self.plan, self.psi.shape[0], self.krylov_fixed) or "auto"
+            if getattr(self, "_tuned", None) is None:
+                from . import autotune
+                shape = autotune.domain_shape(self.plan, self.psi.cpu().numpy())
+                self._tuned = autotune.lookup(self.plan, self.psi.shape[0], self.krylov_fixed, shape) or ("auto", {})
+            k, tun = self._tuned
         if self.plan.kernel != k:
             self.plan.set_kernel(k)
+        self.plan.set_tuning(**tun)
         if self.plan.precision != self.precision:
             self.plan.set_precision(self.precision)
 
@@ -187,6 +202,8 @@ class Ensemble:
         src = torch.as_tensor(np.ascontiguousarray(psi0, dtype=np.complex128)) \
             if not isinstance(psi0, torch.Tensor) else psi0
         self.psi.copy_(src.to(self.psi.device))
+        if self.kernel == "tuned":
+            self._tuned = None   # the rows' domains may differ: look the shape up again
         if self.plan is not None and self.plan.domain_owner == id(self):
             self.plan.set_domain_reuse(True, owner=id(self))
 

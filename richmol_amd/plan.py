@@ -218,6 +218,7 @@ class DevicePlan:
             self.nfield, self.nleaf, ctypes.byref(h)), "rm_plan_create")
         self.handle = h
         self.domain_owner = None
+        self.tuning = dict(_lib.TUNE_DEFAULTS)
         _lib.track(self)
 
     @classmethod
@@ -274,6 +275,20 @@ class DevicePlan:
         _lib.check(self.lib.rm_plan_set_precision(self.handle, _lib.PRECISIONS[precision]),
                    "rm_plan_set_precision")
         self.precision = precision
+
+    def set_tuning(self, **choices):
+        """Launch choices of rm_propagate's subspace path (rm_plan_set_tuning):
+        pair, coop_max, coop_grid, fork; a choice not given returns to its
+        default."""
+        want = dict(_lib.TUNE_DEFAULTS)
+        for k, v in choices.items():
+            if k not in want:
+                raise ValueError(f"unknown tuning choice '{k}' (use {sorted(want)})")
+            want[k] = int(v)
+        for k, v in want.items():
+            if self.tuning.get(k) != v:
+                _lib.check(self.lib.rm_plan_set_tuning(self.handle, _lib.TUNE_KNOBS[k], v), "rm_plan_set_tuning")
+                self.tuning[k] = v
 
     def set_domain_reuse(self, on=True, owner=None):
         """Keep rm_propagate's column domains for its next call on the same

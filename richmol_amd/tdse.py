@@ -152,6 +152,10 @@ class TDSE:
         adaptive Lanczos, here on the device), tol (default 1e-15).
         Device extensions: maxorder (default 100), krylov_fixed (fixed
         Krylov dimension, default 0 = adaptive), stream (torch stream),
+        kernel ('auto' (default): the step runs as a one-step rm_propagate
+        on the invariant-subspace kernels, with the wavepackets' domains kept
+        from the previous call; or a full-vector kernel name of
+        plan.set_kernel ('ell', 'ell:<i>', 'resident', 'global') for rm_step),
         operator_form ('auto' (default), 'flat' or 'kron'): 'kron' keeps the
         operator factored as richmol stores it (Kronecker-block plan; needs
         every J with all m and the full k space and no field thresholds),
@@ -224,8 +228,10 @@ class TDSE:
             plan = _plan.plan_for(H)
             # plans are shared with ensembles that may have picked another
             # kernel or precision
-            if plan.kernel != "auto":
-                plan.set_kernel("auto")
+            kernel = kwargs.get("kernel", "auto")
+            if plan.kernel != kernel:
+                plan.set_kernel(kernel)
+            plan.set_tuning()
             if plan.precision != "fp64":
                 plan.set_precision("fp64")
 
@@ -288,6 +294,8 @@ class TDSE:
             if bool((status != 0).any()):
                 from .ensemble import ExpokitError
                 raise ExpokitError("maximum number of steps reached without convergence")
+        elif plan.kernel == "auto":
+            out = self._step_subspace(plan, H, psi, f, phase, krylov_fixed, maxorder, tol, stream)
         else:
             status = torch.zeros(ncols, dtype=torch.int32, device=psi.device)
             kused = torch.zeros(ncols, dtype=torch.int32, device=psi.device)
@@ -300,6 +308,67 @@ class TDSE:
         if on_device:
             return out
         return out.cpu().numpy()
+
+    def _step_subspace(self, plan, H, psi, f, phase, krylov_fixed, maxorder, tol, stream):
+        """One step as rm_propagate over a one-row field table: the rows are
+        copied into a buffer kept per plan, so that the next call on them
+        finds the same wavepackets and keeps their domains (no host round
+        trip for them); the device checks that they still hold, and a step
+        whose rows left them (status RM_STALE_DOMAIN, rows untouched) is run
+        again with fresh domains.  The field products go up through a pinned
+        staging row."""
+        import torch
+        from . import _lib
+        ncols, n = psi.shape
+        st = self.__dict__.setdefault("_sub_state", {})
+        key = (id(plan), ncols, n, psi.device)
+        s = st.get(key)
+        if s is None or s["plan"] is not plan:
+            st.clear()     # one buffer set per TDSE: the newest plan's
+            dev = psi.device
+            s = st[key] = {
+                "plan": plan,
+                "buf": torch.empty((ncols, n), dtype=torch.complex128, device=dev),
+                "work": torch.empty((ncols, n), dtype=torch.complex128, device=dev),
+                "table": torch.empty((1, max(plan.nfield, 1)), dtype=torch.complex128, device=dev),
+                "host": torch.empty((1, max(plan.nfield, 1)), dtype=torch.complex128).pin_memory(),
+                "kused": torch.empty((1, ncols), dtype=torch.int32, device=dev),
+                "status": torch.empty(ncols, dtype=torch.int32, device=dev),
+                "event": None,
+            }
+        fprod, thresh = plan.field_arrays(H)
+        if s["event"] is not None:
+            s["event"].synchronize()       # the previous upload has left the staging row
+        s["host"][0].copy_(torch.from_numpy(fprod))
+        cs = stream if stream is not None else torch.cuda.current_stream(psi.device)
+        with torch.cuda.stream(cs):
+            s["table"].copy_(s["host"], non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record(cs)
+            s["event"] = ev
+            buf = s["buf"]
+            if buf.data_ptr() != psi.data_ptr():
+                buf.copy_(psi)
+        if plan.domain_owner != id(self):
+            plan.set_domain_reuse(True, owner=id(self))
+        th = [float(t) if t > 0 else None for t in thresh]
+        for attempt in range(2):
+            plan.propagate(buf, s["work"], s["table"], f, phase=phase, thresh=th, krylov_fixed=krylov_fixed,
+                           maxorder=maxorder, tol=tol, kused=s["kused"], status=s["status"], stream=stream)
+            bad = s["status"].cpu().numpy()
+            if attempt == 0 and np.any(bad & _lib.RM_STALE_DOMAIN):
+                plan.set_domain_reuse(True, owner=id(self))   # forget the kept domains
+                with torch.cuda.stream(cs):
+                    buf.copy_(psi)
+                continue
+            break
+        self.last_krylov_dims = s["kused"][0].clone()
+        if krylov_fixed == 0 and np.any(bad & _lib.RM_NOT_CONVERGED):
+            raise ValueError(f"Lanczos reached maximum order of '{maxorder}' without convergence")
+        if np.any(bad & ~(_lib.RM_NOT_CONVERGED | _lib.RM_STALE_DOMAIN)):
+            raise RuntimeError(f"rm_propagate reported status {int(bad.max())}")
+        with torch.cuda.stream(cs):
+            return buf.clone()
 
     def run(self, H, vecs, field, H0=None, obs=None, thresh=None, nsteps=None, **kwargs):
         """The reference's time loop (examples/ocs_alignment.py:89-103) as one
