@@ -107,12 +107,14 @@ def lookup(plan, ncols, krylov_fixed, shape=None):
     return v["kernel"], dict(v.get("tuning", {}))
 
 
-def tune(ens, fields, steps=6, reps=2, save=True, verbose=False):
+def tune(ens, fields, steps=6, reps=2, save=True, verbose=False, per_term=False):
     """Time every candidate on ``steps`` steps of ``fields`` for the
     ensemble's own rows (restored afterwards), select the fastest and keep
-    it for the ensemble's shape.  Returns ((kernel, tuning), {label: ms})."""
+    it for the ensemble's shape.  per_term: ``fields`` holds one (nsteps, 3)
+    array per summed tensor, as Ensemble.prepare takes it.  Returns
+    ((kernel, tuning), {label: ms per call})."""
     import torch
-    table = ens.prepare(fields[:steps])
+    table = ens.prepare([F[:steps] for F in fields] if per_term else fields[:steps], per_term=per_term)
     plan = ens.plan
     psi0 = ens.psi.clone()
     shape = domain_shape(plan, psi0.cpu().numpy())

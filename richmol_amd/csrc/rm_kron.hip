@@ -203,24 +203,34 @@ __global__ __launch_bounds__(256) void k_kp_z(const KTask* __restrict__ tasks, c
     }
     __syncthreads();
     ZT* zt = Z + tk.zoff + (b * tk.dm1 + m0) * 2 * (int64_t)tk.ktot;
-    for (int e = threadIdx.x; e < nr * tk.dk2; e += blockDim.x) {
-      const int r = e / tk.dk2, k2 = e % tk.dk2;
+    // one row per wave at a time, 64 consecutive columns per pass: the row's
+    // taps (MF value, source row, in range or not) are the same on every lane
+    const int lane = threadIdx.x & 63;
+    for (int r = threadIdx.x >> 6; r < nr; r += blockDim.x >> 6) {
       const int m1 = m0 + r;
-      double zre = 0.0, zim = 0.0;
+      double2 f[5];
+      const double* xr[5];
+      bool in[5];
 #pragma unroll
       for (int t = 0; t < 5; ++t) {
-        if (t >= tk.nsig) break;
         const int m2 = m1 - tk.J1 - sg[t] + tk.J2;
-        if (m2 < 0 || m2 >= tk.dm2) continue;
-        const double2 f = s_mf[t * zr + r];
-        const double* xr = s_x + (m2 - lo2) * xld;
-        const double a = xr[k2], c = xr[tk.dk2 + k2];
-        zre += f.x * a - f.y * c;
-        zim += f.x * c + f.y * a;
+        in[t] = t < tk.nsig && m2 >= 0 && m2 < tk.dm2;
+        f[t] = in[t] ? s_mf[t * zr + r] : make_double2(0.0, 0.0);
+        xr[t] = s_x + (in[t] ? m2 - lo2 : 0) * xld;
       }
-      ZT* zrow = zt + (int64_t)r * 2 * tk.ktot;
-      zrow[tk.koff + k2] = (ZT)(zre * s);
-      zrow[tk.ktot + tk.koff + k2] = (ZT)(zim * s);
+      ZT* zrow = zt + (int64_t)r * 2 * tk.ktot + tk.koff;
+      for (int k2 = lane; k2 < tk.dk2; k2 += 64) {
+        double zre = 0.0, zim = 0.0;
+#pragma unroll
+        for (int t = 0; t < 5; ++t) {
+          if (!in[t]) continue;
+          const double a = xr[t][k2], c = xr[t][tk.dk2 + k2];
+          zre += f[t].x * a - f[t].y * c;
+          zim += f[t].x * c + f[t].y * a;
+        }
+        zrow[k2] = (ZT)(zre * s);
+        zrow[tk.ktot + k2] = (ZT)(zim * s);
+      }
     }
     __syncthreads();
   }

@@ -309,14 +309,20 @@ class TDSE:
             return out
         return out.cpu().numpy()
 
+    _RING = 8   # field rows in flight between the host and the device
+
     def _step_subspace(self, plan, H, psi, f, phase, krylov_fixed, maxorder, tol, stream):
-        """One step as rm_propagate over a one-row field table: the rows are
+        """One step as rm_propagate over a one-row field table.  The rows are
         copied into a buffer kept per plan, so that the next call on them
         finds the same wavepackets and keeps their domains (no host round
         trip for them); the device checks that they still hold, and a step
         whose rows left them (status RM_STALE_DOMAIN, rows untouched) is run
-        again with fresh domains.  The field products go up through a pinned
-        staging row."""
+        again with fresh domains.  Rows that are the previous call's output,
+        unchanged (the same tensor, its version counter as returned), lie in
+        their domains by construction: with a fixed Krylov dimension their
+        step then reads no status and the host does not wait for the device.
+        Field rows go up through a ring of pinned staging rows, each reused
+        after its copy has run."""
         import torch
         from . import _lib
         ncols, n = psi.shape
@@ -326,35 +332,49 @@ class TDSE:
         if s is None or s["plan"] is not plan:
             st.clear()     # one buffer set per TDSE: the newest plan's
             dev = psi.device
+            nf = max(plan.nfield, 1)
             s = st[key] = {
                 "plan": plan,
                 "buf": torch.empty((ncols, n), dtype=torch.complex128, device=dev),
                 "work": torch.empty((ncols, n), dtype=torch.complex128, device=dev),
-                "table": torch.empty((1, max(plan.nfield, 1)), dtype=torch.complex128, device=dev),
-                "host": torch.empty((1, max(plan.nfield, 1)), dtype=torch.complex128).pin_memory(),
+                "table": torch.empty((self._RING, nf), dtype=torch.complex128, device=dev),
+                "host": torch.empty((self._RING, nf), dtype=torch.complex128).pin_memory(),
+                "events": [None] * self._RING,
+                "slot": 0,
                 "kused": torch.empty((1, ncols), dtype=torch.int32, device=dev),
                 "status": torch.empty(ncols, dtype=torch.int32, device=dev),
-                "event": None,
+                "last": None, "last_ver": None,
             }
+        trusted = s["last"] is psi and psi._version == s["last_ver"] and plan.domain_owner == id(self)
         fprod, thresh = plan.field_arrays(H)
-        if s["event"] is not None:
-            s["event"].synchronize()       # the previous upload has left the staging row
-        s["host"][0].copy_(torch.from_numpy(fprod))
+        i = s["slot"]
+        s["slot"] = (i + 1) % self._RING
+        if s["events"][i] is not None:
+            s["events"][i].synchronize()     # this staging row's previous copy has run
+        s["host"][i].copy_(torch.from_numpy(fprod))
         cs = stream if stream is not None else torch.cuda.current_stream(psi.device)
+        table = s["table"][i:i + 1]
+        buf = s["buf"]
         with torch.cuda.stream(cs):
-            s["table"].copy_(s["host"], non_blocking=True)
+            table.copy_(s["host"][i:i + 1], non_blocking=True)
             ev = torch.cuda.Event()
             ev.record(cs)
-            s["event"] = ev
-            buf = s["buf"]
+            s["events"][i] = ev
             if buf.data_ptr() != psi.data_ptr():
                 buf.copy_(psi)
         if plan.domain_owner != id(self):
             plan.set_domain_reuse(True, owner=id(self))
         th = [float(t) if t > 0 else None for t in thresh]
+        # no status array on the fast path: the library then reports a
+        # cooperative launch's timeout itself (its return code raises)
+        fast = trusted and krylov_fixed > 0
         for attempt in range(2):
-            plan.propagate(buf, s["work"], s["table"], f, phase=phase, thresh=th, krylov_fixed=krylov_fixed,
-                           maxorder=maxorder, tol=tol, kused=s["kused"], status=s["status"], stream=stream)
+            plan.propagate(buf, s["work"], table, f, phase=phase, thresh=th, krylov_fixed=krylov_fixed,
+                           maxorder=maxorder, tol=tol, kused=s["kused"], status=None if fast else s["status"],
+                           stream=stream)
+            if fast:
+                bad = None
+                break
             bad = s["status"].cpu().numpy()
             if attempt == 0 and np.any(bad & _lib.RM_STALE_DOMAIN):
                 plan.set_domain_reuse(True, owner=id(self))   # forget the kept domains
@@ -362,13 +382,16 @@ class TDSE:
                     buf.copy_(psi)
                 continue
             break
-        self.last_krylov_dims = s["kused"][0].clone()
-        if krylov_fixed == 0 and np.any(bad & _lib.RM_NOT_CONVERGED):
-            raise ValueError(f"Lanczos reached maximum order of '{maxorder}' without convergence")
-        if np.any(bad & ~(_lib.RM_NOT_CONVERGED | _lib.RM_STALE_DOMAIN)):
-            raise RuntimeError(f"rm_propagate reported status {int(bad.max())}")
+        self.last_krylov_dims = s["kused"][0]
+        if bad is not None:
+            if krylov_fixed == 0 and np.any(bad & _lib.RM_NOT_CONVERGED):
+                raise ValueError(f"Lanczos reached maximum order of '{maxorder}' without convergence")
+            if np.any(bad & ~(_lib.RM_NOT_CONVERGED | _lib.RM_STALE_DOMAIN)):
+                raise RuntimeError(f"rm_propagate reported status {int(bad.max())}")
         with torch.cuda.stream(cs):
-            return buf.clone()
+            out = buf.clone()
+        s["last"], s["last_ver"] = out, out._version
+        return out
 
     def run(self, H, vecs, field, H0=None, obs=None, thresh=None, nsteps=None, **kwargs):
         """The reference's time loop (examples/ocs_alignment.py:89-103) as one

@@ -257,3 +257,39 @@ def test_propagate_equals_per_step(device, kernel):
     assert np.array_equal(a, b)
     dev, kdev, psi_d, psi_o = _run_pair(h0, H, fields, X)
     assert np.abs(b - psi_o).max() <= TOL_PSI
+
+
+@pytest.mark.parametrize("fixed", [12, 0])
+def test_update_rows_rewritten_between_calls(device, fixed):
+    """The drop-in loop on device rows, with the caller writing into the
+    returned rows between calls: a value inside their domains (a new version
+    of the tensor: the step reads its status) and amplitude in another m
+    block (outside the kept domains: the device check sends the step through
+    fresh domains).  Every step equals the oracle's on the same rows."""
+    import torch
+    from richmol_amd.tdse import TDSE
+    h0, H, cos2, bas = linear_rotor(8)
+    n = sum(len(bas.mlist[J]) for J in bas.Jlist)
+    X = np.zeros((2, n), dtype=np.complex128)
+    X[0, 0] = 1.0
+    X[1, 5] = 1.0
+    Oh0, OH = OracleCarTens.from_attrs(h0), OracleCarTens.from_attrs(H)
+    td = TDSE(t_end=200.0, dt=10.0, t_units="fs")
+    to = OracleTDSE(t_end=200.0, dt=10.0, t_units="fs")
+    td.time_grid(); to.time_grid()
+    vd, vo = torch.from_numpy(X.copy()).to(device), X.copy()
+    kw = {"krylov_fixed": fixed} if fixed else {}
+    worst = 0.0
+    for i, a in enumerate(np.linspace(4e8, 1.6e9, 12)):
+        if i == 4:      # inside the domains: a scaled amplitude
+            vd[0, 0] *= 0.5
+            vo[0, 0] *= 0.5
+        if i == 8:      # outside them: a basis state of another m block
+            vd[1, n - 1] = 0.3
+            vo[1, n - 1] = 0.3
+        H.field([0.0, 0.0, a])
+        OH.field([0.0, 0.0, a])
+        vd, _ = td.update(H, vd, H0=h0, **kw)
+        vo, _ = to.update(OH, vo, H0=Oh0, fixed_m=fixed or None)
+        worst = max(worst, np.abs(vd.cpu().numpy() - vo).max())
+    assert worst <= TOL_PSI, worst
