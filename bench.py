@@ -578,15 +578,16 @@ def main():
             torch.cuda.synchronize()
             return nsw / (time.perf_counter() - t0), v
 
-        single, v_sub = per_step("auto")
-        single_ell, v_ell = per_step("ell")
+        single, v_auto = per_step("auto")
+        single_sub, v_sub = per_step("subspace")
         result["single_wavepacket_fp64"] = {
             "value": single, "steps": nsw,
-            "path": "TDSE.update per step, one wavepacket, FP64 (host loop; one-step rm_propagate on the "
-                    "subspace kernels, domains kept between calls)",
-            "value_rm_step": single_ell,
-            "path_rm_step": "the same loop with kernel='ell' (rm_step, full-vector ELL kernel)",
-            "max_abs_diff_paths": float((v_sub - v_ell).abs().max()),
+            "path": "TDSE.update per step, one wavepacket, FP64 (host loop; update's default for one "
+                    "row: rm_step on the full-vector kernel)",
+            "value_subspace": single_sub,
+            "path_subspace": "the same loop with kernel='subspace' (one-step rm_propagate on the subspace "
+                             "kernels, domains kept between calls)",
+            "max_abs_diff_paths": float((v_sub - v_auto).abs().max()),
             "speedup_of_value": value / single}
         print(json.dumps({"progress": "extras"}), file=sys.stderr, flush=True)
 

@@ -38,8 +38,13 @@ def field_products(rank, field, thresh=None):
         raise IndexError(
             "field variable must be an iterable with three items which represent "
             "field's X, Y, and Z components") from None
-    fprod = {"".join("xyz"[c] for c in comb): np.prod(fxyz[list(comb)])
-             for comb in itertools.product((0, 1, 2), repeat=rank)}
+    f3 = [fxyz[0].item(), fxyz[1].item(), fxyz[2].item()] if fxyz.dtype.kind in "iufc" else list(fxyz)
+    fprod = {}
+    for comb in itertools.product((0, 1, 2), repeat=rank):
+        v = 1
+        for c in comb:
+            v = v * f3[c]
+        fprod["".join("xyz"[c] for c in comb)] = np.float64(v) if isinstance(v, float) else v
     fprod["0"] = 1
     if thresh is not None:
         fprod = {k: v for k, v in fprod.items() if abs(v) >= thresh}

@@ -31,21 +31,42 @@ def _torch():
 
 
 def _stream_handle(stream=None):
+    if stream is not None:
+        return stream.cuda_stream
     torch = _torch()
-    s = stream if stream is not None else torch.cuda.current_stream()
-    return s.cuda_stream
+    try:
+        # the current stream of the current device, without torch's Python
+        # wrapper around the same lookup
+        sid, dev, dtype = torch._C._cuda_getCurrentStream(torch._C._cuda_getDevice())
+        return torch.cuda.Stream(stream_id=sid, device_index=dev, device_type=dtype).cuda_stream
+    except (AttributeError, TypeError):
+        return torch.cuda.current_stream().cuda_stream
 
 
 def leaf_carts(leaf):
-    """Sorted Cartesian components present in a leaf tensor's M matrices."""
-    return sorted({c for Jp in leaf.mmat for sp in leaf.mmat[Jp]
-                   for w in leaf.mmat[Jp][sp] for c in leaf.mmat[Jp][sp][w]})
+    """Sorted Cartesian components present in a leaf tensor's M matrices
+    (kept on the tensor for its structure id: a changed tensor gets a new
+    one)."""
+    got = leaf.__dict__.get("_carts_of")
+    uid = leaf.__dict__.get("_uid")
+    if got is not None and got[0] == uid:
+        return got[1]
+    out = sorted({c for Jp in leaf.mmat for sp in leaf.mmat[Jp]
+                  for w in leaf.mmat[Jp][sp] for c in leaf.mmat[Jp][sp][w]})
+    leaf.__dict__["_carts_of"] = (uid, out)
+    return out
 
 
 def _basis_signature(T):
-    return (tuple(T.Jlist1), tuple(T.Jlist2),
-            tuple((J, tuple(T.symlist1[J]), tuple(T.dim1[J][s] for s in T.symlist1[J])) for J in T.Jlist1),
-            tuple((J, tuple(T.symlist2[J]), tuple(T.dim2[J][s] for s in T.symlist2[J])) for J in T.Jlist2))
+    got = T.__dict__.get("_basis_sig")
+    key = (T.__dict__.get("_uid"), id(T.Jlist1), id(T.Jlist2), id(T.dim1), id(T.dim2))
+    if got is not None and got[0] == key:
+        return got[1]
+    sig = (tuple(T.Jlist1), tuple(T.Jlist2),
+           tuple((J, tuple(T.symlist1[J]), tuple(T.dim1[J][s] for s in T.symlist1[J])) for J in T.Jlist1),
+           tuple((J, tuple(T.symlist2[J]), tuple(T.dim2[J][s] for s in T.symlist2[J])) for J in T.Jlist2))
+    T.__dict__["_basis_sig"] = (key, sig)
+    return sig
 
 
 def active_carts(T):

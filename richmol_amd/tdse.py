@@ -11,6 +11,7 @@ wavepacket.
 like the reference returns a new array) or a torch complex128 device tensor
 (stays on the device; a new tensor is returned).
 """
+import contextlib
 import functools
 
 import numpy as np
@@ -152,10 +153,13 @@ class TDSE:
         adaptive Lanczos, here on the device), tol (default 1e-15).
         Device extensions: maxorder (default 100), krylov_fixed (fixed
         Krylov dimension, default 0 = adaptive), stream (torch stream),
-        kernel ('auto' (default): the step runs as a one-step rm_propagate
-        on the invariant-subspace kernels, with the wavepackets' domains kept
-        from the previous call; or a full-vector kernel name of
-        plan.set_kernel ('ell', 'ell:<i>', 'resident', 'global') for rm_step),
+        kernel ('subspace': the step runs as a one-step rm_propagate on the
+        invariant-subspace kernels, with the wavepackets' domains kept from
+        the previous call; a full-vector kernel name of plan.set_kernel
+        ('ell', 'ell:<i>', 'resident', 'global'): rm_step; 'auto' (default):
+        'subspace' from SUBSPACE_MIN_ROWS rows of vecs on, rm_step with the
+        plan's own full-vector choice below -- the host work per call sets the
+        time of a step of a few wavepackets, and rm_step's is smaller),
         operator_form ('auto' (default), 'flat' or 'kron'): 'kron' keeps the
         operator factored as richmol stores it (Kronecker-block plan; needs
         every J with all m and the full k space and no field thresholds),
@@ -229,8 +233,10 @@ class TDSE:
             # plans are shared with ensembles that may have picked another
             # kernel or precision
             kernel = kwargs.get("kernel", "auto")
-            if plan.kernel != kernel:
-                plan.set_kernel(kernel)
+            sub_step = kernel == "subspace" or (kernel == "auto" and psi.shape[0] >= SUBSPACE_MIN_ROWS)
+            want = "auto" if kernel == "subspace" else kernel
+            if plan.kernel != want:
+                plan.set_kernel(want)
             plan.set_tuning()
             if plan.precision != "fp64":
                 plan.set_precision("fp64")
@@ -294,7 +300,7 @@ class TDSE:
             if bool((status != 0).any()):
                 from .ensemble import ExpokitError
                 raise ExpokitError("maximum number of steps reached without convergence")
-        elif plan.kernel == "auto":
+        elif sub_step:
             out = self._step_subspace(plan, H, psi, f, phase, krylov_fixed, maxorder, tol, stream)
         else:
             status = torch.zeros(ncols, dtype=torch.int32, device=psi.device)
@@ -352,13 +358,14 @@ class TDSE:
         if s["events"][i] is not None:
             s["events"][i].synchronize()     # this staging row's previous copy has run
         s["host"][i].copy_(torch.from_numpy(fprod))
-        cs = stream if stream is not None else torch.cuda.current_stream(psi.device)
+        # the caller's stream, or (no switch, no lookup) the current one
+        on = torch.cuda.stream(stream) if stream is not None else contextlib.nullcontext()
         table = s["table"][i:i + 1]
         buf = s["buf"]
-        with torch.cuda.stream(cs):
+        with on:
             table.copy_(s["host"][i:i + 1], non_blocking=True)
             ev = torch.cuda.Event()
-            ev.record(cs)
+            ev.record()
             s["events"][i] = ev
             if buf.data_ptr() != psi.data_ptr():
                 buf.copy_(psi)
@@ -378,7 +385,7 @@ class TDSE:
             bad = s["status"].cpu().numpy()
             if attempt == 0 and np.any(bad & _lib.RM_STALE_DOMAIN):
                 plan.set_domain_reuse(True, owner=id(self))   # forget the kept domains
-                with torch.cuda.stream(cs):
+                with on:
                     buf.copy_(psi)
                 continue
             break
@@ -388,7 +395,7 @@ class TDSE:
                 raise ValueError(f"Lanczos reached maximum order of '{maxorder}' without convergence")
             if np.any(bad & ~(_lib.RM_NOT_CONVERGED | _lib.RM_STALE_DOMAIN)):
                 raise RuntimeError(f"rm_propagate reported status {int(bad.max())}")
-        with torch.cuda.stream(cs):
+        with on:
             out = buf.clone()
         s["last"], s["last_ver"] = out, out._version
         return out
@@ -505,6 +512,12 @@ class TDSE:
             raise ValueError(f"Lanczos reached maximum order of '{maxorder}' without convergence")
         return work[:, perm], expval
 
+
+# TDSE.update(kernel='auto'): rows of vecs from which a step runs as a one-step
+# rm_propagate on the subspace kernels (measured per call on the J <= 40 linear
+# rotor: 1 row 76 us vs 53 us through rm_step, 16 rows 79 vs 72 us fixed-m and
+# 118 vs 124 us adaptive, 512 rows 75 vs 97 us)
+SUBSPACE_MIN_ROWS = 16
 
 # 'auto' operator form: bases at least this large with every J carrying all m
 # and the full k space keep the operator factored (a flattened asymmetric top

@@ -95,9 +95,10 @@ def _run_pair(h0, H, fields, X, *, split=True, tol=1e-15, fixed=None, dt=10.0, t
     return dev, kdev, psi_d.cpu().numpy(), psi_o
 
 
-# 'auto': the one-step rm_propagate on the invariant-subspace kernels (the
-# default of update); the others: rm_step with that full-vector kernel
-KERNELS = ["auto", "global", "resident", "ell", "ell:7"]
+# 'subspace': the one-step rm_propagate on the invariant-subspace kernels;
+# 'auto' (update's default): rm_step below 16 rows, as these tests have; the
+# others: rm_step with that full-vector kernel
+KERNELS = ["subspace", "auto", "global", "resident", "ell", "ell:7"]
 
 
 @pytest.mark.parametrize("kernel", KERNELS)
@@ -278,7 +279,7 @@ def test_update_rows_rewritten_between_calls(device, fixed):
     to = OracleTDSE(t_end=200.0, dt=10.0, t_units="fs")
     td.time_grid(); to.time_grid()
     vd, vo = torch.from_numpy(X.copy()).to(device), X.copy()
-    kw = {"krylov_fixed": fixed} if fixed else {}
+    kw = {"krylov_fixed": fixed, "kernel": "subspace"} if fixed else {"kernel": "subspace"}
     worst = 0.0
     for i, a in enumerate(np.linspace(4e8, 1.6e9, 12)):
         if i == 4:      # inside the domains: a scaled amplitude
