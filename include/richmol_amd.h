@@ -205,11 +205,13 @@ int rm_plan_kernel_time(rm_plan* plan, double* total_ms, int64_t* launches);
  * steps in one launch, with the operator A(F) = sum_s F_s B_s assembled per
  * step from the field table.  Applies when no MF threshold is given; results
  * equal the full-vector kernels' to rounding (entries outside the domain stay
- * exactly zero).  Domains of more than RM_SUBSPACE_MAX_ROWS rows are advanced
- * by a cooperative launch (every workgroup of the device on one wavepacket,
- * grid barriers between the Lanczos reductions) when there are at most 8 of
- * them, otherwise by the block kernel (one workgroup per wavepacket, all in
- * one launch).
+ * exactly zero).  Domains of more than 64 rows (RM_TUNE_WAVE_MAX_ROWS; at most
+ * RM_SUBSPACE_MAX_ROWS) are advanced by a cooperative launch (many
+ * workgroups on one wavepacket, grid barriers between the Lanczos
+ * reductions) when there are at most 8 of them and each holds at least
+ * their count x 6400 stored entries (RM_TUNE_COOP_MAX,
+ * RM_TUNE_COOP_MIN_ENTRIES), otherwise by the block kernel (one workgroup
+ * per wavepacket, all in one launch).
  *
  * RM_KERNEL_COOP, RM_KERNEL_BLOCK (rm_propagate only): every wavepacket
  * through the cooperative, or the block, kernel whatever its domain (no MF
@@ -240,13 +242,20 @@ int rm_plan_set_kernel(rm_plan* plan, int32_t variant);
  *  RM_TUNE_COOP_GRID workgroups of a cooperative launch at most (default 0:
  *                    one per 64-row group of the domain, up to one per CU);
  *  RM_TUNE_FORK      1 (default): the launch classes of a call side by side
- *                    on auxiliary streams; 0: one after another.
+ *                    on auxiliary streams; 0: one after another;
+ *  RM_TUNE_WAVE_MAX_ROWS  domains longer than this (default 64, at most
+ *                    RM_SUBSPACE_MAX_ROWS) take the long-domain forms;
+ *  RM_TUNE_COOP_MIN_ENTRIES  the cooperative form only while every long
+ *                    domain holds at least (their count) x this many stored
+ *                    entries (default 6400: it runs them one at a time).
  * Results do not depend on these choices beyond summation order.  A change
  * drops the launch lists kept for domain reuse. */
 #define RM_TUNE_PAIR       0
 #define RM_TUNE_COOP_MAX   1
 #define RM_TUNE_COOP_GRID  2
 #define RM_TUNE_FORK       3
+#define RM_TUNE_WAVE_MAX_ROWS    4
+#define RM_TUNE_COOP_MIN_ENTRIES 5
 int rm_plan_set_tuning(rm_plan* plan, int32_t knob, int32_t value);
 
 /* Launch configuration i of the ELL kernel: threads per workgroup, rows per

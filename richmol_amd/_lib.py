@@ -15,8 +15,9 @@ RM_NOT_CONVERGED = 1
 RM_DEVICE_TIMEOUT = 2
 RM_STALE_DOMAIN = 4
 # rm_plan_set_tuning knobs and their defaults
-TUNE_KNOBS = {"pair": 0, "coop_max": 1, "coop_grid": 2, "fork": 3}
-TUNE_DEFAULTS = {"pair": 1, "coop_max": 8, "coop_grid": 0, "fork": 1}
+TUNE_KNOBS = {"pair": 0, "coop_max": 1, "coop_grid": 2, "fork": 3, "wave_max_rows": 4, "coop_min_entries": 5}
+TUNE_DEFAULTS = {"pair": 1, "coop_max": 8, "coop_grid": 0, "fork": 1, "wave_max_rows": 64,
+                 "coop_min_entries": 6400}
 RM_MAX_FIELD = 32
 RM_MAX_LEAF = 8
 RM_MAX_KRYLOV = 128

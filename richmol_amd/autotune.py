@@ -48,13 +48,14 @@ def _device_name():
 
 
 def domain_shape(plan, X):
-    """(pairable domains, one-wave domains, long domains, rows of the
+    """(domains of <= 32 rows, of 33-64, of 65-1024, longer, rows of the
     longest) of the rows X: the part of the key that decides the launch
     classes."""
     rows, _ = plan.domain_sizes(np.asarray(X))
     rows = rows[rows > 0]
-    return (int(np.sum(rows <= 32)), int(np.sum((rows > 32) & (rows <= SUBSPACE_MAX_ROWS))),
-            int(np.sum(rows > SUBSPACE_MAX_ROWS)), int(rows.max(initial=0)))
+    return (int(np.sum(rows <= 32)), int(np.sum((rows > 32) & (rows <= 64))),
+            int(np.sum((rows > 64) & (rows <= SUBSPACE_MAX_ROWS))), int(np.sum(rows > SUBSPACE_MAX_ROWS)),
+            int(rows.max(initial=0)))
 
 
 def key(plan, ncols, krylov_fixed, shape=None):
@@ -72,20 +73,25 @@ def candidates(plan, ncols, krylov_fixed=0, shape=None):
     from .plan import ell_configs
     out = [("auto", {})]
     if shape is not None:
-        npair, nwave, nlong, lmax = shape
+        npair, nwave, nmid, nlong, lmax = shape
         if npair and 1 <= krylov_fixed <= 16:
             out.append(("auto", {"pair": 0}))
-        if (npair > 0) + (nwave > 0) + (nlong > 0) > 1 or npair > 0 and nwave > 0:
+        if (npair > 0) + (nwave > 0) + (nmid + nlong > 0) > 1:
             out.append(("auto", {"fork": 0}))
-        if nlong:
+        if nmid:
+            out.append(("auto", {"wave_max_rows": SUBSPACE_MAX_ROWS}))     # one wave each up to 1024 rows
+        nbig = nmid + nlong
+        if nbig:
             groups = (lmax + 63) // 64
-            if nlong <= 8:
-                out.append(("auto", {"coop_max": 0}))                      # one workgroup each
+            if nbig <= 8:
+                # one workgroup each / cooperative, in turn, whatever the entries
+                out.append(("auto", {"coop_max": 0}))
+                out.append(("auto", {"coop_min_entries": 0}))
                 for g in (16, 32, 64):
                     if g < groups:
-                        out.append(("auto", {"coop_grid": g}))
+                        out.append(("auto", {"coop_grid": g, "coop_min_entries": 0}))
             else:
-                out.append(("auto", {"coop_max": nlong}))                  # cooperative, in turn
+                out.append(("auto", {"coop_max": nbig, "coop_min_entries": 0}))   # cooperative, in turn
     for i, (ntr, r, c, opl) in enumerate(ell_configs()):
         if plan.n <= ntr * r and (c == 1 or ncols >= 2):
             out.append((f"ell:{i}", {}))

@@ -86,6 +86,8 @@ struct RmTuning {
   int32_t coop_max = 8;    // long-domain wavepackets per call up to which the cooperative form runs
   int32_t coop_grid = 0;   // workgroups of a cooperative launch at most (0: the domain's row groups, one per CU)
   int32_t fork = 1;        // launch classes of a call side by side on auxiliary streams
+  int32_t wave_max_rows = 64;       // domains longer than this leave the one-wave kernels
+  int32_t coop_min_entries = 6400;  // stored entries per cooperative wavepacket at least
 };
 
 struct rm_plan {

@@ -1493,6 +1493,16 @@ extern "C" int rm_plan_set_tuning(rm_plan* p, int32_t knob, int32_t value) {
       if (value < 0) return fail(RM_ERR_ARG, "rm_plan_set_tuning: negative cooperative grid %d", value);
       p->tune.coop_grid = value;
       break;
+    case RM_TUNE_WAVE_MAX_ROWS:
+      if (value < 0 || value > RM_SUBSPACE_MAX_ROWS)
+        return fail(RM_ERR_ARG, "rm_plan_set_tuning: one-wave domain limit %d outside 0..%d", value,
+                    RM_SUBSPACE_MAX_ROWS);
+      p->tune.wave_max_rows = value;
+      break;
+    case RM_TUNE_COOP_MIN_ENTRIES:
+      if (value < 0) return fail(RM_ERR_ARG, "rm_plan_set_tuning: negative entry count %d", value);
+      p->tune.coop_min_entries = value;
+      break;
     default:
       return fail(RM_ERR_ARG, "rm_plan_set_tuning: unknown knob %d", knob);
   }
@@ -1555,9 +1565,11 @@ void sort_long(std::vector<LongItem>& v) {
 
 // The subspace path of rm_propagate.  Arguments are validated by the caller
 // (ld, ncols, Krylov parameters, pointers).  Domains of up to
-// RM_SUBSPACE_MAX_ROWS rows go to the one-wave kernel; longer ones to the
-// cooperative kernel when there are at most p->tune.coop_max of them (8
-// unless tuned), else to the block kernel.  force: RM_KERNEL_COOP / RM_KERNEL_BLOCK send every
+// p->tune.wave_max_rows rows (64 unless tuned, at most RM_SUBSPACE_MAX_ROWS)
+// go to the one-wave kernels; longer ones to the cooperative kernel when
+// there are at most p->tune.coop_max of them and each holds at least
+// their count times p->tune.coop_min_entries stored entries, else to the
+// block kernel.  force: RM_KERNEL_COOP / RM_KERNEL_BLOCK send every
 // wavepacket to that kernel (0: the rule above).  *done = false when the
 // path does not apply; nothing has then been written to psi.
 // The observable plan's matrix in p's permuted ELL form (p->obs_col,
@@ -1732,7 +1744,7 @@ int sub_propagate(rm_plan* p, void* psi, int64_t ld, int64_t ncols, int64_t nste
     for (int64_t b = 0; b < ncols; ++b) {
       const int32_t lo = p->h_dom[2 * b], hi = p->h_dom[2 * b + 1];
       if (hi <= lo) continue;   // zero column: stays zero (kused 0)
-      if (force || hi - lo > RM_SUBSPACE_MAX_ROWS || !p->sub_col) {
+      if (force || hi - lo > std::min<int32_t>(p->tune.wave_max_rows, RM_SUBSPACE_MAX_ROWS) || !p->sub_col) {
         big.push_back(make_int4((int)b, lo, hi - lo, 0));
       } else {
         const int c = sub_class(p->sub_w, hi - lo);
@@ -1741,8 +1753,21 @@ int sub_propagate(rm_plan* p, void* psi, int64_t ld, int64_t ncols, int64_t nste
       }
     }
     if (!big.empty() && !p->sub_rp && (rc = csr_build(p, entry_values(p)))) return rc;
-    const bool to_coop = force == RM_KERNEL_COOP || (force == 0 && (int64_t)big.size() <= p->tune.coop_max);
-    for (const int4& it : big) (to_coop ? sc->kept_coop : sc->kept_blk).push_back(long_item(p, it.x, it.y, it.z, 0));
+    // the cooperative form runs its wavepackets one launch after another on
+    // many workgroups, the block form all of them at once, one workgroup
+    // each: cooperative while the count times ~80 us (a cooperative step)
+    // stays below one workgroup's step on the smallest of them (~0.0125 us
+    // per stored entry), i.e. count * coop_min_entries <= its entries
+    std::vector<LongItem> lbig;
+    int64_t emin = INT64_MAX;
+    for (const int4& it : big) {
+      lbig.push_back(long_item(p, it.x, it.y, it.z, 0));
+      emin = std::min<int64_t>(emin, lbig.back().nent);
+    }
+    const bool to_coop = force == RM_KERNEL_COOP ||
+                         (force == 0 && (int64_t)big.size() <= p->tune.coop_max &&
+                          (int64_t)big.size() * p->tune.coop_min_entries <= emin);
+    for (const LongItem& it : lbig) (to_coop ? sc->kept_coop : sc->kept_blk).push_back(it);
     sort_long(sc->kept_blk);
     for (auto& v : sc->kept_cls)
       std::stable_sort(v.begin(), v.end(), [](const int4& x, const int4& y) { return x.z > y.z; });

@@ -28,6 +28,30 @@ def _run_tuned(h0, H, fields, X, tuning, fixed=0):
     return out
 
 
+@pytest.mark.parametrize("tuning", [{"wave_max_rows": 1024}, {"wave_max_rows": 1024, "coop_min_entries": 0},
+                                    {"wave_max_rows": 16}])
+def test_mid_domain_choices_equal_oracle(device, tuning):
+    """An asymmetric top on its m = 0 basis (J <= 12, Z field: one domain of
+    ~50 rows for the ground state, one of ~40 for an excited state): the
+    one-wave kernels up to 1024 rows, the long forms from 17 rows on, and the
+    default -- each against the oracle, adaptive."""
+    from richmol_amd import rot, units
+    bas = rot.asymtop_basis(9.28, 27.88, 14.51, 12, m_filter=lambda J, m: m == 0)
+    h0 = rot.hamiltonian(bas)
+    H = rot.tensor(np.diag([9.14, 9.87, 9.45]), bas) * (-0.5 * units.AUpol_x_Vm_to_invcm())
+    n = sum(sum(c.shape[1] for _, c in bas.kfun[J].values()) for J in bas.Jlist)
+    X = np.zeros((2, n), dtype=np.complex128)
+    X[0, 0] = 1.0
+    X[1, 3] = 1.0
+    fields = [[0.0, 0.0, a] for a in np.linspace(2e9, 6e9, 5)]
+    base, kb = _run_tuned(h0, H, fields, X, {})
+    got, kg = _run_tuned(h0, H, fields, X, tuning)
+    psi_o, k_o = _oracle_run(h0, H, fields, X)
+    assert np.abs(base - psi_o).max() <= TOL_PSI
+    assert np.abs(got - psi_o).max() <= TOL_PSI
+    assert np.array_equal(kg, k_o) and np.array_equal(kb, k_o)
+
+
 @pytest.mark.parametrize("tuning", [{"pair": 0}, {"fork": 0}, {"pair": 0, "fork": 0}])
 def test_wave_choices_equal_default(device, tuning):
     """J <= 12 all m, Z field, fixed Krylov dimension 12: pairable domains
@@ -50,9 +74,10 @@ def test_wave_choices_equal_default(device, tuning):
 def test_long_domain_choices_equal_default(device, tuning, fixed):
     """J <= 40 with an X+Z field: two dense columns on 1681-row domains (27
     row groups of 64).  A cooperative grid of 2 or 6 workgroups (fewer waves
-    than row groups: the memory form loops over them), the one-workgroup form
-    for both (coop_max 0), or one cooperative and one in order (coop_max 1)
-    -- each equal to the default and to the oracle, same Krylov dimensions."""
+    than row groups: the memory form loops over them), or the one-workgroup
+    form for both (coop_max 0, or 1 below their count) -- each equal to the
+    default (cooperative: two domains of ~2e4 entries each) and to the
+    oracle, same Krylov dimensions."""
     h0, H, cos2, bas = linear_rotor(40)
     rng = np.random.default_rng(31)
     X = rng.normal(size=(2, 1681)) + 1j * rng.normal(size=(2, 1681))
