@@ -291,12 +291,23 @@ __device__ __forceinline__ double inv_beta(double beta) {
   return ib;
 }
 
+// Dynamic LDS of the XL stage (matvec input and row offsets), rounded up to
+// 256 bytes: the LV vectors follow it.
+template <bool F32>
+constexpr size_t xl_bytes() {
+  return ((F32 ? sizeof(float2) : sizeof(double2)) * XL_ROWS + sizeof(int) * (XL_ROWS + 1) + 255) & ~(size_t)255;
+}
+
 // XL (block form, domains of at most XL_ROWS rows): the matvec input is
 // staged in LDS by the threads that write it, and gathered from there.
-template <int MODE, bool FIXED, bool F32, bool XL = false>
+// LV (with XL): the Krylov basis and the step's vectors (V, W, Y, S, P) sit
+// in LDS after the stage instead of the workgroup's global region (the host
+// takes it when they fit); the step's operator array stays global.
+template <int MODE, bool FIXED, bool F32, bool XL = false, bool LV = false>
 __global__ __launch_bounds__(ct_of<MODE>()) void k_long(LongArgs a) {
   using XT = typename XType<F32>::T;
   static_assert(!XL || MODE == BLOCK, "LDS-staged input is for the block form");
+  static_assert(!LV || XL, "LDS vectors come with the LDS-staged input");
   extern __shared__ char s_dyn[];
   XT* s_x = (XT*)s_dyn;                                   // [XL_ROWS] matvec input
   int* s_rp = (int*)(s_dyn + sizeof(XT) * XL_ROWS);       // [XL_ROWS + 1] row offsets from e_lo
@@ -322,12 +333,13 @@ __global__ __launch_bounds__(ct_of<MODE>()) void k_long(LongArgs a) {
   const int sub = lane % LPR, rw = lane / LPR;
   const int kc = a.kcap;
   char* base = a.ws + (MODE == BLOCK ? (int64_t)blockIdx.x * a.wg_bytes : 0);
-  double2* V = (double2*)(base + a.offV);
-  double2* Wb = (double2*)(base + a.offWb);
-  float2* Wf = (float2*)(base + a.offWf);
-  double2* Y = (double2*)(base + a.offY);
-  double2* S = (double2*)(base + a.offS);
-  double2* P = (double2*)(base + a.offP);
+  char* vbase = LV ? s_dyn + xl_bytes<F32>() : base;      // the vectors' region (offsets below offA)
+  double2* V = (double2*)(vbase + a.offV);
+  double2* Wb = (double2*)(vbase + a.offWb);
+  float2* Wf = (float2*)(vbase + a.offWf);
+  double2* Y = (double2*)(vbase + a.offY);
+  double2* S = (double2*)(vbase + a.offS);
+  double2* P = (double2*)(vbase + a.offP);
   XT* Ast = (XT*)(base + a.offA);
   const XT* Xin;
   if constexpr (XL)
@@ -732,10 +744,10 @@ struct LongScratch {
   size_t bytes = 0;
 };
 
-template <int MODE, bool FIXED, bool F32, bool XL>
+template <int MODE, bool FIXED, bool F32, bool XL, bool LV = false>
 hipError_t launch_r(LongArgs& a, int grid, size_t lds, hipStream_t s) {
   if (MODE == BLOCK) {
-    hipLaunchKernelGGL((k_long<MODE, FIXED, F32, XL>), dim3(grid), dim3(ct_of<MODE>()), lds, s, a);
+    hipLaunchKernelGGL((k_long<MODE, FIXED, F32, XL, LV>), dim3(grid), dim3(ct_of<MODE>()), lds, s, a);
     return hipGetLastError();
   }
   void* args[] = {&a};
@@ -743,20 +755,23 @@ hipError_t launch_r(LongArgs& a, int grid, size_t lds, hipStream_t s) {
                                     args, lds, s);
 }
 
-template <int MODE, bool XL = false>
+template <int MODE, bool XL = false, bool LV = false>
 hipError_t launch_m(LongArgs& a, int grid, bool f32, size_t lds, hipStream_t s) {
   const bool fixed = a.fixed_m > 0;
   if (fixed)
-    return f32 ? launch_r<MODE, true, true, XL>(a, grid, lds, s) : launch_r<MODE, true, false, XL>(a, grid, lds, s);
-  return f32 ? launch_r<MODE, false, true, XL>(a, grid, lds, s) : launch_r<MODE, false, false, XL>(a, grid, lds, s);
+    return f32 ? launch_r<MODE, true, true, XL, LV>(a, grid, lds, s)
+               : launch_r<MODE, true, false, XL, LV>(a, grid, lds, s);
+  return f32 ? launch_r<MODE, false, true, XL, LV>(a, grid, lds, s)
+             : launch_r<MODE, false, false, XL, LV>(a, grid, lds, s);
 }
 
 // Workgroups of one form the device holds at once (the smallest over its
-// instantiations) with lds bytes of dynamic LDS each.
-template <int MODE, bool XL = false>
-hipError_t max_resident(int n_cu, size_t lds, int* grid) {
-  const void* ks[] = {(const void*)k_long<MODE, true, true, XL>, (const void*)k_long<MODE, true, false, XL>,
-                      (const void*)k_long<MODE, false, true, XL>, (const void*)k_long<MODE, false, false, XL>};
+// instantiations) with lds bytes of dynamic LDS each; 0 when one of them
+// cannot run with that much.
+template <int MODE, bool XL = false, bool LV = false>
+hipError_t resident(int n_cu, size_t lds, int* grid) {
+  const void* ks[] = {(const void*)k_long<MODE, true, true, XL, LV>, (const void*)k_long<MODE, true, false, XL, LV>,
+                      (const void*)k_long<MODE, false, true, XL, LV>, (const void*)k_long<MODE, false, false, XL, LV>};
   int occ = INT32_MAX;
   for (const void* k : ks) {
     int o = 0;
@@ -764,8 +779,15 @@ hipError_t max_resident(int n_cu, size_t lds, int* grid) {
     if (e != hipSuccess) return e;
     occ = std::min(occ, o);
   }
-  *grid = std::max(1, occ) * n_cu;
+  *grid = occ * n_cu;
   return hipSuccess;
+}
+
+template <int MODE, bool XL = false>
+hipError_t max_resident(int n_cu, size_t lds, int* grid) {
+  hipError_t e = resident<MODE, XL>(n_cu, lds, grid);
+  *grid = std::max(*grid, n_cu);
+  return e;
 }
 
 size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -897,7 +919,26 @@ int long_run(void** scratch, const SubOp* d_ops, const int4* d_blk, const std::v
     ab.ws = wb;
     ab.items = d_blk;
     ab.nitems = (int32_t)nx;
-    HIPCHK((launch_m<BLOCK, true>(ab, (int)std::min<int64_t>((int64_t)nx, gres_x), f32, lds_x, s)));
+    // the vectors of the longest of these domains in LDS beside the stage,
+    // when the kernel still fits a CU with them (fixed Krylov dimensions and
+    // domains of up to ~500 rows); the region layout's vector part ends at offA
+    int64_t lx = 0;
+    for (size_t i = 0; i < nx; ++i) lx = std::max<int64_t>(lx, blk[i].item.z);
+    LongArgs lv = ab;
+    const size_t vbytes = region_layout(lv, lx, 1, kcap, f32, false);
+    const size_t lds_v = (f32 ? xl_bytes<true>() : xl_bytes<false>()) + vbytes;
+    int gres_v = 0;
+    if (lds_v <= 160 * 1024) HIPCHK((resident<BLOCK, true, true>(n_cu, lds_v, &gres_v)));
+    if (gres_v > 0) {
+      // same offsets as the global region's vectors (lmax differs: a layout of their own)
+      lv.ws = wb;
+      lv.offA = ab.offA;
+      lv.wg_bytes = ab.wg_bytes;
+      lv.lmax = lx;
+      HIPCHK((launch_m<BLOCK, true, true>(lv, (int)std::min<int64_t>((int64_t)nx, gres_v), f32, lds_v, s)));
+    } else {
+      HIPCHK((launch_m<BLOCK, true>(ab, (int)std::min<int64_t>((int64_t)nx, gres_x), f32, lds_x, s)));
+    }
   }
   if (blk.size() > nx) {
     ab.ws = wb;
