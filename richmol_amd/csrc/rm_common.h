@@ -47,6 +47,14 @@ __device__ __forceinline__ double2 cscale(double s, double2 a) {
   return make_double2(s * a.x, s * a.y);
 }
 __device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+// *p when c holds, else 0, as a value.  (`c ? p[k] : zero` with a named zero
+// is an lvalue: the compiler selects between the two addresses, keeps the zero
+// in scratch memory and reloads it wherever it is used.)
+__device__ __forceinline__ double2 load_or_zero(bool c, const double2* p) {
+  double2 v = make_double2(0.0, 0.0);
+  if (c) v = *p;
+  return v;
+}
 // acc + a*b
 __device__ __forceinline__ double2 cfma(double2 a, double2 b, double2 acc) {
   return make_double2(acc.x + (a.x * b.x - a.y * b.y), acc.y + (a.x * b.y + a.y * b.x));
@@ -74,6 +82,10 @@ __device__ __forceinline__ float2 to_x<true>(double2 v) {
 }
 __device__ __forceinline__ double2 xfma(double2 a, double2 b, double2 acc) { return cfma(a, b, acc); }
 __device__ __forceinline__ float2 xfma(float2 a, float2 b, float2 acc) { return cfmaf(a, b, acc); }
+__device__ __forceinline__ double2 xmul(double2 a, double2 b) { return cmul(a, b); }
+__device__ __forceinline__ float2 xmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
 
 
 // ---------------------------------------------------------------------------
@@ -409,24 +421,34 @@ __device__ __forceinline__ double half_allsum32(double v) {
   return e + o;
 }
 
-// sqrt(s) and 1/sqrt(s) for s > 0 without a branch: s is first scaled by an
-// exact power of two (2^1000 below 2^-900, 2^-1000 above 2^900) so that the
-// hardware reciprocal square root and its two Newton steps stay in range,
-// and the results are scaled back exactly.  s == 0 gives no meaningful
-// result (the caller's breakdown path replaces it).
-__device__ __forceinline__ void sqrt_rcp_nb(double s, double* beta, double* ib) {
-  const bool tiny = s < 0x1p-900, huge = s > 0x1p+900;
-  const double up = tiny ? 0x1p+1000 : (huge ? 0x1p-1000 : 1.0);
-  const double hup = tiny ? 0x1p+500 : (huge ? 0x1p-500 : 1.0);     // sqrt(up)
-  const double hdn = tiny ? 0x1p-500 : (huge ? 0x1p+500 : 1.0);     // 1 / sqrt(up)
-  const double t = s * up;
+// sqrt(s) and 1/sqrt(s) for s > 0: the hardware reciprocal square root and
+// two Newton steps.  Outside [2^-900, 2^900] s is first scaled by an exact
+// power of two (2^1000 below, 2^-1000 above) so that they stay in range, and
+// the results are scaled back exactly; inside it the scale is 1, so that
+// range is computed straight away and the scaled form redone only on lanes
+// outside it (the range test is off the dependent chain).  s == 0 gives no
+// meaningful result (the caller's breakdown path replaces it).
+__device__ __forceinline__ void sqrt_rcp_newton(double t, double* b, double* r_out) {
   double r = __builtin_amdgcn_rsq(t);
   double e = fma(-t * r, r, 1.0);
   r = fma(0.5 * r, e, r);
   e = fma(-t * r, r, 1.0);
   r = fma(0.5 * r, e, r);
-  *beta = (t * r) * hdn;
-  *ib = r * hup;
+  *b = t * r;
+  *r_out = r;
+}
+__device__ __forceinline__ void sqrt_rcp_nb(double s, double* beta, double* ib) {
+  sqrt_rcp_newton(s, beta, ib);
+  const bool tiny = s < 0x1p-900, huge = s > 0x1p+900;
+  if (tiny || huge) {
+    const double up = tiny ? 0x1p+1000 : 0x1p-1000;
+    const double hup = tiny ? 0x1p+500 : 0x1p-500;     // sqrt(up)
+    const double hdn = tiny ? 0x1p-500 : 0x1p+500;     // 1 / sqrt(up)
+    double b, r;
+    sqrt_rcp_newton(s * up, &b, &r);
+    *beta = b * hdn;
+    *ib = r * hup;
+  }
 }
 
 __device__ __forceinline__ double wave_max(double v) {
@@ -465,21 +487,34 @@ __device__ __forceinline__ double taylor_rinv(int j) {
   return r[j];
 }
 
-// Taylor terms for a scaled norm x <= 1/2: the smallest j with x^j / j! < 2^-60,
-// from the increasing thresholds x_j = (2^-60 j!)^(1/j) (x^j / j! < 2^-60 iff
-// x < x_j), at most TAYLOR_TERMS -- independent comparisons instead of a
-// chain of divisions.
-__device__ __forceinline__ int taylor_terms(double x) {
-  constexpr double thr[TAYLOR_TERMS - 1] = {
-      0x1.0000000000008p-60, 0x1.6a09e667f3bd8p-30, 0x1.d12ed0af1a27ep-20, 0x1.1b4f819c2ff7fp-14,
-      0x1.4d763f00a146ep-11, 0x1.7f34ae136c8e5p-9,  0x1.23257ccb0c5fcp-7,  0x1.54b5ca9e6358bp-6,
-      0x1.4e681294f9348p-5,  0x1.21d6b0d9fac89p-4,  0x1.ca8e511edefb5p-4,  0x1.527c8d13037b4p-3,
-      0x1.d98d19e3bfb28p-3,  0x1.3d6a8eb921352p-2,  0x1.9b19641f27bf7p-2,  0x1.02ca0b1cb9cc0p-1,
-      0x1.3e3dbd2d3b2f7p-1};
-  int n = 1;
+// Taylor terms for a scaled norm x <= 1/2: the smallest j with x^j / j! < 2^-60
+// (x_j = (2^-60 j!)^(1/j) increasing: x^j / j! < 2^-60 iff x < x_j), at most
+// TAYLOR_TERMS -- independent comparisons instead of a chain of divisions.
+// They compare the high 32 bits of x and x_j as integers (for x >= 0 the bit
+// pattern orders as the value), i.e. against x_j rounded down to 20 mantissa
+// bits: at most one term more than the exact count, never fewer.  As integer
+// comparisons they run on the scalar unit when x is the same on every lane
+// (taylor_terms_hi of a lane read).
+constexpr uint32_t TAYLOR_THR_HI[TAYLOR_TERMS - 1] = {
+    0x3c300000u, 0x3e16a09eu, 0x3ebd12edu, 0x3f11b4f8u, 0x3f44d763u, 0x3f67f34au,
+    0x3f823257u, 0x3f954b5cu, 0x3fa4e681u, 0x3fb21d6bu, 0x3fbca8e5u, 0x3fc527c8u,
+    0x3fcd98d1u, 0x3fd3d6a8u, 0x3fd9b196u, 0x3fe02ca0u, 0x3fe3e3dbu};
+__device__ __forceinline__ int taylor_terms_hi(uint32_t xh) {
+  // xh >= t iff (t - 1 - xh) < 0: its sign bit (both below 2^31), summed as
+  // plain integer arithmetic (no condition codes, no vector round trip)
+  uint32_t n = 1;
 #pragma unroll
-  for (int j = 0; j < TAYLOR_TERMS - 1; ++j) n += x >= thr[j] ? 1 : 0;
-  return n;
+  for (int j = 0; j < TAYLOR_TERMS - 1; ++j) n += (TAYLOR_THR_HI[j] - 1u - xh) >> 31;
+  return (int)n;
+}
+// the high word of x without its sign (x = -0 counts as +0)
+__device__ __forceinline__ uint32_t abs_hi(double x) { return (uint32_t)__double2hiint(x) & 0x7fffffffu; }
+__device__ __forceinline__ int taylor_terms(double x) { return taylor_terms_hi(abs_hi(x)); }
+// the counts of lane 0's and lane 32's x (the same on every lane of a half),
+// on the scalar unit
+__device__ __forceinline__ void taylor_terms_halves(double x, int* c0, int* c1) {
+  *c0 = taylor_terms_hi((uint32_t)__builtin_amdgcn_readlane((int)abs_hi(x), 0));
+  *c1 = taylor_terms_hi((uint32_t)__builtin_amdgcn_readlane((int)abs_hi(x), 32));
 }
 
 // c = exp(f*T) e_0 for the s x s tridiagonal T (diag alpha[0..s-1], off-diagonal
@@ -545,12 +580,12 @@ static __device__ void expm_e0(int s, const double2* alpha, const double* beta, 
 }
 
 // DPP move of a double within 16-lane rows; lanes whose source falls outside
-// the row receive 0.  CTRL 0x111 (row_shr:1): lane l gets lane l-1;
+// the row receive 0 (bound control: no zero old value to set up first).  CTRL 0x111 (row_shr:1): lane l gets lane l-1;
 // CTRL 0x101 (row_shl:1): lane l gets lane l+1.
 template <int CTRL>
 __device__ __forceinline__ double dpp_row_shift0(double v) {
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xf, 0xf, false);
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xf, 0xf, true);
   return __hiloint2double(hi, lo);
 }
 
@@ -610,10 +645,12 @@ static __device__ void expm_e0_16(int s, const double2* alpha, const double* bet
   if (norm1 > 0.5) nsub = (int)ceil(2.0 * norm1);
   if (!(nsub >= 1) || nsub > (1 << 20)) nsub = 1;
   const double h = 1.0 / (double)nsub;
-  const int nterm = taylor_terms(h * norm1);
-  // the larger count of the two halves runs (uniform, unrolled); each lane
-  // adds only the terms of its own count
-  const int nmax = max(__builtin_amdgcn_readlane(nterm, 0), __builtin_amdgcn_readlane(nterm, 32));
+  // each half's count (h norm1 is the same on its lanes); the larger one runs
+  // (uniform, unrolled) and each lane adds only the terms of its own count
+  int c0, c1;
+  taylor_terms_halves(h * norm1, &c0, &c1);
+  const int nterm = l < 32 ? c0 : c1;
+  const int nmax = max(c0, c1);
   double2 y = make_double2(i == 0 ? 1.0 : 0.0, 0.0);
   for (int sub = 0; sub < nsub; ++sub) {
     double2 term = y, acc = y;
@@ -643,8 +680,8 @@ constexpr double rfact_ct(int j) { return j <= 1 ? 1.0 : rfact_ct(j - 1) / (doub
 
 // Terms J.. of the real recursion of expm_e0_16_herm, unrolled at compile
 // time (the weight 1/J! and the power of i are constants of the term): w <-
-// (phi T) w, and y += i^J w / J! on the lanes whose own count reaches J;
-// nmax (uniform) ends the sequence.
+// (phi T) w, and y += i^J w / J! (one rounding) on the lanes whose own count
+// reaches J; nmax (uniform) ends the sequence.
 template <int J>
 __device__ __forceinline__ void herm_terms(int nmax, int nterm, double a, double bl, double bu, double& w,
                                            double2& y) {
@@ -652,34 +689,33 @@ __device__ __forceinline__ void herm_terms(int nmax, int nterm, double a, double
     if (J > nmax) return;
     const double prev = dpp_row_shift0<0x111>(w), next = dpp_row_shift0<0x101>(w);
     w = fma(bu, next, fma(bl, prev, a * w));
-    constexpr double rf = rfact_ct(J);
-    const double t = J <= nterm ? rf * w : 0.0;
-    if constexpr ((J & 3) == 1) y.y += t;
-    else if constexpr ((J & 3) == 2) y.x -= t;
-    else if constexpr ((J & 3) == 3) y.y -= t;
-    else y.x += t;
+    constexpr double rf = ((J & 3) >= 2) ? -rfact_ct(J) : rfact_ct(J);   // i^J / J!: re for even J, im for odd
+    double& yc = (J & 1) ? y.y : y.x;
+    if (J <= nterm) yc = fma(rf, w, yc);
     herm_terms<J + 1>(nmax, nterm, a, bl, bu, w, y);
   }
 }
 
+// The exponential from this lane's row i = l % 16 of phi T (a = phi alpha_i,
+// bl = phi beta_i, bu = phi beta_{i+1}, each 0 outside the s x s matrix) and
+// the 1-norm of phi T (the same on every lane of a problem).
 template <bool PAIR = false>
-static __device__ void expm_e0_16_herm(int s, const double2* alpha, const double* beta, double phi, double2* c) {
+static __device__ void expm_e0_16_herm_row(int s, double a, double bl, double bu, double norm1, double2* c) {
   const int l = threadIdx.x & 63;
   const int i = l & 15;
-  double a = 0.0, bl = 0.0, bu = 0.0, col1 = 0.0;
-  if (i < s) {
-    a = phi * alpha[i].x;
-    if (i >= 1) bl = phi * beta[i];
-    if (i + 1 < s) bu = phi * beta[i + 1];
-    col1 = fabs(a) + fabs(bl) + fabs(bu);
-  }
-  const double norm1 = row_max16(col1);
   double2 y = make_double2(i == 0 ? 1.0 : 0.0, 0.0);
   // each half decides for itself (the rows of a half are the rows of its
-  // DPP rows, so the two halves may take different branches below)
+  // DPP rows, so the two halves may take different branches below); its
+  // count comes from the scalar unit (norm1 is the same on its lanes), 0 for
+  // a half that takes sub-steps
   const bool small = !(norm1 > 0.5);
-  const int nterm_r = small ? taylor_terms(norm1) : 0;
-  const int nmax = max(__builtin_amdgcn_readlane(nterm_r, 0), __builtin_amdgcn_readlane(nterm_r, 32));
+  const uint64_t sm = __ballot(small);
+  int c0, c1;
+  taylor_terms_halves(norm1, &c0, &c1);
+  if (!(sm & 1)) c0 = 0;
+  if (!((sm >> 32) & 1)) c1 = 0;
+  const int nterm_r = l < 32 ? c0 : c1;
+  const int nmax = max(c0, c1);
   if (nmax > 0) {
     double w = y.x;
     herm_terms<1>(nmax, nterm_r, a, bl, bu, w, y);
@@ -708,6 +744,18 @@ static __device__ void expm_e0_16_herm(int s, const double2* alpha, const double
   } else if (l < s) {
     c[l] = y;
   }
+}
+template <bool PAIR = false>
+static __device__ void expm_e0_16_herm(int s, const double2* alpha, const double* beta, double phi, double2* c) {
+  const int i = threadIdx.x & 15;
+  double a = 0.0, bl = 0.0, bu = 0.0, col1 = 0.0;
+  if (i < s) {
+    a = phi * alpha[i].x;
+    if (i >= 1) bl = phi * beta[i];
+    if (i + 1 < s) bu = phi * beta[i + 1];
+    col1 = fabs(a) + fabs(bl) + fabs(bu);
+  }
+  expm_e0_16_herm_row<PAIR>(s, a, bl, bu, row_max16(col1), c);
 }
 
 // Adaptive stopping metric after iteration k (one wave; tdse.py:473-476).

@@ -470,7 +470,7 @@ __global__ __launch_bounds__(ct_of<MODE>()) void k_long(LongArgs a) {
           }
         }
       }
-      const XT f0 = to_x<F32>(direct ? fp[0] : zero);
+      const XT f0 = to_x<F32>(load_or_zero(direct, fp));
       // V_0 = phi = P psi (unnormalised, tdse.py:443-446), the first matvec input
       for (int64_t i = r_beg; i < len; i += r_step) {
         const double2 v = op.phase ? cmul(S[i], P[i]) : S[i];

@@ -298,7 +298,7 @@ __global__ __launch_bounds__(ET) void k_expokit(ExpoArgs a) {
             const double2* Vj = V + (int64_t)j * a.lmax;
             ++nmult;
             expo_matvec(op, fp, lo, len, Vj, P);
-            const double2 hprev = j > 0 ? s_H[(j - 1) * MH + j] : zero;
+            const double2 hprev = load_or_zero(j > 0, s_H + (j > 0 ? (j - 1) * MH + j : 0));
             double d0 = 0.0, d1 = 0.0;
             for (int i = tid; i < len; i += ET) {
               double2 w = P[i];

@@ -445,6 +445,12 @@ __device__ __forceinline__ float2 wave_shift<float2>(float2 v, bool up) {
                      __int_as_float(wave_shift_i(__float_as_int(v.y), up)));
 }
 
+// no lane of half h (lanes 32 h .. 32 h + 31) has p (all 64 lanes active)
+__device__ __forceinline__ bool half_none(bool p, int h) {
+  const uint64_t m = __ballot(p);
+  return ((h ? (m >> 32) : m) & 0xffffffffull) == 0;
+}
+
 // v of lane src (ds_bpermute on the two halves of the double, no memory)
 __device__ __forceinline__ double lane_read(double v, int src) {
   const int addr = src << 2;
@@ -531,6 +537,20 @@ __global__ __launch_bounds__(64) void k_sub_pair(SubArgs a) {
       }
       herm = !__any(bad);
     }
+    // BAND: are my half's entries real (every slot and offset, imaginary
+    // parts within 1e-14 of the real ones, as tensors assembled in complex
+    // arithmetic leave them)?  Then with real field products the matvec
+    // multiplies by the real parts alone (decided per half, so a wavepacket's
+    // arithmetic never depends on the one it is paired with).
+    bool breal = false;
+    if constexpr (BAND) {
+      bool im = false;
+#pragma unroll
+      for (int sl = 0; sl < NSR; ++sl)
+#pragma unroll
+        for (int o = 0; o < 3; ++o) im |= fabs(Bo[sl][o].y) > 1e-14 * fabs(Bo[sl][o].x);
+      breal = half_none(im, h);
+    }
     // OBS: the observable's entries of my row as (source lane, value), and
     // summed per offset -1, 0, +1 when every entry inside the domain lies
     // within one row of the diagonal (then the rows come by DPP shifts, as
@@ -551,7 +571,7 @@ __global__ __launch_bounds__(64) void k_sub_pair(SubArgs a) {
         const bool e = ok && j < op.ow;
         const int c = e ? op.ocol[(int64_t)j * n + gq] - lo : -1;
         const bool in = c >= 0 && c < len;
-        const double2 v = in ? op.oval[(int64_t)j * n + gq] : zero;
+        const double2 v = load_or_zero(in, op.oval + (int64_t)j * n + gq);
         osrc[j] = in ? 32 * h + c : l;
         oval[j] = v;
         const int d = c - i;
@@ -563,7 +583,7 @@ __global__ __launch_bounds__(64) void k_sub_pair(SubArgs a) {
       for (int j = OWR; j < op.ow; ++j) {
         const int c = ok ? op.ocol[(int64_t)j * n + gq] - lo : -1;
         const bool in = c >= 0 && c < len;
-        const double2 v = in ? op.oval[(int64_t)j * n + gq] : zero;
+        const double2 v = load_or_zero(in, op.oval + (int64_t)j * n + gq);
         const int d = c - i;
 #pragma unroll
         for (int o = 0; o < 3; ++o)
@@ -587,7 +607,7 @@ __global__ __launch_bounds__(64) void k_sub_pair(SubArgs a) {
           for (int j = OWR; j < op.ow; ++j) {
             const int c = ok ? op.ocol[(int64_t)j * n + gq] - lo : -1;
             const bool in = c >= 0 && c < len;
-            const double2 v = in ? op.oval[(int64_t)j * n + gq] : zero;
+            const double2 v = load_or_zero(in, op.oval + (int64_t)j * n + gq);
             const int src = in ? 32 * h + c : l;
             y = cfma(v, make_double2(lane_read(st.x, src), lane_read(st.y, src)), y);
           }
@@ -611,8 +631,8 @@ __global__ __launch_bounds__(64) void k_sub_pair(SubArgs a) {
     // the step flag and the first two field products of the next step are
     // loaded while the current step computes (their latency is off the chain)
     int sf_next = op.nsteps > 0 ? op.sflag[0] : 0;
-    double2 f0_next = op.nsteps > 0 && op.nslot > 0 ? op.table[0] : zero;
-    double2 f1_next = op.nsteps > 0 && op.nslot > 1 ? op.table[1] : zero;
+    double2 f0_next = load_or_zero(op.nsteps > 0 && op.nslot > 0, op.table);
+    double2 f1_next = load_or_zero(op.nsteps > 0 && op.nslot > 1, op.table + 1);
     for (int step = 0; step < op.nsteps; ++step) {
       const int sf = sf_next;
       const double2 f0 = f0_next, f1 = f1_next;
@@ -650,17 +670,31 @@ __global__ __launch_bounds__(64) void k_sub_pair(SubArgs a) {
           A[j] = to_x<F32>(v);
         }
       }
+      const bool realA = BAND && breal && f0.y == 0.0 && f1.y == 0.0;
       auto matvec = [&]() __attribute__((always_inline)) -> double2 {
         const double2 z0 = make_double2(0.0, 0.0);
-        XT acc = to_x<F32>(z0);
+        XT acc;   // from the first product (an added zero is an instruction)
         if (BAND) {
+          // rows past the domain have no entries: their result is 0 as it is
           const XT x = to_x<F32>(vk);
-          acc = xfma(Ab[0], wave_shift<XT>(x, false), acc);
-          acc = xfma(Ab[1], x, acc);
-          acc = xfma(Ab[2], wave_shift<XT>(x, true), acc);
+          const XT xm = wave_shift<XT>(x, false), xp = wave_shift<XT>(x, true);
+          if (realA) {
+            acc.x = Ab[0].x * xm.x;
+            acc.y = Ab[0].x * xm.y;
+            acc.x = Ab[1].x * x.x + acc.x;
+            acc.y = Ab[1].x * x.y + acc.y;
+            acc.x = Ab[2].x * xp.x + acc.x;
+            acc.y = Ab[2].x * xp.y + acc.y;
+          } else {
+            acc = xmul(Ab[0], xm);
+            acc = xfma(Ab[1], x, acc);
+            acc = xfma(Ab[2], xp, acc);
+          }
+          return to_d2(acc);
         } else {
+          acc = xmul(A[0], s_x[cidx[0]]);
 #pragma unroll
-          for (int j = 0; j < WMR; ++j) acc = xfma(A[j], s_x[cidx[j]], acc);
+          for (int j = 1; j < WMR; ++j) acc = xfma(A[j], s_x[cidx[j]], acc);
         }
         return ok ? to_d2(acc) : z0;
       };
@@ -723,11 +757,13 @@ __global__ __launch_bounds__(64) void k_sub_pair(SubArgs a) {
         Wr = matvec();
         const double a0 = Wr.x * vk.x + Wr.y * vk.y, a1 = Wr.x * vk.y - Wr.y * vk.x;
         al = make_double2(half_allsum32(a0), 0.0);
+        // W - beta V_{k-1} is formed while alpha is summed, then - alpha V_k
+        // (one dependent operation after the sum instead of two)
         if (realal) {
-          Wr = csub(csub(Wr, cscale(al.x, vk)), cscale(beta, vp));
+          Wr = csub(csub(Wr, cscale(beta, vp)), cscale(al.x, vk));
         } else {
           al.y = half_allsum32(a1);
-          Wr = csub(csub(Wr, cmul(al, vk)), cscale(beta, vp));
+          Wr = csub(csub(Wr, cscale(beta, vp)), cmul(al, vk));
         }
         sa[it] = al;
         k = it + 1;
