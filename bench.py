@@ -383,7 +383,7 @@ def main():
                     help="timed steps of the large asymmetric-top (Kronecker-block) leg (0: skip)")
     ap.add_argument("--kron-accuracy-steps", type=int, default=60,
                     help="steps of the large-top FP32-vs-FP64 comparison (0: skip)")
-    ap.add_argument("--kron-kernel", default="gemm", choices=["gemm", "fused"],
+    ap.add_argument("--kron-kernel", default="gemm", choices=["gemm", "mfma", "fused"],
                     help="matvec form of the large asymmetric-top leg")
     ap.add_argument("--kron-only", action="store_true",
                     help="only the large asymmetric-top leg (for profiling its kernels) and exit")

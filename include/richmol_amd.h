@@ -418,9 +418,13 @@ int rm_kplan_info(const rm_kplan* plan, int64_t* n, double* gemm_flops_per_col);
  * m-mixed input (MF_w X) stored, then one rocBLAS product per output J, the
  * build of the next J chunk overlapping the products of the last.
  * RM_KKERNEL_FUSED: one launch per matvec on the matrix cores, the m-mixed
- * input built in LDS chunk by chunk inside the product (no stored copy). */
+ * input built in LDS chunk by chunk inside the product (no stored copy).
+ * RM_KKERNEL_MFMA: the stored form with the library's own matrix-core
+ * product (one launch per J chunk, 64 x 128 output tiles) in place of
+ * rocBLAS. */
 #define RM_KKERNEL_FUSED 0
 #define RM_KKERNEL_GEMM  1
+#define RM_KKERNEL_MFMA  2
 int rm_kplan_set_kernel(rm_kplan* plan, int32_t kernel);
 /* y = H(F) x for ncols wavepackets (device, complex, row b at b*ld, internal
  * order); fprod_dev: device, nslot complex.  precision RM_PREC_FP32: the

@@ -41,6 +41,12 @@ struct KTask {
   int64_t xoff;        // X[J2] (reals) in the split vector layout
 };
 
+// J descriptor of the stored form's matrix-core product (k_kp_mm)
+struct MJ {
+  int32_t M, N, K, pad;   // rows (wavepacket, m1), columns 2 dk1, stacked reals 2 Ktot
+  int64_t zoff, yoff;     // the J's Z block and output block
+};
+
 struct KBlk {          // per block: MF assembly from the field products
   int32_t dm1, nsig, ncart, slot0;
   int64_t mf_off, d_off;
@@ -82,6 +88,16 @@ struct rm_kplan {
   const int4* fz_tiles = nullptr;
   int64_t ntiles = 0;
   std::vector<char> h_fz;
+  // stored form on the matrix cores (k_kp_mm): per call, the J descriptors,
+  // the K-stack pointers and per matvec chunk its tiles
+  char* d_mm = nullptr;
+  int64_t mm_cap = 0;
+  std::vector<char> h_mm;
+  const MJ* mm_j = nullptr;
+  const int4* mm_tiles = nullptr;
+  const double* const* mm_kb64 = nullptr;
+  const float* const* mm_kb32 = nullptr;
+  std::vector<int64_t> mm_chunk;        // KCHUNK + 1 tile offsets
   int64_t* d_voff = nullptr;            // per J: complex offset (per wavepacket)
   int32_t* d_Js = nullptr;
   // per-call workspace
@@ -378,6 +394,106 @@ __global__ __launch_bounds__(512) void k_kp_fused(const int4* __restrict__ tiles
         for (int a = 0; a < 2; ++a) av[a] = sZ[cur][kq][wr * 32 + a * 16 + (lane & 15)];
 #pragma unroll
         for (int c = 0; c < 4; ++c) bv[c] = sK[cur][kq][wc * 64 + c * 16 + (lane & 15)];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int c = 0; c < 4; ++c) acc[a][c] = FAcc<T>::mfma(av[a], bv[c], acc[a][c]);
+      }
+    }
+    __syncthreads();
+  }
+  if (prod) return;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int col = n0 + wc * 64 + c * 16 + (lane & 15);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = r0 + wr * 32 + a * 16 + FAcc<T>::row(lane, v);
+        if (row < j.M && col < j.N) Y[j.yoff + (int64_t)row * j.N + col] = acc[a][c][v];
+      }
+    }
+}
+
+// The stored form's product Y[J1] = Z_J1 K_J1^T (real split form) on the
+// matrix cores: one workgroup of eight waves per (output J, FBM rows, FBN
+// columns) tile of a chunk's tile list.  Waves 0-3 multiply (16 x 16 x 4
+// MFMA, operand and result layouts as in k_kp_fused, 2 x 4 blocks each);
+// waves 4-7 copy FBK-deep slices of the Z rows and of the K stack (row-major
+// (2 Ktot) x (2 dk1)) into the other half of the double-buffered LDS tiles,
+// one slice ahead in registers (the loads of slice c + 2 are in flight while
+// slice c is multiplied); every wave passes the same barriers, one per slice.
+
+template <typename T>
+__global__ __launch_bounds__(512) void k_kp_mm(const int4* __restrict__ tiles, const MJ* __restrict__ jd,
+                                               const T* const* __restrict__ kb, const T* __restrict__ Z,
+                                               T* __restrict__ Y) {
+  using V = typename FAcc<T>::V;
+  __shared__ T sA[2][FBK][FBM];
+  __shared__ T sB[2][FBK][FBN];
+  const int4 tl = tiles[blockIdx.x];
+  const MJ j = jd[tl.x];
+  const T* __restrict__ B = kb[tl.x];
+  const int r0 = tl.y * FBM, n0 = tl.z * FBN;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const bool prod = w >= 4;
+  const int pt = t & 255;
+  const int nchunk = (j.K + FBK - 1) / FBK;
+  // copying thread pt: Z row r0 + pt / 4, stacked reals 4 (pt % 4) .. + 3 of
+  // the slice; K-stack row pt / 16 of the slice, columns 8 (pt % 16) .. + 7
+  const int ar = pt >> 2, ak = (pt & 3) * 4, bk = pt >> 4, bc = (pt & 15) * 8;
+  const bool arow = r0 + ar < j.M;
+  const T* __restrict__ Ar = Z + j.zoff + (int64_t)(arow ? r0 + ar : 0) * j.K;
+  T ra[4], rb[8];
+  auto load = [&](int ch) __attribute__((always_inline)) {
+    const int k0 = ch * FBK;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int k = k0 + ak + u;
+      ra[u] = (arow && k < j.K) ? Ar[k] : (T)0;
+    }
+    const int k = k0 + bk;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int c = n0 + bc + u;
+      rb[u] = (k < j.K && c < j.N) ? B[(int64_t)k * j.N + c] : (T)0;
+    }
+  };
+  auto store = [&](int buf) __attribute__((always_inline)) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) sA[buf][ak + u][ar] = ra[u];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) sB[buf][bk][bc + u] = rb[u];
+  };
+  V acc[2][4];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[a][c] = V{0, 0, 0, 0};
+  const int wr = w & 1, wc = (w >> 1) & 1;
+  if (prod && nchunk > 0) {
+    load(0);
+    store(0);
+    if (nchunk > 1) load(1);
+  }
+  __syncthreads();
+  for (int ch = 0; ch < nchunk; ++ch) {
+    const int cur = ch & 1;
+    if (prod) {
+      if (ch + 1 < nchunk) {
+        store(cur ^ 1);
+        if (ch + 2 < nchunk) load(ch + 2);
+      }
+    } else {
+#pragma unroll
+      for (int ks = 0; ks < FBK / 4; ++ks) {
+        const int kq = ks * 4 + (lane >> 4);
+        T av[2], bv[4];
+#pragma unroll
+        for (int a = 0; a < 2; ++a) av[a] = sA[cur][kq][wr * 32 + a * 16 + (lane & 15)];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) bv[c] = sB[cur][kq][wc * 64 + c * 16 + (lane & 15)];
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -753,7 +869,7 @@ __global__ void k_kp_out(const int64_t* __restrict__ voff, const int32_t* __rest
 void kplan_free(rm_kplan* p) {
   if (!p) return;
   void* bufs[] = {p->d_mdiag, p->d_sig, p->d_blk, p->d_mf, p->ws, p->d_tasks, p->d_voff, p->d_Js, p->d_fp, p->d_meta, p->d_kf64, p->d_kf32, p->d_colmap, p->d_fz,
-                  p->d_ex};
+                  p->d_ex, p->d_mm};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (double* k : p->kbig64)
@@ -1147,6 +1263,50 @@ int ktasks(rm_kplan* p, int32_t ncols, int64_t* ntask, std::vector<int64_t>& zof
     HIPCHK(hipMemcpyAsync(p->d_fz, p->h_fz.data(), p->h_fz.size(), hipMemcpyHostToDevice, s));
     p->fz_tiles = (const int4*)((const char*)p->d_fz + fjb);
   }
+  // stored form on the matrix cores: per chunk of output J its tiles (the
+  // longest stacks first), the J descriptors and the K-stack pointers
+  if (p->kernel == RM_KKERNEL_MFMA) {
+    std::vector<MJ> mj((size_t)p->nJ);
+    std::vector<int4> tl;
+    p->mm_chunk.assign(KCHUNK + 1, 0);
+    for (int c = 0; c < KCHUNK; ++c) {
+      p->mm_chunk[c] = (int64_t)tl.size();
+      const size_t first = tl.size();
+      for (int a = p->chunk_j[c]; a < p->chunk_j[c + 1]; ++a) {
+        if (p->src[a].empty()) continue;
+        MJ& d = mj[(size_t)a];
+        const int64_t dm = jdim(p->Js[a]);
+        if ((int64_t)ncols * dm > INT32_MAX || (int64_t)ncols * dm * 2 * p->ktot[a] > INT32_MAX)
+          return fail(RM_ERR_ARG, "matrix-core product: %d wavepackets exceed its 32-bit offsets", ncols);
+        d.M = (int32_t)(ncols * dm);
+        d.N = (int32_t)(2 * dm);
+        d.K = 2 * p->ktot[a];
+        d.pad = 0;
+        d.zoff = zoff_J[a];
+        d.yoff = p->h_voff[a] * 2 * ncols;
+        const int nrt = (d.M + FBM - 1) / FBM, nct = (d.N + FBN - 1) / FBN;
+        for (int rt = 0; rt < nrt; ++rt)
+          for (int ct = 0; ct < nct; ++ct) tl.push_back(make_int4(a, rt, ct, 0));
+      }
+      std::stable_sort(tl.begin() + first, tl.end(),
+                       [&](const int4& x, const int4& y) { return p->ktot[x.x] > p->ktot[y.x]; });
+    }
+    p->mm_chunk[KCHUNK] = (int64_t)tl.size();
+    if ((int64_t)tl.size() > INT32_MAX) return fail(RM_ERR_ARG, "too many matrix-core product tiles");
+    const size_t b_j = al256(sizeof(MJ) * mj.size()), b_p = al256(sizeof(void*) * (size_t)p->nJ);
+    p->h_mm.assign(b_j + 2 * b_p + sizeof(int4) * tl.size(), 0);
+    memcpy(p->h_mm.data(), mj.data(), sizeof(MJ) * mj.size());
+    memcpy(p->h_mm.data() + b_j, p->kbig64.data(), sizeof(void*) * (size_t)p->nJ);
+    memcpy(p->h_mm.data() + b_j + b_p, p->kbig32.data(), sizeof(void*) * (size_t)p->nJ);
+    if (!tl.empty()) memcpy(p->h_mm.data() + b_j + 2 * b_p, tl.data(), sizeof(int4) * tl.size());
+    int rc0 = grow(&p->d_mm, &p->mm_cap, (int64_t)p->h_mm.size());
+    if (rc0) return rc0;
+    HIPCHK(hipMemcpyAsync(p->d_mm, p->h_mm.data(), p->h_mm.size(), hipMemcpyHostToDevice, s));
+    p->mm_j = (const MJ*)p->d_mm;
+    p->mm_kb64 = (const double* const*)(p->d_mm + b_j);
+    p->mm_kb32 = (const float* const*)(p->d_mm + b_j + b_p);
+    p->mm_tiles = (const int4*)(p->d_mm + b_j + 2 * b_p);
+  }
   int rc = grow(&p->d_tasks, &p->tasks_cap, std::max<int64_t>(*ntask, 1));
   if (rc) return rc;
   if ((rc = grow(&p->d_meta, &p->meta_cap, (int64_t)p->h_meta.size()))) return rc;
@@ -1198,6 +1358,26 @@ int kmatvec(rm_kplan* p, const double* X, const double* ib, void* Y, void* Z, in
     }
     HIPCHK(hipEventRecord(p->ev[c], s));
     HIPCHK(hipStreamWaitEvent(p->gs, p->ev[c], 0));
+    if (p->kernel == RM_KKERNEL_MFMA) {
+      for (int a = j0; a < j1; ++a) {
+        if (!p->src[a].empty()) continue;
+        // no source block reaches this J: its output rows are zero
+        const int64_t d = jdim(p->Js[a]), yoff = p->h_voff[a] * 2 * ncols;
+        const int64_t cnt = f32 ? (int64_t)ncols * d * d : 2 * (int64_t)ncols * d * d;   // in doubles
+        double* y0 = f32 ? (double*)((float*)Y + yoff) : (double*)Y + yoff;
+        hipLaunchKernelGGL(k_kp_zero, dim3(grid_for(cnt, 256)), dim3(256), 0, p->gs, y0, cnt);
+      }
+      const int64_t m0 = p->mm_chunk[c], mt = p->mm_chunk[c + 1] - m0;
+      if (mt > 0) {
+        if (f32)
+          hipLaunchKernelGGL(k_kp_mm<float>, dim3((unsigned)mt), dim3(512), 0, p->gs, p->mm_tiles + m0, p->mm_j,
+                             p->mm_kb32, (const float*)Z, (float*)Y);
+        else
+          hipLaunchKernelGGL(k_kp_mm<double>, dim3((unsigned)mt), dim3(512), 0, p->gs, p->mm_tiles + m0, p->mm_j,
+                             p->mm_kb64, (const double*)Z, (double*)Y);
+      }
+      continue;
+    }
     for (int a = j0; a < j1; ++a) {
       const int64_t d = jdim(p->Js[a]);
       const int64_t yoff = p->h_voff[a] * 2 * ncols;
@@ -1451,7 +1631,7 @@ static int kplan_propagate(rm_kplan* p, void* psi, int64_t ld, int32_t ncols, in
 
 extern "C" int rm_kplan_set_kernel(rm_kplan* p, int32_t kernel) {
   if (!p) return fail(RM_ERR_ARG, "rm_kplan_set_kernel: NULL plan");
-  if (kernel != RM_KKERNEL_FUSED && kernel != RM_KKERNEL_GEMM)
+  if (kernel != RM_KKERNEL_FUSED && kernel != RM_KKERNEL_GEMM && kernel != RM_KKERNEL_MFMA)
     return fail(RM_ERR_ARG, "rm_kplan_set_kernel: unknown kernel %d", kernel);
   p->kernel = kernel;
   return RM_OK;

@@ -365,11 +365,13 @@ class DeviceKronPlan:
         except Exception:
             pass
 
-    KERNELS = {"fused": 0, "gemm": 1}
+    KERNELS = {"fused": 0, "gemm": 1, "mfma": 2}
 
     def set_kernel(self, kernel):
         """'gemm' (default): the m-mixed input stored, rocBLAS per output J;
-        'fused': one MFMA launch per matvec, the m-mixed input built in LDS."""
+        'mfma': the m-mixed input stored, the library's own matrix-core
+        product per J chunk; 'fused': one MFMA launch per matvec, the m-mixed
+        input built in LDS."""
         self._lib.check(self._lib.load().rm_kplan_set_kernel(self.handle, self.KERNELS[kernel]),
                         "rm_kplan_set_kernel")
         self.kernel = kernel

@@ -24,7 +24,7 @@ def _to_int(kop, Xf):
     return Xi
 
 
-@pytest.mark.parametrize("kernel", ["fused", "gemm"])
+@pytest.mark.parametrize("kernel", ["fused", "gemm", "mfma"])
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("fp32", 2e-6)])
 def test_matvec_vs_host_factorisation(device, precision, tol, kernel):
     """J <= 12 (2625 states; m blocks of up to 25 rows span two Z tiles), 5
@@ -68,8 +68,9 @@ def test_matvec_strided_rows(device):
 def test_fused_matvec_long_stacks(device):
     """J <= 24 (2 401 states; output blocks of up to 49 x 98 reals, stacks of
     up to nine source blocks, several row and column tiles per J): the fused
-    kernel equals the stored-Z rocBLAS form to rounding, and the host
-    factorisation, in FP64; 3 wavepackets."""
+    kernel and the stored form's own matrix-core product equal the stored-Z
+    rocBLAS form to rounding, and the host factorisation, in FP64; 3
+    wavepackets."""
     import torch
     from richmol_amd.kron import slot_products
     kop, bas = large_top(24)
@@ -80,7 +81,7 @@ def test_fused_matvec_long_stacks(device):
     fp = torch.from_numpy(slot_products(kop, F)[0]).to(device)
     x = torch.from_numpy(X).to(device)
     out = {}
-    for k in ("fused", "gemm"):
+    for k in ("fused", "gemm", "mfma"):
         dp.set_kernel(k)
         y = torch.zeros_like(x)
         dp.matvec(x, y, fp)
@@ -88,7 +89,9 @@ def test_fused_matvec_long_stacks(device):
     ref = kop.matvec(X, kop.field_products([F[0][0], F[1][0]]))
     scale = np.abs(ref).max()
     assert np.abs(out["fused"] - out["gemm"]).max() <= 1e-13 * scale
+    assert np.abs(out["mfma"] - out["gemm"]).max() <= 1e-13 * scale
     assert np.abs(out["fused"] - ref).max() <= 1e-13 * scale
+    assert np.abs(out["mfma"] - ref).max() <= 1e-13 * scale
 
 
 def _oracle_run(bas, fields, Xf, fixed, dt):
@@ -131,7 +134,7 @@ def _columns(n, seed):
     return X
 
 
-@pytest.mark.parametrize("kernel", ["gemm", "fused"])
+@pytest.mark.parametrize("kernel", ["gemm", "fused", "mfma"])
 @pytest.mark.parametrize("fixed", [12, 5])
 def test_propagate_matches_oracle(device, fixed, kernel):
     """J <= 5 (286 states), four wavepackets (basis states, dense, unnormalised),
@@ -153,7 +156,7 @@ def test_propagate_matches_oracle(device, fixed, kernel):
         assert np.abs(np.linalg.norm(got, axis=1) - np.linalg.norm(X, axis=1)).max() <= 1e-12
 
 
-@pytest.mark.parametrize("kernel", ["gemm", "fused"])
+@pytest.mark.parametrize("kernel", ["gemm", "fused", "mfma"])
 def test_propagate_fp32_products(device, kernel):
     """FP32 dense products (complex64 operands and output; recurrence, norms,
     exp(fT), phases and wavepackets FP64): amplitudes within 2e-6 and
