@@ -420,8 +420,8 @@ int rm_kplan_info(const rm_kplan* plan, int64_t* n, double* gemm_flops_per_col);
  * RM_KKERNEL_FUSED: one launch per matvec on the matrix cores, the m-mixed
  * input built in LDS chunk by chunk inside the product (no stored copy).
  * RM_KKERNEL_MFMA: the stored form with the library's own matrix-core
- * product (one launch per J chunk, 64 x 128 output tiles) in place of
- * rocBLAS. */
+ * product in place of rocBLAS (one launch per J chunk; FP64 64 x 128 output
+ * tiles, FP32 128 x 128). */
 #define RM_KKERNEL_FUSED 0
 #define RM_KKERNEL_GEMM  1
 #define RM_KKERNEL_MFMA  2

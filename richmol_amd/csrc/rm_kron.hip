@@ -94,10 +94,11 @@ struct rm_kplan {
   int64_t mm_cap = 0;
   std::vector<char> h_mm;
   const MJ* mm_j = nullptr;
-  const int4* mm_tiles = nullptr;
+  const int4* mm_tiles = nullptr;       // FBM x FBN tiles (FP64, k_kp_mm)
+  const int4* mm_tiles_w = nullptr;     // MBM x MBN tiles (FP32, k_kp_mm_w)
   const double* const* mm_kb64 = nullptr;
   const float* const* mm_kb32 = nullptr;
-  std::vector<int64_t> mm_chunk;        // KCHUNK + 1 tile offsets
+  std::vector<int64_t> mm_chunk, mm_chunk_w;   // KCHUNK + 1 tile offsets each
   int64_t* d_voff = nullptr;            // per J: complex offset (per wavepacket)
   int32_t* d_Js = nullptr;
   // per-call workspace
@@ -503,6 +504,99 @@ __global__ __launch_bounds__(512) void k_kp_mm(const int4* __restrict__ tiles, c
     __syncthreads();
   }
   if (prod) return;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int col = n0 + wc * 64 + c * 16 + (lane & 15);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = r0 + wr * 32 + a * 16 + FAcc<T>::row(lane, v);
+        if (row < j.M && col < j.N) Y[j.yoff + (int64_t)row * j.N + col] = acc[a][c][v];
+      }
+    }
+}
+
+// The same product for FP32 operands, with every wave multiplying: one
+// workgroup of eight waves per (output J, MBM rows, MBN columns) tile, each
+// wave a 32 x 64 block (2 x 4 MFMA blocks).  Each thread issues its loads of
+// slice c + 1 before the products of slice c and writes them to the other
+// LDS buffer after them, one barrier per slice; the LDS rows are padded so
+// that the k rows read by one instruction fall on different banks.  (For
+// FP64 operands the form above, with copying waves beside the multiplying
+// ones, measures faster; for FP32 this one.)
+constexpr int MBM = 128, MBN = 128;
+constexpr int MPAD = 16;
+
+template <typename T>
+__global__ __launch_bounds__(512) void k_kp_mm_w(const int4* __restrict__ tiles, const MJ* __restrict__ jd,
+                                                 const T* const* __restrict__ kb, const T* __restrict__ Z,
+                                                 T* __restrict__ Y) {
+  using V = typename FAcc<T>::V;
+  __shared__ T sA[2][FBK][MBM + MPAD];
+  __shared__ T sB[2][FBK][MBN + MPAD];
+  const int4 tl = tiles[blockIdx.x];
+  const MJ j = jd[tl.x];
+  const T* __restrict__ B = kb[tl.x];
+  const int r0 = tl.y * MBM, n0 = tl.z * MBN;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int nchunk = (j.K + FBK - 1) / FBK;
+  // thread t copies Z row r0 + t / 4, stacked reals 4 (t % 4) .. + 3 of a
+  // slice, and K-stack row t / 32 of the slice, columns 4 (t % 32) .. + 3
+  const int ar = t >> 2, ak = (t & 3) * 4, bk = t >> 5, bc = (t & 31) * 4;
+  const bool arow = r0 + ar < j.M;
+  const T* __restrict__ Ar = Z + j.zoff + (int64_t)(arow ? r0 + ar : 0) * j.K;
+  T ra[4], rb[4];
+  auto load = [&](int ch) __attribute__((always_inline)) {
+    const int k0 = ch * FBK;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int k = k0 + ak + u;
+      ra[u] = (arow && k < j.K) ? Ar[k] : (T)0;
+    }
+    const int k = k0 + bk;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int c = n0 + bc + u;
+      rb[u] = (k < j.K && c < j.N) ? B[(int64_t)k * j.N + c] : (T)0;
+    }
+  };
+  auto store = [&](int buf) __attribute__((always_inline)) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) sA[buf][ak + u][ar] = ra[u];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) sB[buf][bk][bc + u] = rb[u];
+  };
+  V acc[2][4];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[a][c] = V{0, 0, 0, 0};
+  const int wr = w & 3, wc = w >> 2;   // 4 x 2 waves of 32 x 64
+  if (nchunk > 0) {
+    load(0);
+    store(0);
+  }
+  __syncthreads();
+  for (int ch = 0; ch < nchunk; ++ch) {
+    const int cur = ch & 1;
+    if (ch + 1 < nchunk) load(ch + 1);
+#pragma unroll
+    for (int ks = 0; ks < FBK / 4; ++ks) {
+      const int kq = ks * 4 + (lane >> 4);
+      T av[2], bv[4];
+#pragma unroll
+      for (int a = 0; a < 2; ++a) av[a] = sA[cur][kq][wr * 32 + a * 16 + (lane & 15)];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) bv[c] = sB[cur][kq][wc * 64 + c * 16 + (lane & 15)];
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[a][c] = FAcc<T>::mfma(av[a], bv[c], acc[a][c]);
+    }
+    if (ch + 1 < nchunk) store(cur ^ 1);
+    __syncthreads();
+  }
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -1267,11 +1361,13 @@ int ktasks(rm_kplan* p, int32_t ncols, int64_t* ntask, std::vector<int64_t>& zof
   // longest stacks first), the J descriptors and the K-stack pointers
   if (p->kernel == RM_KKERNEL_MFMA) {
     std::vector<MJ> mj((size_t)p->nJ);
-    std::vector<int4> tl;
+    std::vector<int4> tl, tw;
     p->mm_chunk.assign(KCHUNK + 1, 0);
+    p->mm_chunk_w.assign(KCHUNK + 1, 0);
     for (int c = 0; c < KCHUNK; ++c) {
       p->mm_chunk[c] = (int64_t)tl.size();
-      const size_t first = tl.size();
+      p->mm_chunk_w[c] = (int64_t)tw.size();
+      const size_t first = tl.size(), first_w = tw.size();
       for (int a = p->chunk_j[c]; a < p->chunk_j[c + 1]; ++a) {
         if (p->src[a].empty()) continue;
         MJ& d = mj[(size_t)a];
@@ -1287,18 +1383,25 @@ int ktasks(rm_kplan* p, int32_t ncols, int64_t* ntask, std::vector<int64_t>& zof
         const int nrt = (d.M + FBM - 1) / FBM, nct = (d.N + FBN - 1) / FBN;
         for (int rt = 0; rt < nrt; ++rt)
           for (int ct = 0; ct < nct; ++ct) tl.push_back(make_int4(a, rt, ct, 0));
+        const int nrw = (d.M + MBM - 1) / MBM, ncw = (d.N + MBN - 1) / MBN;
+        for (int rt = 0; rt < nrw; ++rt)
+          for (int ct = 0; ct < ncw; ++ct) tw.push_back(make_int4(a, rt, ct, 0));
       }
-      std::stable_sort(tl.begin() + first, tl.end(),
-                       [&](const int4& x, const int4& y) { return p->ktot[x.x] > p->ktot[y.x]; });
+      const auto longer = [&](const int4& x, const int4& y) { return p->ktot[x.x] > p->ktot[y.x]; };
+      std::stable_sort(tl.begin() + first, tl.end(), longer);
+      std::stable_sort(tw.begin() + first_w, tw.end(), longer);
     }
     p->mm_chunk[KCHUNK] = (int64_t)tl.size();
+    p->mm_chunk_w[KCHUNK] = (int64_t)tw.size();
     if ((int64_t)tl.size() > INT32_MAX) return fail(RM_ERR_ARG, "too many matrix-core product tiles");
     const size_t b_j = al256(sizeof(MJ) * mj.size()), b_p = al256(sizeof(void*) * (size_t)p->nJ);
-    p->h_mm.assign(b_j + 2 * b_p + sizeof(int4) * tl.size(), 0);
+    const size_t b_t = al256(sizeof(int4) * tl.size());
+    p->h_mm.assign(b_j + 2 * b_p + b_t + sizeof(int4) * tw.size(), 0);
     memcpy(p->h_mm.data(), mj.data(), sizeof(MJ) * mj.size());
     memcpy(p->h_mm.data() + b_j, p->kbig64.data(), sizeof(void*) * (size_t)p->nJ);
     memcpy(p->h_mm.data() + b_j + b_p, p->kbig32.data(), sizeof(void*) * (size_t)p->nJ);
     if (!tl.empty()) memcpy(p->h_mm.data() + b_j + 2 * b_p, tl.data(), sizeof(int4) * tl.size());
+    if (!tw.empty()) memcpy(p->h_mm.data() + b_j + 2 * b_p + b_t, tw.data(), sizeof(int4) * tw.size());
     int rc0 = grow(&p->d_mm, &p->mm_cap, (int64_t)p->h_mm.size());
     if (rc0) return rc0;
     HIPCHK(hipMemcpyAsync(p->d_mm, p->h_mm.data(), p->h_mm.size(), hipMemcpyHostToDevice, s));
@@ -1306,6 +1409,7 @@ int ktasks(rm_kplan* p, int32_t ncols, int64_t* ntask, std::vector<int64_t>& zof
     p->mm_kb64 = (const double* const*)(p->d_mm + b_j);
     p->mm_kb32 = (const float* const*)(p->d_mm + b_j + b_p);
     p->mm_tiles = (const int4*)(p->d_mm + b_j + 2 * b_p);
+    p->mm_tiles_w = (const int4*)(p->d_mm + b_j + 2 * b_p + b_t);
   }
   int rc = grow(&p->d_tasks, &p->tasks_cap, std::max<int64_t>(*ntask, 1));
   if (rc) return rc;
@@ -1367,11 +1471,12 @@ int kmatvec(rm_kplan* p, const double* X, const double* ib, void* Y, void* Z, in
         double* y0 = f32 ? (double*)((float*)Y + yoff) : (double*)Y + yoff;
         hipLaunchKernelGGL(k_kp_zero, dim3(grid_for(cnt, 256)), dim3(256), 0, p->gs, y0, cnt);
       }
-      const int64_t m0 = p->mm_chunk[c], mt = p->mm_chunk[c + 1] - m0;
+      const std::vector<int64_t>& co = f32 ? p->mm_chunk_w : p->mm_chunk;
+      const int64_t m0 = co[c], mt = co[c + 1] - m0;
       if (mt > 0) {
         if (f32)
-          hipLaunchKernelGGL(k_kp_mm<float>, dim3((unsigned)mt), dim3(512), 0, p->gs, p->mm_tiles + m0, p->mm_j,
-                             p->mm_kb32, (const float*)Z, (float*)Y);
+          hipLaunchKernelGGL(k_kp_mm_w<float>, dim3((unsigned)mt), dim3(512), 0, p->gs, p->mm_tiles_w + m0,
+                             p->mm_j, p->mm_kb32, (const float*)Z, (float*)Y);
         else
           hipLaunchKernelGGL(k_kp_mm<double>, dim3((unsigned)mt), dim3(512), 0, p->gs, p->mm_tiles + m0, p->mm_j,
                              p->mm_kb64, (const double*)Z, (double*)Y);
