@@ -1670,11 +1670,13 @@ int sub_propagate(rm_plan* p, void* psi, int64_t ld, int64_t ncols, int64_t nste
   const bool reuse = !expo && p->dom_reuse && sc->kept && sc->kept_psi == psi && sc->kept_ld == ld &&
                      sc->kept_ncols == ncols && sc->kept_force == force && sc->kept_pairs == pairs;
   // step flags in chunks of <= 1024 terms, at most 16 per step; from 64
-  // steps on one block per step (a step with a field ends its scan at the
-  // first non-zero element, in its first pass: more blocks per step would
-  // only repeat that first pass)
-  const int32_t parts = nsteps >= 64 ? 1
-                                     : (int32_t)std::max<int64_t>(1, std::min<int64_t>(16, (p->nterm + 1023) / 1024));
+  // steps on, or for up to 8192 terms, one block per step (a step with a
+  // field ends its scan at the first non-zero element, in its first pass:
+  // more blocks per step would only repeat that first pass, and a step
+  // without one scans at most 32 passes)
+  const int32_t parts = (nsteps >= 64 || p->nterm <= 8192)
+                            ? 1
+                            : (int32_t)std::max<int64_t>(1, std::min<int64_t>(16, (p->nterm + 1023) / 1024));
   const int64_t chunk = std::max<int64_t>(1, (p->nterm + parts - 1) / parts);
   if (nsteps * parts > INT32_MAX) return RM_OK;
   if ((rc = grow(&p->sflag, &p->sflag_cap, nsteps))) return rc;

@@ -446,12 +446,19 @@ class DevicePlan:
         _check_cols(psi, self.n); _check_cols(work, self.n)
         if psi.stride(0) != work.stride(0):
             raise ValueError("psi and work need the same row stride")
-        th = np.full(max(self.nleaf, 1), -1.0)
-        if thresh is not None:
+        if thresh is None:
+            # no thresholds: one array kept per plan (its address passed every call)
+            if getattr(self, "_th_none", None) is None:
+                self._th_none = np.full(max(self.nleaf, 1), -1.0)
+                self._th_none_ptr = _lib.ptr(self._th_none, ctypes.c_double)
+            th_ptr = self._th_none_ptr
+        else:
+            th = np.full(max(self.nleaf, 1), -1.0)
             th[: len(thresh)] = [t if (t is not None and t > 0) else -1.0 for t in thresh]
+            th_ptr = _lib.ptr(th, ctypes.c_double)
         vp = ctypes.c_void_p
         args = (self.handle, vp(psi.data_ptr()), vp(work.data_ptr()), psi.stride(0), psi.shape[0],
-                table.shape[0], vp(table.data_ptr()), _lib.ptr(th, ctypes.c_double),
+                table.shape[0], vp(table.data_ptr()), th_ptr,
                 float(f.real), float(f.imag), vp(phase.data_ptr() if phase is not None else None),
                 int(krylov_fixed), int(maxorder), float(tol),
                 vp(kused.data_ptr() if kused is not None else None),
