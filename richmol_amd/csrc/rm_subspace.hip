@@ -535,7 +535,7 @@ __global__ __launch_bounds__(64) void k_sub_pair(SubArgs a) {
         const bool diag_ok = !ok || fabs(Bo[sl][1].y) <= 1e-14 * fabs(Bo[sl][1].x);
         bad |= !(pair_ok && diag_ok);
       }
-      herm = !__any(bad);
+      herm = half_none(bad, h);   // per half, as the operator may be Hermitian on some domains only
     }
     // BAND: are my half's entries real (every slot and offset, imaginary
     // parts within 1e-14 of the real ones, as tensors assembled in complex
@@ -705,7 +705,9 @@ __global__ __launch_bounds__(64) void k_sub_pair(SubArgs a) {
       }
       // Hermitian domain operator and real field products: alpha is real,
       // and with a real-time exponent f = i phi the Krylov exponential takes
-      // the real recursion (a wave-wide vote: both halves share the operator)
+      // the real recursion (decided per half; where the halves differ both
+      // forms run under their lane masks, and the exponentials' lane reads of
+      // the other half only bound the unrolled number of terms)
       const bool realal = herm && __all(f0.y == 0.0 && f1.y == 0.0);
       const bool realexp = realal && __all(op.f.x == 0.0);
       double2 Wr = matvec(), vp = zero;

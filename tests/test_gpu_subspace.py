@@ -207,6 +207,56 @@ def test_pair_kernel_result_independent_of_partner(device, precision):
         assert np.abs(a - psi_o).max() <= TOL_PSI
 
 
+def test_pair_kernel_operator_complex_on_some_domains(device):
+    """A Z-field operator made complex and non-Hermitian on the m = 0 chains
+    only (H + 0.1j H restricted to m = 0): a wave pairs a domain that takes
+    the real forms (real coefficients, real alpha, the real exponential) with
+    one that takes the complex ones, decided per half.  Every column's bits
+    equal its run alone (and in another order), and the oracle's sum of the
+    two tensors to 1e-10."""
+    import copy
+    from scipy.sparse import csr_matrix
+    h0, H, cos2, bas = linear_rotor(12)
+    H = copy.deepcopy(H)
+    # the m = 0 -> m = 0 entries of every M matrix (row and column index m + J)
+    Hm = copy.deepcopy(H)
+    for (J1, J2), a1 in Hm.mmat.items():
+        for a2 in a1.values():
+            for a3 in a2.values():
+                for cart, m in list(a3.items()):
+                    c = m.tocoo()
+                    keep = (c.row == J1) & (c.col == J2)
+                    a3[cart] = csr_matrix((c.data[keep], (c.row[keep], c.col[keep])), shape=m.shape)
+    Hm = Hm * 0.1j
+    H.field([0.0, 0.0, 1.0])
+    Hm.field([0.0, 0.0, 1.0])
+    Hc = H + Hm
+    n = 169
+    jm = [(0, 0), (1, 1), (2, 0), (3, -1), (4, 0), (2, 2), (6, 0), (5, -3)]
+    X = _basis_columns(n, [J * J + J + m for J, m in jm])
+    t = np.arange(25, 31) * 10.0 + 5.0
+    fields = [[0.0, 0.0, gaussian_pulse_field(tt)] for tt in t]
+    a, _, _ = _device_run(h0, Hc, fields, X, fixed=12)
+    order = np.array([1, 0, 3, 2, 5, 4, 7, 6])
+    b, _, _ = _device_run(h0, Hc, fields, X[order], fixed=12)
+    assert np.array_equal(a[order], b)
+    for c in (0, 1, 5):
+        one, _, _ = _device_run(h0, Hc, fields, X[c:c + 1], fixed=12)
+        assert np.array_equal(one[0], a[c])
+    Oh0 = OracleCarTens.from_attrs(h0)
+    OA, OB = OracleCarTens.from_attrs(H), OracleCarTens.from_attrs(Hm)
+    to = OracleTDSE(t_start=0.0, t_end=10.0 * len(fields), dt=10.0, t_units="fs")
+    to.time_grid()
+    psi = X.copy()
+    for F in fields:
+        OA.field(F)
+        OB.field(F)
+        psi, _ = to.update(OA + OB, psi, H0=Oh0, fixed_m=12)
+    assert np.abs(a - psi).max() <= TOL_PSI
+    # the m = 0 columns did feel the complex part
+    assert np.abs(np.abs(a[0]) - np.abs(_device_run(h0, H, fields, X[:1], fixed=12)[0][0])).max() > 1e-6
+
+
 def test_auto_equals_subspace_and_ell_agrees(device):
     """'auto' takes the subspace kernel for rm_propagate (bitwise the same
     run) and agrees with the full-vector ELL kernel to rounding."""
